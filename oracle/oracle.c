@@ -1,0 +1,255 @@
+/*
+ * oracle.c -- CPU restatement of HONEI's tags::CPU sparse-solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP backend.  Loaded by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg; never by
+ * the product library.  Built with -ffp-contract=off (see Makefile).
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_T double
+#define OR_S f64
+#define OR_IS_FLOAT 0
+#include "oracle_impl.h"
+#undef OR_T
+#undef OR_S
+#undef OR_IS_FLOAT
+
+#define OR_T float
+#define OR_S f32
+#define OR_IS_FLOAT 1
+#include "oracle_impl.h"
+#undef OR_T
+#undef OR_S
+#undef OR_IS_FLOAT
+
+/* honei/la/sparse_matrix_ell-impl.hh:85-101 */
+void or_ell_shape(uint64_t rows, const uint64_t *rowptr, uint64_t threads,
+                  uint64_t *stride, uint64_t *num_cols_per_row)
+{
+    uint64_t ncpr = 1;
+    for (uint64_t i = 0; i < rows; ++i) {
+        uint64_t used = rowptr[i + 1] - rowptr[i];
+        if (used > ncpr)
+            ncpr = used;
+    }
+    *num_cols_per_row = (uint64_t)ceil((double)ncpr / (double)threads);
+    const uint64_t alignment = 32;
+    *stride = alignment * (((rows * threads) + alignment - 1) / alignment);
+}
+
+/* SplitMix64 finaliser (Steele, Lea, Flood: "Fast splittable pseudorandom
+ * number generators", OOPSLA 2014). */
+static uint64_t or_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint64_t or_rng_bits(uint64_t seed, uint64_t stream, uint64_t index)
+{
+    const uint64_t gamma = 0x9E3779B97F4A7C15ull;
+    uint64_t key = or_mix64(seed + gamma * (stream + 1));
+    return or_mix64(key + gamma * (index + 1));
+}
+
+double or_rng_uniform(uint64_t seed, uint64_t stream, uint64_t index, double lo, double hi)
+{
+    double u = (double)(or_rng_bits(seed, stream, index) >> 11) * 0x1.0p-53;
+    return lo + (hi - lo) * u;
+}
+
+/* TypeTraits<double>::convert / TypeTraits<float>::convert
+ * (honei/util/type_traits.cc:107-123): plain C casts, round to nearest. */
+void or_convert_d2f(float *dst, const double *src, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        dst[i] = (float)src[i];
+}
+
+void or_convert_f2d(double *dst, const float *src, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        dst[i] = (double)src[i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multigrid<Tag, OuterTag, NONE, JAC, CYCLE::V, MIXED> on Q1 bands.         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct or_mg_level {
+    uint64_t n;
+    const float *const *a;
+    const float *dinv;
+    float *c, *d, *rhs, *x, *tmp;
+} or_mg_level;
+
+/* _multigrid_kernel<float>: honei/math/multigrid.hh:1620-1933 (V-cycle only;
+ * the local W-cycle branch of the prolongation loop is unreachable for V). */
+static void or_mg_kernel(or_mg_level *lv, const or_mg_params *p, const float tol,
+                         const float adapt, uint64_t *inner_iterations)
+{
+    const uint64_t minl = p->min_level, maxl = p->max_level;
+    or_mg_level *top = &lv[maxl];
+    float defect, initial_defect;
+
+    if (p->initial_zero)
+        memcpy(top->d, top->rhs, sizeof(float) * top->n);
+    else
+        or_q1_defect_f32(top->d, top->rhs, top->a, top->x, top->n);
+
+    defect = or_norm_l2_f32(top->d, top->n, 1);
+    initial_defect = defect;
+
+    if (p->convergence_check && defect <= tol)
+        return;
+
+    uint64_t iter;
+    for (iter = 1; iter <= p->n_max_iter; ++iter) {
+        uint64_t L = maxl;
+        int restriction_started = 1;
+        /* restriction loop */
+        while (minl != maxl) {
+            or_mg_level *l = &lv[L];
+            if (restriction_started) {
+                or_element_product3_f32(l->c, l->dinv, l->d, l->n);
+                or_jacobi_q1_f32(l->c, l->a, l->d, l->dinv, l->n, p->n_pre_smooth - 1, l->tmp);
+                or_sum_f32(l->x, l->c, l->n);
+            } else {
+                or_element_product3_f32(l->x, l->dinv, l->d, l->n);
+                or_jacobi_q1_f32(l->x, l->a, l->d, l->dinv, l->n, p->n_pre_smooth - 1, l->tmp);
+            }
+            or_q1_defect_f32(l->d, l->rhs, l->a, l->x, l->n);
+            --L;
+            or_restriction_f32(lv[L].d, lv[L + 1].d, lv[L + 1].n, p->macro_border_mask);
+            memcpy(lv[L].rhs, lv[L].d, sizeof(float) * lv[L].n);
+            if (L == minl)
+                break;
+            restriction_started = 0;
+        }
+        /* coarse grid correction */
+        {
+            uint64_t cg_iters;
+            or_cg_konv_q1_f32(lv[L].a, lv[L].d, lv[L].x, lv[L].n, FLT_EPSILON, 0, &cg_iters);
+        }
+        if (minl == maxl) {
+            or_q1_defect_f32(lv[L].d, lv[L].rhs, lv[L].a, lv[L].x, lv[L].n);
+            break;
+        }
+        /* prolongation loop */
+        while (1) {
+            ++L;
+            or_mg_level *l = &lv[L];
+            or_prolongation_f32(l->c, lv[L - 1].x, l->n, p->macro_border_mask);
+            float alpha;
+            if (adapt == 0.0f) {
+                float d1 = or_dot_f32(l->d, l->c, l->n);
+                or_q1_product_f32(l->tmp, l->a, l->c, l->n);
+                float d2 = or_dot_f32(l->tmp, l->c, l->n);
+                alpha = d1 / d2;
+            } else {
+                alpha = adapt;
+            }
+            or_scaled_sum_f32(l->x, l->c, alpha, l->n);
+            or_jacobi_q1_f32(l->x, l->a, l->rhs, l->dinv, l->n, p->n_pre_smooth, l->tmp);
+            or_q1_defect_f32(l->d, l->rhs, l->a, l->x, l->n);
+            if (L == maxl)
+                break;
+        }
+        defect = or_norm_l2_f32(top->d, top->n, 1);
+        if (defect <= initial_defect * tol)
+            break;
+    }
+    *inner_iterations = (iter > p->n_max_iter) ? iter - 1 : iter;
+}
+
+/* Multigrid<...MIXED>::value(BandedMatrixQx<Q1Type,double>&, b, x, ...):
+ * honei/math/multigrid.hh:2253-2393. */
+int or_mg_mixed_q1(const double *const *A64, const double *b, double *x,
+                   const float *const *a32, const float *const *dinv32,
+                   const or_mg_params *p, uint64_t *outer_iters, uint64_t *inner_iters,
+                   double *final_defect, double *initial_defect_out)
+{
+    if (p->max_level < p->min_level || p->max_level > 14 || p->n_pre_smooth < 1)
+        return -1;
+    const uint64_t maxl = p->max_level;
+    const uint64_t n = ((1ull << maxl) + 1) * ((1ull << maxl) + 1);
+    /* MGInfo<float, ...> stores tolerance and the correction factor as float */
+    const float tol = (float)p->tolerance;
+    const float adapt = (float)p->adapt_correction_factor;
+
+    or_mg_level lv[15];
+    memset(lv, 0, sizeof(lv));
+    for (uint64_t L = p->min_level; L <= maxl; ++L) {
+        uint64_t nl = ((1ull << L) + 1) * ((1ull << L) + 1);
+        lv[L].n = nl;
+        lv[L].a = a32 + 9 * L;
+        lv[L].dinv = dinv32[L];
+        lv[L].c = (float *)calloc(nl, sizeof(float));
+        lv[L].d = (float *)calloc(nl, sizeof(float));
+        lv[L].rhs = (float *)calloc(nl, sizeof(float));
+        lv[L].x = (float *)calloc(nl, sizeof(float));
+        lv[L].tmp = (float *)calloc(nl, sizeof(float));
+    }
+
+    double *result = (double *)calloc(n, sizeof(double));
+    double *ig = (double *)malloc(sizeof(double) * n);
+    double *od = (double *)malloc(sizeof(double) * n);
+    double *prod = (double *)malloc(sizeof(double) * n);
+    double *x_outer = (double *)malloc(sizeof(double) * n);
+    memcpy(ig, x, sizeof(double) * n);
+
+    uint64_t sqrt_N = (uint64_t)sqrt((double)n);
+    for (uint64_t i = 0; i < sqrt_N; ++i) ig[i] = b[i];
+    for (uint64_t i = n - sqrt_N; i < n; ++i) ig[i] = b[i];
+    for (uint64_t i = 0; i < n; i += sqrt_N) ig[i] = b[i];
+    or_convert_d2f(lv[maxl].x, ig, n);
+
+    uint64_t inner_total = 0, outer = 0;
+    double scale_factor = 1.0;
+    or_q1_product_f64(prod, A64, ig, n);
+    memcpy(od, b, sizeof(double) * n);
+    or_difference_f64(od, prod, n);
+    double initial_defect = or_norm_l2_f64(od, n, 1);
+    double def_norm = initial_defect;
+    const double outer_eps = (double)tol;
+    int status = 0;
+
+    while (1) {
+        if (p->max_outer && outer >= p->max_outer) {
+            status = 1;
+            break;
+        }
+        or_convert_d2f(lv[maxl].rhs, od, n);
+        uint64_t inner = 0;
+        or_mg_kernel(lv, p, tol, adapt, &inner);
+        ++outer;
+        inner_total += inner;
+        or_convert_f2d(x_outer, lv[maxl].x, n);
+        or_scaled_sum_f64(result, x_outer, scale_factor, n);
+        or_q1_defect_f64(od, b, A64, result, n);
+        def_norm = or_norm_l2_f64(od, n, 1);
+        if (def_norm < outer_eps * initial_defect)
+            break;
+        scale_factor = def_norm;
+        double inv = 1.0 / scale_factor;
+        or_scale_f64(od, inv, n);
+    }
+    memcpy(x, result, sizeof(double) * n);
+    *outer_iters = outer;
+    *inner_iters = inner_total;
+    if (final_defect) *final_defect = def_norm;
+    if (initial_defect_out) *initial_defect_out = initial_defect;
+
+    for (uint64_t L = p->min_level; L <= maxl; ++L) {
+        free(lv[L].c); free(lv[L].d); free(lv[L].rhs); free(lv[L].x); free(lv[L].tmp);
+    }
+    free(result); free(ig); free(od); free(prod); free(x_outer);
+    return status;
+}

@@ -1,0 +1,616 @@
+/*
+ * oracle_impl.h -- type-generic body of the CPU oracle.  Included twice by
+ * oracle.c with OR_T = double / float and OR_S = f64 / f32.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ */
+#define OR_CAT2(a, b) a##_##b
+#define OR_CAT(a, b) OR_CAT2(a, b)
+#define FN(name) OR_CAT(name, OR_S)
+#define T OR_T
+
+/* honei/la/sparse_matrix_ell-impl.hh:78-117: rows of a SparseMatrix into the
+ * column-major "threads" layout; explicit zeros are skipped, Arl counts the
+ * stored entries. */
+void FN(or_ell_from_csr)(uint64_t rows, const uint64_t *rowptr, const uint64_t *col,
+                         const T *val, uint64_t threads, uint64_t stride,
+                         uint64_t num_cols_per_row, uint64_t *Aj, T *Ax, uint64_t *Arl)
+{
+    uint64_t size = num_cols_per_row * stride;
+    for (uint64_t i = 0; i < size; ++i) {
+        Aj[i] = 0;
+        Ax[i] = (T)0;
+    }
+    for (uint64_t r = 0; r < rows; ++r) {
+        uint64_t used = rowptr[r + 1] - rowptr[r];
+        Arl[r] = (uint64_t)ceil((double)used / (double)threads);
+        uint64_t target = 0;
+        for (uint64_t i = rowptr[r]; i < rowptr[r + 1]; ++i) {
+            if (val[i] != (T)0) {
+                uint64_t pos = (target % threads) + r * threads + (target / threads) * stride;
+                Aj[pos] = col[i];
+                Ax[pos] = val[i];
+                ++target;
+            }
+        }
+    }
+}
+
+/* honei/la/sparse_matrix_ell-impl.hh:186-207 (row lengths of a raw ELL
+ * matrix as read from file: count slots until the first zero value). */
+void FN(or_ell_row_length)(uint64_t rows, uint64_t stride, uint64_t threads,
+                           uint64_t size, const T *Ax, uint64_t *Arl)
+{
+    for (uint64_t row = 0; row < rows; ++row) {
+        uint64_t count = 0;
+        for (uint64_t i = row * threads; i < size; i += stride) {
+            int stop = 0;
+            for (uint64_t t = 0; t < threads; ++t) {
+                if (Ax[i + t] == (T)0) {
+                    stop = 1;
+                    break;
+                }
+                if (t == 0)
+                    ++count;
+            }
+            if (stop)
+                break;
+        }
+        Arl[row] = count;
+    }
+}
+
+/* Product<tags::CPU>::value(y, SparseMatrixELL, x, row_start, row_end):
+ * honei/la/product.hh:148-206. */
+void FN(or_ell_product)(T *y, const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
+                        const T *x, uint64_t row_start, uint64_t row_end,
+                        uint64_t stride, uint64_t threads)
+{
+    for (uint64_t row = row_start; row < row_end; ++row) {
+        T sum = 0;
+        uint64_t j = row * threads;
+        for (uint64_t c = 0; c < Arl[row]; ++c, j += stride)
+            for (uint64_t t = 0; t < threads; ++t)
+                sum += Ax[j + t] * x[Aj[j + t]];
+        y[row] = sum;
+    }
+}
+
+/* Defect<tags::CPU>::value(r, rhs, SparseMatrixELL, x):
+ * honei/math/defect.hh:375-397 (product into a temporary, then rhs - temp). */
+void FN(or_ell_defect)(T *r, const T *rhs, const uint64_t *Aj, const T *Ax,
+                       const uint64_t *Arl, const T *x, uint64_t rows,
+                       uint64_t stride, uint64_t threads)
+{
+    T *temp = (T *)malloc(sizeof(T) * (rows ? rows : 1));
+    FN(or_ell_product)(temp, Aj, Ax, Arl, x, 0, rows, stride, threads);
+    for (uint64_t i = 0; i < rows; ++i)
+        r[i] = rhs[i] - temp[i];
+    free(temp);
+}
+
+/* Product<tags::CPU>::value(y, BandedMatrixQx<Q1Type>, x):
+ * honei/la/product.hh:370-403, summation order DD,LL,LD,LU,DL,DU,UL,UD,UU. */
+void FN(or_q1_product)(T *y, const T *const *bands, const T *x, uint64_t n)
+{
+    const long size = (long)n;
+    const long root = (long)sqrt((double)n);
+    const T *ll = bands[0], *ld = bands[1], *lu = bands[2];
+    const T *dl = bands[3], *dd = bands[4], *du = bands[5];
+    const T *ul = bands[6], *ud = bands[7], *uu = bands[8];
+    for (long i = 0; i < size; ++i) {
+        T s = dd[i] * x[i];
+        if (i - root - 1 >= 0) s += ll[i] * x[i - root - 1];
+        if (i - root >= 0)     s += ld[i] * x[i - root];
+        if (i - root + 1 >= 0) s += lu[i] * x[i - root + 1];
+        if (i - 1 >= 0)        s += dl[i] * x[i - 1];
+        if (i + 1 < size)      s += du[i] * x[i + 1];
+        if (i + root - 1 < size) s += ul[i] * x[i + root - 1];
+        if (i + root < size)   s += ud[i] * x[i + root];
+        if (i + root + 1 < size) s += uu[i] * x[i + root + 1];
+        y[i] = s;
+    }
+}
+
+/* Defect<tags::CPU>::value(r, rhs, BandedMatrixQx<Q1Type>, x):
+ * honei/math/defect.hh:193-197 = Difference(r, rhs, Product(A, x)). */
+void FN(or_q1_defect)(T *r, const T *rhs, const T *const *bands, const T *x, uint64_t n)
+{
+    T *temp = (T *)malloc(sizeof(T) * (n ? n : 1));
+    FN(or_q1_product)(temp, bands, x, n);
+    for (uint64_t i = 0; i < n; ++i)
+        r[i] = rhs[i] - temp[i];
+    free(temp);
+}
+
+/* DotProduct<tags::CPU>: honei/la/dot_product.hh:72-87 (sequential sum). */
+T FN(or_dot)(const T *x, const T *y, uint64_t n)
+{
+    T result = 0;
+    for (uint64_t i = 0; i < n; ++i)
+        result += x[i] * y[i];
+    return result;
+}
+
+/* Norm<vnt_l_two, false/true, tags::CPU>: honei/la/norm.hh:290-308, 363-381
+ * (squared norm = DotProduct(x, x); root -> sqrt in T). */
+T FN(or_norm_l2)(const T *x, uint64_t n, int32_t root)
+{
+    T s = FN(or_dot)(x, x, n);
+    return root ? (T)sqrt(s) : s;
+}
+
+/* ScaledSum<tags::CPU>, honei/la/scaled_sum.hh:73-89: x += b * y. */
+void FN(or_scaled_sum)(T *x, const T *y, T b, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        x[i] += b * y[i];
+}
+
+/* scaled_sum.hh:91-111: r = b * y + x. */
+void FN(or_scaled_sum3)(T *r, const T *x, const T *y, T b, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        r[i] = b * y[i] + x[i];
+}
+
+/* scaled_sum.hh:177-197: a += b * c (elementwise). */
+void FN(or_scaled_sum_vv)(T *a, const T *b, const T *c, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        a[i] += b[i] * c[i];
+}
+
+/* Sum<tags::CPU>: x += y (honei/la/sum.hh). */
+void FN(or_sum)(T *x, const T *y, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        x[i] += y[i];
+}
+
+/* Scale<tags::CPU>, honei/la/scale.hh:229-240: x *= a with a of type T. */
+void FN(or_scale)(T *x, T a, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        x[i] *= a;
+}
+
+/* honei/la/scale.hh:111-121 with a double factor on a T vector (the overload
+ * selected by Scale<Tag>::value(DenseVector<float>&, 0.7)): the product is
+ * formed in double and rounded to T. */
+void FN(or_scale_by_double)(T *x, double a, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        x[i] = (T)((double)x[i] * a);
+}
+
+/* Difference<tags::CPU>: x -= y (honei/la/difference.hh). */
+void FN(or_difference)(T *x, const T *y, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        x[i] -= y[i];
+}
+
+/* Difference<tags::CPU>::value(r, x, y): r = x - y. */
+void FN(or_difference3)(T *r, const T *x, const T *y, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        r[i] = x[i] - y[i];
+}
+
+/* ElementProduct<tags::CPU>: a *= b (honei/la/element_product.hh). */
+void FN(or_element_product)(T *a, const T *b, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        a[i] *= b[i];
+}
+
+/* ElementProduct<tags::CPU>::value(r, a, b): r = a * b. */
+void FN(or_element_product3)(T *r, const T *a, const T *b, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        r[i] = a[i] * b[i];
+}
+
+/* ElementInverse<tags::CPU>, honei/la/element_inverse.hh:66-80: x = 1/x
+ * for x != 0, zeros are kept. */
+void FN(or_element_inverse)(T *x, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        if (x[i] == (T)0)
+            continue;
+        x[i] = (T)1 / x[i];
+    }
+}
+
+/* ApplyDirichletBoundaries: honei/math/apply_dirichlet_boundaries.hh:31-80. */
+void FN(or_apply_dirichlet)(T *grid, uint64_t N, const uint64_t *mask)
+{
+    uint64_t M = (uint64_t)(int)sqrt((double)N);
+    if (mask[0] == 2) grid[0] = 0;
+    if (mask[1] == 2) grid[M - 1] = 0;
+    if (mask[2] == 2) grid[N - 1] = 0;
+    if (mask[3] == 2) grid[N - M] = 0;
+    if (mask[4] == 2)
+        for (uint64_t i = 1; i < M - 1; i++) grid[i] = 0;
+    if (mask[5] == 2)
+        for (uint64_t i = 2 * M - 1; i < N - M; i += M) grid[i] = 0;
+    if (mask[6] == 2)
+        for (uint64_t i = N - M + 1; i < N - 1; i++) grid[i] = 0;
+    if (mask[7] == 2)
+        for (uint64_t i = M; i < N - M; i += M) grid[i] = 0;
+}
+
+/* Grid side of a (2^L+1)^2 vector, computed the way the transfer operators
+ * do it: (unsigned long)sqrt((Prec_)n) (restriction.hh:48, prolongation.hh:53).
+ * For the float instantiation the argument is rounded to float first. */
+static uint64_t FN(or_side)(uint64_t n)
+{
+#if OR_IS_FLOAT
+    return (uint64_t)sqrtf((float)n);
+#else
+    return (uint64_t)sqrt((double)n);
+#endif
+}
+
+/* Restriction<Tag, NONE>: honei/math/restriction.hh:43-172 (full weighting,
+ * scattered in the reference's loop order). */
+void FN(or_restriction)(T *coarse, const T *fine, uint64_t n_fine, const uint64_t *mask)
+{
+    uint64_t n_x_fine = FN(or_side)(n_fine) - 1;
+    uint64_t n_x_coarse = n_x_fine / 2;
+    uint64_t n_y_fine = n_x_fine + 1;
+    uint64_t n_y_coarse = n_x_coarse + 1;
+    uint64_t i_step = 2 * n_y_fine;
+    long i_aux, i_aux_1;
+
+    for (uint64_t i = 1; i <= n_y_coarse; ++i)
+        for (uint64_t j = 0, k = 0; j < n_y_coarse; ++j, k += 2)
+            coarse[1 + n_y_coarse * (i - 1) - 1 + j] = fine[2 * n_y_fine * (i - 1) + k];
+
+    for (uint64_t i = 1; i <= n_x_coarse; ++i) {
+        i_aux = (long)(1 + n_y_fine + 2 * n_y_fine * (i - 1));
+        i_aux_1 = (long)(1 + n_y_coarse * i);
+        for (uint64_t j = 0; j < n_y_coarse; ++j)
+            coarse[i_aux_1 - 1 + j] += 0.5 * fine[i_aux - 1 + j * 2];
+        i_aux_1 = i_aux_1 - (long)n_y_coarse;
+        for (uint64_t j = 0; j < n_y_coarse; ++j)
+            coarse[i_aux_1 - 1 + j] += 0.5 * fine[i_aux - 1 + j * 2];
+        for (uint64_t j = 0; j < n_y_coarse; ++j)
+            coarse[i - 1 + j * n_y_coarse] += 0.5 * fine[2 * i - 1 + j * i_step];
+        for (uint64_t j = 0; j < n_y_coarse; ++j)
+            coarse[i + j * n_y_coarse] += 0.5 * fine[2 * i - 1 + j * i_step];
+        ++i_aux;
+        for (uint64_t j = 0; j < n_x_coarse; ++j)
+            coarse[i_aux_1 - 1 + j] += 0.25 * fine[i_aux - 1 + j * 2];
+        for (uint64_t j = 0; j < n_x_coarse; ++j)
+            coarse[i_aux_1 + j] += 0.25 * fine[i_aux - 1 + j * 2];
+        for (uint64_t j = 0; j < n_x_coarse; ++j)
+            coarse[n_y_coarse * i + j] += 0.25 * fine[i_aux - 1 + j * 2];
+        for (uint64_t j = 0; j < n_x_coarse; ++j)
+            coarse[1 + n_y_coarse * i + j] += 0.25 * fine[i_aux - 1 + j * 2];
+    }
+    FN(or_apply_dirichlet)(coarse, n_y_coarse * n_y_coarse, mask);
+}
+
+/* Prolongation<Tag, NONE>: honei/math/prolongation.hh:43-139 (bilinear). */
+void FN(or_prolongation)(T *fine, const T *coarse, uint64_t n_fine, const uint64_t *mask)
+{
+    uint64_t n_x_fine = FN(or_side)(n_fine) - 1;
+    uint64_t n_x_coarse = n_x_fine / 2;
+    uint64_t n_y_fine = n_x_fine + 1;
+    uint64_t n_y_coarse = n_x_coarse + 1;
+    uint64_t i_step = 2 * n_y_fine;
+
+    for (uint64_t i = 0; i < n_fine; ++i)
+        fine[i] = 0.0;
+    for (uint64_t i = 1; i <= n_y_coarse; ++i)
+        for (uint64_t j = 0, k = 0; j < n_y_coarse; ++j, k += 2)
+            fine[i_step * (i - 1) + k] = coarse[i + n_x_coarse * (i - 1) - 1 + j];
+    for (uint64_t i = 1; i <= n_x_coarse; ++i) {
+        uint64_t iaux = 2 * i;
+        for (uint64_t j = 0; j < n_y_coarse; ++j) {
+            fine[iaux - 1 + j * i_step] += coarse[i - 1 + j * n_y_coarse] * 0.5;
+            fine[iaux - 1 + j * i_step] += coarse[i + j * n_y_coarse] * 0.5;
+        }
+        iaux = n_y_fine + 1 + (i - 1) * i_step;
+        for (uint64_t j = 0; j < n_y_coarse; ++j) {
+            fine[iaux - 1 + j * 2] += fine[iaux - n_y_fine - 1 + j * 2] * 0.5;
+            fine[iaux - 1 + j * 2] += fine[iaux + n_y_fine - 1 + j * 2] * 0.5;
+        }
+    }
+    for (uint64_t i = 1; i <= n_x_coarse; ++i) {
+        uint64_t iaux = n_x_fine + 3 + (i - 1) * i_step;
+        for (uint64_t j = 0; j < n_x_coarse; ++j) {
+            fine[iaux - 1 + j * 2] += fine[iaux - n_x_fine - 3 + j * 2] * 0.25;
+            fine[iaux - 1 + j * 2] += fine[iaux - n_x_fine - 1 + j * 2] * 0.25;
+            fine[iaux - 1 + j * 2] += fine[iaux + n_x_fine - 1 + j * 2] * 0.25;
+            fine[iaux - 1 + j * 2] += fine[iaux + n_x_fine + 1 + j * 2] * 0.25;
+        }
+    }
+    FN(or_apply_dirichlet)(fine, n_fine, mask);
+}
+
+/* Jacobi<Tag>::value(to_smooth, Q1, rhs, x, iters, omega, dinv):
+ * honei/math/jacobi.hh:438-480 with jacobi_kernel jacobi.hh:117-139 when
+ * to_smooth and x are the same vector: x += dinv * (rhs - A x), iters times. */
+void FN(or_jacobi_q1)(T *x, const T *const *bands, const T *rhs, const T *dinv,
+                      uint64_t n, uint64_t iters, T *temp)
+{
+    for (uint64_t it = 0; it < iters; ++it) {
+        FN(or_q1_defect)(temp, rhs, bands, x, n);
+        FN(or_scaled_sum_vv)(x, temp, dinv, n);
+    }
+}
+
+/* Same for ELL: jacobi.hh:268-279, 505-523. */
+void FN(or_jacobi_ell)(T *x, const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
+                       uint64_t stride, uint64_t threads, const T *rhs, const T *dinv,
+                       uint64_t n, uint64_t iters, T *temp)
+{
+    for (uint64_t it = 0; it < iters; ++it) {
+        FN(or_ell_defect)(temp, rhs, Aj, Ax, Arl, x, n, stride, threads);
+        FN(or_scaled_sum_vv)(x, temp, dinv, n);
+    }
+}
+
+/* FillMatrix<Tag, POISSON, DIRICHLET_0>: honei/math/fill_matrix.hh:37-76. */
+void FN(or_fill_poisson_dirichlet0)(T *const *bands, uint64_t N)
+{
+    uint64_t M = (uint64_t)sqrt((double)N);
+    for (uint64_t i = 1; i <= N; ++i) {
+        int bnd = (i <= M || i > N - M || (i % M) == 1 || (i % M) == 0);
+        for (int b = 0; b < 9; ++b)
+            bands[b][i - 1] = bnd ? (T)0.0 : (T)(-1.0 / 3.0);
+        bands[4][i - 1] = bnd ? (T)1.0 : (T)(8.0 / 3.0);
+    }
+}
+
+/* FillMatrix<Tag, POISSON, DIRICHLET_NEUMANN>: fill_matrix.hh:79-131. */
+void FN(or_fill_poisson_dirichlet_neumann)(T *const *bands, uint64_t N)
+{
+    uint64_t M = (uint64_t)sqrt((double)N);
+    T *ll = bands[0], *ld = bands[1], *lu = bands[2], *dl = bands[3], *dd = bands[4];
+    T *du = bands[5], *ul = bands[6], *ud = bands[7], *uu = bands[8];
+    for (uint64_t i = 1; i <= N; ++i) {
+        uint64_t k = i - 1;
+        if (i <= M || i > N - M || (i % M) == 1) {
+            dd[k] = (T)1.0;
+            ll[k] = ld[k] = lu[k] = dl[k] = du[k] = ul[k] = ud[k] = uu[k] = (T)0.0;
+        } else if ((i % M) == 0 && i > M && i < N) {
+            dd[k] = (T)(4.0 / 3.0);
+            ll[k] = (T)(-1.0 / 3.0);
+            ld[k] = (T)(-1.0 / 6.0);
+            dl[k] = (T)(-1.0 / 3.0);
+            ul[k] = (T)(-1.0 / 3.0);
+            ud[k] = (T)(-1.0 / 6.0);
+            lu[k] = du[k] = uu[k] = (T)0.0;
+        } else {
+            dd[k] = (T)(8.0 / 3.0);
+            ll[k] = ld[k] = lu[k] = dl[k] = du[k] = ul[k] = ud[k] = uu[k] = (T)(-1.0 / 3.0);
+        }
+    }
+}
+
+/* FillVector<Tag, POISSON, DIRICHLET_NEUMANN>: fill_vector.hh:72-158. */
+void FN(or_fill_vector_dirichlet_neumann)(T *v, uint64_t N)
+{
+    uint64_t M = (uint64_t)sqrt((double)N);
+    int level = 0;
+    while (level <= 30 && (uint64_t)(((1ull << level) + 1) * ((1ull << level) + 1)) != N)
+        ++level;
+    for (uint64_t i = 1; i <= N; i++) {
+        if (i <= M || i > N - M || (i % M) == 1)
+            v[i - 1] = (T)2.0;
+        else if ((i % M) == 0 && i > M && i < N)
+            v[i - 1] = (T)(0.5 * (1.0 / pow(4.0, level)));
+        else
+            v[i - 1] = (T)(1.0 / pow(4.0, level));
+    }
+}
+
+/* Generic matrix application used by the CG restatements. */
+typedef struct FN(or_matrix) {
+    int kind; /* 0 = Q1 bands, 1 = ELL */
+    const T *const *bands;
+    const uint64_t *Aj;
+    const T *Ax;
+    const uint64_t *Arl;
+    uint64_t stride, threads;
+} FN(or_matrix);
+
+static void FN(or_apply)(const FN(or_matrix) *A, T *y, const T *x, uint64_t n)
+{
+    if (A->kind == 0)
+        FN(or_q1_product)(y, A->bands, x, n);
+    else
+        FN(or_ell_product)(y, A->Aj, A->Ax, A->Arl, x, 0, n, A->stride, A->threads);
+}
+
+static void FN(or_defect_any)(const FN(or_matrix) *A, T *r, const T *b, const T *x, uint64_t n)
+{
+    if (A->kind == 0)
+        FN(or_q1_defect)(r, b, A->bands, x, n);
+    else
+        FN(or_ell_defect)(r, b, A->Aj, A->Ax, A->Arl, x, n, A->stride, A->threads);
+}
+
+#if OR_IS_FLOAT
+#define OR_EPS FLT_EPSILON
+#else
+#define OR_EPS DBL_EPSILON
+#endif
+
+/* CGSolver<Tag, methods::NONE>::value: honei/math/cg.hh:63-133. */
+static int FN(or_cg_none)(const FN(or_matrix) *A, const T *b, T *x, uint64_t n,
+                          uint64_t max_iters, T eps_relative, uint64_t *used_iters)
+{
+    T *p = (T *)malloc(sizeof(T) * n), *r = (T *)malloc(sizeof(T) * n), *v = (T *)malloc(sizeof(T) * n);
+    T alpha, alpha_old, lambda, initial_defect, current_defect = 0, temp;
+    uint64_t iterations = 0;
+    int converged = 0;
+
+    FN(or_defect_any)(A, r, b, x, n);
+    memcpy(p, r, sizeof(T) * n);
+    alpha = FN(or_norm_l2)(r, n, 0);
+    initial_defect = (T)sqrt(alpha);
+    *used_iters = 0;
+    while (iterations < max_iters) {
+        FN(or_apply)(A, v, p, n);
+        temp = FN(or_dot)(v, p, n);
+        lambda = alpha / (fabs(temp) > OR_EPS ? temp : (T)OR_EPS);
+        FN(or_scaled_sum)(x, p, lambda, n);
+        T mlambda = -lambda;
+        FN(or_scaled_sum)(r, v, mlambda, n);
+        alpha_old = alpha;
+        alpha = FN(or_norm_l2)(r, n, 0);
+        T talpha = alpha / (fabs(alpha_old) > OR_EPS ? alpha_old : (T)OR_EPS);
+        FN(or_scale)(p, talpha, n);
+        FN(or_sum)(p, r, n);
+        ++iterations;
+        current_defect = (T)sqrt(alpha);
+        if (current_defect < eps_relative * initial_defect) {
+            *used_iters = iterations;
+            converged = 1;
+            break;
+        }
+        if (current_defect < eps_relative) {
+            *used_iters = iterations;
+            converged = 1;
+            break;
+        }
+        if (iterations == max_iters)
+            *used_iters = iterations;
+    }
+    free(p); free(r); free(v);
+    return converged ? 0 : 1;
+}
+
+int FN(or_cg_none_q1)(const T *const *bands, const T *b, T *x, uint64_t n,
+                      uint64_t max_iters, T eps, uint64_t *used_iters)
+{
+    FN(or_matrix) A = {0, bands, 0, 0, 0, 0, 0};
+    return FN(or_cg_none)(&A, b, x, n, max_iters, eps, used_iters);
+}
+
+int FN(or_cg_none_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
+                       uint64_t stride, uint64_t threads, const T *b, T *x, uint64_t n,
+                       uint64_t max_iters, T eps, uint64_t *used_iters)
+{
+    FN(or_matrix) A = {1, 0, Aj, Ax, Arl, stride, threads};
+    return FN(or_cg_none)(&A, b, x, n, max_iters, eps, used_iters);
+}
+
+static T FN(or_sign)(T v) { return v / (T)fabs(v); }
+
+/* CGSolver<Tag, methods::VAR>::value with a DenseVector preconditioner P
+ * (Product(z, P, r) is the elementwise product): honei/math/cg.hh:141-219. */
+int FN(or_cg_var_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
+                      uint64_t stride, uint64_t threads, const T *P, const T *b, T *x,
+                      uint64_t n, uint64_t max_iters, T eps_relative, uint64_t *used_iters)
+{
+    FN(or_matrix) A = {1, 0, Aj, Ax, Arl, stride, threads};
+    T *p = (T *)malloc(sizeof(T) * n), *r = (T *)malloc(sizeof(T) * n);
+    T *v = (T *)malloc(sizeof(T) * n), *z = (T *)malloc(sizeof(T) * n);
+    T alpha, alpha_new, lambda, initial_defect, temp;
+    uint64_t iterations = 0;
+    int converged = 0;
+
+    FN(or_defect_any)(&A, r, b, x, n);
+    FN(or_element_product3)(p, P, r, n);
+    initial_defect = FN(or_norm_l2)(r, n, 1);
+    alpha_new = FN(or_dot)(r, p, n);
+    *used_iters = 0;
+    while (iterations < max_iters) {
+        FN(or_apply)(&A, v, p, n);
+        temp = FN(or_dot)(v, p, n);
+        lambda = alpha_new / (fabs(temp) > OR_EPS ? temp : (T)OR_EPS * FN(or_sign)(temp));
+        ++iterations;
+        FN(or_scaled_sum)(x, p, lambda, n);
+        T mlambda = -lambda;
+        FN(or_scaled_sum)(r, v, mlambda, n);
+        T current_defect = FN(or_norm_l2)(r, n, 1);
+        if (current_defect < eps_relative * initial_defect) {
+            *used_iters = iterations;
+            converged = 1;
+            break;
+        }
+        if (current_defect < eps_relative) {
+            *used_iters = iterations;
+            converged = 1;
+            break;
+        }
+        if (iterations == max_iters) {
+            *used_iters = iterations;
+            break;
+        }
+        FN(or_element_product3)(z, P, r, n);
+        alpha = alpha_new;
+        alpha_new = FN(or_dot)(r, z, n);
+        T talpha_new = alpha_new / (fabs(alpha) > OR_EPS ? alpha : (T)OR_EPS * FN(or_sign)(alpha));
+        FN(or_scale)(p, talpha_new, n);
+        FN(or_sum)(p, z, n);
+    }
+    free(p); free(r); free(v); free(z);
+    return converged ? 0 : 1;
+}
+
+/* ConjugateGradients<Tag, NONE>::value(Q1, rhs, x, konv_rad):
+ * honei/math/conjugate_gradients.hh:490-526 with cg_kernel :108-151.
+ * max_iters is a safety cap the reference does not have (0 = none). */
+int FN(or_cg_konv_q1)(const T *const *bands, const T *rhs, T *x, uint64_t n,
+                      double konv_rad, uint64_t max_iters, uint64_t *iters)
+{
+    T *g = (T *)malloc(sizeof(T) * n), *u = (T *)malloc(sizeof(T) * n);
+    T *energy = (T *)malloc(sizeof(T) * n), *ng = (T *)malloc(sizeof(T) * n);
+    uint64_t it = 0;
+    int capped = 0;
+
+    for (uint64_t i = 0; i < n; ++i)
+        x[i] = 0;
+    FN(or_q1_product)(g, bands, x, n);
+    FN(or_difference)(g, rhs, n);
+    memcpy(u, g, sizeof(T) * n);
+    FN(or_scale)(u, (T)-1., n);
+    T norm_x = FN(or_norm_l2)(x, n, 0);
+    T norm_x_last = (T)(norm_x - 1);
+    while (norm_x - norm_x_last > konv_rad) {
+        if (max_iters && it >= max_iters) {
+            capped = 1;
+            break;
+        }
+        /* cg_kernel */
+        T upper = FN(or_norm_l2)(g, n, 0);
+        FN(or_q1_product)(energy, bands, u, n);
+        T lower = FN(or_dot)(energy, u, n);
+        memcpy(energy, u, sizeof(T) * n); /* u_c = copy(u); energy = u_c */
+        if (fabs(lower) >= OR_EPS)
+            FN(or_scale)(energy, (T)(upper / lower), n);
+        else
+            FN(or_scale)(energy, (T)(upper / (T)OR_EPS), n);
+        FN(or_sum)(energy, x, n);
+        lower = upper;
+        FN(or_q1_product)(ng, bands, energy, n);
+        FN(or_difference)(ng, rhs, n);
+        upper = FN(or_norm_l2)(ng, n, 0);
+        if (fabs(lower) >= OR_EPS)
+            FN(or_scale)(u, (T)(upper / lower), n);
+        else
+            FN(or_scale)(u, (T)(upper / (T)OR_EPS), n);
+        FN(or_difference)(u, ng, n);
+        memcpy(g, ng, sizeof(T) * n);
+        memcpy(x, energy, sizeof(T) * n);
+        ++it;
+        norm_x_last = norm_x;
+        norm_x = FN(or_norm_l2)(x, n, 0);
+    }
+    *iters = it;
+    free(g); free(u); free(energy); free(ng);
+    return capped;
+}
+
+#undef OR_EPS
+#undef T
+#undef FN
+#undef OR_CAT
+#undef OR_CAT2

@@ -1,0 +1,428 @@
+"""ctypes front end of the CPU oracle (liboracle.so) and, where it was built,
+of the compiled HONEI reference (oracle/_ref/libhoneiref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py.  The product (honei_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libhoneiref.so")
+
+u64 = C.c_uint64
+vp = C.c_void_p
+BAND_NAMES = ("LL", "LD", "LU", "DL", "DD", "DU", "UL", "UD", "UU")
+
+
+def _p(a):
+    return a.ctypes.data_as(vp) if a is not None else None
+
+
+def _pp(arrays):
+    arr = (vp * len(arrays))(*[a.ctypes.data for a in arrays])
+    return arr
+
+
+def _sfx(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+
+
+class MGParams(C.Structure):
+    _fields_ = [
+        ("min_level", u64), ("max_level", u64), ("n_max_iter", u64),
+        ("initial_zero", C.c_int32), ("convergence_check", C.c_int32),
+        ("tolerance", C.c_double), ("n_pre_smooth", u64),
+        ("adapt_correction_factor", C.c_double), ("macro_border_mask", u64 * 8),
+        ("max_outer", u64),
+    ]
+
+
+class Oracle:
+    """The C restatement of HONEI's tags::CPU path."""
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run oracle/Makefile)")
+        self.lib = C.CDLL(path)
+        L = self.lib
+        for s, T in (("f64", C.c_double), ("f32", C.c_float)):
+            getattr(L, f"or_dot_{s}").restype = T
+            getattr(L, f"or_norm_l2_{s}").restype = T
+            for fn in ("or_scaled_sum", "or_scaled_sum3", "or_scale"):
+                f = getattr(L, f"{fn}_{s}")
+                if fn == "or_scaled_sum":
+                    f.argtypes = [vp, vp, T, u64]
+                elif fn == "or_scaled_sum3":
+                    f.argtypes = [vp, vp, vp, T, u64]
+                else:
+                    f.argtypes = [vp, T, u64]
+            getattr(L, f"or_scale_by_double_{s}").argtypes = [vp, C.c_double, u64]
+            getattr(L, f"or_cg_none_q1_{s}").argtypes = [vp, vp, vp, u64, u64, T, vp]
+            getattr(L, f"or_cg_none_ell_{s}").argtypes = [vp, vp, vp, u64, u64, vp, vp, u64, u64, T, vp]
+            getattr(L, f"or_cg_var_ell_{s}").argtypes = [vp, vp, vp, u64, u64, vp, vp, vp, u64, u64, T, vp]
+            getattr(L, f"or_cg_konv_q1_{s}").argtypes = [vp, vp, vp, u64, C.c_double, u64, vp]
+        L.or_rng_bits.restype = u64
+        L.or_rng_bits.argtypes = [u64, u64, u64]
+        L.or_rng_uniform.restype = C.c_double
+        L.or_rng_uniform.argtypes = [u64, u64, u64, C.c_double, C.c_double]
+        L.or_mg_mixed_q1.argtypes = [vp, vp, vp, vp, vp, C.POINTER(MGParams), vp, vp, vp, vp]
+
+    # -- ELL ---------------------------------------------------------------
+    def ell_from_csr(self, rows, rowptr, col, val, threads=1):
+        rowptr = np.ascontiguousarray(rowptr, np.uint64)
+        col = np.ascontiguousarray(col, np.uint64)
+        val = np.ascontiguousarray(val)
+        stride, ncpr = u64(), u64()
+        self.lib.or_ell_shape(u64(rows), _p(rowptr), u64(threads), C.byref(stride), C.byref(ncpr))
+        size = stride.value * ncpr.value
+        Aj = np.zeros(size, np.uint64)
+        Ax = np.zeros(size, val.dtype)
+        Arl = np.zeros(rows, np.uint64)
+        getattr(self.lib, f"or_ell_from_csr_{_sfx(val.dtype)}")(
+            u64(rows), _p(rowptr), _p(col), _p(val), u64(threads), stride, ncpr, _p(Aj), _p(Ax), _p(Arl))
+        return dict(rows=rows, stride=stride.value, ncpr=ncpr.value, threads=threads,
+                    Aj=Aj, Ax=Ax, Arl=Arl)
+
+    def ell_row_length(self, rows, stride, threads, Ax):
+        Arl = np.zeros(rows, np.uint64)
+        getattr(self.lib, f"or_ell_row_length_{_sfx(Ax.dtype)}")(
+            u64(rows), u64(stride), u64(threads), u64(Ax.size), _p(Ax), _p(Arl))
+        return Arl
+
+    def ell_product(self, E, x, row_start=0, row_end=None):
+        rows = E["rows"]
+        row_end = rows if row_end is None else row_end
+        y = np.zeros(rows, x.dtype)
+        getattr(self.lib, f"or_ell_product_{_sfx(x.dtype)}")(
+            _p(y), _p(E["Aj"]), _p(E["Ax"]), _p(E["Arl"]), _p(x), u64(row_start), u64(row_end),
+            u64(E["stride"]), u64(E["threads"]))
+        return y
+
+    def ell_defect(self, E, rhs, x):
+        r = np.zeros(E["rows"], x.dtype)
+        getattr(self.lib, f"or_ell_defect_{_sfx(x.dtype)}")(
+            _p(r), _p(rhs), _p(E["Aj"]), _p(E["Ax"]), _p(E["Arl"]), _p(x), u64(E["rows"]),
+            u64(E["stride"]), u64(E["threads"]))
+        return r
+
+    # -- Q1 ----------------------------------------------------------------
+    def q1_product(self, bands, x):
+        y = np.zeros_like(x)
+        getattr(self.lib, f"or_q1_product_{_sfx(x.dtype)}")(_p(y), _pp(bands), _p(x), u64(x.size))
+        return y
+
+    def q1_defect(self, bands, rhs, x):
+        r = np.zeros_like(x)
+        getattr(self.lib, f"or_q1_defect_{_sfx(x.dtype)}")(_p(r), _p(rhs), _pp(bands), _p(x), u64(x.size))
+        return r
+
+    def fill_poisson_dirichlet0(self, n, dtype=np.float64):
+        bands = [np.zeros(n, dtype) for _ in range(9)]
+        getattr(self.lib, f"or_fill_poisson_dirichlet0_{_sfx(dtype)}")(_pp(bands), u64(n))
+        return bands
+
+    def fill_poisson_dirichlet_neumann(self, n, dtype=np.float64):
+        bands = [np.zeros(n, dtype) for _ in range(9)]
+        getattr(self.lib, f"or_fill_poisson_dirichlet_neumann_{_sfx(dtype)}")(_pp(bands), u64(n))
+        return bands
+
+    def fill_vector_dirichlet_neumann(self, n, dtype=np.float64):
+        v = np.zeros(n, dtype)
+        getattr(self.lib, f"or_fill_vector_dirichlet_neumann_{_sfx(dtype)}")(_p(v), u64(n))
+        return v
+
+    # -- BLAS-1 ------------------------------------------------------------
+    def dot(self, x, y):
+        return getattr(self.lib, f"or_dot_{_sfx(x.dtype)}")(_p(x), _p(y), u64(x.size))
+
+    def norm(self, x, root=False):
+        return getattr(self.lib, f"or_norm_l2_{_sfx(x.dtype)}")(_p(x), u64(x.size), C.c_int32(int(root)))
+
+    def _inplace(self, name, x, *args):
+        x = x.copy()
+        getattr(self.lib, f"{name}_{_sfx(x.dtype)}")(_p(x), *args)
+        return x
+
+    def scaled_sum(self, x, y, b):
+        return self._inplace("or_scaled_sum", x, _p(y), b, u64(x.size))
+
+    def scaled_sum3(self, x, y, b):
+        r = np.zeros_like(x)
+        getattr(self.lib, f"or_scaled_sum3_{_sfx(x.dtype)}")(_p(r), _p(x), _p(y), b, u64(x.size))
+        return r
+
+    def scaled_sum_vv(self, a, b, c):
+        return self._inplace("or_scaled_sum_vv", a, _p(b), _p(c), u64(a.size))
+
+    def sum(self, x, y):
+        return self._inplace("or_sum", x, _p(y), u64(x.size))
+
+    def scale(self, x, a):
+        return self._inplace("or_scale", x, a, u64(x.size))
+
+    def scale_by_double(self, x, a):
+        return self._inplace("or_scale_by_double", x, C.c_double(a), u64(x.size))
+
+    def difference(self, x, y):
+        return self._inplace("or_difference", x, _p(y), u64(x.size))
+
+    def difference3(self, x, y):
+        r = np.zeros_like(x)
+        getattr(self.lib, f"or_difference3_{_sfx(x.dtype)}")(_p(r), _p(x), _p(y), u64(x.size))
+        return r
+
+    def element_product(self, a, b):
+        return self._inplace("or_element_product", a, _p(b), u64(a.size))
+
+    def element_product3(self, a, b):
+        r = np.zeros_like(a)
+        getattr(self.lib, f"or_element_product3_{_sfx(a.dtype)}")(_p(r), _p(a), _p(b), u64(a.size))
+        return r
+
+    def element_inverse(self, x):
+        return self._inplace("or_element_inverse", x, u64(x.size))
+
+    def convert_d2f(self, x):
+        out = np.zeros(x.size, np.float32)
+        self.lib.or_convert_d2f(_p(out), _p(np.ascontiguousarray(x, np.float64)), u64(x.size))
+        return out
+
+    def convert_f2d(self, x):
+        out = np.zeros(x.size, np.float64)
+        self.lib.or_convert_f2d(_p(out), _p(np.ascontiguousarray(x, np.float32)), u64(x.size))
+        return out
+
+    # -- transfers / smoother ------------------------------------------------
+    def restriction(self, fine, mask=(2,) * 8):
+        m = np.asarray(mask, np.uint64)
+        side_f = int(round(np.sqrt(fine.size)))
+        side_c = (side_f - 1) // 2 + 1
+        coarse = np.zeros(side_c * side_c, fine.dtype)
+        getattr(self.lib, f"or_restriction_{_sfx(fine.dtype)}")(_p(coarse), _p(fine), u64(fine.size), _p(m))
+        return coarse
+
+    def prolongation(self, coarse, n_fine, mask=(2,) * 8):
+        m = np.asarray(mask, np.uint64)
+        fine = np.zeros(n_fine, coarse.dtype)
+        getattr(self.lib, f"or_prolongation_{_sfx(coarse.dtype)}")(_p(fine), _p(coarse), u64(n_fine), _p(m))
+        return fine
+
+    def apply_dirichlet(self, grid, mask=(2,) * 8):
+        m = np.asarray(mask, np.uint64)
+        return self._inplace("or_apply_dirichlet", grid, u64(grid.size), _p(m))
+
+    def jacobi_q1(self, x, bands, rhs, dinv, iters):
+        x = x.copy()
+        tmp = np.zeros_like(x)
+        getattr(self.lib, f"or_jacobi_q1_{_sfx(x.dtype)}")(
+            _p(x), _pp(bands), _p(rhs), _p(dinv), u64(x.size), u64(iters), _p(tmp))
+        return x
+
+    def jacobi_ell(self, x, E, rhs, dinv, iters):
+        x = x.copy()
+        tmp = np.zeros_like(x)
+        getattr(self.lib, f"or_jacobi_ell_{_sfx(x.dtype)}")(
+            _p(x), _p(E["Aj"]), _p(E["Ax"]), _p(E["Arl"]), u64(E["stride"]), u64(E["threads"]),
+            _p(rhs), _p(dinv), u64(x.size), u64(iters), _p(tmp))
+        return x
+
+    # -- solvers -------------------------------------------------------------
+    def cg_none_q1(self, bands, b, x0, max_iters=10000, eps=1e-8):
+        x = x0.copy()
+        used = u64()
+        st = getattr(self.lib, f"or_cg_none_q1_{_sfx(b.dtype)}")(
+            _pp(bands), _p(b), _p(x), u64(b.size), u64(max_iters), eps, C.byref(used))
+        return x, used.value, st
+
+    def cg_none_ell(self, E, b, x0, max_iters=10000, eps=1e-8):
+        x = x0.copy()
+        used = u64()
+        st = getattr(self.lib, f"or_cg_none_ell_{_sfx(b.dtype)}")(
+            _p(E["Aj"]), _p(E["Ax"]), _p(E["Arl"]), u64(E["stride"]), u64(E["threads"]),
+            _p(b), _p(x), u64(b.size), u64(max_iters), eps, C.byref(used))
+        return x, used.value, st
+
+    def cg_var_ell(self, E, P, b, x0, max_iters=10000, eps=1e-8):
+        x = x0.copy()
+        used = u64()
+        st = getattr(self.lib, f"or_cg_var_ell_{_sfx(b.dtype)}")(
+            _p(E["Aj"]), _p(E["Ax"]), _p(E["Arl"]), u64(E["stride"]), u64(E["threads"]),
+            _p(P), _p(b), _p(x), u64(b.size), u64(max_iters), eps, C.byref(used))
+        return x, used.value, st
+
+    def cg_konv_q1(self, bands, rhs, konv_rad, max_iters=0):
+        x = np.zeros_like(rhs)
+        it = u64()
+        st = getattr(self.lib, f"or_cg_konv_q1_{_sfx(rhs.dtype)}")(
+            _pp(bands), _p(rhs), _p(x), u64(rhs.size), C.c_double(konv_rad), u64(max_iters), C.byref(it))
+        return x, it.value, st
+
+    def mg_mixed_q1(self, A64, b, x0, a32_levels, dinv32_levels, *, min_level, max_level,
+                    n_max_iter=2, tolerance=1e-8, n_pre_smooth=2, adapt=1.0, mask=(2,) * 8,
+                    max_outer=0):
+        """a32_levels / dinv32_levels: dict level -> list of 9 bands / array."""
+        x = x0.copy()
+        dummy = np.zeros(1, np.float32)
+        a_ptrs = (vp * (9 * (max_level + 1)))()
+        d_ptrs = (vp * (max_level + 1))()
+        for L in range(max_level + 1):
+            for k in range(9):
+                a_ptrs[9 * L + k] = (a32_levels[L][k] if L in a32_levels else dummy).ctypes.data
+            d_ptrs[L] = (dinv32_levels[L] if L in dinv32_levels else dummy).ctypes.data
+        p = MGParams(min_level, max_level, n_max_iter, 1, 0, tolerance, n_pre_smooth, adapt,
+                     (u64 * 8)(*mask), max_outer)
+        outer, inner = u64(), u64()
+        fd, idf = C.c_double(), C.c_double()
+        st = self.lib.or_mg_mixed_q1(_pp(A64), _p(b), _p(x), a_ptrs, d_ptrs, C.byref(p),
+                                     C.byref(outer), C.byref(inner), C.byref(fd), C.byref(idf))
+        return dict(x=x, outer=outer.value, inner=inner.value, final_defect=fd.value,
+                    initial_defect=idf.value, status=st)
+
+    # -- RNG -----------------------------------------------------------------
+    def rng_uniform(self, seed, stream, index, lo, hi):
+        return self.lib.or_rng_uniform(u64(seed), u64(stream), u64(index), lo, hi)
+
+
+class Reference:
+    """HONEI's own tags::CPU templates, compiled from /root/reference by
+    oracle/ref/Makefile.  Available only where that build ran."""
+
+    def __init__(self, path: str = REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+        L = self.lib
+        for s, T in (("f64", C.c_double), ("f32", C.c_float)):
+            getattr(L, f"ref_dot_{s}").restype = T
+            getattr(L, f"ref_norm_{s}").restype = T
+            for fn in ("ref_cg_none_q1", "ref_cg_none_ell", "ref_cg_var_ell"):
+                getattr(L, f"{fn}_{s}").restype = u64
+            getattr(L, f"ref_cg_none_q1_{s}").argtypes = [vp, vp, vp, u64, u64, T]
+            getattr(L, f"ref_cg_none_ell_{s}").argtypes = [u64, u64, u64, u64, vp, vp, vp, vp, u64, T]
+            getattr(L, f"ref_cg_var_ell_{s}").argtypes = [u64, u64, u64, u64, vp, vp, vp, vp, vp, u64, T]
+            getattr(L, f"ref_cg_konv_q1_{s}").argtypes = [vp, vp, vp, u64, C.c_double]
+        L.ref_mg_mixed_q1.argtypes = [vp, vp, vp, vp, u64, u64, u64, C.c_double, u64, C.c_double,
+                                      vp, vp, vp]
+
+    def ell_from_csr(self, rows, cols, rowptr, col, val, threads=1):
+        rowptr = np.ascontiguousarray(rowptr, np.uint64)
+        col = np.ascontiguousarray(col, np.uint64)
+        val = np.ascontiguousarray(val)
+        stride, ncpr = u64(), u64()
+        f = getattr(self.lib, f"ref_ell_from_csr_{_sfx(val.dtype)}")
+        f(u64(rows), u64(cols), _p(rowptr), _p(col), _p(val), u64(threads), C.byref(stride),
+          C.byref(ncpr), None, None, None)
+        size = stride.value * ncpr.value
+        Aj = np.zeros(size, np.uint64)
+        Ax = np.zeros(size, val.dtype)
+        Arl = np.zeros(rows, np.uint64)
+        f(u64(rows), u64(cols), _p(rowptr), _p(col), _p(val), u64(threads), C.byref(stride),
+          C.byref(ncpr), _p(Aj), _p(Ax), _p(Arl))
+        return dict(rows=rows, stride=stride.value, ncpr=ncpr.value, threads=threads,
+                    Aj=Aj, Ax=Ax, Arl=Arl)
+
+    def ell_product(self, E, x, cols=None):
+        rows = E["rows"]
+        cols = x.size if cols is None else cols
+        y = np.zeros(rows, x.dtype)
+        Arl = np.zeros(rows, np.uint64)
+        getattr(self.lib, f"ref_ell_product_{_sfx(x.dtype)}")(
+            _p(y), u64(rows), u64(cols), u64(E["stride"]), u64(E["ncpr"]), u64(E["threads"]),
+            _p(E["Aj"]), _p(E["Ax"]), _p(x), _p(Arl))
+        return y, Arl
+
+    def ell_defect(self, E, rhs, x):
+        r = np.zeros(E["rows"], x.dtype)
+        getattr(self.lib, f"ref_ell_defect_{_sfx(x.dtype)}")(
+            _p(r), _p(rhs), u64(E["rows"]), u64(E["stride"]), u64(E["ncpr"]), u64(E["threads"]),
+            _p(E["Aj"]), _p(E["Ax"]), _p(x))
+        return r
+
+    def q1_product(self, bands, x):
+        y = np.zeros_like(x)
+        getattr(self.lib, f"ref_q1_product_{_sfx(x.dtype)}")(_p(y), _pp(bands), _p(x), u64(x.size))
+        return y
+
+    def q1_defect(self, bands, rhs, x):
+        r = np.zeros_like(x)
+        getattr(self.lib, f"ref_q1_defect_{_sfx(x.dtype)}")(_p(r), _p(rhs), _pp(bands), _p(x), u64(x.size))
+        return r
+
+    def dot(self, x, y):
+        return getattr(self.lib, f"ref_dot_{_sfx(x.dtype)}")(_p(x), _p(y), u64(x.size))
+
+    def norm(self, x, root=False):
+        return getattr(self.lib, f"ref_norm_{_sfx(x.dtype)}")(_p(x), u64(x.size), C.c_int(int(root)))
+
+    def restriction(self, fine, mask=(2,) * 8):
+        m = np.asarray(mask, np.uint64)
+        side_f = int(round(np.sqrt(fine.size)))
+        side_c = (side_f - 1) // 2 + 1
+        coarse = np.zeros(side_c * side_c, fine.dtype)
+        getattr(self.lib, f"ref_restriction_{_sfx(fine.dtype)}")(
+            _p(coarse), u64(coarse.size), _p(fine), u64(fine.size), _p(m))
+        return coarse
+
+    def prolongation(self, coarse, n_fine, mask=(2,) * 8):
+        m = np.asarray(mask, np.uint64)
+        fine = np.zeros(n_fine, coarse.dtype)
+        getattr(self.lib, f"ref_prolongation_{_sfx(coarse.dtype)}")(
+            _p(fine), u64(n_fine), _p(coarse), u64(coarse.size), _p(m))
+        return fine
+
+    def jacobi_q1(self, x, bands, rhs, dinv, iters):
+        x = x.copy()
+        getattr(self.lib, f"ref_jacobi_q1_{_sfx(x.dtype)}")(
+            _p(x), _pp(bands), _p(rhs), _p(dinv), u64(x.size), u64(iters))
+        return x
+
+    def cg_none_q1(self, bands, b, x0, max_iters=10000, eps=1e-8):
+        x = x0.copy()
+        used = getattr(self.lib, f"ref_cg_none_q1_{_sfx(b.dtype)}")(
+            _pp(bands), _p(b), _p(x), u64(b.size), u64(max_iters), eps)
+        return x, used
+
+    def cg_none_ell(self, E, b, x0, max_iters=10000, eps=1e-8):
+        x = x0.copy()
+        used = getattr(self.lib, f"ref_cg_none_ell_{_sfx(b.dtype)}")(
+            u64(E["rows"]), u64(E["stride"]), u64(E["ncpr"]), u64(E["threads"]), _p(E["Aj"]),
+            _p(E["Ax"]), _p(b), _p(x), u64(max_iters), eps)
+        return x, used
+
+    def cg_var_ell(self, E, P, b, x0, max_iters=10000, eps=1e-8):
+        x = x0.copy()
+        used = getattr(self.lib, f"ref_cg_var_ell_{_sfx(b.dtype)}")(
+            u64(E["rows"]), u64(E["stride"]), u64(E["ncpr"]), u64(E["threads"]), _p(E["Aj"]),
+            _p(E["Ax"]), _p(P), _p(b), _p(x), u64(max_iters), eps)
+        return x, used
+
+    def cg_konv_q1(self, bands, rhs, konv_rad):
+        x = np.zeros_like(rhs)
+        getattr(self.lib, f"ref_cg_konv_q1_{_sfx(rhs.dtype)}")(
+            _pp(bands), _p(rhs), _p(x), u64(rhs.size), C.c_double(konv_rad))
+        return x
+
+    def mg_mixed_q1(self, A64, b, x0, a32_levels, *, min_level, max_level, n_max_iter=2,
+                    tolerance=1e-8, n_pre_smooth=2, adapt=1.0, mask=(2,) * 8):
+        x = x0.copy()
+        dummy = np.zeros(9, np.float32)
+        a_ptrs = (vp * (9 * (max_level + 1)))()
+        for L in range(max_level + 1):
+            for k in range(9):
+                a_ptrs[9 * L + k] = (a32_levels[L][k] if L in a32_levels else dummy).ctypes.data
+        m = np.asarray(mask, np.uint64)
+        outer, inner = u64(), u64()
+        self.lib.ref_mg_mixed_q1(_pp(A64), _p(b), _p(x), a_ptrs, u64(min_level), u64(max_level),
+                                 u64(n_max_iter), C.c_double(tolerance), u64(n_pre_smooth),
+                                 C.c_double(adapt), _p(m), C.byref(outer), C.byref(inner))
+        return dict(x=x, outer=outer.value, inner=inner.value)
+
+
+def have_reference() -> bool:
+    return os.path.exists(REF_SO)

@@ -1,0 +1,325 @@
+/*
+ * refdrv.cc -- C entry points over HONEI's own tags::CPU templates, compiled
+ * from the reference sources in place (see oracle/ref/Makefile).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Used in the development container to check the
+ * C restatement (oracle.c) against the reference itself and to write golden
+ * fixtures under tests/golden/.  It never ships: nothing in the product
+ * library, and nothing that runs on a GPU host, loads it.
+ */
+#include <honei/la/dense_vector.hh>
+#include <honei/la/banded_matrix_qx.hh>
+#include <honei/la/sparse_matrix.hh>
+#include <honei/la/sparse_matrix_ell.hh>
+#include <honei/la/product.hh>
+#include <honei/la/dot_product.hh>
+#include <honei/la/norm.hh>
+#include <honei/la/scaled_sum.hh>
+#include <honei/la/element_inverse.hh>
+#include <honei/la/scale.hh>
+#include <honei/math/defect.hh>
+#include <honei/math/cg.hh>
+#include <honei/math/conjugate_gradients.hh>
+#include <honei/math/jacobi.hh>
+#include <honei/math/multigrid.hh>
+#include <honei/math/restriction.hh>
+#include <honei/math/prolongation.hh>
+#include <honei/math/fill_matrix.hh>
+#include <honei/util/configuration.hh>
+
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <stdint.h>
+#include <string>
+
+using namespace honei;
+
+namespace
+{
+    template <typename DT_> DenseVector<DT_> dv(const DT_ * src, unsigned long n)
+    {
+        DenseVector<DT_> v(n);
+        for (unsigned long i(0) ; i < n ; ++i)
+            v[i] = src[i];
+        return v;
+    }
+
+    template <typename DT_> void out(DT_ * dst, const DenseVector<DT_> & v)
+    {
+        for (unsigned long i(0) ; i < v.size() ; ++i)
+            dst[i] = v[i];
+    }
+
+    template <typename DT_> BandedMatrixQx<Q1Type, DT_> q1(const DT_ * const * b, unsigned long n)
+    {
+        return BandedMatrixQx<Q1Type, DT_>(n, dv(b[0], n), dv(b[1], n), dv(b[2], n), dv(b[3], n),
+                dv(b[4], n), dv(b[5], n), dv(b[6], n), dv(b[7], n), dv(b[8], n));
+    }
+
+    template <typename DT_> SparseMatrixELL<DT_> ell(unsigned long rows, unsigned long cols,
+            unsigned long stride, unsigned long ncpr, unsigned long threads,
+            const uint64_t * Aj, const DT_ * Ax)
+    {
+        DenseVector<unsigned long> aj(ncpr * stride);
+        DenseVector<DT_> ax(ncpr * stride);
+        for (unsigned long i(0) ; i < ncpr * stride ; ++i)
+        {
+            aj[i] = Aj[i];
+            ax[i] = Ax[i];
+        }
+        return SparseMatrixELL<DT_>(rows, cols, stride, ncpr, aj, ax, threads);
+    }
+
+    template <typename DT_> void ell_from_csr(unsigned long rows, unsigned long cols, const uint64_t * rowptr,
+            const uint64_t * col, const DT_ * val, unsigned long threads,
+            uint64_t * stride, uint64_t * ncpr, uint64_t * Aj, DT_ * Ax, uint64_t * Arl)
+    {
+        Configuration::instance()->set_value("ell::threads", (int)threads);
+        SparseMatrix<DT_> sm(rows, cols);
+        for (unsigned long r(0) ; r < rows ; ++r)
+            for (uint64_t i(rowptr[r]) ; i < rowptr[r + 1] ; ++i)
+                sm(r, col[i], val[i]);
+        SparseMatrixELL<DT_> e(sm);
+        *stride = e.stride();
+        *ncpr = e.num_cols_per_row();
+        if (Aj != 0)
+        {
+            for (unsigned long i(0) ; i < e.Aj().size() ; ++i)
+            {
+                Aj[i] = e.Aj()[i];
+                Ax[i] = e.Ax()[i];
+            }
+            for (unsigned long i(0) ; i < rows ; ++i)
+                Arl[i] = e.Arl()[i];
+        }
+        Configuration::instance()->set_value("ell::threads", 1);
+    }
+}
+
+extern "C"
+{
+
+#define REF_TYPED(DT, S)                                                                                   \
+void ref_ell_from_csr_##S(uint64_t rows, uint64_t cols, const uint64_t * rowptr, const uint64_t * col,     \
+        const DT * val, uint64_t threads, uint64_t * stride, uint64_t * ncpr, uint64_t * Aj, DT * Ax,       \
+        uint64_t * Arl)                                                                                    \
+{                                                                                                          \
+    ell_from_csr<DT>(rows, cols, rowptr, col, val, threads, stride, ncpr, Aj, Ax, Arl);                     \
+}                                                                                                          \
+                                                                                                           \
+void ref_ell_product_##S(DT * y, uint64_t rows, uint64_t cols, uint64_t stride, uint64_t ncpr,            \
+        uint64_t threads, const uint64_t * Aj, const DT * Ax, const DT * x, uint64_t * Arl_out)            \
+{                                                                                                          \
+    SparseMatrixELL<DT> A(ell<DT>(rows, cols, stride, ncpr, threads, Aj, Ax));                            \
+    DenseVector<DT> xv(dv(x, cols)), yv(rows);                                                             \
+    Product<tags::CPU>::value(yv, A, xv);                                                                  \
+    out(y, yv);                                                                                            \
+    if (Arl_out)                                                                                           \
+        for (unsigned long i(0) ; i < rows ; ++i) Arl_out[i] = A.Arl()[i];                                 \
+}                                                                                                          \
+                                                                                                           \
+void ref_ell_defect_##S(DT * r, const DT * rhs, uint64_t rows, uint64_t stride, uint64_t ncpr,            \
+        uint64_t threads, const uint64_t * Aj, const DT * Ax, const DT * x)                                \
+{                                                                                                          \
+    SparseMatrixELL<DT> A(ell<DT>(rows, rows, stride, ncpr, threads, Aj, Ax));                            \
+    DenseVector<DT> xv(dv(x, rows)), rv(rows), bv(dv(rhs, rows));                                          \
+    Defect<tags::CPU>::value(rv, bv, A, xv);                                                               \
+    out(r, rv);                                                                                            \
+}                                                                                                          \
+                                                                                                           \
+void ref_q1_product_##S(DT * y, const DT * const * bands, const DT * x, uint64_t n)                       \
+{                                                                                                          \
+    BandedMatrixQx<Q1Type, DT> A(q1(bands, n));                                                            \
+    DenseVector<DT> xv(dv(x, n)), yv(n);                                                                   \
+    Product<tags::CPU>::value(yv, A, xv);                                                                  \
+    out(y, yv);                                                                                            \
+}                                                                                                          \
+                                                                                                           \
+void ref_q1_defect_##S(DT * r, const DT * rhs, const DT * const * bands, const DT * x, uint64_t n)        \
+{                                                                                                          \
+    BandedMatrixQx<Q1Type, DT> A(q1(bands, n));                                                            \
+    DenseVector<DT> xv(dv(x, n)), rv(n), bv(dv(rhs, n));                                                   \
+    Defect<tags::CPU>::value(rv, bv, A, xv);                                                               \
+    out(r, rv);                                                                                            \
+}                                                                                                          \
+                                                                                                           \
+DT ref_dot_##S(const DT * x, const DT * y, uint64_t n)                                                    \
+{                                                                                                          \
+    return DotProduct<tags::CPU>::value(dv(x, n), dv(y, n));                                               \
+}                                                                                                          \
+                                                                                                           \
+DT ref_norm_##S(const DT * x, uint64_t n, int root)                                                       \
+{                                                                                                          \
+    DenseVector<DT> v(dv(x, n));                                                                           \
+    return root ? Norm<vnt_l_two, true, tags::CPU>::value(v) : Norm<vnt_l_two, false, tags::CPU>::value(v);\
+}                                                                                                          \
+                                                                                                           \
+void ref_restriction_##S(DT * coarse, uint64_t n_coarse, const DT * fine, uint64_t n_fine,                \
+        const uint64_t * mask)                                                                             \
+{                                                                                                          \
+    DenseVector<DT> c(n_coarse, DT(0)), f(dv(fine, n_fine));                                               \
+    DenseVector<unsigned long> m(8);                                                                       \
+    for (int i(0) ; i < 8 ; ++i) m[i] = mask[i];                                                           \
+    BandedMatrixQx<Q1Type, DT> dummy(9, DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)),              \
+            DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)),               \
+            DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)),               \
+            DenseVector<DT>(9, DT(0)));                                                                    \
+    Restriction<tags::CPU, methods::NONE>::value(c, f, m, dummy);                                          \
+    out(coarse, c);                                                                                        \
+}                                                                                                          \
+                                                                                                           \
+void ref_prolongation_##S(DT * fine, uint64_t n_fine, const DT * coarse, uint64_t n_coarse,               \
+        const uint64_t * mask)                                                                             \
+{                                                                                                          \
+    DenseVector<DT> f(n_fine, DT(0)), c(dv(coarse, n_coarse));                                             \
+    DenseVector<unsigned long> m(8);                                                                       \
+    for (int i(0) ; i < 8 ; ++i) m[i] = mask[i];                                                           \
+    BandedMatrixQx<Q1Type, DT> dummy(9, DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)),              \
+            DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)),               \
+            DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)), DenseVector<DT>(9, DT(0)),               \
+            DenseVector<DT>(9, DT(0)));                                                                    \
+    Prolongation<tags::CPU, methods::NONE>::value(f, c, m, dummy);                                         \
+    out(fine, f);                                                                                          \
+}                                                                                                          \
+                                                                                                           \
+void ref_jacobi_q1_##S(DT * x, const DT * const * bands, const DT * rhs, const DT * dinv, uint64_t n,     \
+        uint64_t iters)                                                                                    \
+{                                                                                                          \
+    BandedMatrixQx<Q1Type, DT> A(q1(bands, n));                                                            \
+    DenseVector<DT> xv(dv(x, n)), bv(dv(rhs, n)), di(dv(dinv, n));                                        \
+    Jacobi<tags::CPU>::value(xv, A, bv, xv, iters, DT(0.7), di);                                          \
+    out(x, xv);                                                                                            \
+}                                                                                                          \
+                                                                                                           \
+uint64_t ref_cg_none_q1_##S(const DT * const * bands, const DT * b, DT * x, uint64_t n, uint64_t max_iters,\
+        DT eps)                                                                                            \
+{                                                                                                          \
+    BandedMatrixQx<Q1Type, DT> A(q1(bands, n));                                                            \
+    DenseVector<DT> bv(dv(b, n)), xv(dv(x, n));                                                            \
+    unsigned long used(0);                                                                                 \
+    CGSolver<tags::CPU, methods::NONE>::value(A, A, bv, xv, (unsigned long)max_iters, used, eps);         \
+    out(x, xv);                                                                                            \
+    return used;                                                                                           \
+}                                                                                                          \
+                                                                                                           \
+uint64_t ref_cg_none_ell_##S(uint64_t rows, uint64_t stride, uint64_t ncpr, uint64_t threads,             \
+        const uint64_t * Aj, const DT * Ax, const DT * b, DT * x, uint64_t max_iters, DT eps)              \
+{                                                                                                          \
+    SparseMatrixELL<DT> A(ell<DT>(rows, rows, stride, ncpr, threads, Aj, Ax));                            \
+    DenseVector<DT> bv(dv(b, rows)), xv(dv(x, rows));                                                      \
+    unsigned long used(0);                                                                                 \
+    CGSolver<tags::CPU, methods::NONE>::value(A, A, bv, xv, (unsigned long)max_iters, used, eps);         \
+    out(x, xv);                                                                                            \
+    return used;                                                                                           \
+}                                                                                                          \
+                                                                                                           \
+uint64_t ref_cg_var_ell_##S(uint64_t rows, uint64_t stride, uint64_t ncpr, uint64_t threads,              \
+        const uint64_t * Aj, const DT * Ax, const DT * P, const DT * b, DT * x, uint64_t max_iters,        \
+        DT eps)                                                                                            \
+{                                                                                                          \
+    SparseMatrixELL<DT> A(ell<DT>(rows, rows, stride, ncpr, threads, Aj, Ax));                            \
+    DenseVector<DT> bv(dv(b, rows)), xv(dv(x, rows)), pv(dv(P, rows));                                     \
+    unsigned long used(0);                                                                                 \
+    CGSolver<tags::CPU, methods::VAR>::value(A, pv, bv, xv, (unsigned long)max_iters, used, eps);         \
+    out(x, xv);                                                                                            \
+    return used;                                                                                           \
+}                                                                                                          \
+                                                                                                           \
+void ref_cg_konv_q1_##S(const DT * const * bands, const DT * rhs, DT * x, uint64_t n, double konv_rad)    \
+{                                                                                                          \
+    BandedMatrixQx<Q1Type, DT> A(q1(bands, n));                                                            \
+    DenseVector<DT> bv(dv(rhs, n)), xv(dv(x, n));                                                          \
+    ConjugateGradients<tags::CPU, methods::NONE>::value(A, bv, xv, konv_rad);                             \
+    out(x, xv);                                                                                            \
+}
+
+REF_TYPED(double, f64)
+REF_TYPED(float, f32)
+
+/* Multigrid<CPU, CPU, NONE, JAC, V, MIXED> set up as in
+ * honei/math/poisson_mg_mixed_banded_TEST.cc:176-423; bands a32[level*9+b]
+ * for levels min..max, dinv computed there as ElementInverse + Scale(0.7).
+ * Returns the outer iteration count printed by the solver; *inner gets the
+ * inner total. */
+int ref_mg_mixed_q1(const double * const * A64, const double * b, double * x,
+        const float * const * a32, uint64_t min_level, uint64_t max_level, uint64_t n_max_iter,
+        double tolerance, uint64_t n_pre_smooth, double adapt, const uint64_t * mask,
+        uint64_t * outer, uint64_t * inner)
+{
+    unsigned long n(((1ul << max_level) + 1) * ((1ul << max_level) + 1));
+    BandedMatrixQx<Q1Type, double> A(q1(A64, n));
+    MGInfo<float, BandedMatrixQx<Q1Type, float> > info;
+    info.is_smoother = false;
+    for (int i(0) ; i < 8 ; ++i)
+        info.macro_border_mask[i] = mask[i];
+    info.min_level = min_level;
+    info.max_level = max_level;
+    info.n_max_iter = n_max_iter;
+    info.initial_zero = true;
+    info.tolerance = tolerance;
+    info.convergence_check = false;
+    info.n_pre_smooth = n_pre_smooth;
+    info.n_post_smooth = n_pre_smooth;
+    info.n_max_iter_coarse = 0;
+    info.tolerance_coarse = 1e-2;
+    info.adapt_correction_factor = adapt;
+    for (unsigned long i(0) ; i < min_level ; ++i)
+    {
+        unsigned long size((((1ul << i) + 1) * ((1ul << i) + 1)));
+        if (i == 0)
+            size = 9;
+        DenseVector<float> dummy_band(size, float(0));
+        BandedMatrixQx<Q1Type, float> ac_a(size, dummy_band, dummy_band, dummy_band, dummy_band, dummy_band,
+                dummy_band, dummy_band, dummy_band, dummy_band);
+        info.a.push_back(ac_a);
+        info.c.push_back(DenseVector<float>(size, float(0)));
+        info.d.push_back(DenseVector<float>(size, float(0)));
+        info.rhs.push_back(DenseVector<float>(size, float(0)));
+        info.x.push_back(DenseVector<float>(size, float(0)));
+        info.diags_inverted.push_back(dummy_band.copy());
+    }
+    for (unsigned long i(min_level) ; i <= max_level ; ++i)
+    {
+        unsigned long size((((1ul << i) + 1) * ((1ul << i) + 1)));
+        info.c.push_back(DenseVector<float>(size, float(0)));
+        info.d.push_back(DenseVector<float>(size, float(0)));
+        info.x.push_back(DenseVector<float>(size, float(0)));
+        info.diags_inverted.push_back(DenseVector<float>(size, float(0)));
+        info.rhs.push_back(DenseVector<float>(size, float(0)));
+        info.a.push_back(q1(a32 + 9 * i, size));
+    }
+    for (unsigned long i(0) ; i <= max_level ; ++i)
+    {
+        DenseVector<float> scaled_diag_inverted(info.a[i].band(DD).copy());
+        ElementInverse<tags::CPU>::value(scaled_diag_inverted);
+        Scale<tags::CPU>::value(scaled_diag_inverted, 0.7);
+        info.diags_inverted[i] = scaled_diag_inverted.copy();
+    }
+    DenseVector<double> bv(dv(b, n)), xv(dv(x, n));
+    std::stringstream captured;
+    std::streambuf * old(std::cout.rdbuf(captured.rdbuf()));
+    Multigrid<tags::CPU, tags::CPU, methods::NONE, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
+            A, bv, xv, (unsigned long)11, std::numeric_limits<double>::epsilon(), info);
+    std::cout.rdbuf(old);
+    out(x, xv);
+    std::string line;
+    *outer = 0;
+    *inner = 0;
+    while (std::getline(captured, line))
+    {
+        if (line.find("TN of outer iters: ") == 0)
+            *outer = std::strtoul(line.c_str() + 19, 0, 10);
+        if (line.find("TN of inner iters: ") == 0)
+            *inner = std::strtoul(line.c_str() + 19, 0, 10);
+    }
+    return (int)*outer;
+}
+
+/* The ell::threads value the reference's ELL constructor reads. */
+void ref_set_ell_threads(int threads)
+{
+    Configuration::instance()->set_value("ell::threads", threads);
+}
+
+}

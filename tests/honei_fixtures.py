@@ -1,0 +1,134 @@
+"""Readers for the HONEI test-data formats kept under tests/golden/honei_testdata
+(copied data files of honei/math/testdata).
+
+* FEAST Q1 ``.bin``: int32 n, then 12 float64 arrays of n: DD, LL, LD, LU, DL,
+  DU, UL, UD, UU, b, ana_sol, ref_sol (honei/math/poisson_mg_mixed_banded_TEST.cc:67-118).
+* ELL binary ``.ell``: 5 x uint64 header size, rows, columns, stride,
+  num_cols_per_row, then uint64 Aj[size] and float64 Ax[size]
+  (honei/math/matrix_io.hh:468-583).
+* EXP text vectors (``rhs_6``, ``sol_6``): '#'-lines skipped, one value per
+  line read with atof (honei/math/vector_io.hh:52-116).
+* DV binary vectors: uint64 size + float64[size] (vector_io.hh:205-250).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TESTDATA = os.path.join(GOLDEN, "honei_testdata")
+
+# Band order of BandedMatrixQx (honei/la/band_type.hh): LL LD LU DL DD DU UL UD UU
+BIN_ORDER = ("DD", "LL", "LD", "LU", "DL", "DU", "UL", "UD", "UU")
+BAND_ORDER = ("LL", "LD", "LU", "DL", "DD", "DU", "UL", "UD", "UU")
+
+
+def load_bin(n_or_path):
+    path = n_or_path if isinstance(n_or_path, str) else os.path.join(TESTDATA, f"{n_or_path}.bin")
+    with open(path, "rb") as f:
+        n = int(np.frombuffer(f.read(4), np.int32)[0])
+        arrs = np.frombuffer(f.read(), np.float64)
+    assert arrs.size >= 12 * n, (path, arrs.size, n)
+    arrs = arrs[: 12 * n].reshape(12, n).copy()
+    named = dict(zip(BIN_ORDER, arrs[:9]))
+    bands = [named[k].copy() for k in BAND_ORDER]
+    return dict(n=n, bands=bands, b=arrs[9].copy(), ana_sol=arrs[10].copy(), ref_sol=arrs[11].copy())
+
+
+def load_ell(path):
+    path = path if os.path.isabs(path) else os.path.join(TESTDATA, path)
+    with open(path, "rb") as f:
+        hdr = np.frombuffer(f.read(40), np.uint64)
+        size, rows, cols, stride, ncpr = (int(v) for v in hdr)
+        Aj = np.frombuffer(f.read(8 * size), np.uint64).copy()
+        Ax = np.frombuffer(f.read(8 * size), np.float64).copy()
+    return dict(size=size, rows=rows, cols=cols, stride=stride, ncpr=ncpr, threads=1, Aj=Aj, Ax=Ax)
+
+
+def load_exp(path):
+    path = path if os.path.isabs(path) else os.path.join(TESTDATA, path)
+    vals = []
+    with open(path, "r") as f:
+        lines = f.read().split("\n")
+    # std::getline loop: the fragment after the last newline is dropped
+    for line in lines[:-1]:
+        if "#" in line:
+            continue
+        s = line.strip()
+        try:
+            vals.append(float(s) if s else 0.0)
+        except ValueError:
+            vals.append(0.0)
+    return np.asarray(vals, np.float64)
+
+
+def load_dv(path):
+    path = path if os.path.isabs(path) else os.path.join(TESTDATA, path)
+    with open(path, "rb") as f:
+        size = int(np.frombuffer(f.read(8), np.uint64)[0])
+        return np.frombuffer(f.read(8 * size), np.float64).copy()
+
+
+def level_of(n):
+    for L in range(0, 16):
+        if ((1 << L) + 1) ** 2 == n:
+            return L
+    raise ValueError(n)
+
+
+def side_of(L):
+    return (1 << L) + 1
+
+
+def dinv_reference(dd32):
+    """diags_inverted as the MG tests build it: ElementInverse then
+    Scale(v, 0.7) with a double factor (poisson_mg_mixed_banded_TEST.cc:411-419)."""
+    inv = dd32.copy()
+    nz = inv != 0
+    inv[nz] = np.float32(1.0) / inv[nz]
+    return (inv.astype(np.float64) * 0.7).astype(np.float32)
+
+
+def csr_from_dense(D):
+    rows, cols = D.shape
+    rowptr = [0]
+    col, val = [], []
+    for r in range(rows):
+        nz = np.nonzero(D[r])[0]
+        col.extend(nz.tolist())
+        val.extend(D[r, nz].tolist())
+        rowptr.append(len(col))
+    return np.asarray(rowptr, np.uint64), np.asarray(col, np.uint64), np.asarray(val, D.dtype)
+
+
+def q1_dense(bands, n):
+    """Dense matrix of a Q1 banded matrix, with the product's index guards."""
+    m = int(np.sqrt(n))
+    offs = (-m - 1, -m, -m + 1, -1, 0, 1, m - 1, m, m + 1)
+    D = np.zeros((n, n), bands[0].dtype)
+    for b, o in zip(bands, offs):
+        for i in range(n):
+            j = i + o
+            if 0 <= j < n:
+                D[i, j] += b[i]
+    return D
+
+
+def q1_csr(bands, n, drop_zeros=True):
+    """CSR (ascending columns) of a Q1 banded matrix, as SparseMatrix holds it."""
+    m = int(np.sqrt(n))
+    offs = np.array((-m - 1, -m, -m + 1, -1, 0, 1, m - 1, m, m + 1), np.int64)
+    B = np.stack(bands)  # 9 x n
+    i = np.arange(n, dtype=np.int64)
+    J = i[None, :] + offs[:, None]
+    keep = (J >= 0) & (J < n)
+    if drop_zeros:
+        keep &= B != 0
+    # columns ascend with the offset order for every row
+    counts = keep.sum(axis=0)
+    rowptr = np.zeros(n + 1, np.uint64)
+    rowptr[1:] = np.cumsum(counts)
+    col = J.T[keep.T].astype(np.uint64)
+    val = B.T[keep.T]
+    return rowptr, col, val
