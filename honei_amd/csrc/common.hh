@@ -1,0 +1,87 @@
+// common.hh -- error plumbing, stream helpers and launch-shape constants
+// shared by the gfx950 kernels and the C-ABI of libhsolve.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "hsolve.h"
+
+namespace hsolve
+{
+    // Error raised inside the library; converted to a status code at the C-ABI.
+    struct Error : public std::runtime_error
+    {
+        int code;
+        Error(int c, const std::string & msg) : std::runtime_error(msg), code(c) {}
+    };
+
+    void set_last_error(const std::string & msg);
+
+    inline void hip_check(hipError_t e, const char * what, const char * file, int line)
+    {
+        if (e != hipSuccess)
+        {
+            throw Error(HSOLVE_ERR_HIP, std::string(what) + " failed: " + hipGetErrorString(e) + " at " +
+                    file + ":" + std::to_string(line));
+        }
+    }
+
+    inline void require(bool cond, int code, const std::string & msg)
+    {
+        if (! cond)
+            throw Error(code, msg);
+    }
+
+    inline hipStream_t as_stream(void * s) { return reinterpret_cast<hipStream_t>(s); }
+
+    // Launch-shape constants for gfx950: 64-wide waves, 256 CUs in 8 XCDs.
+    constexpr int kWave = 64;
+    constexpr int kBlock = 256;
+    constexpr int kReduceBlocks = 2048;   // partial sums per one-pass reduction
+
+    inline unsigned grid_for(uint64_t work, unsigned per_block)
+    {
+        uint64_t g = (work + per_block - 1) / per_block;
+        if (g == 0)
+            g = 1;
+        return (unsigned)g;
+    }
+
+    // Per-stream scratch used by the one-pass reductions: a partials array
+    // and a ticket counter that the last block resets, so a reduction is
+    // safe to replay inside a captured graph without any memset.
+    struct ReduceScratch
+    {
+        double * partials;      // kReduceBlocks doubles
+        unsigned * ticket;      // 1 counter, zero at rest
+    };
+    ReduceScratch & reduce_scratch(hipStream_t stream);
+}
+
+#define HIP_CHECK(expr) ::hsolve::hip_check((expr), #expr, __FILE__, __LINE__)
+
+// Wrap a C-ABI body: exceptions become status codes and a stored message.
+#define HSOLVE_API_BEGIN try {
+#define HSOLVE_API_END                                                         \
+    }                                                                          \
+    catch (const ::hsolve::Error & e)                                          \
+    {                                                                          \
+        ::hsolve::set_last_error(e.what());                                    \
+        return e.code;                                                         \
+    }                                                                          \
+    catch (const std::bad_alloc &)                                             \
+    {                                                                          \
+        ::hsolve::set_last_error("host allocation failed");                    \
+        return HSOLVE_ERR_ALLOC;                                               \
+    }                                                                          \
+    catch (const std::exception & e)                                           \
+    {                                                                          \
+        ::hsolve::set_last_error(e.what());                                    \
+        return HSOLVE_ERR_INVALID;                                             \
+    }                                                                          \
+    return HSOLVE_OK;

@@ -1,0 +1,296 @@
+"""Device-resident containers over libhsolve, for Python callers (tests,
+benchmarks, the serving loop).  Every operation runs on the GPU through the
+C-ABI; nothing here computes on the host."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import MGParams, check, lib, u64, vp
+
+_DT = {np.dtype(np.float32): 0, np.dtype(np.float64): 1}
+
+
+def _sfx(dtype):
+    return "double" if np.dtype(dtype) == np.float64 else "float"
+
+
+class DeviceBuffer:
+    """A hipMalloc'ed byte range, freed with the object."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = vp()
+        check(lib().hsolve_malloc(C.byref(p), u64(max(self.nbytes, 1))), "malloc")
+        self.ptr = p.value
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().hsolve_free(vp(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceVector:
+    """DenseVector<DT> kept on the device."""
+
+    def __init__(self, size: int, dtype=np.float64, data=None):
+        self.dtype = np.dtype(dtype)
+        self.size = int(size)
+        self.buf = DeviceBuffer(self.size * self.dtype.itemsize)
+        if data is not None:
+            self.upload(data)
+
+    @classmethod
+    def from_numpy(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        return cls(arr.size, arr.dtype, arr)
+
+    @classmethod
+    def zeros(cls, size, dtype=np.float64):
+        v = cls(size, dtype)
+        v.fill(0)
+        return v
+
+    @property
+    def ptr(self):
+        return self.buf.ptr
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr, self.dtype)
+        assert arr.size == self.size
+        check(lib().hsolve_upload(arr.ctypes.data_as(vp), vp(self.ptr), u64(arr.nbytes), vp(stream)), "upload")
+
+    def get(self, stream=None):
+        out = np.empty(self.size, self.dtype)
+        check(lib().hsolve_download(vp(self.ptr), out.ctypes.data_as(vp), u64(out.nbytes), vp(stream)), "download")
+        return out
+
+    def fill(self, value, stream=None):
+        if self.dtype == np.float64:
+            check(lib().hsolve_fill_double(vp(self.ptr), float(value), u64(self.size), vp(stream)), "fill")
+        else:
+            check(lib().hsolve_fill_float(vp(self.ptr), float(value), u64(self.size), vp(stream)), "fill")
+
+    def copy_from(self, other, stream=None):
+        assert other.size == self.size and other.dtype == self.dtype
+        check(lib().hsolve_copy(vp(other.ptr), vp(self.ptr), u64(self.size * self.dtype.itemsize), vp(stream)),
+              "copy")
+
+
+class Q1Matrix:
+    """BandedMatrixQx<Q1Type, DT>: nine device bands LL LD LU DL DD DU UL UD UU."""
+
+    def __init__(self, bands):
+        bands = [np.ascontiguousarray(b) for b in bands]
+        assert len(bands) == 9
+        self.dtype = bands[0].dtype
+        self.size = bands[0].size
+        self.root = int(round(np.sqrt(self.size)))
+        self.bands = [DeviceVector.from_numpy(b) for b in bands]
+        self.ptrs = (vp * 9)(*[b.ptr for b in self.bands])
+
+    def band_ptrs(self):
+        return [vp(b.ptr) for b in self.bands]
+
+
+class EllMatrix:
+    """SparseMatrixELL<DT> uploaded into a device handle (uint32 columns)."""
+
+    def __init__(self, rows, columns, stride, num_cols_per_row, threads, Aj, Ax, Arl=None):
+        Aj = np.ascontiguousarray(Aj, np.uint64)
+        Ax = np.ascontiguousarray(Ax)
+        self.dtype = Ax.dtype
+        self.rows, self.columns = int(rows), int(columns)
+        h = vp()
+        arl_p = None
+        if Arl is not None:
+            self._arl = np.ascontiguousarray(Arl, np.uint64)
+            arl_p = self._arl.ctypes.data_as(vp)
+        check(lib().hsolve_ell_create(_DT[self.dtype], u64(rows), u64(columns), u64(stride),
+                                      u64(num_cols_per_row), u64(threads), Aj.ctypes.data_as(vp),
+                                      Ax.ctypes.data_as(vp), arl_p, C.byref(h)), "ell_create")
+        self.handle = h.value
+        vals = [u64() for _ in range(7)]
+        check(lib().hsolve_ell_info(vp(self.handle), *[C.byref(v) for v in vals]), "ell_info")
+        (self.rows, self.columns, self.stride, self.ncpr, self.threads, self.nnz,
+         self.device_bytes) = (v.value for v in vals)
+
+    @classmethod
+    def from_dict(cls, E, columns=None):
+        return cls(E["rows"], columns if columns is not None else E.get("cols", E["rows"]), E["stride"],
+                   E["ncpr"], E.get("threads", 1), E["Aj"], E["Ax"], E.get("Arl"))
+
+    def spmv(self, x, y, rhs=None, stream=None):
+        check(lib().hsolve_ell_spmv(vp(self.handle), vp(x.ptr), vp(y.ptr), vp(rhs.ptr) if rhs else None,
+                                    vp(stream)), "ell_spmv")
+
+    def jacobi(self, rhs, dinv, x_old, x_new, stream=None):
+        check(lib().hsolve_ell_jacobi(vp(self.handle), vp(rhs.ptr), vp(dinv.ptr), vp(x_old.ptr), vp(x_new.ptr),
+                                      vp(stream)), "ell_jacobi")
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().hsolve_ell_destroy(vp(self.handle))
+                self.handle = None
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- ops
+def q1_product(A: Q1Matrix, x: DeviceVector, y: DeviceVector, stream=None):
+    f = getattr(lib(), f"hsolve_product_bmdv_q1_{_sfx(A.dtype)}")
+    check(f(*A.band_ptrs(), vp(x.ptr), vp(y.ptr), u64(A.size), u64(A.root), vp(stream)), "product_q1")
+
+
+def q1_defect(A: Q1Matrix, rhs: DeviceVector, x: DeviceVector, y: DeviceVector, stream=None):
+    f = getattr(lib(), f"hsolve_defect_q1_{_sfx(A.dtype)}")
+    check(f(vp(rhs.ptr), *A.band_ptrs(), vp(x.ptr), vp(y.ptr), u64(A.size), u64(A.root), vp(stream)), "defect_q1")
+
+
+def q1_jacobi(A: Q1Matrix, rhs, dinv, x_old, x_new, stream=None):
+    f = getattr(lib(), f"hsolve_jacobi_q1_{_sfx(A.dtype)}")
+    check(f(vp(rhs.ptr), A.ptrs, vp(dinv.ptr), vp(x_old.ptr), vp(x_new.ptr), u64(A.size), vp(stream)), "jacobi_q1")
+
+
+def dot(a: DeviceVector, b: DeviceVector, stream=None):
+    if a.dtype == np.float64:
+        out = C.c_double()
+        check(lib().hsolve_dot_product_two_double_host(vp(a.ptr), vp(b.ptr), u64(a.size), C.byref(out),
+                                                       vp(stream)), "dot")
+    else:
+        out = C.c_float()
+        check(lib().hsolve_dot_product_two_float_host(vp(a.ptr), vp(b.ptr), u64(a.size), C.byref(out),
+                                                      vp(stream)), "dot")
+    return out.value
+
+
+def norm(a: DeviceVector, root=False, stream=None):
+    if a.dtype == np.float64:
+        out = C.c_double()
+        check(lib().hsolve_norm_l2_one_double_host(vp(a.ptr), u64(a.size), int(root), C.byref(out), vp(stream)),
+              "norm")
+    else:
+        out = C.c_float()
+        check(lib().hsolve_norm_l2_one_float_host(vp(a.ptr), u64(a.size), int(root), C.byref(out), vp(stream)),
+              "norm")
+    return out.value
+
+
+def restriction(coarse: DeviceVector, fine: DeviceVector, mask=(2,) * 8, stream=None):
+    m = np.asarray(mask, np.uint64)
+    f = getattr(lib(), f"hsolve_restriction_{_sfx(fine.dtype)}")
+    check(f(vp(coarse.ptr), u64(coarse.size), vp(fine.ptr), u64(fine.size), m.ctypes.data_as(vp), vp(stream)),
+          "restriction")
+
+
+def prolongation(fine: DeviceVector, coarse: DeviceVector, mask=(2,) * 8, stream=None):
+    m = np.asarray(mask, np.uint64)
+    f = getattr(lib(), f"hsolve_prolongation_{_sfx(fine.dtype)}")
+    check(f(vp(fine.ptr), u64(fine.size), vp(coarse.ptr), u64(coarse.size), m.ctypes.data_as(vp), vp(stream)),
+          "prolongation")
+
+
+def prolongation_axpy(fine_x: DeviceVector, coarse: DeviceVector, alpha, mask=(2,) * 8, stream=None):
+    m = np.asarray(mask, np.uint64)
+    f = getattr(lib(), f"hsolve_prolongation_axpy_{_sfx(fine_x.dtype)}")
+    check(f(vp(fine_x.ptr), u64(fine_x.size), vp(coarse.ptr), u64(coarse.size), alpha, m.ctypes.data_as(vp),
+            vp(stream)), "prolongation_axpy")
+
+
+def cg_konv_q1(A: Q1Matrix, rhs: DeviceVector, x: DeviceVector, konv_rad, max_iters=0, stream=None):
+    it = DeviceVector(1, np.uint64)
+    f = getattr(lib(), f"hsolve_cg_konv_q1_{_sfx(A.dtype)}")
+    check(f(A.ptrs, vp(rhs.ptr), vp(x.ptr), u64(A.size), float(konv_rad), u64(max_iters), vp(it.ptr), vp(stream)),
+          "cg_konv_q1")
+    return int(it.get()[0])
+
+
+# ----------------------------------------------------------------- solvers
+def cg_q1(A: Q1Matrix, b: DeviceVector, x: DeviceVector, max_iters=10000, eps=1e-8, stream=None, raise_on_cap=True):
+    used, defect = u64(), C.c_double()
+    rc = lib().hsolve_cg_q1(_DT[A.dtype], A.ptrs, u64(A.size), vp(b.ptr), vp(x.ptr), u64(max_iters), float(eps),
+                            C.byref(used), C.byref(defect), vp(stream))
+    if rc != 5 or raise_on_cap:
+        check(rc, "cg_q1")
+    return used.value, defect.value
+
+
+def cg_ell(A: EllMatrix, b: DeviceVector, x: DeviceVector, max_iters=10000, eps=1e-8, stream=None,
+           raise_on_cap=True):
+    used, defect = u64(), C.c_double()
+    rc = lib().hsolve_cg_ell(vp(A.handle), vp(b.ptr), vp(x.ptr), u64(max_iters), float(eps), C.byref(used),
+                             C.byref(defect), vp(stream))
+    if rc != 5 or raise_on_cap:
+        check(rc, "cg_ell")
+    return used.value, defect.value
+
+
+def pcg_ell(A: EllMatrix, P: DeviceVector, b: DeviceVector, x: DeviceVector, max_iters=10000, eps=1e-8,
+            stream=None, raise_on_cap=True):
+    used, defect = u64(), C.c_double()
+    rc = lib().hsolve_pcg_ell(vp(A.handle), vp(P.ptr), vp(b.ptr), vp(x.ptr), u64(max_iters), float(eps),
+                              C.byref(used), C.byref(defect), vp(stream))
+    if rc != 5 or raise_on_cap:
+        check(rc, "pcg_ell")
+    return used.value, defect.value
+
+
+def mg_params(min_level, max_level, n_max_iter=2, tolerance=1e-8, n_pre_smooth=2, adapt=1.0, mask=(2,) * 8,
+              initial_zero=True, convergence_check=False, max_outer=0):
+    return MGParams(min_level, max_level, n_max_iter, int(initial_zero), int(convergence_check), tolerance,
+                    n_pre_smooth, adapt, (u64 * 8)(*mask), max_outer)
+
+
+class MGSolver:
+    """Multigrid<MI355X, MI355X, NONE, JAC, CYCLE::V, MIXED> hierarchy, device resident."""
+
+    def __init__(self, params: MGParams, bands64=None, bands32_levels=None, dinv32_levels=None):
+        self.params = params
+        h = vp()
+        if bands64 is None:
+            check(lib().hsolve_mg_create_poisson(C.byref(params), C.byref(h)), "mg_create_poisson")
+        else:
+            maxl = params.max_level
+            self._keep = [np.ascontiguousarray(b, np.float64) for b in bands64]
+            b64 = (vp * 9)(*[b.ctypes.data for b in self._keep])
+            b32 = (vp * (9 * (maxl + 1)))()
+            d32 = (vp * (maxl + 1))()
+            for L in range(params.min_level, maxl + 1):
+                for k in range(9):
+                    a = np.ascontiguousarray(bands32_levels[L][k], np.float32)
+                    self._keep.append(a)
+                    b32[9 * L + k] = a.ctypes.data
+                if dinv32_levels is not None and L in dinv32_levels:
+                    a = np.ascontiguousarray(dinv32_levels[L], np.float32)
+                    self._keep.append(a)
+                    d32[L] = a.ctypes.data
+            check(lib().hsolve_mg_create(C.byref(params), b64, b32, d32, C.byref(h)), "mg_create")
+            self._keep = None
+        self.handle = h.value
+        nb = u64()
+        check(lib().hsolve_mg_device_bytes(vp(self.handle), C.byref(nb)), "mg_device_bytes")
+        self.device_bytes = nb.value
+
+    def solve(self, b: DeviceVector, x: DeviceVector, stream=None, raise_on_cap=True):
+        o, i, fd, idf = u64(), u64(), C.c_double(), C.c_double()
+        rc = lib().hsolve_mg_solve(vp(self.handle), vp(b.ptr), vp(x.ptr), C.byref(o), C.byref(i), C.byref(fd),
+                                   C.byref(idf), vp(stream))
+        if rc != 5 or raise_on_cap:
+            check(rc, "mg_solve")
+        return dict(outer=o.value, inner=i.value, final_defect=fd.value, initial_defect=idf.value, status=rc)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().hsolve_mg_destroy(vp(self.handle))
+                self.handle = None
+        except Exception:
+            pass

@@ -1,0 +1,280 @@
+/*
+ * hsolve.h -- C-ABI of the MI355X (gfx950) backend for HONEI's sparse
+ * solver path.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (HSOLVE_OK == 0) and never
+ *     throws; hsolve_last_error() gives the message of the last failure on
+ *     the calling thread.  Every HIP call inside is checked.
+ *   - Vector and matrix arguments are device pointers unless named *_host.
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *     Kernels are enqueued asynchronously; functions that return a host
+ *     value synchronise that stream.
+ *   - Sizes are uint64_t (the reference's `unsigned long` on LP64).
+ *
+ * The first block mirrors the reference's CUDA C-ABI
+ * (honei/backends/cuda/operations.hh:28-130, transfer.hh, multi_gpu.hh):
+ * same names with the `hsolve_` prefix instead of `cuda_`, same argument
+ * order and meaning, minus the per-call `blocksize`/`gridsize` tuning
+ * arguments (launch shapes are chosen for gfx950 inside) and plus an
+ * explicit stream.  Reductions write their result to a device scalar or,
+ * in the *_host forms, return it.
+ *
+ * The second block is new: device-resident matrix handles (32-bit column
+ * indices, compact row lengths), fused smoother/transfer kernels and the
+ * solver entry points (CG, MG-IR) that the C++ template layer in
+ * honei_amd/include implements on top of the kernels.
+ */
+#ifndef HSOLVE_H
+#define HSOLVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum hsolve_status_code {
+    HSOLVE_OK = 0,
+    HSOLVE_ERR_INVALID = 1,       /* bad argument (null pointer, bad layout) */
+    HSOLVE_ERR_SIZE = 2,          /* sizes do not match (VectorSizeDoesNotMatch) */
+    HSOLVE_ERR_HIP = 3,           /* a HIP runtime call failed */
+    HSOLVE_ERR_ALLOC = 4,         /* device allocation failed (InternalError) */
+    HSOLVE_ERR_NOT_CONVERGED = 5, /* solver hit its iteration cap */
+    HSOLVE_ERR_BUDGET = 6         /* request exceeds the device-memory budget */
+};
+
+const char *hsolve_last_error(void);
+const char *hsolve_version(void);
+
+/* ---------------- runtime: transfer.hh:26-60, multi_gpu.hh:24-40 ---------- */
+int hsolve_device_count(int *count);                     /* cuda_device_count */
+int hsolve_get_device(int *device);                      /* cuda_get_device */
+int hsolve_set_device(int device);                       /* cuda_set_device */
+int hsolve_malloc(void **ptr, uint64_t bytes);           /* cuda_malloc */
+int hsolve_malloc_host(void **ptr, uint64_t bytes);      /* cuda_malloc_host (pinned) */
+int hsolve_free(void *ptr);                              /* cuda_free */
+int hsolve_free_host(void *ptr);                         /* cuda_free_host */
+int hsolve_upload(const void *src_host, void *dst, uint64_t bytes, void *stream);    /* cuda_upload(_async) */
+int hsolve_download(const void *src, void *dst_host, uint64_t bytes, void *stream);  /* cuda_download(_async) */
+int hsolve_copy(const void *src, void *dst, uint64_t bytes, void *stream);           /* cuda_copy */
+int hsolve_fill_zero(void *dst, uint64_t bytes, void *stream);                       /* cuda_fill_zero (kernel, no memset) */
+int hsolve_fill_float(void *dst, float value, uint64_t size, void *stream);
+int hsolve_fill_double(void *dst, double value, uint64_t size, void *stream);
+int hsolve_synchronize(void *stream);                    /* cuda_thread_synchronize */
+int hsolve_stream_create(void **stream);
+int hsolve_stream_destroy(void *stream);
+
+/* ---------------- operations.hh mirror ------------------------------------- */
+/* ELL SpMV y = A x on rows [row_start, row_end) (row_end 0 = all rows), in
+ * HONEI's SMELL layout: element k of lane t of row r at r*threads + t +
+ * k*stride; Aj uint64 column indices, Arl uint64 row step counts.
+ * Replaces cuda_product_smell_dv_{float,double} (product.cu:356-402). */
+int hsolve_product_smell_dv_float(const void *x, void *y, const void *Aj, const void *Ax,
+                                  const void *Arl, uint64_t row_start, uint64_t row_end,
+                                  uint64_t num_cols_per_row, uint64_t stride, uint64_t threads,
+                                  void *stream);
+int hsolve_product_smell_dv_double(const void *x, void *y, const void *Aj, const void *Ax,
+                                   const void *Arl, uint64_t row_start, uint64_t row_end,
+                                   uint64_t num_cols_per_row, uint64_t stride, uint64_t threads,
+                                   void *stream);
+/* ELL defect result = rhs - A b.  Replaces cuda_defect_smell_dv_* (defect.cu:367-417). */
+int hsolve_defect_smell_dv_float(const void *rhs, void *result, const void *Aj, const void *Ax,
+                                 const void *Arl, const void *b, uint64_t row_start,
+                                 uint64_t row_end, uint64_t num_cols_per_row, uint64_t stride,
+                                 uint64_t threads, void *stream);
+int hsolve_defect_smell_dv_double(const void *rhs, void *result, const void *Aj, const void *Ax,
+                                  const void *Arl, const void *b, uint64_t row_start,
+                                  uint64_t row_end, uint64_t num_cols_per_row, uint64_t stride,
+                                  uint64_t threads, void *stream);
+/* Q1 banded SpMV y = A x, m = sqrt(size).  Replaces cuda_product_bmdv_q1_*
+ * (product.cu:300-354); sums in tags::CPU order DD,LL,LD,LU,DL,DU,UL,UD,UU. */
+int hsolve_product_bmdv_q1_float(const void *ll, const void *ld, const void *lu, const void *dl,
+                                 const void *dd, const void *du, const void *ul, const void *ud,
+                                 const void *uu, const void *x, void *y, uint64_t size, uint64_t m,
+                                 void *stream);
+int hsolve_product_bmdv_q1_double(const void *ll, const void *ld, const void *lu, const void *dl,
+                                  const void *dd, const void *du, const void *ul, const void *ud,
+                                  const void *uu, const void *x, void *y, uint64_t size,
+                                  uint64_t m, void *stream);
+/* Q1 defect y = rhs - A x.  Replaces cuda_defect_q1_* (defect.cu:305-365). */
+int hsolve_defect_q1_float(const void *rhs, const void *ll, const void *ld, const void *lu,
+                           const void *dl, const void *dd, const void *du, const void *ul,
+                           const void *ud, const void *uu, const void *x, void *y, uint64_t size,
+                           uint64_t m, void *stream);
+int hsolve_defect_q1_double(const void *rhs, const void *ll, const void *ld, const void *lu,
+                            const void *dl, const void *dd, const void *du, const void *ul,
+                            const void *ud, const void *uu, const void *x, void *y,
+                            uint64_t size, uint64_t m, void *stream);
+/* Dot product and squared L2 norm (root = 0) or L2 norm (root = 1), one
+ * pass, deterministic order; result_dev is a device scalar of the vector's
+ * type.  Replace cuda_dot_product_two_* (dot_product.cu:77-155) and
+ * cuda_norm_l2_one_* (norm.cu:77-155). */
+int hsolve_dot_product_two_float(const void *a, const void *b, uint64_t size, void *result_dev,
+                                 void *stream);
+int hsolve_dot_product_two_double(const void *a, const void *b, uint64_t size, void *result_dev,
+                                  void *stream);
+int hsolve_norm_l2_one_float(const void *a, uint64_t size, int root, void *result_dev,
+                             void *stream);
+int hsolve_norm_l2_one_double(const void *a, uint64_t size, int root, void *result_dev,
+                              void *stream);
+int hsolve_dot_product_two_float_host(const void *a, const void *b, uint64_t size, float *result,
+                                      void *stream);
+int hsolve_dot_product_two_double_host(const void *a, const void *b, uint64_t size,
+                                       double *result, void *stream);
+int hsolve_norm_l2_one_float_host(const void *a, uint64_t size, int root, float *result,
+                                  void *stream);
+int hsolve_norm_l2_one_double_host(const void *a, uint64_t size, int root, double *result,
+                                   void *stream);
+/* r = x + b*y (r may equal x).  cuda_scaled_sum_three_*_s (scaled_sum.cu:68-99). */
+int hsolve_scaled_sum_three_float_s(void *r, const void *x, const void *y, float b,
+                                    uint64_t size, void *stream);
+int hsolve_scaled_sum_three_double_s(void *r, const void *x, const void *y, double b,
+                                     uint64_t size, void *stream);
+/* x += y * z.  cuda_scaled_sum_three_* (scaled_sum.cu:102-133). */
+int hsolve_scaled_sum_three_float(void *x, const void *y, const void *z, uint64_t size,
+                                  void *stream);
+int hsolve_scaled_sum_three_double(void *x, const void *y, const void *z, uint64_t size,
+                                   void *stream);
+/* a += b.  cuda_sum_two_* (sum.cu:48-77). */
+int hsolve_sum_two_float(void *a, const void *b, uint64_t size, void *stream);
+int hsolve_sum_two_double(void *a, const void *b, uint64_t size, void *stream);
+/* x *= a.  cuda_scale_one_* (scale.cu:48-75). */
+int hsolve_scale_one_float(void *x, float a, uint64_t size, void *stream);
+int hsolve_scale_one_double(void *x, double a, uint64_t size, void *stream);
+/* a -= b; r = a - b.  cuda_difference_{two,three}_* (difference.cu:68-131). */
+int hsolve_difference_two_float(void *a, const void *b, uint64_t size, void *stream);
+int hsolve_difference_two_double(void *a, const void *b, uint64_t size, void *stream);
+int hsolve_difference_three_float(void *r, const void *a, const void *b, uint64_t size,
+                                  void *stream);
+int hsolve_difference_three_double(void *r, const void *a, const void *b, uint64_t size,
+                                   void *stream);
+/* r = a * b elementwise.  cuda_element_product_three_* (element_product.cu:48-79). */
+int hsolve_element_product_three_float(void *r, const void *a, const void *b, uint64_t size,
+                                       void *stream);
+int hsolve_element_product_three_double(void *r, const void *a, const void *b, uint64_t size,
+                                        void *stream);
+/* x = 1/x where x != 0.  cuda_element_inverse_one_* (element_inverse.cu:50-75). */
+int hsolve_element_inverse_one_float(void *x, uint64_t size, void *stream);
+int hsolve_element_inverse_one_double(void *x, uint64_t size, void *stream);
+/* Q1 bilinear prolongation / full-weighting restriction on (2^L+1)^2 grids,
+ * then zeroing of Dirichlet nodes per the 8-entry host mask (2 = Dirichlet).
+ * cuda_prolongation_* (prolongation.cu:191-234), cuda_restriction_*
+ * (restriction.cu:240-285); arithmetic follows the tags::CPU loops bit for bit. */
+int hsolve_prolongation_float(void *fine, uint64_t size_fine, const void *coarse,
+                              uint64_t size_coarse, const uint64_t *macro_border_mask_host,
+                              void *stream);
+int hsolve_prolongation_double(void *fine, uint64_t size_fine, const void *coarse,
+                               uint64_t size_coarse, const uint64_t *macro_border_mask_host,
+                               void *stream);
+int hsolve_restriction_float(void *coarse, uint64_t size_coarse, const void *fine,
+                             uint64_t size_fine, const uint64_t *macro_border_mask_host,
+                             void *stream);
+int hsolve_restriction_double(void *coarse, uint64_t size_coarse, const void *fine,
+                              uint64_t size_fine, const uint64_t *macro_border_mask_host,
+                              void *stream);
+/* FP32 <-> FP64 conversion.  cuda_convert_float_double / _double_float
+ * (transfer.cu:110-138). */
+int hsolve_convert_float_double(const void *src_float, void *dst_double, uint64_t size,
+                                void *stream);
+int hsolve_convert_double_float(const void *src_double, void *dst_float, uint64_t size,
+                                void *stream);
+
+/* ---------------- new fused kernels ---------------------------------------- */
+/* One damped-Jacobi sweep in one pass: x_new = x_old + dinv * (rhs - A x_old)
+ * (Jacobi<Tag>::value MG overload, honei/math/jacobi.hh:438-480); x_new must
+ * not alias x_old. */
+int hsolve_jacobi_q1_float(const void *rhs, const void *const *bands9, const void *dinv,
+                           const void *x_old, void *x_new, uint64_t size, void *stream);
+int hsolve_jacobi_q1_double(const void *rhs, const void *const *bands9, const void *dinv,
+                            const void *x_old, void *x_new, uint64_t size, void *stream);
+/* fine_x += alpha * (P coarse) without materialising P coarse
+ * (Prolongation then ScaledSum, honei/math/multigrid.hh:1818-1843). */
+int hsolve_prolongation_axpy_float(void *fine_x, uint64_t size_fine, const void *coarse,
+                                   uint64_t size_coarse, float alpha,
+                                   const uint64_t *macro_border_mask_host, void *stream);
+int hsolve_prolongation_axpy_double(void *fine_x, uint64_t size_fine, const void *coarse,
+                                    uint64_t size_coarse, double alpha,
+                                    const uint64_t *macro_border_mask_host, void *stream);
+/* ConjugateGradients<Tag, NONE>::value(Q1, rhs, x, konv_rad) in one
+ * workgroup (honei/math/conjugate_gradients.hh:490-526); size <= 16641.
+ * iters_dev (uint64 device scalar, may be NULL) receives the iteration
+ * count; max_iters is a termination guarantee the reference lacks. */
+int hsolve_cg_konv_q1_float(const void *const *bands9, const void *rhs, void *x, uint64_t size,
+                            double konv_rad, uint64_t max_iters, void *iters_dev, void *stream);
+int hsolve_cg_konv_q1_double(const void *const *bands9, const void *rhs, void *x,
+                             uint64_t size, double konv_rad, uint64_t max_iters, void *iters_dev,
+                             void *stream);
+
+/* ---------------- device-resident ELL handle -------------------------------- */
+typedef struct hsolve_ell_s *hsolve_ell;
+/* Uploads a SparseMatrixELL given in HONEI's host layout (uint64 Aj/Arl),
+ * converting columns to uint32 and row lengths to the narrowest unsigned
+ * type once.  dtype: 0 = float, 1 = double.  Arl_host may be NULL: row
+ * lengths are then derived by scanning for the first zero value
+ * (sparse_matrix_ell-impl.hh:186-207). */
+int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t stride,
+                      uint64_t num_cols_per_row, uint64_t threads, const uint64_t *Aj_host,
+                      const void *Ax_host, const uint64_t *Arl_host, hsolve_ell *out);
+int hsolve_ell_destroy(hsolve_ell m);
+int hsolve_ell_info(hsolve_ell m, uint64_t *rows, uint64_t *columns, uint64_t *stride,
+                    uint64_t *num_cols_per_row, uint64_t *threads, uint64_t *nnz,
+                    uint64_t *device_bytes);
+/* y = A x, or y = rhs - A x when rhs != NULL */
+int hsolve_ell_spmv(hsolve_ell m, const void *x, void *y, const void *rhs, void *stream);
+/* x_new = x_old + dinv * (rhs - A x_old) */
+int hsolve_ell_jacobi(hsolve_ell m, const void *rhs, const void *dinv, const void *x_old,
+                      void *x_new, void *stream);
+
+/* ---------------- solvers ---------------------------------------------------- */
+/* CGSolver<Tag, methods::NONE>::value (honei/math/cg.hh:63-133) on a Q1
+ * matrix (bands9: 9 device pointers, LL..UU) or an ELL handle; FP64 or
+ * FP32 by dtype (0 float, 1 double); b, x device vectors; x is the initial
+ * guess and the result.  used_iters and final relative defect on output.
+ * Stops when ||r|| < eps*||r0|| or ||r|| < eps or iters == max_iters. */
+int hsolve_cg_q1(int dtype, const void *const *bands9, uint64_t size, const void *b, void *x,
+                 uint64_t max_iters, double eps, uint64_t *used_iters, double *defect_out,
+                 void *stream);
+int hsolve_cg_ell(hsolve_ell m, const void *b, void *x, uint64_t max_iters, double eps,
+                  uint64_t *used_iters, double *defect_out, void *stream);
+/* CGSolver<Tag, methods::VAR> with a diagonal preconditioner P (device
+ * vector, z = P * r elementwise), honei/math/cg.hh:141-219. */
+int hsolve_pcg_ell(hsolve_ell m, const void *P, const void *b, void *x, uint64_t max_iters,
+                   double eps, uint64_t *used_iters, double *defect_out, void *stream);
+
+/* Mixed-precision MG-IR: Multigrid<Tag, OuterTag, NONE, JAC, CYCLE::V, MIXED>
+ * on Q1 bands (honei/math/multigrid.hh:1616-2393).  A hierarchy holds the
+ * FP64 finest matrix, FP32 matrices and inverted diagonals on every level
+ * and all iterates, device resident. */
+typedef struct hsolve_mg_params_s {
+    uint64_t min_level, max_level;
+    uint64_t n_max_iter;          /* V-cycles per inner solve */
+    int32_t initial_zero, convergence_check;
+    double tolerance;             /* stored as FP32 like MGInfo<float> */
+    uint64_t n_pre_smooth;        /* pre- and (reference quirk) post-smoothing sweeps */
+    double adapt_correction_factor;
+    uint64_t macro_border_mask[8];
+    uint64_t max_outer;           /* 0 = no cap (reference behaviour) */
+} hsolve_mg_params;
+typedef struct hsolve_mg_s *hsolve_mg;
+/* bands64_host: 9 host pointers (finest, FP64).  bands32_host: 9*(max_level+1)
+ * host pointers, entry [L*9+b] for L in [min_level, max_level] (others
+ * ignored); dinv32_host[L] likewise, or NULL to use the inverted diagonal
+ * scaled by 0.7 in double as the reference tests build it. */
+int hsolve_mg_create(const hsolve_mg_params *params, const double *const *bands64_host,
+                     const float *const *bands32_host, const float *const *dinv32_host,
+                     hsolve_mg *out);
+/* Variant that assembles every level with FillMatrix<POISSON, DIRICHLET_0>
+ * on the device (no host bands). */
+int hsolve_mg_create_poisson(const hsolve_mg_params *params, hsolve_mg *out);
+int hsolve_mg_destroy(hsolve_mg mg);
+int hsolve_mg_device_bytes(hsolve_mg mg, uint64_t *bytes);
+/* b, x: FP64 device vectors of the finest size; x in: initial guess, out: result. */
+int hsolve_mg_solve(hsolve_mg mg, const void *b, void *x, uint64_t *outer_iters,
+                    uint64_t *inner_iters, double *final_defect, double *initial_defect,
+                    void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
