@@ -1,0 +1,274 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X backend on HONEI's sparse-solver path.
+
+Headline (one "step" = one ELL SpMV y = A x over the whole matrix):
+  ELL SpMV, Matrix A = Q1 9-point stencil on 4097 x 4097 nodes
+  (16,785,409 rows, 150,937,609 nonzeros, U[0.5,1.5]-scaled coefficients,
+  strictly diagonally dominant), FP64, inputs resident in HBM.
+  value = algorithmic bytes per SpMV / time per SpMV, with the byte model
+  z(s+4) + n(2s+4) (values + 32-bit column indices + x + y + 4-byte row
+  lengths).
+
+Also reported under "extra": defect and FP32 forms, Matrix B (random ELL,
+2^24 rows, 5..13 per row), Q1 banded SpMV, and Poisson solves/s of
+mixed-precision MG-IR (FP32 V-cycles, FP64 refinement) and plain FP64 CG.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Poisson solves/s at rel res 1e-8 (MG-IR vs FP64 CG), p50/p99; ELL SpMV GB/s"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_ell_spmv_matrix_a_4097_f64.txt")
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def timed(fn, steps, warmup, stream=None):
+    """Average ms per call of fn() over `steps` calls, HIP events on `stream`."""
+    from honei_amd import device as hip
+    for _ in range(warmup):
+        fn()
+    hip.synchronize(stream)
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    ms = e0.elapsed_ms(e1)
+    hip.synchronize(stream)
+    return ms / steps
+
+
+def read_traffic():
+    """HBM bytes per launch from the committed PMC summary, if present."""
+    try:
+        vals = {}
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if ":" in line and not line.startswith("#"):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
+        return float(vals["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def spmv_case(hip, P, E, steps, warmup, defect=False):
+    n = E["rows"]
+    dtype = E["Ax"].dtype
+    s = dtype.itemsize
+    z = P.ell_nnz(E)
+    M = hip.EllMatrix.from_dict(E)
+    x = hip.DeviceVector.from_numpy(
+        P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype))
+    y = hip.DeviceVector.zeros(n, dtype)
+    rhs = None
+    if defect:
+        rhs = hip.DeviceVector.from_numpy(
+            P.rng_uniform(P.SEED, P.S_RHS, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype))
+    ms = timed(lambda: M.spmv(x, y, rhs=rhs), steps, warmup)
+    nbytes = P.ell_defect_bytes(n, z, s) if defect else P.ell_spmv_bytes(n, z, s)
+    out = dict(rows=n, nnz=z, ms=ms, bytes=nbytes, GBps=nbytes / (ms * 1e-3) / 1e9)
+    out["frac_hbm"] = out["GBps"] / HBM_PEAK_GBS
+    del M, x, y, rhs
+    return out
+
+
+def q1_case(hip, P, side, dtype, steps, warmup):
+    n = side * side
+    A = hip.Q1Matrix(P.matrix_a_bands(side, dtype=dtype))
+    x = hip.DeviceVector.from_numpy(
+        P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype))
+    y = hip.DeviceVector.zeros(n, dtype)
+    ms = timed(lambda: hip.q1_product(A, x, y), steps, warmup)
+    nbytes = 11 * n * np.dtype(dtype).itemsize
+    return dict(rows=n, ms=ms, bytes=nbytes, GBps=nbytes / (ms * 1e-3) / 1e9,
+                frac_hbm=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+
+
+def mg_case(hip, P, level, min_level, reps, tol=1e-8):
+    n = P.side_of_level(level) ** 2
+    mg = hip.MGSolver(hip.mg_params(min_level, level, n_max_iter=2, tolerance=tol))
+    b = hip.DeviceVector.from_numpy(P.rhs_uniform(n))
+    x = hip.DeviceVector.zeros(n)
+    lat = []
+    info = None
+    for r in range(reps + 1):
+        x.fill(0.0)
+        hip.synchronize()
+        t0 = time.perf_counter()
+        info = mg.solve(b, x)
+        t1 = time.perf_counter()
+        if r > 0:
+            lat.append((t1 - t0) * 1e3)
+    lat = np.asarray(lat)
+    return dict(n=n, levels=f"{min_level}..{level}", tol=tol, outer=info["outer"], inner=info["inner"],
+                rel_defect=info["final_defect"] / info["initial_defect"],
+                ms_p50=float(np.percentile(lat, 50)), ms_p99=float(np.percentile(lat, 99)),
+                solves_per_s=float(1e3 / lat.mean()), device_bytes=mg.device_bytes), x
+
+
+def cg_case(hip, P, side, max_iters, tol=1e-8):
+    n = side * side
+    A = hip.Q1Matrix(P.poisson_q1_bands(n))
+    b = hip.DeviceVector.from_numpy(P.rhs_uniform(n))
+    x = hip.DeviceVector.zeros(n)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    used, rel = hip.cg_q1(A, b, x, max_iters, tol, raise_on_cap=False)
+    t1 = time.perf_counter()
+    return dict(n=n, iters=used, rel_defect=rel, ms=(t1 - t0) * 1e3, solves_per_s=1.0 / (t1 - t0))
+
+
+def cpu_baseline_spmv(E, budget_s=10.0):
+    """The oracle (single-thread C restatement of Product<tags::CPU> on ELL)
+    over the same Matrix A, repeated until about budget_s seconds."""
+    from honei_amd import problems as P
+    from oracle.pyoracle import Oracle
+    O = Oracle()
+    n = E["rows"]
+    x = P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        O.ell_product(E, x)
+        reps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    nbytes = P.ell_spmv_bytes(n, P.ell_nnz(E), 8)
+    return dict(value=nbytes / dt / 1e9, unit="GB/s", cores=1, kind="port",
+                sample=f"{reps} full Product<tags::CPU> ELL SpMVs of Matrix A FP64 ({n} rows), "
+                       f"{dt * 1e3:.0f} ms each, on {cpu_model()}")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--quick", action="store_true", help="headline only (no extras, no CPU baseline)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--side", type=int, default=4097)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 2 GiB device copy kernel (for byte-counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by grid row blocks, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from honei_amd import device as hip
+    from honei_amd import problems as P
+
+    side = args.side
+    t0 = time.perf_counter()
+    E64 = P.q1_to_ell(P.matrix_a_bands(side, dtype=np.float64))
+    log(f"Matrix A {side}^2 built in {time.perf_counter() - t0:.1f}s: rows={E64['rows']} nnz={P.ell_nnz(E64)}")
+
+    extra = {}
+    if args.copy_calibration:
+        nb = 2 << 30
+        src, dst = hip.DeviceBuffer(nb), hip.DeviceBuffer(nb)
+        ms = timed(lambda: hip.copy_kernel(src, dst, nb), max(args.steps // 5, 3), 1)
+        extra["copy_kernel_2GiB"] = dict(bytes=2 * nb, ms=ms, GBps=2 * nb / (ms * 1e-3) / 1e9)
+        log("copy", extra["copy_kernel_2GiB"])
+        del src, dst
+
+    head = spmv_case(hip, P, E64, args.steps, args.warmup)
+    log("headline", head)
+    traffic = read_traffic()
+    if not args.quick:
+        extra["ell_defect_matrix_a_f64"] = spmv_case(hip, P, E64, args.steps, args.warmup, defect=True)
+        E32 = dict(E64)
+        E32["Ax"] = E64["Ax"].astype(np.float32)
+        extra["ell_spmv_matrix_a_f32"] = spmv_case(hip, P, E32, args.steps, args.warmup)
+        extra["ell_defect_matrix_a_f32"] = spmv_case(hip, P, E32, args.steps, args.warmup, defect=True)
+        del E32
+        log("matrix A extras done")
+        t1 = time.perf_counter()
+        EB = P.matrix_b_ell(1 << 24)
+        log(f"Matrix B built in {time.perf_counter() - t1:.1f}s")
+        extra["ell_spmv_matrix_b_f64"] = spmv_case(hip, P, EB, args.steps, args.warmup)
+        EB["Ax"] = EB["Ax"].astype(np.float32)
+        extra["ell_spmv_matrix_b_f32"] = spmv_case(hip, P, EB, args.steps, args.warmup)
+        del EB
+        extra["q1_spmv_matrix_a_f64"] = q1_case(hip, P, side, np.float64, args.steps, args.warmup)
+        log("spmv extras done")
+        mg, _ = mg_case(hip, P, 12, 2, reps=5, tol=1e-8)
+        extra["mg_ir_poisson_4097_tol1e-8"] = mg
+        mg10, _ = mg_case(hip, P, 12, 2, reps=3, tol=1e-10)
+        extra["mg_ir_poisson_4097_tol1e-10"] = mg10
+        log("mg", mg, mg10)
+        cg = cg_case(hip, P, 4097, 100000)
+        extra["cg_fp64_poisson_4097_tol1e-8"] = cg
+        extra["mg_ir_speedup_vs_fp64_cg_4097"] = cg["ms"] / mg["ms_p50"]
+        log("cg", cg)
+
+    cpu = None
+    if not args.quick and not args.no_cpu:
+        cpu = cpu_baseline_spmv(E64)
+        log("cpu", cpu)
+
+    res = {
+        "metric": METRIC,
+        "value": round(head["GBps"], 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"ELL SpMV y=Ax, Matrix A: Q1 9-point stencil {side}x{side} "
+                        f"({head['rows']} rows, {head['nnz']} nnz), FP64, inputs resident in HBM",
+            "bytes_model": "z*(s+4) + n*(2s+4), s=8",
+            "bytes_per_step": head["bytes"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(head["GBps"], 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(head["GBps"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -54,6 +54,11 @@ _SIGS = {
     "hsolve_synchronize": [vp],
     "hsolve_stream_create": [C.POINTER(vp)],
     "hsolve_stream_destroy": [vp],
+    "hsolve_event_create": [C.POINTER(vp)],
+    "hsolve_event_destroy": [vp],
+    "hsolve_event_record": [vp, vp],
+    "hsolve_event_elapsed_ms": [vp, vp, C.POINTER(flt)],
+    "hsolve_copy_kernel": [vp, vp, u64, vp],
     "hsolve_product_smell_dv_float": [vp, vp, vp, vp, vp, u64, u64, u64, u64, u64, vp],
     "hsolve_product_smell_dv_double": [vp, vp, vp, vp, vp, u64, u64, u64, u64, u64, vp],
     "hsolve_defect_smell_dv_float": [vp, vp, vp, vp, vp, vp, u64, u64, u64, u64, u64, vp],

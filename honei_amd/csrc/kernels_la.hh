@@ -306,6 +306,14 @@ namespace k
             dst[i] = (DST)src[i];
     }
 
+    static __global__ __launch_bounds__(256) void copy16_kernel(const uint4 * __restrict__ src,
+            uint4 * __restrict__ dst, uint64_t n)
+    {
+        const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+            dst[i] = src[i];
+    }
+
     static __global__ __launch_bounds__(256) void zero_bytes_kernel(uint8_t * __restrict__ p, uint64_t bytes)
     {
         const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;

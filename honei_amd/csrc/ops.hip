@@ -358,6 +358,55 @@ int hsolve_stream_destroy(void * stream)
     HSOLVE_API_END
 }
 
+int hsolve_event_create(void ** event)
+{
+    HSOLVE_API_BEGIN
+    require(event != nullptr, HSOLVE_ERR_INVALID, "null event");
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    *event = (void *)e;
+    HSOLVE_API_END
+}
+
+int hsolve_event_destroy(void * event)
+{
+    HSOLVE_API_BEGIN
+    if (event)
+        HIP_CHECK(hipEventDestroy((hipEvent_t)event));
+    HSOLVE_API_END
+}
+
+int hsolve_event_record(void * event, void * stream)
+{
+    HSOLVE_API_BEGIN
+    HIP_CHECK(hipEventRecord((hipEvent_t)event, ST));
+    HSOLVE_API_END
+}
+
+int hsolve_event_elapsed_ms(void * start, void * stop, float * ms)
+{
+    HSOLVE_API_BEGIN
+    HIP_CHECK(hipEventSynchronize((hipEvent_t)stop));
+    HIP_CHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    HSOLVE_API_END
+}
+
+int hsolve_copy_kernel(const void * src, void * dst, uint64_t bytes, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(src && dst && (bytes % 16) == 0, HSOLVE_ERR_INVALID, "copy_kernel: null pointer or bytes % 16 != 0");
+    const uint64_t n = bytes / 16;
+    if (n)
+    {
+        unsigned grid = grid_for(n, kBlock);
+        if (grid > 65536)
+            grid = 65536;
+        hipLaunchKernelGGL(k::copy16_kernel, dim3(grid), dim3(kBlock), 0, ST, (const uint4 *)src, (uint4 *)dst, n);
+        HIP_CHECK(hipGetLastError());
+    }
+    HSOLVE_API_END
+}
+
 // ------------------------------------------------------------- ELL (raw)
 #define HS_ELL_RAW(NAME, DT, DEFECT, RHS)                                                             \
     {                                                                                                 \

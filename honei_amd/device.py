@@ -144,6 +144,39 @@ class EllMatrix:
             pass
 
 
+class Event:
+    """hipEvent_t for timing work enqueued on a stream."""
+
+    def __init__(self):
+        e = vp()
+        check(lib().hsolve_event_create(C.byref(e)), "event_create")
+        self.ev = e.value
+
+    def record(self, stream=None):
+        check(lib().hsolve_event_record(vp(self.ev), vp(stream)), "event_record")
+
+    def elapsed_ms(self, stop):
+        ms = C.c_float()
+        check(lib().hsolve_event_elapsed_ms(vp(self.ev), vp(stop.ev), C.byref(ms)), "event_elapsed")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "ev", None):
+                lib().hsolve_event_destroy(vp(self.ev))
+                self.ev = None
+        except Exception:
+            pass
+
+
+def synchronize(stream=None):
+    check(lib().hsolve_synchronize(vp(stream)), "synchronize")
+
+
+def copy_kernel(src: DeviceBuffer, dst: DeviceBuffer, nbytes: int, stream=None):
+    check(lib().hsolve_copy_kernel(vp(src.ptr), vp(dst.ptr), u64(nbytes), vp(stream)), "copy_kernel")
+
+
 # ----------------------------------------------------------------- ops
 def q1_product(A: Q1Matrix, x: DeviceVector, y: DeviceVector, stream=None):
     f = getattr(lib(), f"hsolve_product_bmdv_q1_{_sfx(A.dtype)}")

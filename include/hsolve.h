@@ -64,6 +64,16 @@ int hsolve_fill_double(void *dst, double value, uint64_t size, void *stream);
 int hsolve_synchronize(void *stream);                    /* cuda_thread_synchronize */
 int hsolve_stream_create(void **stream);
 int hsolve_stream_destroy(void *stream);
+/* HIP events for timing work on a given stream (no reference counterpart:
+ * the reference times with gettimeofday around synchronous calls,
+ * benchmark/benchmark.cc:194-400). */
+int hsolve_event_create(void **event);
+int hsolve_event_destroy(void *event);
+int hsolve_event_record(void *event, void *stream);
+int hsolve_event_elapsed_ms(void *start, void *stop, float *ms);   /* synchronises stop */
+/* Plain device copy kernel (16-byte vector loads/stores), used to calibrate
+ * hardware byte counters against a kernel of known traffic. */
+int hsolve_copy_kernel(const void *src, void *dst, uint64_t bytes, void *stream);
 
 /* ---------------- operations.hh mirror ------------------------------------- */
 /* ELL SpMV y = A x on rows [row_start, row_end) (row_end 0 = all rows), in
