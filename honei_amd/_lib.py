@@ -106,6 +106,7 @@ _SIGS = {
     "hsolve_ell_create": [C.c_int, u64, u64, u64, u64, u64, vp, vp, vp, C.POINTER(vp)],
     "hsolve_ell_destroy": [vp],
     "hsolve_ell_info": [vp] + [C.POINTER(u64)] * 7,
+    "hsolve_ell_set_variant": [vp, C.c_int],
     "hsolve_ell_spmv": [vp, vp, vp, vp, vp],
     "hsolve_ell_jacobi": [vp, vp, vp, vp, vp, vp],
     "hsolve_cg_q1": [C.c_int, vp, u64, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],

@@ -2,12 +2,15 @@
 //
 // The host layout (HONEI's SMELL: uint64 column indices, uint64 row step
 // counts, column-major slots of `stride` rows) is converted once at upload
-// into the backend layout: the same slot order with uint32 column indices
-// and the row lengths in the narrowest unsigned type that holds the longest
-// row.  That halves the index stream (12 instead of 16 bytes per FP64
-// nonzero) and makes the row-length stream 1 byte per row for ordinary
-// stencils, without changing which products are summed or their order.
-#include "kernels_la.hh"
+// into the backend layout: the same slots in the same order, with
+//  * column indices as int16 offsets from the row when every stored column
+//    lies within +-32767 of its row (any banded or stencil matrix), else as
+//    uint32 columns, and
+//  * row lengths in the narrowest unsigned type that holds the longest row.
+// For an FP64 9-point stencil that is 10 instead of 16 bytes per nonzero and
+// 1 instead of 8 bytes per row length.  Which products are summed, and in
+// which order, does not change.
+#include "ell_handle.hh"
 
 #include <algorithm>
 #include <cmath>
@@ -17,104 +20,76 @@
 namespace hsolve
 {
     void * stream_work(hipStream_t st, size_t bytes);
-    template <typename DT>
-    void launch_dot(const DT * a, const DT * b, uint64_t n, DT * out, int root, hipStream_t st);
 }
-
-struct hsolve_ell_s
-{
-    int dtype;                 // 0 float, 1 double
-    uint64_t rows, cols, stride, ncpr, threads, nnz;
-    int len_bytes;             // 1, 2 or 4
-    uint32_t * Aj;             // device, ncpr * stride
-    void * Ax;                 // device, ncpr * stride values
-    void * Arl;                // device, rows lengths
-    uint64_t device_bytes;
-};
 
 using namespace hsolve;
 
 namespace
 {
-    template <typename DT, typename LT, bool DEFECT>
-    void ell_launch(const hsolve_ell_s * m, const DT * x, DT * y, const DT * rhs, hipStream_t st)
+    // Kernel variant of a handle: explicit choice, or for the threads == 1
+    // layout two rows per lane with unconditional slot loads when at least
+    // 90% of the slots below num_cols_per_row are used (stencils), and with
+    // XCD-contiguous blocks otherwise (measured on MI355X: the FP64 stencil
+    // is fastest without the remap, scattered matrices gain from it).
+    int ell_variant(const hsolve_ell_s * m)
+    {
+        if (m->variant & HSOLVE_ELL_SET)
+            return m->variant;
+        if (m->nnz * 10 >= m->rows * m->ncpr * 9)
+            return HSOLVE_ELL_FULL;
+        return HSOLVE_ELL_XCD;
+    }
+
+    template <typename DT, typename IT, typename LT, int MODE, int K>
+    bool ell_launch_r2(const hsolve_ell_s * m, int v, const DT * x, DT * y, const IT * Aj, const DT * Ax,
+                       const LT * Arl, const DT * rhs, const DT * dinv, hipStream_t st)
+    {
+        if (m->ncpr > (uint64_t)K)
+            return false;
+        const unsigned grid = grid_for((m->rows + 1) / 2, kBlock);
+        const bool full = v & HSOLVE_ELL_FULL, xcd = v & HSOLVE_ELL_XCD;
+#define HS_R2(F, X)                                                                                           \
+        hipLaunchKernelGGL((k::ell_spmv_r2<DT, IT, LT, K, MODE, F, X>), dim3(grid), dim3(kBlock), 0, st, x, y, Aj, \
+                           Ax, Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride, (uint32_t)m->ncpr)
+        if (full && xcd) HS_R2(true, true);
+        else if (full) HS_R2(true, false);
+        else if (xcd) HS_R2(false, true);
+        else HS_R2(false, false);
+#undef HS_R2
+        return true;
+    }
+
+    template <typename DT, int MODE>
+    void ell_launch(const hsolve_ell_s * m, const DT * x, DT * y, const DT * rhs, const DT * dinv, hipStream_t st)
     {
         const unsigned grid = grid_for(m->rows, kBlock);
-        const uint32_t * Aj = m->Aj;
         const DT * Ax = (const DT *)m->Ax;
-        const LT * Arl = (const LT *)m->Arl;
-        if (m->threads != 1)
-            hipLaunchKernelGGL((k::ell_spmv_tn<DT, uint32_t, LT, DEFECT>), dim3(grid), dim3(kBlock), 0, st, x, y,
-                               Aj, Ax, Arl, rhs, (uint64_t)0, m->rows, m->stride, (uint32_t)m->threads);
+        const int v = ell_variant(m);
+        // two rows per lane reads the slot pair at row_end - 1: safe when the
+        // stride leaves a padding row or the row count is even
+        const bool r2_ok = m->threads == 1 && ! (v & HSOLVE_ELL_ONE_ROW) && (m->stride > m->rows || m->rows % 2 == 0);
+        with_ell_types(m, [&](auto Aj, auto Arl) {
+            using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
+            using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
+            if (r2_ok && (ell_launch_r2<DT, IT, LT, MODE, 4>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
+                          ell_launch_r2<DT, IT, LT, MODE, 9>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
+                          ell_launch_r2<DT, IT, LT, MODE, 13>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
+                          ell_launch_r2<DT, IT, LT, MODE, 16>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st)))
+                return;
+            if (m->threads != 1)
+                hipLaunchKernelGGL((k::ell_spmv_tn<DT, IT, LT, MODE>), dim3(grid), dim3(kBlock), 0, st, x, y, Aj, Ax,
+                                   Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride, (uint32_t)m->threads);
 #define HS_K(K)                                                                                               \
-        else if (m->ncpr <= K)                                                                                \
-            hipLaunchKernelGGL((k::ell_spmv_t1<DT, uint32_t, LT, K, DEFECT>), dim3(grid), dim3(kBlock), 0, st, x, \
-                               y, Aj, Ax, Arl, rhs, (uint64_t)0, m->rows, m->stride);
-        HS_K(4) HS_K(8) HS_K(9) HS_K(12) HS_K(13) HS_K(16) HS_K(24) HS_K(32)
+            else if (m->ncpr <= K)                                                                            \
+                hipLaunchKernelGGL((k::ell_spmv_t1<DT, IT, LT, K, MODE>), dim3(grid), dim3(kBlock), 0, st, x, y, Aj, \
+                                   Ax, Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride);
+            HS_K(4) HS_K(8) HS_K(9) HS_K(12) HS_K(13) HS_K(16) HS_K(24) HS_K(32)
 #undef HS_K
-        else
-            hipLaunchKernelGGL((k::ell_spmv_t1<DT, uint32_t, LT, 0, DEFECT>), dim3(grid), dim3(kBlock), 0, st, x, y,
-                               Aj, Ax, Arl, rhs, (uint64_t)0, m->rows, m->stride);
+            else
+                hipLaunchKernelGGL((k::ell_spmv_t1<DT, IT, LT, 0, MODE>), dim3(grid), dim3(kBlock), 0, st, x, y, Aj,
+                                   Ax, Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride);
+        });
         HIP_CHECK(hipGetLastError());
-    }
-
-    template <typename DT, typename LT>
-    void ell_jacobi_launch(const hsolve_ell_s * m, const DT * rhs, const DT * dinv, const DT * x_old, DT * x_new,
-                           hipStream_t st)
-    {
-        const unsigned grid = grid_for(m->rows, kBlock);
-        const uint32_t * Aj = m->Aj;
-        const DT * Ax = (const DT *)m->Ax;
-        const LT * Arl = (const LT *)m->Arl;
-        if (m->threads != 1)
-        {
-            // defect into scratch, then x_new = x_old + t * dinv
-            DT * t = (DT *)stream_work(st, sizeof(DT) * m->rows + 256);
-            ell_launch<DT, LT, true>(m, x_old, t, rhs, st);
-            unsigned g2 = grid_for(m->rows, kBlock);
-            if (g2 > 16384) g2 = 16384;
-            hipLaunchKernelGGL((k::elem_kernel<DT, k::E_SCALED_SUM_V>), dim3(g2), dim3(kBlock), 0, st, x_new,
-                               (const DT *)nullptr, (const DT *)t, dinv, DT(0), m->rows);
-            // x_new currently holds garbage + t*dinv unless it started as x_old
-            HIP_CHECK(hipGetLastError());
-            return;
-        }
-#define HS_K(K)                                                                                               \
-        if (m->ncpr <= K)                                                                                     \
-        {                                                                                                     \
-            hipLaunchKernelGGL((k::ell_jacobi_t1<DT, uint32_t, LT, K>), dim3(grid), dim3(kBlock), 0, st, x_old, \
-                               x_new, Aj, Ax, Arl, rhs, dinv, m->rows, m->stride);                            \
-            HIP_CHECK(hipGetLastError());                                                                     \
-            return;                                                                                           \
-        }
-        HS_K(4) HS_K(8) HS_K(9) HS_K(12) HS_K(13) HS_K(16) HS_K(24) HS_K(32)
-#undef HS_K
-        hipLaunchKernelGGL((k::ell_jacobi_t1<DT, uint32_t, LT, 0>), dim3(grid), dim3(kBlock), 0, st, x_old, x_new,
-                           Aj, Ax, Arl, rhs, dinv, m->rows, m->stride);
-        HIP_CHECK(hipGetLastError());
-    }
-
-    template <typename DT, bool DEFECT>
-    void ell_dispatch_len(const hsolve_ell_s * m, const DT * x, DT * y, const DT * rhs, hipStream_t st)
-    {
-        if (m->len_bytes == 1)
-            ell_launch<DT, uint8_t, DEFECT>(m, x, y, rhs, st);
-        else if (m->len_bytes == 2)
-            ell_launch<DT, uint16_t, DEFECT>(m, x, y, rhs, st);
-        else
-            ell_launch<DT, uint32_t, DEFECT>(m, x, y, rhs, st);
-    }
-
-    template <typename DT>
-    void ell_jacobi_dispatch(const hsolve_ell_s * m, const DT * rhs, const DT * dinv, const DT * x_old, DT * x_new,
-                             hipStream_t st)
-    {
-        if (m->len_bytes == 1)
-            ell_jacobi_launch<DT, uint8_t>(m, rhs, dinv, x_old, x_new, st);
-        else if (m->len_bytes == 2)
-            ell_jacobi_launch<DT, uint16_t>(m, rhs, dinv, x_old, x_new, st);
-        else
-            ell_jacobi_launch<DT, uint32_t>(m, rhs, dinv, x_old, x_new, st);
     }
 
     template <typename DT>
@@ -167,7 +142,7 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
     require(dtype == 0 || dtype == 1, HSOLVE_ERR_INVALID, "ell_create: dtype must be 0 (float) or 1 (double)");
     require(threads >= 1, HSOLVE_ERR_INVALID, "ell_create: threads must be >= 1");
     require(stride >= rows * threads, HSOLVE_ERR_INVALID, "ell_create: stride < rows * threads");
-    require(columns <= 0xFFFFFFFFull, HSOLVE_ERR_SIZE, "ell_create: more than 2^32 columns");
+    require(columns <= 0xFFFFFFFFull && rows <= 0xFFFFFFFFull, HSOLVE_ERR_SIZE, "ell_create: more than 2^32 rows/columns");
     *out = nullptr;
     const uint64_t size = num_cols_per_row * stride;
     const size_t vs = dtype ? sizeof(double) : sizeof(float);
@@ -180,17 +155,40 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
     else
         rl = row_lengths_by_scan<float>(rows, stride, threads, size, (const float *)Ax_host);
     uint64_t maxlen = 0, nnz = 0;
+    int64_t maxoff = 0;
     for (uint64_t r = 0; r < rows; ++r)
     {
         require(rl[r] <= num_cols_per_row, HSOLVE_ERR_INVALID, "ell_create: row length > num_cols_per_row");
         maxlen = std::max(maxlen, rl[r]);
         nnz += rl[r] * threads;
+        for (uint64_t k = 0; k < rl[r]; ++k)
+            for (uint64_t t = 0; t < threads; ++t)
+            {
+                const uint64_t c = Aj_host[r * threads + t + k * stride];
+                require(c < columns, HSOLVE_ERR_INVALID, "ell_create: column index out of range");
+                maxoff = std::max<int64_t>(maxoff, std::llabs((int64_t)c - (int64_t)r));
+            }
     }
-    std::vector<uint32_t> aj32(size);
-    for (uint64_t i = 0; i < size; ++i)
+    const int idx_kind = (maxoff <= 32767) ? 1 : 0;
+    std::vector<uint8_t> idx(size * (idx_kind ? sizeof(int16_t) : sizeof(uint32_t)));
+    if (idx_kind)
     {
-        require(Aj_host[i] < (columns ? columns : 1), HSOLVE_ERR_INVALID, "ell_create: column index out of range");
-        aj32[i] = (uint32_t)Aj_host[i];
+        int16_t * d = (int16_t *)idx.data();
+        for (uint64_t j = 0; j < size; ++j)
+            d[j] = 0;
+        for (uint64_t r = 0; r < rows; ++r)
+            for (uint64_t k = 0; k < rl[r]; ++k)
+                for (uint64_t t = 0; t < threads; ++t)
+                {
+                    const uint64_t j = r * threads + t + k * stride;
+                    d[j] = (int16_t)((int64_t)Aj_host[j] - (int64_t)r);
+                }
+    }
+    else
+    {
+        uint32_t * d = (uint32_t *)idx.data();
+        for (uint64_t j = 0; j < size; ++j)
+            d[j] = (uint32_t)std::min<uint64_t>(Aj_host[j], columns ? columns - 1 : 0);
     }
     std::vector<uint8_t> lens;
     int lb;
@@ -200,16 +198,16 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
 
     hsolve_ell_s * m = new hsolve_ell_s();
     m->dtype = dtype; m->rows = rows; m->cols = columns; m->stride = stride; m->ncpr = num_cols_per_row;
-    m->threads = threads; m->nnz = nnz; m->len_bytes = lb;
+    m->threads = threads; m->nnz = nnz; m->idx_kind = idx_kind; m->len_bytes = lb; m->variant = 0;
     m->Aj = nullptr; m->Ax = nullptr; m->Arl = nullptr;
     try
     {
-        hipError_t e1 = hipMalloc((void **)&m->Aj, sizeof(uint32_t) * (size ? size : 1));
+        hipError_t e1 = hipMalloc(&m->Aj, idx.size() ? idx.size() : 1);
         hipError_t e2 = hipMalloc(&m->Ax, vs * (size ? size : 1));
         hipError_t e3 = hipMalloc(&m->Arl, lens.size() ? lens.size() : 1);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
             throw Error(HSOLVE_ERR_ALLOC, "ell_create: device allocation failed");
-        HIP_CHECK(hipMemcpy(m->Aj, aj32.data(), sizeof(uint32_t) * size, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(m->Aj, idx.data(), idx.size(), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(m->Ax, Ax_host, vs * size, hipMemcpyHostToDevice));
         if (! lens.empty())
             HIP_CHECK(hipMemcpy(m->Arl, lens.data(), lens.size(), hipMemcpyHostToDevice));
@@ -222,7 +220,7 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
         delete m;
         throw;
     }
-    m->device_bytes = sizeof(uint32_t) * size + vs * size + lens.size();
+    m->device_bytes = idx.size() + vs * size + lens.size();
     *out = m;
     HSOLVE_API_END
 }
@@ -255,6 +253,15 @@ int hsolve_ell_info(hsolve_ell m, uint64_t * rows, uint64_t * columns, uint64_t 
     HSOLVE_API_END
 }
 
+int hsolve_ell_set_variant(hsolve_ell m, int variant)
+{
+    HSOLVE_API_BEGIN
+    require(m != nullptr, HSOLVE_ERR_INVALID, "ell_set_variant: null handle");
+    require(variant >= 0 && variant < 8, HSOLVE_ERR_INVALID, "ell_set_variant: variant bits out of range");
+    m->variant = variant ? (variant | HSOLVE_ELL_SET) : 0;
+    HSOLVE_API_END
+}
+
 int hsolve_ell_spmv(hsolve_ell m, const void * x, void * y, const void * rhs, void * stream)
 {
     HSOLVE_API_BEGIN
@@ -266,16 +273,16 @@ int hsolve_ell_spmv(hsolve_ell m, const void * x, void * y, const void * rhs, vo
     if (m->dtype)
     {
         if (rhs)
-            ell_dispatch_len<double, true>(m, (const double *)x, (double *)y, (const double *)rhs, st);
+            ell_launch<double, 1>(m, (const double *)x, (double *)y, (const double *)rhs, nullptr, st);
         else
-            ell_dispatch_len<double, false>(m, (const double *)x, (double *)y, nullptr, st);
+            ell_launch<double, 0>(m, (const double *)x, (double *)y, nullptr, nullptr, st);
     }
     else
     {
         if (rhs)
-            ell_dispatch_len<float, true>(m, (const float *)x, (float *)y, (const float *)rhs, st);
+            ell_launch<float, 1>(m, (const float *)x, (float *)y, (const float *)rhs, nullptr, st);
         else
-            ell_dispatch_len<float, false>(m, (const float *)x, (float *)y, nullptr, st);
+            ell_launch<float, 0>(m, (const float *)x, (float *)y, nullptr, nullptr, st);
     }
     HSOLVE_API_END
 }
@@ -286,19 +293,12 @@ int hsolve_ell_jacobi(hsolve_ell m, const void * rhs, const void * dinv, const v
     HSOLVE_API_BEGIN
     require(m && rhs && dinv && x_old && x_new, HSOLVE_ERR_INVALID, "ell_jacobi: null pointer");
     require(x_old != x_new, HSOLVE_ERR_INVALID, "ell_jacobi: x_new must not alias x_old");
+    require(m->rows == m->cols, HSOLVE_ERR_SIZE, "ell_jacobi: matrix is not square");
     hipStream_t st = as_stream(stream);
-    if (m->threads != 1)
-    {
-        // the two-pass form needs x_new = x_old first
-        const size_t vs = m->dtype ? sizeof(double) : sizeof(float);
-        HIP_CHECK(hipMemcpyAsync(x_new, x_old, vs * m->rows, hipMemcpyDeviceToDevice, st));
-    }
     if (m->dtype)
-        ell_jacobi_dispatch<double>(m, (const double *)rhs, (const double *)dinv, (const double *)x_old,
-                                    (double *)x_new, st);
+        ell_launch<double, 2>(m, (const double *)x_old, (double *)x_new, (const double *)rhs, (const double *)dinv, st);
     else
-        ell_jacobi_dispatch<float>(m, (const float *)rhs, (const float *)dinv, (const float *)x_old, (float *)x_new,
-                                   st);
+        ell_launch<float, 2>(m, (const float *)x_old, (float *)x_new, (const float *)rhs, (const float *)dinv, st);
     HSOLVE_API_END
 }
 

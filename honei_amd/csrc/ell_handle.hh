@@ -1,0 +1,51 @@
+// ell_handle.hh -- the device-resident SparseMatrixELL handle and the
+// dispatch over its storage variants.
+#pragma once
+
+#include "kernels_la.hh"
+
+struct hsolve_ell_s
+{
+    int dtype;                 // 0 float, 1 double
+    uint64_t rows, cols, stride, ncpr, threads, nnz;
+    int idx_kind;              // 0: uint32 columns, 1: int16 offsets (column - row)
+    int len_bytes;             // row-length type: 1, 2 or 4 bytes
+    void * Aj;                 // ncpr * stride indices
+    void * Ax;                 // ncpr * stride values
+    void * Arl;                // rows row lengths
+    uint64_t device_bytes;
+    int variant;               // kernel variant bits (HSOLVE_ELL_*), 0 = automatic
+};
+
+// kernel variant bits of hsolve_ell_set_variant
+enum
+{
+    HSOLVE_ELL_ONE_ROW = 1,    // one row per lane (else two rows per lane)
+    HSOLVE_ELL_FULL = 2,       // load all num_cols_per_row slots without waiting for row lengths
+    HSOLVE_ELL_XCD = 4,        // XCD-contiguous block order
+    HSOLVE_ELL_SET = 8         // marks an explicit choice
+};
+
+namespace hsolve
+{
+    // Calls f.template operator()<IT, LT>(Aj, Arl) with the handle's typed
+    // index and row-length arrays.
+    template <typename F>
+    void with_ell_types(const hsolve_ell_s * m, F && f)
+    {
+        auto by_len = [&](auto it_tag) {
+            using IT = decltype(it_tag);
+            const IT * Aj = (const IT *)m->Aj;
+            if (m->len_bytes == 1)
+                f(Aj, (const uint8_t *)m->Arl);
+            else if (m->len_bytes == 2)
+                f(Aj, (const uint16_t *)m->Arl);
+            else
+                f(Aj, (const uint32_t *)m->Arl);
+        };
+        if (m->idx_kind == 1)
+            by_len(int16_t(0));
+        else
+            by_len(uint32_t(0));
+    }
+}

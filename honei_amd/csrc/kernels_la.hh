@@ -22,23 +22,27 @@ namespace k
         return __builtin_nontemporal_load(p);
     }
 
-    // ------------------------------------------------------------ ELL (T=1)
-    // One lane per row.  Slots are walked in ascending order up to the row's
-    // length; MAXK (compile-time bound on num_cols_per_row) lets the loads of
-    // all slots be issued before the dependent sum, for memory-level
-    // parallelism, while the sum itself keeps the reference order.
-    template <typename DT, typename IT, typename LT, int MAXK, bool DEFECT>
-    __global__ __launch_bounds__(256) void ell_spmv_t1(
-            const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
-            const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
-            uint64_t row_start, uint64_t row_end, uint64_t stride)
+    // ------------------------------------------------------------------ ELL
+    // Column index decoding: unsigned index types hold the column itself,
+    // signed ones (int16_t) the offset of the column from the row, a
+    // backend-internal compression chosen at upload when every stored
+    // column lies within +-32767 of its row.
+    template <typename IT> __device__ __forceinline__ uint64_t col_of(IT a, uint64_t row)
     {
-        const uint64_t row = row_start + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (row >= row_end)
-            return;
-        const uint32_t len = (uint32_t)Arl[row];
-        const IT * aj = Aj + row;
-        const DT * ax = Ax + row;
+        if constexpr (IT(-1) < IT(0))
+            return (uint64_t)((int64_t)row + (int64_t)a);
+        else
+            return (uint64_t)a;
+    }
+
+    // Sum over the first `len` slots of one row (threads == 1 layout), in
+    // ascending slot order as product.hh:192-202.  MAXK (compile-time bound
+    // on num_cols_per_row) lets every slot's loads be issued before the
+    // dependent sum, for memory-level parallelism.
+    template <typename DT, typename IT, int MAXK>
+    __device__ __forceinline__ DT ell_row_sum(const DT * __restrict__ x, const IT * __restrict__ aj,
+            const DT * __restrict__ ax, uint32_t len, uint64_t row, uint64_t stride)
+    {
         DT sum = DT(0);
         if constexpr (MAXK > 0)
         {
@@ -57,24 +61,138 @@ namespace k
             for (int k = 0; k < MAXK; ++k)
             {
                 if ((uint32_t)k < len)
-                    sum += a[k] * x[c[k]];
+                    sum += a[k] * x[col_of<IT>(c[k], row)];
             }
         }
         else
         {
             for (uint32_t k = 0; k < len; ++k)
-                sum += ld_stream(ax + (uint64_t)k * stride) * x[ld_stream(aj + (uint64_t)k * stride)];
+                sum += ld_stream(ax + (uint64_t)k * stride) * x[col_of<IT>(ld_stream(aj + (uint64_t)k * stride), row)];
         }
-        y[row] = DEFECT ? rhs[row] - sum : sum;
+        return sum;
+    }
+
+    // One lane per row.  MODE 0: y = A x; 1: y = rhs - A x;
+    // 2: y = x + (rhs - A x) * dinv (damped-Jacobi sweep: Defect, then
+    //    ScaledSum(x, temp, dinv), jacobi.hh:268-279).
+    template <typename DT, typename IT, typename LT, int MAXK, int MODE>
+    __global__ __launch_bounds__(256) void ell_spmv_t1(
+            const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
+            const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
+            const DT * __restrict__ dinv, uint64_t row_start, uint64_t row_end, uint64_t stride)
+    {
+        const uint64_t row = row_start + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (row >= row_end)
+            return;
+        const uint32_t len = (uint32_t)Arl[row];
+        const DT sum = ell_row_sum<DT, IT, MAXK>(x, Aj + row, Ax + row, len, row, stride);
+        if (MODE == 0)
+            y[row] = sum;
+        else if (MODE == 1)
+            y[row] = rhs[row] - sum;
+        else
+        {
+            const DT t = rhs[row] - sum;
+            y[row] = x[row] + t * dinv[row];
+        }
+    }
+
+    // XCD-aware block order: the hardware deals consecutive workgroups to the
+    // 8 XCDs round-robin; this bijection hands every XCD one contiguous range
+    // of logical blocks, so the x entries its blocks gather stay in its L2.
+    __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+    {
+        const uint32_t q = nb >> 3, rem = nb & 7u;
+        const uint32_t xcd = b & 7u, pos = b >> 3;
+        return xcd * q + (xcd < rem ? xcd : rem) + pos;
+    }
+
+    template <typename T> struct Vec2;
+    template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+    template <> struct Vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+    template <> struct Vec2<int16_t> { typedef short type __attribute__((ext_vector_type(2))); };
+    template <> struct Vec2<uint32_t> { typedef unsigned type __attribute__((ext_vector_type(2))); };
+
+    // Two consecutive rows per lane (threads == 1 layout): the value and
+    // index streams are read as 2-element vectors, halving the load
+    // instructions per byte.  FULL loads every slot below num_cols_per_row
+    // without waiting for the row lengths (for matrices whose rows are
+    // nearly all full, e.g. stencils); the sums still stop at each row's
+    // length, in ascending slot order.  Requires row_start even and the
+    // slot arrays readable at row_end (stride > rows or rows even).
+    template <typename DT, typename IT, typename LT, int MAXK, int MODE, bool FULL, bool XCD>
+    __global__ __launch_bounds__(256) void ell_spmv_r2(
+            const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
+            const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
+            const DT * __restrict__ dinv, uint64_t row_start, uint64_t row_end, uint64_t stride,
+            uint32_t ncpr)
+    {
+        typedef typename Vec2<DT>::type V;
+        typedef typename Vec2<IT>::type VI;
+        const uint32_t b = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+        const uint64_t r0 = row_start + ((uint64_t)b * blockDim.x + threadIdx.x) * 2;
+        if (r0 >= row_end)
+            return;
+        const bool two = r0 + 1 < row_end;
+        const uint32_t len0 = (uint32_t)Arl[r0];
+        const uint32_t len1 = two ? (uint32_t)Arl[r0 + 1] : 0u;
+        const uint32_t lmax = len0 > len1 ? len0 : len1;
+        DT a0[MAXK], a1[MAXK];
+        IT c0[MAXK], c1[MAXK];
+#pragma unroll
+        for (int k = 0; k < MAXK; ++k)
+        {
+            if (FULL ? (uint32_t)k < ncpr : (uint32_t)k < lmax)
+            {
+                const V v = __builtin_nontemporal_load((const V *)(Ax + r0 + (uint64_t)k * stride));
+                const VI c = __builtin_nontemporal_load((const VI *)(Aj + r0 + (uint64_t)k * stride));
+                a0[k] = v.x;
+                a1[k] = v.y;
+                c0[k] = (IT)c.x;
+                c1[k] = (IT)c.y;
+            }
+        }
+        DT s0 = DT(0), s1 = DT(0);
+#pragma unroll
+        for (int k = 0; k < MAXK; ++k)
+        {
+            if ((uint32_t)k < len0)
+                s0 += a0[k] * x[col_of<IT>(c0[k], r0)];
+            if ((uint32_t)k < len1)
+                s1 += a1[k] * x[col_of<IT>(c1[k], r0 + 1)];
+        }
+        if (MODE == 0)
+        {
+            y[r0] = s0;
+            if (two)
+                y[r0 + 1] = s1;
+        }
+        else if (MODE == 1)
+        {
+            y[r0] = rhs[r0] - s0;
+            if (two)
+                y[r0 + 1] = rhs[r0 + 1] - s1;
+        }
+        else
+        {
+            const DT t0 = rhs[r0] - s0;
+            y[r0] = x[r0] + t0 * dinv[r0];
+            if (two)
+            {
+                const DT t1 = rhs[r0 + 1] - s1;
+                y[r0 + 1] = x[r0 + 1] + t1 * dinv[r0 + 1];
+            }
+        }
     }
 
     // ELL with threads > 1: still one lane per row, walking k then t so the
     // summation order is the reference's (product.hh:192-202).
-    template <typename DT, typename IT, typename LT, bool DEFECT>
+    template <typename DT, typename IT, typename LT, int MODE>
     __global__ __launch_bounds__(256) void ell_spmv_tn(
             const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
             const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
-            uint64_t row_start, uint64_t row_end, uint64_t stride, uint32_t threads)
+            const DT * __restrict__ dinv, uint64_t row_start, uint64_t row_end, uint64_t stride,
+            uint32_t threads)
     {
         const uint64_t row = row_start + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
         if (row >= row_end)
@@ -84,46 +202,16 @@ namespace k
         DT sum = DT(0);
         for (uint32_t k = 0; k < len; ++k, j += stride)
             for (uint32_t t = 0; t < threads; ++t)
-                sum += Ax[j + t] * x[Aj[j + t]];
-        y[row] = DEFECT ? rhs[row] - sum : sum;
-    }
-
-    // ELL damped-Jacobi sweep: x_new = x_old + (rhs - A x_old) * dinv
-    // (Defect, then ScaledSum(x, temp, dinv), jacobi.hh:268-279).
-    template <typename DT, typename IT, typename LT, int MAXK>
-    __global__ __launch_bounds__(256) void ell_jacobi_t1(
-            const DT * __restrict__ x_old, DT * __restrict__ x_new, const IT * __restrict__ Aj,
-            const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
-            const DT * __restrict__ dinv, uint64_t rows, uint64_t stride)
-    {
-        const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (row >= rows)
-            return;
-        const uint32_t len = (uint32_t)Arl[row];
-        DT sum = DT(0);
-        if constexpr (MAXK > 0)
-        {
-            DT a[MAXK > 0 ? MAXK : 1];
-            IT c[MAXK > 0 ? MAXK : 1];
-#pragma unroll
-            for (int k = 0; k < MAXK; ++k)
-                if ((uint32_t)k < len)
-                {
-                    a[k] = ld_stream(Ax + row + (uint64_t)k * stride);
-                    c[k] = ld_stream(Aj + row + (uint64_t)k * stride);
-                }
-#pragma unroll
-            for (int k = 0; k < MAXK; ++k)
-                if ((uint32_t)k < len)
-                    sum += a[k] * x_old[c[k]];
-        }
+                sum += Ax[j + t] * x[col_of<IT>(Aj[j + t], row)];
+        if (MODE == 0)
+            y[row] = sum;
+        else if (MODE == 1)
+            y[row] = rhs[row] - sum;
         else
         {
-            for (uint32_t k = 0; k < len; ++k)
-                sum += Ax[row + (uint64_t)k * stride] * x_old[Aj[row + (uint64_t)k * stride]];
+            const DT t = rhs[row] - sum;
+            y[row] = x[row] + t * dinv[row];
         }
-        const DT t = rhs[row] - sum;
-        x_new[row] = x_old[row] + t * dinv[row];
     }
 
     // ------------------------------------------------------------------ Q1

@@ -96,21 +96,23 @@ namespace hsolve
     {
         if (row_end <= row_start)
             return;
+        constexpr int MODE = DEFECT ? 1 : 0;
+        const DT * dinv = nullptr;
         const unsigned grid = grid_for(row_end - row_start, kBlock);
         if (threads != 1)
         {
-            hipLaunchKernelGGL((k::ell_spmv_tn<DT, IT, LT, DEFECT>), dim3(grid), dim3(kBlock), 0, st, x, y,
-                               Aj, Ax, Arl, rhs, row_start, row_end, stride, (uint32_t)threads);
+            hipLaunchKernelGGL((k::ell_spmv_tn<DT, IT, LT, MODE>), dim3(grid), dim3(kBlock), 0, st, x, y,
+                               Aj, Ax, Arl, rhs, dinv, row_start, row_end, stride, (uint32_t)threads);
         }
 #define HS_ELL(K)                                                                                  \
         else if (ncpr <= K)                                                                        \
-            hipLaunchKernelGGL((k::ell_spmv_t1<DT, IT, LT, K, DEFECT>), dim3(grid), dim3(kBlock), 0, st, \
-                               x, y, Aj, Ax, Arl, rhs, row_start, row_end, stride);
+            hipLaunchKernelGGL((k::ell_spmv_t1<DT, IT, LT, K, MODE>), dim3(grid), dim3(kBlock), 0, st,   \
+                               x, y, Aj, Ax, Arl, rhs, dinv, row_start, row_end, stride);
         HS_ELL(4) HS_ELL(8) HS_ELL(9) HS_ELL(12) HS_ELL(16) HS_ELL(24) HS_ELL(32)
 #undef HS_ELL
         else
-            hipLaunchKernelGGL((k::ell_spmv_t1<DT, IT, LT, 0, DEFECT>), dim3(grid), dim3(kBlock), 0, st, x, y,
-                               Aj, Ax, Arl, rhs, row_start, row_end, stride);
+            hipLaunchKernelGGL((k::ell_spmv_t1<DT, IT, LT, 0, MODE>), dim3(grid), dim3(kBlock), 0, st, x, y,
+                               Aj, Ax, Arl, rhs, dinv, row_start, row_end, stride);
         HIP_CHECK(hipGetLastError());
     }
 

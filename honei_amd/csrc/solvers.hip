@@ -13,23 +13,13 @@
 // has already stopped, so the host enqueues whole chunks of iterations and
 // synchronises once per chunk, while the iteration count stays exactly the
 // reference's.
+#include "ell_handle.hh"
 #include "kernels_mg.hh"
 
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <vector>
-
-struct hsolve_ell_s
-{
-    int dtype;
-    uint64_t rows, cols, stride, ncpr, threads, nnz;
-    int len_bytes;
-    uint32_t * Aj;
-    void * Ax;
-    void * Arl;
-    uint64_t device_bytes;
-};
 
 namespace hsolve
 {
@@ -49,9 +39,9 @@ namespace
         }
     };
 
-    template <typename DT, typename LT> struct EllOp
+    template <typename DT, typename IT, typename LT> struct EllOp
     {
-        const uint32_t * Aj;
+        const IT * Aj;
         const DT * Ax;
         const LT * Arl;
         uint64_t stride;
@@ -59,19 +49,13 @@ namespace
         __device__ __forceinline__ DT row(const DT * __restrict__ x, int64_t i) const
         {
             const uint32_t len = (uint32_t)Arl[i];
-            DT sum = DT(0);
             if (threads == 1)
-            {
-                for (uint32_t k = 0; k < len; ++k)
-                    sum += k::ld_stream(Ax + i + (uint64_t)k * stride) * x[k::ld_stream(Aj + i + (uint64_t)k * stride)];
-            }
-            else
-            {
-                uint64_t j = (uint64_t)i * threads;
-                for (uint32_t k = 0; k < len; ++k, j += stride)
-                    for (uint32_t t = 0; t < threads; ++t)
-                        sum += Ax[j + t] * x[Aj[j + t]];
-            }
+                return k::ell_row_sum<DT, IT, 0>(x, Aj + i, Ax + i, len, (uint64_t)i, stride);
+            DT sum = DT(0);
+            uint64_t j = (uint64_t)i * threads;
+            for (uint32_t k = 0; k < len; ++k, j += stride)
+                for (uint32_t t = 0; t < threads; ++t)
+                    sum += Ax[j + t] * x[k::col_of<IT>(Aj[j + t], (uint64_t)i)];
             return sum;
         }
     };
@@ -393,23 +377,12 @@ namespace
                     uint64_t * used, double * defect, hipStream_t st, int * conv)
     {
         const uint64_t n = m->rows;
-        if (m->len_bytes == 1)
-        {
-            EllOp<DT, uint8_t> A{m->Aj, (const DT *)m->Ax, (const uint8_t *)m->Arl, m->stride, (uint32_t)m->threads};
-            run_cg<DT, EllOp<DT, uint8_t>, VAR>(A, n, b, x, P, max_iters, eps, used, defect, st, conv);
-        }
-        else if (m->len_bytes == 2)
-        {
-            EllOp<DT, uint16_t> A{m->Aj, (const DT *)m->Ax, (const uint16_t *)m->Arl, m->stride,
-                                  (uint32_t)m->threads};
-            run_cg<DT, EllOp<DT, uint16_t>, VAR>(A, n, b, x, P, max_iters, eps, used, defect, st, conv);
-        }
-        else
-        {
-            EllOp<DT, uint32_t> A{m->Aj, (const DT *)m->Ax, (const uint32_t *)m->Arl, m->stride,
-                                  (uint32_t)m->threads};
-            run_cg<DT, EllOp<DT, uint32_t>, VAR>(A, n, b, x, P, max_iters, eps, used, defect, st, conv);
-        }
+        with_ell_types(m, [&](auto Aj, auto Arl) {
+            using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
+            using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
+            EllOp<DT, IT, LT> A{Aj, (const DT *)m->Ax, Arl, m->stride, (uint32_t)m->threads};
+            run_cg<DT, EllOp<DT, IT, LT>, VAR>(A, n, b, x, P, max_iters, eps, used, defect, st, conv);
+        });
     }
 }  // namespace
 }  // namespace hsolve

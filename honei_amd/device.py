@@ -127,6 +127,9 @@ class EllMatrix:
         return cls(E["rows"], columns if columns is not None else E.get("cols", E["rows"]), E["stride"],
                    E["ncpr"], E.get("threads", 1), E["Aj"], E["Ax"], E.get("Arl"))
 
+    def set_variant(self, variant: int):
+        check(lib().hsolve_ell_set_variant(vp(self.handle), int(variant)), "ell_set_variant")
+
     def spmv(self, x, y, rhs=None, stream=None):
         check(lib().hsolve_ell_spmv(vp(self.handle), vp(x.ptr), vp(y.ptr), vp(rhs.ptr) if rhs else None,
                                     vp(stream)), "ell_spmv")

@@ -230,6 +230,11 @@ int hsolve_ell_destroy(hsolve_ell m);
 int hsolve_ell_info(hsolve_ell m, uint64_t *rows, uint64_t *columns, uint64_t *stride,
                     uint64_t *num_cols_per_row, uint64_t *threads, uint64_t *nnz,
                     uint64_t *device_bytes);
+/* Kernel variant of the handle (tuning knob, no reference counterpart):
+ * 0 = automatic; else bits 1 = one row per lane (default two), 2 = load all
+ * num_cols_per_row slots without waiting for row lengths, 4 = XCD-contiguous
+ * block order.  Results are identical for every variant. */
+int hsolve_ell_set_variant(hsolve_ell m, int variant);
 /* y = A x, or y = rhs - A x when rhs != NULL */
 int hsolve_ell_spmv(hsolve_ell m, const void *x, void *y, const void *rhs, void *stream);
 /* x_new = x_old + dinv * (rhs - A x_old) */

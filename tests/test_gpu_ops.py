@@ -106,13 +106,17 @@ def test_ell_matrix_b_and_a_small(hip, oracle, dtype):
         n = E["rows"]
         x = P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype)
         ref = oracle.ell_product(E, x)
+        rhs = P.rng_uniform(P.SEED, P.S_RHS, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype)
+        ref_d = oracle.ell_defect(E, rhs, x)
         M = hip.EllMatrix.from_dict(E)
         yd = hip.DeviceVector.zeros(n, dtype)
-        M.spmv(hip.DeviceVector.from_numpy(x), yd)
-        assert np.array_equal(yd.get(), ref)
-        rhs = P.rng_uniform(P.SEED, P.S_RHS, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype)
-        M.spmv(hip.DeviceVector.from_numpy(x), yd, rhs=hip.DeviceVector.from_numpy(rhs))
-        assert np.array_equal(yd.get(), oracle.ell_defect(E, rhs, x))
+        xd, rd = hip.DeviceVector.from_numpy(x), hip.DeviceVector.from_numpy(rhs)
+        for variant in range(8):          # every kernel variant gives the same bits
+            M.set_variant(variant)
+            M.spmv(xd, yd)
+            assert np.array_equal(yd.get(), ref), variant
+            M.spmv(xd, yd, rhs=rd)
+            assert np.array_equal(yd.get(), ref_d), variant
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
