@@ -95,6 +95,8 @@ _SIGS = {
     "hsolve_prolongation_double": [vp, u64, vp, u64, vp, vp],
     "hsolve_restriction_float": [vp, u64, vp, u64, vp, vp],
     "hsolve_restriction_double": [vp, u64, vp, u64, vp, vp],
+    "hsolve_apply_dirichlet_float": [vp, u64, vp, vp],
+    "hsolve_apply_dirichlet_double": [vp, u64, vp, vp],
     "hsolve_convert_float_double": [vp, vp, u64, vp],
     "hsolve_convert_double_float": [vp, vp, u64, vp],
     "hsolve_jacobi_q1_float": [vp, vp, vp, vp, vp, u64, vp],

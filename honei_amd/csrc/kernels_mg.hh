@@ -144,6 +144,16 @@ namespace k
             fine[idx] = v;
     }
 
+    template <typename DT>
+    __global__ __launch_bounds__(256) void dirichlet_kernel(DT * __restrict__ grid, int64_t M, Mask mk)
+    {
+        const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (idx >= M * M)
+            return;
+        if (dirichlet_node(idx, M, mk))
+            grid[idx] = DT(0);
+    }
+
     // First two damped-Jacobi sweeps from a zero start in one pass:
     //   c1 = dinv * d                         (Jacobi MG overload (1), ElementProduct)
     //   c2 = c1 + (d - A c1) * dinv           (one sweep with rhs d)

@@ -693,6 +693,28 @@ int hsolve_restriction_double(void * coarse, uint64_t size_coarse, const void * 
     HSOLVE_API_END
 }
 
+int hsolve_apply_dirichlet_float(void * grid, uint64_t size, const uint64_t * mask, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(grid != nullptr, HSOLVE_ERR_INVALID, "apply_dirichlet: null grid");
+    const int64_t M = grid_side(size, "apply_dirichlet");
+    hipLaunchKernelGGL((k::dirichlet_kernel<float>), dim3(grid_for(size, kBlock)), dim3(kBlock), 0, ST, (float *)grid,
+                       M, mask_of(mask));
+    HIP_CHECK(hipGetLastError());
+    HSOLVE_API_END
+}
+
+int hsolve_apply_dirichlet_double(void * grid, uint64_t size, const uint64_t * mask, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(grid != nullptr, HSOLVE_ERR_INVALID, "apply_dirichlet: null grid");
+    const int64_t M = grid_side(size, "apply_dirichlet");
+    hipLaunchKernelGGL((k::dirichlet_kernel<double>), dim3(grid_for(size, kBlock)), dim3(kBlock), 0, ST,
+                       (double *)grid, M, mask_of(mask));
+    HIP_CHECK(hipGetLastError());
+    HSOLVE_API_END
+}
+
 // ------------------------------------------------------- fused smoothers
 int hsolve_jacobi_q1_float(const void * rhs, const void * const * bands9, const void * dinv, const void * x_old,
                            void * x_new, uint64_t size, void * stream)

@@ -1,0 +1,3 @@
+// honei/la/dot_product.hh -- include path kept alike to HONEI; the templates are in operations.hh.
+#pragma once
+#include <honei/la/operations.hh>

@@ -1,0 +1,3 @@
+// honei/math/matrix_io.hh -- include path kept alike to HONEI; the code is in io.hh.
+#pragma once
+#include <honei/math/io.hh>

@@ -183,6 +183,14 @@ int hsolve_restriction_float(void *coarse, uint64_t size_coarse, const void *fin
 int hsolve_restriction_double(void *coarse, uint64_t size_coarse, const void *fine,
                               uint64_t size_fine, const uint64_t *macro_border_mask_host,
                               void *stream);
+/* Zero the Dirichlet nodes of a (2^L+1)^2 grid vector per the 8-entry host
+ * mask: ApplyDirichletBoundaries<Tag>::value
+ * (honei/math/apply_dirichlet_boundaries.hh:31-80; CUDA applies it inside
+ * the transfer kernels only). */
+int hsolve_apply_dirichlet_float(void *grid, uint64_t size, const uint64_t *macro_border_mask_host,
+                                 void *stream);
+int hsolve_apply_dirichlet_double(void *grid, uint64_t size, const uint64_t *macro_border_mask_host,
+                                  void *stream);
 /* FP32 <-> FP64 conversion.  cuda_convert_float_double / _double_float
  * (transfer.cu:110-138). */
 int hsolve_convert_float_double(const void *src_float, void *dst_double, uint64_t size,

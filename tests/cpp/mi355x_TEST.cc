@@ -1,0 +1,488 @@
+// mi355x_TEST.cc -- the C++ template layer (honei_amd/include/honei) used the
+// way HONEI's own tests use a backend tag: containers, Product/Defect/...
+// and solvers instantiated on tags::GPU::MI355X, each result checked
+// against the oracle (oracle/oracle.h, the CPU restatement of tags::CPU),
+// which this test links as its checker.
+//
+// Usage: mi355x_TEST <fixture dir>   (exit status 0 when every check passes)
+#include <honei/la/product.hh>
+#include <honei/la/dot_product.hh>
+#include <honei/la/norm.hh>
+#include <honei/la/scaled_sum.hh>
+#include <honei/la/sum.hh>
+#include <honei/la/scale.hh>
+#include <honei/la/difference.hh>
+#include <honei/la/element_product.hh>
+#include <honei/la/element_inverse.hh>
+#include <honei/la/algorithm.hh>
+#include <honei/math/defect.hh>
+#include <honei/math/cg.hh>
+#include <honei/math/conjugate_gradients.hh>
+#include <honei/math/jacobi.hh>
+#include <honei/math/multigrid.hh>
+#include <honei/math/restriction.hh>
+#include <honei/math/prolongation.hh>
+#include <honei/math/matrix_io.hh>
+#include <honei/math/vector_io.hh>
+#include <honei/math/fill_matrix.hh>
+
+#include <oracle.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <string>
+#include <vector>
+
+using namespace honei;
+typedef tags::GPU::MI355X Tag;
+
+static std::string g_dir;
+static int g_failures = 0;
+static int g_checks = 0;
+
+#define TEST_CHECK(cond)                                                                             \
+    do {                                                                                             \
+        ++g_checks;                                                                                  \
+        if (! (cond))                                                                                \
+        {                                                                                            \
+            ++g_failures;                                                                            \
+            std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);                           \
+        }                                                                                            \
+    } while (0)
+
+#define TEST_CHECK_EQUAL_WITHIN_EPS(a, b, eps) TEST_CHECK(std::fabs((double)(a) - (double)(b)) <= (double)(eps))
+
+template <typename DT> static DenseVector<DT> random_vector(unsigned long n, unsigned seed, DT lo = DT(-1), DT hi = DT(1))
+{
+    std::mt19937_64 g(seed);
+    std::uniform_real_distribution<double> u((double)lo, (double)hi);
+    DenseVector<DT> v(n);
+    DT * e = v.elements();
+    for (unsigned long i = 0; i < n; ++i)
+        e[i] = (DT)u(g);
+    return v;
+}
+
+template <typename DT> static bool bit_equal(const DenseVector<DT> & a, const DT * b)
+{
+    return std::memcmp(a.read_elements(), b, sizeof(DT) * a.size()) == 0;
+}
+
+// oracle entry points by value type
+static void or_ell_product(double * y, const uint64_t * Aj, const double * Ax, const uint64_t * Arl, const double * x,
+                           uint64_t rows, uint64_t stride, uint64_t threads)
+{ or_ell_product_f64(y, Aj, Ax, Arl, x, 0, rows, stride, threads); }
+static void or_ell_product(float * y, const uint64_t * Aj, const float * Ax, const uint64_t * Arl, const float * x,
+                           uint64_t rows, uint64_t stride, uint64_t threads)
+{ or_ell_product_f32(y, Aj, Ax, Arl, x, 0, rows, stride, threads); }
+static void or_q1_defect(double * r, const double * rhs, const double * const * b, const double * x, uint64_t n)
+{ or_q1_defect_f64(r, rhs, b, x, n); }
+static void or_q1_defect(float * r, const float * rhs, const float * const * b, const float * x, uint64_t n)
+{ or_q1_defect_f32(r, rhs, b, x, n); }
+
+static BandedMatrixQx<Q1Type, double> read_bin(unsigned long n, DenseVector<double> & b, DenseVector<double> & ref_sol)
+{
+    const std::string path = g_dir + "/" + std::to_string(n) + ".bin";
+    FILE * f = std::fopen(path.c_str(), "rb");
+    if (! f)
+        throw InternalError("cannot open " + path);
+    int nn = 0;
+    std::vector<double> arr(12 * n);
+    bool ok = std::fread(&nn, sizeof(int), 1, f) == 1 && (unsigned long)nn == n &&
+              std::fread(arr.data(), sizeof(double), 12 * n, f) == 12 * n;
+    std::fclose(f);
+    if (! ok)
+        throw InternalError("bad " + path);
+    // file order DD LL LD LU DL DU UL UD UU, b, ana_sol, ref_sol
+    const int file_of_band[9] = {1, 2, 3, 4, 0, 5, 6, 7, 8};
+    std::vector<DenseVector<double>> bands;
+    for (int k = 0; k < 9; ++k)
+    {
+        DenseVector<double> v(n);
+        std::memcpy(v.elements(), arr.data() + file_of_band[k] * n, sizeof(double) * n);
+        bands.push_back(v);
+    }
+    b = DenseVector<double>(n);
+    std::memcpy(b.elements(), arr.data() + 9 * n, sizeof(double) * n);
+    ref_sol = DenseVector<double>(n);
+    std::memcpy(ref_sol.elements(), arr.data() + 11 * n, sizeof(double) * n);
+    return BandedMatrixQx<Q1Type, double>(n, bands[0], bands[1], bands[2], bands[3], bands[4], bands[5], bands[6],
+                                          bands[7], bands[8]);
+}
+
+template <typename DT> static BandedMatrixQx<Q1Type, DT> poisson(unsigned long n)
+{
+    BandedMatrixQx<Q1Type, DT> A(n);
+    FillMatrix<Tag, applications::POISSON, boundary_types::DIRICHLET::DIRICHLET_0>::value(A);
+    return A;
+}
+
+template <typename DT> static void bands_host(const BandedMatrixQx<Q1Type, DT> & A, const DT * out[9])
+{
+    for (int k = 0; k < 9; ++k)
+        out[k] = A.band(k).read_elements();
+}
+
+// ------------------------------------------------------------------ tests
+template <typename DT> static void test_ell_product_defect()
+{
+    SparseMatrixELL<DT> A(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/poisson/A_6.ell", DT(0)));
+    DenseVector<DT> x(random_vector<DT>(A.columns(), 7)), y(A.rows()), rhs(random_vector<DT>(A.rows(), 8));
+    Product<Tag>::value(y, A, x);
+    std::vector<DT> ref(A.rows());
+    or_ell_product(ref.data(), (const uint64_t *)A.Aj().read_elements(), A.Ax().read_elements(),
+                   (const uint64_t *)A.Arl().read_elements(), x.read_elements(), A.rows(), A.stride(), A.threads());
+    TEST_CHECK(bit_equal(y, ref.data()));
+    // a row range goes through the reference-layout entry point
+    DenseVector<DT> y2(A.rows(), DT(0));
+    Product<Tag>::value(y2, A, x, 0, A.rows() / 2);
+    TEST_CHECK(std::memcmp(y2.read_elements(), ref.data(), sizeof(DT) * (A.rows() / 2)) == 0);
+    DenseVector<DT> r(A.rows());
+    Defect<Tag>::value(r, rhs, A, x);
+    for (unsigned long i = 0; i < A.rows(); ++i)
+        ref[i] = rhs[i] - ref[i];
+    TEST_CHECK(bit_equal(r, ref.data()));
+    // a prolongation matrix (rows != columns) as PROLMAT transfer
+    SparseMatrixELL<DT> P(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/poisson/prol_6.ell", DT(0)));
+    DenseVector<DT> c(random_vector<DT>(P.columns(), 9)), f(P.rows());
+    DenseVector<unsigned long> mask(8, 2ul);
+    Prolongation<Tag, methods::PROLMAT>::value(f, c, mask, P);
+    std::vector<DT> reff(P.rows());
+    or_ell_product(reff.data(), (const uint64_t *)P.Aj().read_elements(), P.Ax().read_elements(),
+                   (const uint64_t *)P.Arl().read_elements(), c.read_elements(), P.rows(), P.stride(), P.threads());
+    TEST_CHECK(bit_equal(f, reff.data()));
+    // size mismatch throws like HONEI
+    bool thrown = false;
+    try
+    {
+        DenseVector<DT> bad(A.columns() + 1);
+        Product<Tag>::value(y, A, bad);
+    }
+    catch (const VectorSizeDoesNotMatch &)
+    {
+        thrown = true;
+    }
+    TEST_CHECK(thrown);
+}
+
+template <typename DT> static void test_ell_from_sparse_matrix()
+{
+    // product_TEST.cc:773-845 known-answer matrix, threads 1..16
+    for (unsigned long size = 11; size < (1 << 9); size <<= 1)
+    {
+        SparseMatrix<DT> sms(size, size + 3);
+        for (unsigned long i = 0; i < size * (size + 3); i += 5)
+            sms(i / (size + 3), i % (size + 3), DT(i) / DT(1.234));
+        for (unsigned long threads = 1; threads <= 16; threads *= 2)
+        {
+            Configuration::instance()->set_value("ell::threads", (int)threads);
+            SparseMatrixELL<DT> sm0(sms);
+            DenseVector<DT> dv1(size + 3, DT(4));
+            dv1[0] = 1;
+            dv1[1] = 2;
+            DenseVector<DT> prod(size, DT(4711));
+            Product<Tag>::value(prod, sm0, dv1);
+            std::vector<DT> ref(size);
+            or_ell_product(ref.data(), (const uint64_t *)sm0.Aj().read_elements(), sm0.Ax().read_elements(),
+                           (const uint64_t *)sm0.Arl().read_elements(), dv1.read_elements(), size, sm0.stride(),
+                           threads);
+            TEST_CHECK(bit_equal(prod, ref.data()));
+            TEST_CHECK(sm0(size - 1, 2) == sms(size - 1, 2));
+        }
+        Configuration::instance()->set_value("ell::threads", 1);
+    }
+}
+
+template <typename DT> static void test_q1_and_blas1()
+{
+    const unsigned long n = 4225;
+    BandedMatrixQx<Q1Type, DT> A(poisson<DT>(n));
+    DenseVector<DT> x(random_vector<DT>(n, 1)), y(n), rhs(random_vector<DT>(n, 2)), z(random_vector<DT>(n, 3));
+    Product<Tag>::value(y, A, x);
+    const DT * b[9];
+    bands_host(A, b);
+    std::vector<DT> ref(n);
+    or_q1_defect(ref.data(), rhs.read_elements(), b, x.read_elements(), n);
+    DenseVector<DT> r(Defect<Tag>::value(rhs, A, x));
+    TEST_CHECK(bit_equal(r, ref.data()));
+    // dot / norm within the reduction-order tolerance
+    double exact = 0, sq = 0, mag = 0;
+    for (unsigned long i = 0; i < n; ++i)
+    {
+        exact += (double)x[i] * (double)z[i];
+        mag += std::fabs((double)x[i] * (double)z[i]);
+        sq += (double)x[i] * (double)x[i];
+    }
+    const double tol = sizeof(DT) == 8 ? 1e-13 : 1e-6;
+    TEST_CHECK_EQUAL_WITHIN_EPS(DotProduct<Tag>::value(x, z), exact, tol * mag);
+    TEST_CHECK_EQUAL_WITHIN_EPS((Norm<vnt_l_two, false, Tag>::value(x)), sq, tol * sq);
+    TEST_CHECK_EQUAL_WITHIN_EPS((Norm<vnt_l_two, true, Tag>::value(x)), std::sqrt(sq), tol * std::sqrt(sq));
+    // elementwise ops: bit-exact with the sequential definition
+    DenseVector<DT> a(x.copy());
+    ScaledSum<Tag>::value(a, z, DT(0.25));
+    bool ok = true;
+    for (unsigned long i = 0; i < n; ++i)
+        ok = ok && a[i] == DT(x[i] + DT(0.25) * z[i]);
+    TEST_CHECK(ok);
+    DenseVector<DT> s(n);
+    ScaledSum<Tag>::value(s, x, z, DT(-3));
+    DenseVector<DT> v(x.copy());
+    ScaledSum<Tag>::value(v, z, rhs);
+    Sum<Tag>::value(a, z);
+    Scale<Tag>::value(a, DT(0.5));
+    DenseVector<DT> d(n);
+    Difference<Tag>::value(d, x, z);
+    DenseVector<DT> e(x.copy());
+    ElementProduct<Tag>::value(e, z);
+    DenseVector<DT> inv(x.copy());
+    ElementInverse<Tag>::value(inv);
+    ok = true;
+    for (unsigned long i = 0; i < n; ++i)
+    {
+        const DT ai = DT(DT(DT(x[i] + DT(0.25) * z[i]) + z[i]) * DT(0.5));
+        ok = ok && s[i] == DT(DT(-3) * z[i] + x[i]) && v[i] == DT(x[i] + z[i] * rhs[i]) && a[i] == ai &&
+             d[i] == DT(x[i] - z[i]) && e[i] == DT(x[i] * z[i]) && inv[i] == DT(DT(1) / x[i]);
+    }
+    TEST_CHECK(ok);
+    // convert FP64 -> FP32 -> FP64 and fill/copy
+    DenseVector<float> f32(n);
+    DenseVector<double> f64(n, 0.1);
+    convert<Tag>(f32, f64);
+    TEST_CHECK(f32[17] == 0.1f);
+    DenseVector<double> back(n);
+    convert<Tag>(back, f32);
+    TEST_CHECK(back[17] == (double)0.1f);
+    fill<Tag>(e, DT(3));
+    DenseVector<DT> cp(n);
+    copy<Tag>(e, cp);
+    TEST_CHECK(cp[n - 1] == DT(3));
+}
+
+template <typename DT> static void test_transfers_and_jacobi()
+{
+    DenseVector<unsigned long> mask(8, 2ul);
+    for (unsigned long L = 1; L <= 8; ++L)
+    {
+        const unsigned long nf = ((1ul << L) + 1) * ((1ul << L) + 1), nc = ((1ul << (L - 1)) + 1) * ((1ul << (L - 1)) + 1);
+        DenseVector<DT> f(random_vector<DT>(nf, 10 + L)), c(random_vector<DT>(nc, 20 + L));
+        DenseVector<DT> cc(nc), ff(nf);
+        BandedMatrixQx<Q1Type, DT> dummy(9);
+        Restriction<Tag, methods::NONE>::value(cc, f, mask, dummy);
+        Prolongation<Tag, methods::NONE>::value(ff, c, mask, dummy);
+        std::vector<DT> rc(nc), rf(nf);
+        const uint64_t m[8] = {2, 2, 2, 2, 2, 2, 2, 2};
+        if (sizeof(DT) == 8)
+        {
+            or_restriction_f64((double *)rc.data(), (const double *)f.read_elements(), nf, m);
+            or_prolongation_f64((double *)rf.data(), (const double *)c.read_elements(), nf, m);
+        }
+        else
+        {
+            or_restriction_f32((float *)rc.data(), (const float *)f.read_elements(), nf, m);
+            or_prolongation_f32((float *)rf.data(), (const float *)c.read_elements(), nf, m);
+        }
+        TEST_CHECK(bit_equal(cc, rc.data()));
+        TEST_CHECK(bit_equal(ff, rf.data()));
+    }
+    // Jacobi MG overloads: first sweep from zero, then 3 sweeps in place
+    const unsigned long n = 1089;
+    BandedMatrixQx<Q1Type, DT> A(poisson<DT>(n));
+    DenseVector<DT> rhs(random_vector<DT>(n, 5)), dinv(n), x(n);
+    for (unsigned long i = 0; i < n; ++i)
+        dinv[i] = DT((double)(DT(1) / A.band(DD)[i]) * 0.7);
+    Jacobi<Tag>::value(A, rhs, x, DT(0.7), dinv);
+    Jacobi<Tag>::value(x, A, rhs, x, 3ul, DT(0.7), dinv);
+    std::vector<DT> rx(n), tmp(n);
+    for (unsigned long i = 0; i < n; ++i)
+        rx[i] = dinv[i] * rhs[i];
+    const DT * b[9];
+    bands_host(A, b);
+    if (sizeof(DT) == 8)
+        or_jacobi_q1_f64((double *)rx.data(), (const double * const *)b, (const double *)rhs.read_elements(),
+                         (const double *)dinv.read_elements(), n, 3, (double *)tmp.data());
+    else
+        or_jacobi_q1_f32((float *)rx.data(), (const float * const *)b, (const float *)rhs.read_elements(),
+                         (const float *)dinv.read_elements(), n, 3, (float *)tmp.data());
+    TEST_CHECK(bit_equal(x, rx.data()));
+}
+
+static void test_cg()
+{
+    // plain FP64 CG on Q1 129^2 (banded and ELL)
+    const unsigned long n = 129 * 129;
+    BandedMatrixQx<Q1Type, double> A(poisson<double>(n));
+    DenseVector<double> b(random_vector<double>(n, 42)), x(n, 0.0);
+    for (unsigned long i = 0; i < n; ++i)
+    {
+        const unsigned long r = i / 129, c = i % 129;
+        if (r == 0 || c == 0 || r == 128 || c == 128)
+            b[i] = 0;
+    }
+    unsigned long used = 0;
+    CGSolver<Tag, methods::NONE>::value(A, A, b, x, 10000ul, used, 1e-8);
+    std::vector<double> xr(n, 0.0);
+    uint64_t used_ref = 0;
+    const double * bb[9];
+    bands_host(A, bb);
+    or_cg_none_q1_f64(bb, b.read_elements(), xr.data(), n, 10000, 1e-8, &used_ref);
+    TEST_CHECK(used + 1 >= used_ref && used <= used_ref + 1);
+    double num = 0, den = 0;
+    for (unsigned long i = 0; i < n; ++i)
+    {
+        num += (x[i] - xr[i]) * (x[i] - xr[i]);
+        den += xr[i] * xr[i];
+    }
+    TEST_CHECK(std::sqrt(num / den) <= 1e-6);
+    // Jacobi-preconditioned CG on the FEATFLOW2 fixture (cg_TEST.cc:359-440)
+    SparseMatrixELL<double> E(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/poisson/A_6.ell", double(0)));
+    DenseVector<double> rhs(VectorIO<io_formats::EXP>::read_vector(g_dir + "/poisson/rhs_6", double(0)));
+    DenseVector<double> res(VectorIO<io_formats::EXP>::read_vector(g_dir + "/poisson/init_6", double(0)));
+    DenseVector<double> sol(VectorIO<io_formats::DV>::read_vector(g_dir + "/poisson/sol_6.dv", double(0)));
+    DenseVector<double> dinv(E.rows(), 0.0);
+    for (unsigned long i = 0; i < E.rows(); ++i)
+        dinv[i] = 0.7 / E(i, i);
+    unsigned long used2 = 0;
+    CGSolver<Tag, methods::VAR>::value(E, dinv, rhs, res, 700ul, used2, 1e-8);
+    bool ok = used2 < 700;
+    for (unsigned long i = 0; i < E.rows(); ++i)
+        ok = ok && std::fabs(res[i] - sol[i]) <= 3e-5;
+    TEST_CHECK(ok);
+    // coarse-grid CG (bit-exact, one workgroup)
+    BandedMatrixQx<Q1Type, float> C(poisson<float>(25));
+    DenseVector<float> cr(random_vector<float>(25, 3)), cx(25);
+    for (unsigned long i = 0; i < 25; ++i)
+        if (i < 5 || i >= 20 || i % 5 == 0 || i % 5 == 4)
+            cr[i] = 0;
+    ConjugateGradients<Tag, methods::NONE>::value(C, cr, cx, std::numeric_limits<float>::epsilon());
+    std::vector<float> ccx(25);
+    uint64_t it = 0;
+    const float * cb[9];
+    bands_host(C, cb);
+    or_cg_konv_q1_f32(cb, cr.read_elements(), ccx.data(), 25, std::numeric_limits<float>::epsilon(), 0, &it);
+    TEST_CHECK(bit_equal(cx, ccx.data()));
+}
+
+static void test_multigrid_mixed()
+{
+    // poisson_mg_mixed_banded_TEST.cc:243-427, with the .bin hierarchy
+    DenseVector<double> b(1), ref_sol(1);
+    BandedMatrixQx<Q1Type, double> A(read_bin(4225, b, ref_sol));
+    MGInfo<float, BandedMatrixQx<Q1Type, float>> info;
+    info.min_level = 1;
+    info.max_level = 6;
+    info.n_max_iter = 2;
+    info.initial_zero = true;
+    info.tolerance = 1e-8;
+    info.convergence_check = false;
+    info.n_pre_smooth = 2;
+    info.n_post_smooth = 2;
+    info.adapt_correction_factor = 1.;
+    std::vector<const float *> a32(9 * 7, nullptr);
+    std::vector<const float *> d32(7, nullptr);
+    for (unsigned long L = 0; L <= 6; ++L)
+    {
+        const unsigned long n = L == 0 ? 9 : ((1ul << L) + 1) * ((1ul << L) + 1);
+        BandedMatrixQx<Q1Type, float> a32m(n);
+        DenseVector<float> dinv(n, 0.f);
+        if (L >= 1)
+        {
+            DenseVector<double> bb(1), rs(1);
+            BandedMatrixQx<Q1Type, double> al(read_bin(n, bb, rs));
+            for (int k = 0; k < 9; ++k)
+                for (unsigned long i = 0; i < n; ++i)
+                    a32m.band(k)[i] = (float)al.band(k)[i];
+            DenseVector<float> di(a32m.band(DD).copy());
+            for (unsigned long i = 0; i < n; ++i)
+                dinv[i] = (float)((double)(di[i] != 0 ? 1.0f / di[i] : 0.0f) * 0.7);
+        }
+        info.a.push_back(a32m);
+        info.diags_inverted.push_back(dinv);
+    }
+    for (unsigned long L = 1; L <= 6; ++L)
+    {
+        for (int k = 0; k < 9; ++k)
+            a32[9 * L + k] = info.a[L].band(k).read_elements();
+        d32[L] = info.diags_inverted[L].read_elements();
+    }
+    DenseVector<double> result(4225, 0.0);
+    Multigrid<Tag, Tag, methods::NONE, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
+            A, b, result, 11ul, std::numeric_limits<double>::epsilon(), info);
+    bool ok = true;
+    for (unsigned long i = 0; i < 4225; ++i)
+        ok = ok && std::fabs(ref_sol[i] - result[i]) <= 1e-4;
+    TEST_CHECK(ok);
+    // the oracle's iteration counts
+    or_mg_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.min_level = 1; p.max_level = 6; p.n_max_iter = 2; p.initial_zero = 1; p.tolerance = 1e-8;
+    p.n_pre_smooth = 2; p.adapt_correction_factor = 1.0;
+    for (int i = 0; i < 8; ++i) p.macro_border_mask[i] = 2;
+    std::vector<double> xr(4225, 0.0);
+    uint64_t outer = 0, inner = 0;
+    const double * b64[9];
+    bands_host(A, b64);
+    or_mg_mixed_q1(b64, b.read_elements(), xr.data(), a32.data(), d32.data(), &p, &outer, &inner, nullptr, nullptr);
+    TEST_CHECK(info.outer_iterations == outer && info.inner_iterations == inner);
+    // a second solve reuses the resident hierarchy
+    DenseVector<double> result2(4225, 0.0);
+    Multigrid<Tag, Tag, methods::NONE, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
+            A, b, result2, 11ul, std::numeric_limits<double>::epsilon(), info);
+    TEST_CHECK(bit_equal(result2, result.read_elements()));
+}
+
+static void test_lock_semantics()
+{
+    DenseVector<double> v(1000, 1.0), w(1000);
+    Scale<Tag>::value(v, 2.0);           // device write
+    TEST_CHECK(v[10] == 2.0);            // host read downloads
+    v[10] = 5.0;                         // host write
+    Sum<Tag>::value(v, v.copy());        // device op sees the host write
+    TEST_CHECK(v[10] == 10.0 && v[11] == 4.0);
+    DenseVector<double> alias(v);        // shared handle
+    alias[0] = -1.0;
+    TEST_CHECK(v[0] == -1.0);
+    DenseVector<double> deep(v.copy());
+    deep[0] = 7.0;
+    TEST_CHECK(v[0] == -1.0);
+}
+
+int main(int argc, char ** argv)
+{
+    g_dir = argc > 1 ? argv[1] : "tests/golden/honei_testdata";
+    struct Case { const char * name; std::function<void()> f; };
+    std::vector<Case> cases = {
+        {"ell_product_defect<double>", test_ell_product_defect<double>},
+        {"ell_product_defect<float>", test_ell_product_defect<float>},
+        {"ell_from_sparse_matrix<double>", test_ell_from_sparse_matrix<double>},
+        {"ell_from_sparse_matrix<float>", test_ell_from_sparse_matrix<float>},
+        {"q1_and_blas1<double>", test_q1_and_blas1<double>},
+        {"q1_and_blas1<float>", test_q1_and_blas1<float>},
+        {"transfers_and_jacobi<double>", test_transfers_and_jacobi<double>},
+        {"transfers_and_jacobi<float>", test_transfers_and_jacobi<float>},
+        {"cg", test_cg},
+        {"multigrid_mixed", test_multigrid_mixed},
+        {"lock_semantics", test_lock_semantics},
+    };
+    for (auto & c : cases)
+    {
+        const int before = g_failures;
+        std::printf("RUN %s\n", c.name);
+        std::fflush(stdout);
+        try
+        {
+            c.f();
+        }
+        catch (const std::exception & e)
+        {
+            ++g_failures;
+            std::printf("  EXCEPTION %s\n", e.what());
+        }
+        std::printf("%s %s\n", g_failures == before ? "PASS" : "FAIL", c.name);
+        std::fflush(stdout);
+    }
+    std::printf("%d checks, %d failures\n", g_checks, g_failures);
+    return g_failures ? 1 : 0;
+}
