@@ -178,7 +178,15 @@ def main():
     ap.add_argument("--side", type=int, default=4097)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 2 GiB device copy kernel (for byte-counter calibration)")
+    ap.add_argument("--mg-only", type=int, default=0, metavar="REPS",
+                    help="only time REPS MG-IR solves on the 4097^2 Poisson problem (for profiling)")
     args = ap.parse_args()
+    if args.mg_only:
+        from honei_amd import device as hip
+        from honei_amd import problems as P
+        mg, _ = mg_case(hip, P, 12, 2, reps=args.mg_only, tol=1e-8)
+        print(json.dumps({"mg_ir_poisson_4097_tol1e-8": mg}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "

@@ -261,6 +261,154 @@ namespace k
         }
     }
 
+    // 16-byte vectors: 4 floats or 2 doubles per lane
+    template <typename DT> struct Vec16;
+    template <> struct Vec16<float>
+    {
+        static constexpr int V = 4;
+        typedef float type __attribute__((ext_vector_type(4)));
+    };
+    template <> struct Vec16<double>
+    {
+        static constexpr int V = 2;
+        typedef double type __attribute__((ext_vector_type(2)));
+    };
+
+    // Q1 row sums of V consecutive rows i0..i0+V-1 that lie away from both
+    // ends of the vector (every index guard of q1_row holds): bands come in
+    // as one 16-byte load each, x from the three row windows.  Each row's
+    // sum runs in q1_row's order.
+    template <typename DT>
+    __device__ __forceinline__ void q1_rows_v(const Bands & B, const DT * __restrict__ x, int64_t i0, int64_t m,
+                                              DT (&out)[Vec16<DT>::V])
+    {
+        constexpr int V = Vec16<DT>::V;
+        typedef typename Vec16<DT>::type VT;
+        const DT * const * b = reinterpret_cast<const DT * const *>(B.b);
+        VT bv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            bv[k] = __builtin_nontemporal_load((const VT *)(b[k] + i0));
+        DT xm[V + 2], xc[V + 2], xp[V + 2];
+#pragma unroll
+        for (int j = 0; j < V + 2; ++j)
+        {
+            xm[j] = x[i0 - m - 1 + j];
+            xc[j] = x[i0 - 1 + j];
+            xp[j] = x[i0 + m - 1 + j];
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+        {
+            DT s = bv[4][v] * xc[v + 1];
+            s += bv[0][v] * xm[v];
+            s += bv[1][v] * xm[v + 1];
+            s += bv[2][v] * xm[v + 2];
+            s += bv[3][v] * xc[v];
+            s += bv[5][v] * xc[v + 2];
+            s += bv[6][v] * xp[v];
+            s += bv[7][v] * xp[v + 1];
+            s += bv[8][v] * xp[v + 2];
+            out[v] = s;
+        }
+    }
+
+    // Vectorised form of q1_kernel: V = 16 / sizeof(DT) rows per lane; lane
+    // `lane` covers rows lane*V .. lane*V+V-1.  Needs 16-byte aligned bands,
+    // y, rhs and dinv (checked by the launchers).
+    template <typename DT, int MODE>
+    __device__ __forceinline__ void q1_body_v(const Bands & B, const DT * __restrict__ x,
+            DT * __restrict__ y, const DT * __restrict__ rhs, const DT * __restrict__ dinv,
+            int64_t n, int64_t m, int64_t lane)
+    {
+        constexpr int V = Vec16<DT>::V;
+        typedef typename Vec16<DT>::type VT;
+        const int64_t i0 = lane * V;
+        if (i0 >= n)
+            return;
+        if (i0 - m - 1 < 0 || i0 + V + m >= n)
+        {
+            for (int v = 0; v < V && i0 + v < n; ++v)
+            {
+                const int64_t i = i0 + v;
+                const DT s = q1_row<DT>(B, x, i, n, m);
+                if (MODE == 0)
+                    y[i] = s;
+                else if (MODE == 1)
+                    y[i] = rhs[i] - s;
+                else
+                {
+                    const DT t = rhs[i] - s;
+                    y[i] = x[i] + t * dinv[i];
+                }
+            }
+            return;
+        }
+        DT s[V];
+        q1_rows_v<DT>(B, x, i0, m, s);
+        VT o;
+        if (MODE == 0)
+        {
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                o[v] = s[v];
+        }
+        else
+        {
+            const VT r = __builtin_nontemporal_load((const VT *)(rhs + i0));
+            if (MODE == 1)
+            {
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    o[v] = r[v] - s[v];
+            }
+            else
+            {
+                const VT di = __builtin_nontemporal_load((const VT *)(dinv + i0));
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                {
+                    const DT t = r[v] - s[v];
+                    o[v] = x[i0 + v] + t * di[v];
+                }
+            }
+        }
+        *(VT *)(y + i0) = o;
+    }
+
+    template <typename DT, int MODE>
+    __global__ __launch_bounds__(256) void q1_kernel_v(Bands B, const DT * __restrict__ x,
+            DT * __restrict__ y, const DT * __restrict__ rhs, const DT * __restrict__ dinv,
+            int64_t n, int64_t m, const int * __restrict__ active)
+    {
+        if (active && *active == 0)
+            return;
+        q1_body_v<DT, MODE>(B, x, y, rhs, dinv, n, m, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    }
+
+    inline bool aligned16(const void * p) { return ((uintptr_t)p & 15u) == 0; }
+
+    // Launches the vectorised Q1 kernel when every streamed array is 16-byte
+    // aligned, else the one-row-per-lane form.  MODE as q1_kernel.
+    template <typename DT, int MODE>
+    inline void q1_launch(const Bands & B, const DT * x, DT * y, const DT * rhs, const DT * dinv, int64_t n,
+                          int64_t m, const int * active, hipStream_t st)
+    {
+        bool vec = aligned16(y) && (MODE == 0 || aligned16(rhs)) && (MODE != 2 || aligned16(dinv));
+        for (int k = 0; k < 9; ++k)
+            vec = vec && aligned16(B.b[k]);
+        if (vec)
+        {
+            constexpr int V = Vec16<DT>::V;
+            const uint64_t lanes = ((uint64_t)n + V - 1) / V;
+            hipLaunchKernelGGL((q1_kernel_v<DT, MODE>), dim3(grid_for(lanes, 256)), dim3(256), 0, st, B, x, y, rhs,
+                               dinv, n, m, active);
+        }
+        else
+            hipLaunchKernelGGL((q1_kernel<DT, MODE>), dim3(grid_for((uint64_t)n, 256)), dim3(256), 0, st, B, x, y, rhs,
+                               dinv, n, m, active);
+    }
+
     // ------------------------------------------------------------ reductions
     __device__ __forceinline__ double wave_sum(double v)
     {
@@ -314,6 +462,67 @@ namespace k
         if (threadIdx.x == 0)
         {
             write_result(total);
+            *ticket = 0u;
+        }
+    }
+
+    // Two sums at once: wave shuffles, then the waves' totals in order.
+    // Valid in thread 0 on return.
+    __device__ __forceinline__ void block_sum2(double & a, double & b, double * lds)
+    {
+        a = wave_sum(a);
+        b = wave_sum(b);
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        if (lane == 0)
+        {
+            lds[2 * wid] = a;
+            lds[2 * wid + 1] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            double ra = 0.0, rb = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            {
+                ra += lds[2 * w];
+                rb += lds[2 * w + 1];
+            }
+            a = ra;
+            b = rb;
+        }
+        __syncthreads();
+    }
+
+    // Last-block finish of a two-value reduction over the blocks along x;
+    // `fin(a, b)` runs on one thread of the last block with the totals, and
+    // that block resets the ticket for the next use.
+    template <typename F>
+    __device__ __forceinline__ void finish2(double a, double b, double * partials, unsigned * ticket, F && fin)
+    {
+        __shared__ double lds[32];
+        __shared__ bool last;
+        block_sum2(a, b, lds);
+        if (threadIdx.x == 0)
+        {
+            partials[2 * blockIdx.x] = a;
+            partials[2 * blockIdx.x + 1] = b;
+            __threadfence();
+            last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+        }
+        __syncthreads();
+        if (! last)
+            return;
+        __threadfence();
+        double va = 0.0, vb = 0.0;
+        for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+        {
+            va += __builtin_nontemporal_load(partials + 2 * i);
+            vb += __builtin_nontemporal_load(partials + 2 * i + 1);
+        }
+        block_sum2(va, vb, lds);
+        if (threadIdx.x == 0)
+        {
+            fin(va, vb);
             *ticket = 0u;
         }
     }

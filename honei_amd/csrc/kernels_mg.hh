@@ -36,6 +36,22 @@ namespace k
         return false;
     }
 
+    // The same predicate for node (r, s) of the side-M grid, without 64-bit
+    // division (the transfer kernels run on a 2-D grid of rows x columns).
+    __device__ __forceinline__ bool dirichlet_rs(int r, int s, int M, const Mask & mk)
+    {
+        const bool r0 = r == 0, rM = r == M - 1, s0 = s == 0, sM = s == M - 1;
+        if (r0 && s0) return mk.dir[0];
+        if (r0 && sM) return mk.dir[1];
+        if (rM && sM) return mk.dir[2];
+        if (rM && s0) return mk.dir[3];
+        if (r0) return mk.dir[4];
+        if (sM) return mk.dir[5];
+        if (rM) return mk.dir[6];
+        if (s0) return mk.dir[7];
+        return false;
+    }
+
     template <typename DT>
     __device__ __forceinline__ DT add_d(DT acc, double w, DT v)
     {
@@ -47,15 +63,12 @@ namespace k
     // s+1 (those in [1, Nc-1]) and, inside one iteration, in the statement
     // order (a)..(h) below.
     template <typename DT>
-    __global__ __launch_bounds__(256) void restriction_kernel(DT * __restrict__ coarse,
-            const DT * __restrict__ fine, int64_t Nc, Mask mk, const int * __restrict__ active)
+    __device__ __forceinline__ void restriction_body(DT * __restrict__ coarse, const DT * __restrict__ fine,
+                                                     int64_t Nc, const Mask & mk, int64_t r, int64_t s)
     {
-        if (active && *active == 0)
+        if (s >= Nc)
             return;
-        const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (idx >= Nc * Nc)
-            return;
-        const int64_t r = idx / Nc, s = idx - r * Nc, Nf = 2 * Nc - 1;
+        const int64_t idx = r * Nc + s, Nf = 2 * Nc - 1;
         const DT * f = fine;
         DT v = f[(2 * r) * Nf + 2 * s];
         int64_t c[4] = {r, r + 1, s, s + 1};
@@ -83,65 +96,186 @@ namespace k
             if (r == i && s <= Nc - 2)     v = add_d(v, 0.25, f[(2 * r - 1) * Nf + 2 * s + 1]); // (g)
             if (r == i && s >= 1)          v = add_d(v, 0.25, f[(2 * r - 1) * Nf + 2 * s - 1]); // (h)
         }
-        if (dirichlet_node(idx, Nc, mk))
+        if (dirichlet_rs((int)r, (int)s, (int)Nc, mk))
             v = DT(0);
         coarse[idx] = v;
     }
 
-    // Value of the bilinear prolongation at fine node (R, S) before the
-    // Dirichlet mask, in the reference's addition order.
+    // coarse row blockIdx.y, columns along x
     template <typename DT>
-    __device__ __forceinline__ DT prolongate_node(const DT * __restrict__ c, int64_t R, int64_t S,
-                                                  int64_t Nc)
+    __global__ __launch_bounds__(256) void restriction_kernel(DT * __restrict__ coarse,
+            const DT * __restrict__ fine, int64_t Nc, Mask mk, const int * __restrict__ active)
     {
-        const bool re = (R & 1) == 0, se = (S & 1) == 0;
-        if (re && se)
-            return c[(R >> 1) * Nc + (S >> 1)];
-        DT v = DT(0);
-        if (re)
-        {   // fine row 2j, column 2i-1
-            const int64_t j = R >> 1, i = (S + 1) >> 1;
-            v = add_d(v, 0.5, c[j * Nc + i - 1]);
-            v = add_d(v, 0.5, c[j * Nc + i]);
-        }
-        else if (se)
-        {   // fine row 2i-1, column 2j
-            const int64_t i = (R + 1) >> 1, j = S >> 1;
-            v = add_d(v, 0.5, c[(i - 1) * Nc + j]);
-            v = add_d(v, 0.5, c[i * Nc + j]);
-        }
-        else
-        {   // fine row 2i-1, column 2j+1
-            const int64_t i = (R + 1) >> 1, j = (S - 1) >> 1;
-            v = add_d(v, 0.25, c[(i - 1) * Nc + j]);
-            v = add_d(v, 0.25, c[(i - 1) * Nc + j + 1]);
-            v = add_d(v, 0.25, c[i * Nc + j]);
-            v = add_d(v, 0.25, c[i * Nc + j + 1]);
-        }
-        return v;
+        if (active && *active == 0)
+            return;
+        restriction_body<DT>(coarse, fine, Nc, mk, blockIdx.y, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     }
 
-    // AXPY == false: fine = P coarse (Prolongation<Tag, NONE>).
-    // AXPY == true:  fine += alpha * (P coarse) (Prolongation then
+    // Prolongation<Tag, NONE> (prolongation.hh:43-139) on a side-Nf grid.
+    // AXPY == false: fine = P coarse.
+    // AXPY == true:  fine += alpha * (P coarse) (Prolongation followed by
     //                ScaledSum(x, c, alpha): x = x + alpha*c).
+    // One lane per pair of fine columns (2j, 2j+1) of fine row R =
+    // blockIdx.y: the row parity is uniform over the block and the pair
+    // shares its coarse loads, so no lane diverges.  Per fine node, in the
+    // reference's addition order (accumulated in double from 0):
+    //   (2i, 2j)      c[i][j]                      (copied)
+    //   (2i, 2j+1)    0.5 c[i][j], 0.5 c[i][j+1]
+    //   (2i-1, 2j)    0.5 c[i-1][j], 0.5 c[i][j]
+    //   (2i-1, 2j+1)  0.25 c[i-1][j], c[i-1][j+1], c[i][j], c[i][j+1]
+    // then the Dirichlet mask of the fine grid zeroes boundary nodes.
     template <typename DT, bool AXPY>
-    __global__ __launch_bounds__(256) void prolongation_kernel(DT * __restrict__ fine,
-            const DT * __restrict__ coarse, int64_t Nf, DT alpha, Mask mk,
+    __device__ __forceinline__ void prolongation2_body(DT * __restrict__ fine, const DT * __restrict__ coarse,
+                                                       int Nf, DT alpha, const Mask & mk, int R, int j)
+    {
+        const int Nc = (Nf + 1) / 2;
+        if (j >= Nc)
+            return;
+        const bool has1 = j + 1 < Nc;
+        DT v0, v1 = DT(0);
+        if ((R & 1) == 0)
+        {
+            const DT * c = coarse + (int64_t)(R >> 1) * Nc;
+            const DT a = c[j];
+            v0 = a;
+            if (has1)
+            {
+                v1 = add_d(DT(0), 0.5, a);
+                v1 = add_d(v1, 0.5, c[j + 1]);
+            }
+        }
+        else
+        {
+            const int i = (R + 1) >> 1;
+            const DT * cm = coarse + (int64_t)(i - 1) * Nc;
+            const DT * cp = coarse + (int64_t)i * Nc;
+            const DT a = cm[j], b = cp[j];
+            v0 = add_d(DT(0), 0.5, a);
+            v0 = add_d(v0, 0.5, b);
+            if (has1)
+            {
+                v1 = add_d(DT(0), 0.25, a);
+                v1 = add_d(v1, 0.25, cm[j + 1]);
+                v1 = add_d(v1, 0.25, b);
+                v1 = add_d(v1, 0.25, cp[j + 1]);
+            }
+        }
+        const int S0 = 2 * j;
+        if (dirichlet_rs(R, S0, Nf, mk))
+            v0 = DT(0);
+        DT * f = fine + (int64_t)R * Nf + S0;
+        if (AXPY)
+            f[0] = f[0] + alpha * v0;
+        else
+            f[0] = v0;
+        if (has1)
+        {
+            if (dirichlet_rs(R, S0 + 1, Nf, mk))
+                v1 = DT(0);
+            if (AXPY)
+                f[1] = f[1] + alpha * v1;
+            else
+                f[1] = v1;
+        }
+    }
+
+    template <typename DT, bool AXPY>
+    __global__ __launch_bounds__(256) void prolongation2_kernel(DT * __restrict__ fine,
+            const DT * __restrict__ coarse, int Nf, DT alpha, Mask mk, const int * __restrict__ active)
+    {
+        if (active && *active == 0)
+            return;
+        prolongation2_body<DT, AXPY>(fine, coarse, Nf, alpha, mk, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
+    }
+
+    // First two damped-Jacobi sweeps from a zero start in one pass:
+    //   c1 = dinv * d                         (Jacobi MG overload (1), ElementProduct)
+    //   c2 = c1 + (d - A c1) * dinv           (one sweep with rhs d)
+    // with c1 recomputed at the neighbours instead of stored.  ACC selects
+    // the output: 0 -> out = c2, 1 -> out = out + c2 (Sum<Tag>(x, c)).
+    // V = 16 / sizeof(DT) rows per lane; bands and out must be 16-byte
+    // aligned.  Lanes whose rows touch either end of the vector take the
+    // guarded per-row form; every row sums in q1_row's order.
+    template <typename DT, int ACC>
+    __device__ __forceinline__ void jacobi_zero2_body(const Bands & B, const DT * __restrict__ d,
+            const DT * __restrict__ dinv, DT * __restrict__ out, int64_t n, int64_t m, int64_t lane)
+    {
+        constexpr int V = Vec16<DT>::V;
+        typedef typename Vec16<DT>::type VT;
+        const int64_t i0 = lane * V;
+        if (i0 >= n)
+            return;
+        const DT * const * b = reinterpret_cast<const DT * const *>(B.b);
+        if (i0 - m - 1 < 0 || i0 + V + m >= n)
+        {
+            for (int v = 0; v < V && i0 + v < n; ++v)
+            {
+                const int64_t i = i0 + v;
+#define HS_C1(j) (dinv[(j)] * d[(j)])
+                DT s = ld_stream(b[4] + i) * HS_C1(i);
+                if (i - m - 1 >= 0) s += ld_stream(b[0] + i) * HS_C1(i - m - 1);
+                if (i - m >= 0)     s += ld_stream(b[1] + i) * HS_C1(i - m);
+                if (i - m + 1 >= 0) s += ld_stream(b[2] + i) * HS_C1(i - m + 1);
+                if (i - 1 >= 0)     s += ld_stream(b[3] + i) * HS_C1(i - 1);
+                if (i + 1 < n)      s += ld_stream(b[5] + i) * HS_C1(i + 1);
+                if (i + m - 1 < n)  s += ld_stream(b[6] + i) * HS_C1(i + m - 1);
+                if (i + m < n)      s += ld_stream(b[7] + i) * HS_C1(i + m);
+                if (i + m + 1 < n)  s += ld_stream(b[8] + i) * HS_C1(i + m + 1);
+#undef HS_C1
+                const DT di = dinv[i];
+                const DT c1 = di * d[i];
+                const DT t = d[i] - s;
+                const DT c2 = c1 + t * di;
+                out[i] = ACC ? out[i] + c2 : c2;
+            }
+            return;
+        }
+        VT bv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            bv[k] = __builtin_nontemporal_load((const VT *)(b[k] + i0));
+        DT cm[V + 2], cc[V + 2], cp[V + 2], dc[V + 2], ic[V + 2];
+#pragma unroll
+        for (int jj = 0; jj < V + 2; ++jj)
+        {
+            cm[jj] = dinv[i0 - m - 1 + jj] * d[i0 - m - 1 + jj];
+            ic[jj] = dinv[i0 - 1 + jj];
+            dc[jj] = d[i0 - 1 + jj];
+            cc[jj] = ic[jj] * dc[jj];
+            cp[jj] = dinv[i0 + m - 1 + jj] * d[i0 + m - 1 + jj];
+        }
+        VT o;
+        if (ACC)
+            o = *(const VT *)(out + i0);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+        {
+            DT s = bv[4][v] * cc[v + 1];
+            s += bv[0][v] * cm[v];
+            s += bv[1][v] * cm[v + 1];
+            s += bv[2][v] * cm[v + 2];
+            s += bv[3][v] * cc[v];
+            s += bv[5][v] * cc[v + 2];
+            s += bv[6][v] * cp[v];
+            s += bv[7][v] * cp[v + 1];
+            s += bv[8][v] * cp[v + 2];
+            const DT di = ic[v + 1];
+            const DT c1 = di * dc[v + 1];
+            const DT t = dc[v + 1] - s;
+            const DT c2 = c1 + t * di;
+            o[v] = ACC ? o[v] + c2 : c2;
+        }
+        *(VT *)(out + i0) = o;
+    }
+
+    template <typename DT, int ACC>
+    __global__ __launch_bounds__(256) void jacobi_zero2_q1_v(Bands B, const DT * __restrict__ d,
+            const DT * __restrict__ dinv, DT * __restrict__ out, int64_t n, int64_t m,
             const int * __restrict__ active)
     {
         if (active && *active == 0)
             return;
-        const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (idx >= Nf * Nf)
-            return;
-        const int64_t R = idx / Nf, S = idx - R * Nf, Nc = (Nf + 1) / 2;
-        DT v = prolongate_node<DT>(coarse, R, S, Nc);
-        if (dirichlet_node(idx, Nf, mk))
-            v = DT(0);
-        if (AXPY)
-            fine[idx] = fine[idx] + alpha * v;
-        else
-            fine[idx] = v;
+        jacobi_zero2_body<DT, ACC>(B, d, dinv, out, n, m, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     }
 
     template <typename DT>
@@ -152,43 +286,6 @@ namespace k
             return;
         if (dirichlet_node(idx, M, mk))
             grid[idx] = DT(0);
-    }
-
-    // First two damped-Jacobi sweeps from a zero start in one pass:
-    //   c1 = dinv * d                         (Jacobi MG overload (1), ElementProduct)
-    //   c2 = c1 + (d - A c1) * dinv           (one sweep with rhs d)
-    // with c1 recomputed at the neighbours instead of stored.  ACC selects
-    // the output: 0 -> out = c2, 1 -> out = out + c2 (Sum<Tag>(x, c)).
-    template <typename DT, int ACC>
-    __global__ __launch_bounds__(256) void jacobi_zero2_q1(Bands B, const DT * __restrict__ d,
-            const DT * __restrict__ dinv, DT * __restrict__ out, int64_t n, int64_t m,
-            const int * __restrict__ active)
-    {
-        if (active && *active == 0)
-            return;
-        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (i >= n)
-            return;
-        const DT * const * b = reinterpret_cast<const DT * const *>(B.b);
-#define HS_C1(j) (dinv[(j)] * d[(j)])
-        DT s = ld_stream(b[4] + i) * HS_C1(i);
-        if (i - m - 1 >= 0) s += ld_stream(b[0] + i) * HS_C1(i - m - 1);
-        if (i - m >= 0)     s += ld_stream(b[1] + i) * HS_C1(i - m);
-        if (i - m + 1 >= 0) s += ld_stream(b[2] + i) * HS_C1(i - m + 1);
-        if (i - 1 >= 0)     s += ld_stream(b[3] + i) * HS_C1(i - 1);
-        if (i + 1 < n)      s += ld_stream(b[5] + i) * HS_C1(i + 1);
-        if (i + m - 1 < n)  s += ld_stream(b[6] + i) * HS_C1(i + m - 1);
-        if (i + m < n)      s += ld_stream(b[7] + i) * HS_C1(i + m);
-        if (i + m + 1 < n)  s += ld_stream(b[8] + i) * HS_C1(i + m + 1);
-#undef HS_C1
-        const DT di = dinv[i];
-        const DT c1 = di * d[i];
-        const DT t = d[i] - s;
-        const DT c2 = c1 + t * di;
-        if (ACC)
-            out[i] = out[i] + c2;
-        else
-            out[i] = c2;
     }
 
     // out = dinv * d (ACC 0) or out += dinv * d (ACC 1): the single first
@@ -234,12 +331,10 @@ namespace k
     // sequential CPU sums, so the iteration is bit-identical to tags::CPU.
     // `active` (may be null): when it points to 0 the kernel does nothing.
     template <typename DT>
-    __global__ __launch_bounds__(256) void cg_konv_q1_kernel(Bands B, const DT * __restrict__ rhs,
+    __device__ __forceinline__ void cg_konv_body(const Bands & B, const DT * __restrict__ rhs,
             DT * __restrict__ x, DT * __restrict__ work, int64_t n, int64_t m, double konv_rad,
-            uint64_t max_iters, uint64_t * iters_out, const int * active)
+            uint64_t max_iters, uint64_t * iters_out)
     {
-        if (active && *active == 0)
-            return;
         __shared__ DT sh[4];
         DT * g = work;
         DT * u = work + n;
@@ -303,6 +398,16 @@ namespace k
         }
         if (iters_out && threadIdx.x == 0)
             *iters_out = it;
+    }
+
+    template <typename DT>
+    __global__ __launch_bounds__(256) void cg_konv_q1_kernel(Bands B, const DT * __restrict__ rhs,
+            DT * __restrict__ x, DT * __restrict__ work, int64_t n, int64_t m, double konv_rad,
+            uint64_t max_iters, uint64_t * iters_out, const int * active)
+    {
+        if (active && *active == 0)
+            return;
+        cg_konv_body<DT>(B, rhs, x, work, n, m, konv_rad, max_iters, iters_out);
     }
 }
 }

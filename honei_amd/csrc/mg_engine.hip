@@ -1,0 +1,841 @@
+// mg_engine.hip -- kernels and tick sequence of the FP32-MG / FP64-IR engine
+// (declarations and the slot/tick model: mg_engine.hh).
+//
+// Reference: Multigrid<Tag, OuterTag, NONE, JAC, CYCLE::V, MIXED>::value
+// (honei/math/multigrid.hh:2245-2393) around the FP32 V-cycle
+// (multigrid.hh:1616-1905):
+//   outer: x0 with b's Dirichlet values injected (:2277-2298); result = 0;
+//          d = b - A x0 (FP64); loop { rhs32 = (float)(d / |d|) (unscaled
+//          the first time); inner FP32 MG solve; result += |d| * x32;
+//          d = b - A result; stop when |d| < eps |d0| }
+//   inner: |d| of the initial FP32 defect, V-cycles until
+//          |d_k| <= tol |d_0| or n_max_iter cycles
+//   cycle: per level down: Jacobi sweeps from zero (x = dinv*d, then
+//          n_pre_smooth-1 sweeps), defect, restriction; coarse
+//          ConjugateGradients<NONE> (konv_rad form); per level up:
+//          prolongation + correction, n_pre_smooth post-sweeps, defect on
+//          the finest level.
+#include "mg_engine.hh"
+
+#include <cstring>
+
+namespace hsolve
+{
+namespace mg
+{
+namespace
+{
+    __device__ __forceinline__ bool live(const Slot & sl) { return sl.s != nullptr; }
+
+    // ------------------------------------------------------------ tick gates
+    __global__ void plan_kernel(const Slot * T, unsigned nslots)
+    {
+        const unsigned z = blockIdx.x * blockDim.x + threadIdx.x;
+        if (z >= nslots)
+            return;
+        Scalars * s = T[z].s;
+        if (s == nullptr)
+            return;
+        const int ph = s->phase;
+        s->do_start = (ph == PH_START) ? 1 : 0;
+        s->do_outer = (ph == PH_SOLVE && s->need_outer) ? 1 : 0;
+        s->open = 0;
+        if (ph == PH_START || ph == PH_SOLVE)
+            s->ticks += 1;
+    }
+
+    // ------------------------------------------------------ start of a solve
+    // result = initial guess (0 or the caller's) with b injected on the first
+    // row, the last row and the first column (multigrid.hh:2282-2297);
+    // x32 of the finest level = (float)result
+    __global__ __launch_bounds__(256) void start_init_kernel(const Slot * T, int L, int64_t n, int64_t sq)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl) || sl.s->do_start == 0)
+            return;
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n)
+            return;
+        double g = sl.s->zero_guess ? 0.0 : sl.result[i];
+        if (i < sq || i >= n - sq || (i % sq) == 0)
+            g = sl.b[i];
+        sl.result[i] = g;
+        sl.v[L][F_X][i] = (float)g;
+    }
+
+    // iterates of the coarser levels start at zero
+    __global__ __launch_bounds__(256) void start_zero_kernel(const Slot * T, int minl, int L)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl) || sl.s->do_start == 0)
+            return;
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        for (int l = minl; l < L; ++l)
+            if (i < (int64_t)size_of(l))
+                sl.v[l][F_X][i] = 0.0f;
+    }
+
+    // ------------------------------------------------------- FP64 outer step
+    // result += scale * (double)x32
+    __global__ __launch_bounds__(256) void outer_accumulate_kernel(const Slot * T, int L, int64_t n)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl) || sl.s->do_outer == 0)
+            return;
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n)
+            return;
+        const double scale = sl.s->scale;
+        sl.result[i] = sl.result[i] + scale * (double)sl.v[L][F_X][i];
+    }
+
+    // od = b - A64 result and its norm; opens the next inner solve or ends
+    // the outer loop.  On a start: the initial defect.
+    __global__ __launch_bounds__(256) void outer_defect_kernel(k::Bands B, const Slot * T, int64_t n, int64_t m,
+            double outer_eps, uint64_t max_outer)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl))
+            return;
+        Scalars * s = sl.s;
+        const int start = s->do_start;
+        if (start == 0 && s->do_outer == 0)
+            return;
+        const double * __restrict__ b = sl.b;
+        const double * __restrict__ x = sl.result;
+        double * __restrict__ od = sl.od;
+        const bool vec = ((uintptr_t)b & 15u) == 0;
+        double acc = 0.0;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x * 2;
+        for (int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i0 < n; i0 += stride)
+        {
+            if (vec && i0 - m - 1 >= 0 && i0 + 2 + m < n)
+            {
+                double prod[2];
+                k::q1_rows_v<double>(B, x, i0, m, prod);
+                typedef k::Vec16<double>::type V2;
+                const V2 bv = __builtin_nontemporal_load((const V2 *)(b + i0));
+                V2 o;
+                o[0] = bv[0] - prod[0];
+                o[1] = bv[1] - prod[1];
+                *(V2 *)(od + i0) = o;
+                acc += o[0] * o[0];
+                acc += o[1] * o[1];
+            }
+            else
+            {
+                for (int64_t i = i0; i < i0 + 2 && i < n; ++i)
+                {
+                    const double prod = k::q1_row<double>(B, x, i, n, m);
+                    const double di = b[i] - prod;
+                    od[i] = di;
+                    acc += di * di;
+                }
+            }
+        }
+        k::finish2(acc, 0.0, sl.partials, &s->ticket, [&](double dd, double) {
+            const double nrm = sqrt(dd);
+            if (start)
+            {
+                s->initial_defect = nrm;
+                s->def_norm = nrm;
+                s->scale = 1.0;
+                s->inv = 1.0;
+                s->outer_iters = 0;
+                s->inner_total = 0;
+                s->phase = PH_SOLVE;
+                s->open = 1;
+            }
+            else
+            {
+                s->def_norm = nrm;
+                s->outer_iters += 1;
+                s->inner_total += s->inner_last;
+                s->need_outer = 0;
+                if (nrm < outer_eps * s->initial_defect)
+                {
+                    s->outer_active = 0;
+                    s->phase = PH_DONE;
+                }
+                else
+                {
+                    s->scale = nrm;
+                    s->inv = 1.0 / nrm;
+                    if (max_outer != 0 && s->outer_iters >= max_outer)
+                    {
+                        s->outer_active = 0;
+                        s->status = 1;
+                        s->phase = PH_DONE;
+                    }
+                    else
+                        s->open = 1;
+                }
+            }
+        });
+    }
+
+    // rhs32 = (float)(od * inv) (inv == 1 on the first step, so the cast of
+    // the unscaled defect); a start also clears the result array
+    __global__ __launch_bounds__(256) void to_inner_kernel(const Slot * T, int L, int64_t n)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl) || sl.s->open == 0)
+            return;
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n)
+            return;
+        const double inv = sl.s->inv;
+        sl.v[L][F_RHS][i] = (float)(sl.od[i] * inv);
+        if (sl.s->do_start)
+            sl.result[i] = 0.0;
+    }
+
+    // Opens the inner solve: its initial defect d is rhs itself when
+    // initial_zero (multigrid.hh:1626-1628), else rhs - A x.
+    template <int INITIAL_ZERO>
+    __global__ __launch_bounds__(256) void open_inner_kernel(k::Bands B, const Slot * T, int L, int64_t n, int64_t m)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl) || sl.s->open == 0)
+            return;
+        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const float * rhs = sl.v[L][F_RHS];
+        float * d = sl.v[L][F_D];
+        if (INITIAL_ZERO)
+        {
+            typedef k::Vec16<float>::type F4;
+            const int64_t i0 = lane * 4;
+            if (i0 + 4 <= n)
+                *(F4 *)(d + i0) = *(const F4 *)(rhs + i0);
+            else
+                for (int64_t i = i0; i < n; ++i)
+                    d[i] = rhs[i];
+        }
+        else
+            k::q1_body_v<float, 1>(B, sl.v[L][F_X], d, rhs, nullptr, n, m, lane);
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            sl.s->run = 1;
+            sl.s->need_outer = 0;
+        }
+    }
+
+    // |d| of the finest FP32 defect.  MODE 0: start of an inner solve (gate
+    // `open`); MODE 1: end of a V-cycle (gate `run`): count it, stop on
+    // |d| <= tol |d_0| or after n_max_iter cycles.
+    template <int MODE>
+    __global__ __launch_bounds__(256) void inner_norm_kernel(const Slot * T, int L, int64_t n, float tol,
+            int convergence_check, uint64_t n_max_iter)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl))
+            return;
+        Scalars * s = sl.s;
+        if (MODE == 0 ? s->open == 0 : s->run == 0)
+            return;
+        const float * __restrict__ d = sl.v[L][F_D];
+        double acc = 0.0;
+        typedef k::Vec16<float>::type F4;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t n4 = n >> 2;
+        const F4 * d4 = reinterpret_cast<const F4 *>(d);
+        int64_t i = tid;
+        for (; i + 3 * stride < n4; i += 4 * stride)
+        {
+            F4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                v[u] = __builtin_nontemporal_load(d4 + i + u * stride);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                acc += (double)v[u].x * (double)v[u].x;
+                acc += (double)v[u].y * (double)v[u].y;
+                acc += (double)v[u].z * (double)v[u].z;
+                acc += (double)v[u].w * (double)v[u].w;
+            }
+        }
+        for (; i < n4; i += stride)
+        {
+            const F4 v = __builtin_nontemporal_load(d4 + i);
+            acc += (double)v.x * (double)v.x;
+            acc += (double)v.y * (double)v.y;
+            acc += (double)v.z * (double)v.z;
+            acc += (double)v.w * (double)v.w;
+        }
+        for (int64_t t = 4 * n4 + tid; t < n; t += stride)
+        {
+            const double v = (double)d[t];
+            acc += v * v;
+        }
+        k::finish2(acc, 0.0, sl.partials, &s->ticket, [&](double ss, double) {
+            const float nrm = sqrtf((float)ss);
+            if (MODE == 0)
+            {
+                s->inner_initial = nrm;
+                s->inner_defect = nrm;
+                if (convergence_check && nrm <= tol)
+                {
+                    // nothing to do; the count keeps the previous solve's value
+                    s->run = 0;
+                    s->need_outer = 1;
+                }
+                else
+                    s->inner_last = 0;
+            }
+            else
+            {
+                s->inner_defect = nrm;
+                s->inner_last += 1;
+                if (nrm <= s->inner_initial * tol || s->inner_last >= n_max_iter)
+                {
+                    s->run = 0;
+                    s->need_outer = 1;
+                }
+            }
+        });
+    }
+
+    // one-level hierarchy: the cycle loop is left after the coarse solve
+    __global__ void one_level_end_kernel(const Slot * T, unsigned nslots)
+    {
+        const unsigned z = blockIdx.x * blockDim.x + threadIdx.x;
+        if (z >= nslots)
+            return;
+        Scalars * s = T[z].s;
+        if (s == nullptr || s->run == 0)
+            return;
+        s->inner_last += 1;
+        s->run = 0;
+        s->need_outer = 1;
+    }
+
+    // --------------------------------------------------------------- V-cycle
+    struct LevelArgs
+    {
+        k::Bands B;
+        const float * dinv;
+        int64_t n, m;
+    };
+
+#define HS_GATE_RUN                                     \
+    const Slot & sl = T[blockIdx.z];                    \
+    if (! live(sl) || sl.s->run == 0)                   \
+        return;
+
+    // x (+)= dinv * d: the single first sweep when n_pre_smooth == 1
+    template <int ACC>
+    __global__ __launch_bounds__(256) void v_zero1_kernel(LevelArgs a, const Slot * T, int L, int fsrc, int fout)
+    {
+        HS_GATE_RUN
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= a.n)
+            return;
+        const float * d = sl.v[L][fsrc];
+        float * out = sl.v[L][fout];
+        const float c = a.dinv[i] * d[i];
+        out[i] = ACC ? out[i] + c : c;
+    }
+
+    template <int ACC>
+    __global__ __launch_bounds__(256) void v_zero2_kernel(LevelArgs a, const Slot * T, int L, int fsrc, int fout)
+    {
+        HS_GATE_RUN
+        k::jacobi_zero2_body<float, ACC>(a.B, sl.v[L][fsrc], a.dinv, sl.v[L][fout], a.n, a.m,
+                                         (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    }
+
+    // y = x + dinv * (rhs - A x)
+    __global__ __launch_bounds__(256) void v_sweep_kernel(LevelArgs a, const Slot * T, int L, int fx, int fy, int frhs)
+    {
+        HS_GATE_RUN
+        k::q1_body_v<float, 2>(a.B, sl.v[L][fx], sl.v[L][fy], sl.v[L][frhs], a.dinv, a.n, a.m,
+                               (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    }
+
+    // d = rhs - A x
+    __global__ __launch_bounds__(256) void v_defect_kernel(LevelArgs a, const Slot * T, int L)
+    {
+        HS_GATE_RUN
+        k::q1_body_v<float, 1>(a.B, sl.v[L][F_X], sl.v[L][F_D], sl.v[L][F_RHS], nullptr, a.n, a.m,
+                               (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    }
+
+    // dst = src (ADD 0) or dst += src (ADD 1), four entries per lane
+    template <int ADD>
+    __global__ __launch_bounds__(256) void v_move_kernel(const Slot * T, int L, int64_t n, int fsrc, int fdst)
+    {
+        HS_GATE_RUN
+        typedef k::Vec16<float>::type F4;
+        const float * src = sl.v[L][fsrc];
+        float * dst = sl.v[L][fdst];
+        const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+        if (i0 + 4 <= n)
+        {
+            const F4 s = *(const F4 *)(src + i0);
+            if (ADD)
+            {
+                F4 d = *(const F4 *)(dst + i0);
+                d.x = d.x + s.x;
+                d.y = d.y + s.y;
+                d.z = d.z + s.z;
+                d.w = d.w + s.w;
+                *(F4 *)(dst + i0) = d;
+            }
+            else
+                *(F4 *)(dst + i0) = s;
+        }
+        else
+            for (int64_t i = i0; i < n; ++i)
+                dst[i] = ADD ? dst[i] + src[i] : src[i];
+    }
+
+    // rhs of level L-1 = restriction of d of level L
+    __global__ __launch_bounds__(256) void v_restrict_kernel(const Slot * T, int L, int64_t Nc, k::Mask mk)
+    {
+        HS_GATE_RUN
+        k::restriction_body<float>(sl.v[L - 1][F_RHS], sl.v[L][F_D], Nc, mk, blockIdx.y,
+                                   (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    }
+
+    // ConjugateGradients<NONE> (konv_rad form) on the coarsest level
+    __global__ __launch_bounds__(256) void v_coarse_kernel(LevelArgs a, const Slot * T, int L, int frhs,
+                                                           double konv_rad, uint64_t max_iters)
+    {
+        HS_GATE_RUN
+        k::cg_konv_body<float>(a.B, sl.v[L][frhs], sl.v[L][F_X], sl.cg_work, a.n, a.m, konv_rad, max_iters,
+                               &sl.s->cg_iters);
+    }
+
+    // level L's fout (+)= alpha * P (x of level L-1)
+    template <bool AXPY>
+    __global__ __launch_bounds__(256) void v_prolong_kernel(const Slot * T, int L, int Nf, float alpha, k::Mask mk,
+                                                            int fout)
+    {
+        HS_GATE_RUN
+        k::prolongation2_body<float, AXPY>(sl.v[L][fout], sl.v[L - 1][F_X], Nf, alpha, mk, blockIdx.y,
+                                           blockIdx.x * blockDim.x + threadIdx.x);
+    }
+
+    // adapt_correction_factor == 0: alpha = (d.c) / ((A c).c)
+    __global__ __launch_bounds__(256) void v_adapt_alpha_kernel(LevelArgs a, const Slot * T, int L)
+    {
+        HS_GATE_RUN
+        const float * d = sl.v[L][F_D];
+        const float * c = sl.v[L][F_C];
+        double a1 = 0.0, a2 = 0.0;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride)
+        {
+            const float t = k::q1_row<float>(a.B, c, i, a.n, a.m);
+            a1 += (double)d[i] * (double)c[i];
+            a2 += (double)t * (double)c[i];
+        }
+        Scalars * s = sl.s;
+        k::finish2(a1, a2, sl.partials, &s->ticket, [&](double d1, double d2) { s->alpha = (float)d1 / (float)d2; });
+    }
+
+    __global__ __launch_bounds__(256) void v_axpy_alpha_kernel(const Slot * T, int L, int64_t n)
+    {
+        HS_GATE_RUN
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n)
+            return;
+        const float alpha = sl.s->alpha;
+        float * x = sl.v[L][F_X];
+        const float * c = sl.v[L][F_C];
+        x[i] = x[i] + alpha * c[i];
+    }
+#undef HS_GATE_RUN
+
+    __global__ void report_kernel(const Slot * T, unsigned nslots, Report * out)
+    {
+        const unsigned z = blockIdx.x * blockDim.x + threadIdx.x;
+        if (z >= nslots)
+            return;
+        const Scalars * s = T[z].s;
+        Report r;
+        if (s == nullptr)
+        {
+            r.phase = PH_IDLE;
+            r.status = 0;
+            r.outer_iters = r.inner_total = r.cg_iters = r.ticks = 0;
+            r.initial_defect = r.def_norm = 0.0;
+        }
+        else
+        {
+            r.phase = s->phase;
+            r.status = s->status;
+            r.outer_iters = s->outer_iters;
+            r.inner_total = s->inner_total;
+            r.cg_iters = s->cg_iters;
+            r.ticks = s->ticks;
+            r.initial_defect = s->initial_defect;
+            r.def_norm = s->def_norm;
+        }
+        out[z] = r;
+    }
+
+    // ----------------------------------------------------------------- setup
+    __host__ __device__ inline uint64_t mix64(uint64_t z)
+    {
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+
+    // b_i = -1 + 2 u_i, u_i = (bits_i >> 11) 2^-53 with SplitMix64 counters;
+    // 0 on the boundary nodes of the square grid
+    __global__ __launch_bounds__(256) void rhs_uniform_kernel(double * b, int64_t n, int64_t sq, uint64_t key)
+    {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n)
+            return;
+        const int64_t r = i / sq, c = i - r * sq;
+        if (r == 0 || r == sq - 1 || c == 0 || c == sq - 1)
+        {
+            b[i] = 0.0;
+            return;
+        }
+        const uint64_t bits = mix64(key + 0x9E3779B97F4A7C15ull * ((uint64_t)i + 1));
+        const double u = (double)(bits >> 11) * 0x1.0p-53;
+        b[i] = -1.0 + 2.0 * u;
+    }
+
+    // FillMatrix<POISSON, DIRICHLET_0> (fill_matrix.hh:37-76)
+    template <typename DT>
+    __global__ void fill_poisson_kernel(DT * b0, uint64_t bs, int64_t N, int64_t M)
+    {
+        const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= N)
+            return;
+        const int64_t i = k + 1;
+        const bool bnd = (i <= M || i > N - M || (i % M) == 1 || (i % M) == 0);
+        const DT off = bnd ? DT(0.0) : DT(-1.0 / 3.0);
+        for (int b = 0; b < 9; ++b)
+            b0[(uint64_t)b * bs + k] = (b == 4) ? (bnd ? DT(1.0) : DT(8.0 / 3.0)) : off;
+    }
+
+    // dinv = (float)((double)(1/dd) * 0.7): ElementInverse then Scale by a
+    // double factor, as the reference tests build MGInfo::diags_inverted
+    __global__ void dinv_kernel(const float * __restrict__ dd, float * __restrict__ dinv, int64_t n)
+    {
+        const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= n)
+            return;
+        float v = dd[k];
+        if (v != 0.0f)
+            v = 1.0f / v;
+        dinv[k] = (float)((double)v * 0.7);
+    }
+
+    inline uint64_t align256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
+    inline unsigned lanes4(uint64_t n) { return grid_for((n + 3) / 4, kBlock); }
+}  // namespace
+
+void * Hierarchy::alloc(size_t b)
+{
+    void * ptr = nullptr;
+    hipError_t e = hipMalloc(&ptr, b ? b : 1);
+    if (e != hipSuccess)
+        throw Error(HSOLVE_ERR_ALLOC, "MG hierarchy allocation of " + std::to_string(b) + " bytes failed");
+    allocations.push_back(ptr);
+    bytes += b;
+    return ptr;
+}
+
+Hierarchy::~Hierarchy()
+{
+    for (void * a : allocations)
+        (void)hipFree(a);
+}
+
+Params make_params(const hsolve_mg_params & p)
+{
+    Params q;
+    q.min_level = p.min_level;
+    q.n_max_iter = p.n_max_iter;
+    q.initial_zero = p.initial_zero;
+    q.convergence_check = p.convergence_check;
+    q.tolerance = (float)p.tolerance;
+    q.n_pre_smooth = p.n_pre_smooth;
+    q.adapt = (float)p.adapt_correction_factor;
+    for (int i = 0; i < 8; ++i)
+        q.mask.dir[i] = (p.macro_border_mask[i] == 2) ? 1 : 0;
+    q.max_outer = p.max_outer;
+    return q;
+}
+
+void hierarchy_alloc(Hierarchy & h, const Params & p, uint64_t max_level)
+{
+    require(p.min_level >= 1 && p.min_level <= max_level && max_level < (uint64_t)kMaxLevels, HSOLVE_ERR_INVALID,
+            "mg: need 1 <= min_level <= max_level <= 14");
+    require(p.n_pre_smooth >= 1, HSOLVE_ERR_INVALID, "mg: n_pre_smooth must be >= 1");
+    require(p.n_max_iter >= 1, HSOLVE_ERR_INVALID, "mg: n_max_iter must be >= 1");
+    require(size_of(p.min_level) <= 16641, HSOLVE_ERR_INVALID, "mg: coarsest level above 129^2");
+    h.p = p;
+    h.max_level = max_level;
+    for (uint64_t L = p.min_level; L <= max_level; ++L)
+    {
+        HierLevel & l = h.lv[L];
+        l.side = side_of(L);
+        l.n = size_of(L);
+        l.bs = band_stride(l.n);
+        l.bands = (float *)h.alloc(sizeof(float) * 9 * l.bs);
+        l.dinv = (float *)h.alloc(sizeof(float) * l.bs);
+    }
+}
+
+void hierarchy_alloc64(Hierarchy & h, uint64_t L)
+{
+    require(L >= h.p.min_level && L <= h.max_level, HSOLVE_ERR_INVALID, "mg: FP64 level outside the hierarchy");
+    if (h.bands64[L])
+        return;
+    h.bs64[L] = band_stride(size_of(L));
+    h.bands64[L] = (double *)h.alloc(sizeof(double) * 9 * h.bs64[L]);
+}
+
+void hierarchy_dinv(Hierarchy & h, uint64_t L, hipStream_t st)
+{
+    HierLevel & l = h.lv[L];
+    hipLaunchKernelGGL(dinv_kernel, dim3(grid_for(l.n, kBlock)), dim3(kBlock), 0, st,
+                       (const float *)(l.bands + 4 * l.bs), l.dinv, (int64_t)l.n);
+    HIP_CHECK(hipGetLastError());
+}
+
+void hierarchy_fill_poisson(Hierarchy & h, hipStream_t st)
+{
+    for (uint64_t L = h.p.min_level; L <= h.max_level; ++L)
+    {
+        HierLevel & l = h.lv[L];
+        hipLaunchKernelGGL((fill_poisson_kernel<float>), dim3(grid_for(l.n, kBlock)), dim3(kBlock), 0, st, l.bands,
+                           l.bs, (int64_t)l.n, l.side);
+        hierarchy_dinv(h, L, st);
+        if (h.bands64[L])
+            hipLaunchKernelGGL((fill_poisson_kernel<double>), dim3(grid_for(l.n, kBlock)), dim3(kBlock), 0, st,
+                               h.bands64[L], h.bs64[L], (int64_t)l.n, l.side);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+uint64_t slot_bytes(const Hierarchy & h, uint64_t L, bool own_b, bool own_result)
+{
+    const uint64_t n = size_of(L);
+    uint64_t b = align256(sizeof(Scalars)) + align256(sizeof(double) * 2 * kNormBlocks);
+    b += align256(sizeof(double) * n);                                  // od
+    if (own_b)
+        b += align256(sizeof(double) * n);
+    if (own_result)
+        b += align256(sizeof(double) * n);
+    b += align256(sizeof(float) * 4 * size_of(h.p.min_level));          // coarse CG
+    for (uint64_t l = h.p.min_level; l <= L; ++l)
+        b += 5 * align256(sizeof(float) * size_of(l));
+    return b;
+}
+
+Slot slot_carve(const Hierarchy & h, uint64_t L, char * base, bool own_b, bool own_result)
+{
+    Slot sl;
+    std::memset(&sl, 0, sizeof(sl));
+    const uint64_t n = size_of(L);
+    char * c = base;
+    auto take = [&](uint64_t bytes) {
+        char * r = c;
+        c += align256(bytes);
+        return r;
+    };
+    sl.s = (Scalars *)take(sizeof(Scalars));
+    sl.partials = (double *)take(sizeof(double) * 2 * kNormBlocks);
+    sl.od = (double *)take(sizeof(double) * n);
+    if (own_b)
+        sl.b = (const double *)take(sizeof(double) * n);
+    if (own_result)
+        sl.result = (double *)take(sizeof(double) * n);
+    sl.cg_work = (float *)take(sizeof(float) * 4 * size_of(h.p.min_level));
+    for (uint64_t l = h.p.min_level; l <= L; ++l)
+        for (int f = 0; f < 5; ++f)
+            sl.v[l][f] = (float *)take(sizeof(float) * size_of(l));
+    return sl;
+}
+
+Scalars start_scalars(bool zero_guess)
+{
+    Scalars s;
+    std::memset(&s, 0, sizeof(s));
+    s.phase = PH_START;
+    s.outer_active = 1;
+    s.zero_guess = zero_guess ? 1 : 0;
+    return s;
+}
+
+void enqueue_rhs_uniform(double * b, uint64_t L, uint64_t seed, uint64_t stream, hipStream_t st)
+{
+    const uint64_t key = mix64(seed + 0x9E3779B97F4A7C15ull * (stream + 1));
+    const uint64_t n = size_of(L);
+    hipLaunchKernelGGL(rhs_uniform_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, b, (int64_t)n,
+                       side_of(L), key);
+    HIP_CHECK(hipGetLastError());
+}
+
+void enqueue_report(const Slot * table, unsigned nslots, Report * reports, hipStream_t st)
+{
+    hipLaunchKernelGGL(report_kernel, dim3(grid_for(nslots, 64)), dim3(64), 0, st, table, nslots, reports);
+    HIP_CHECK(hipGetLastError());
+}
+
+void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, hipStream_t st)
+{
+    const Params & p = h.p;
+    const uint64_t minl = p.min_level;
+    require(maxl >= minl && maxl <= h.max_level && h.bands64[maxl] != nullptr, HSOLVE_ERR_INVALID,
+            "mg tick: level not prepared");
+    const HierLevel & top = h.lv[maxl];
+    const int64_t n = (int64_t)top.n;
+    const dim3 blk(kBlock);
+    auto largs = [&](uint64_t L) {
+        const HierLevel & l = h.lv[L];
+        LevelArgs a;
+        a.B = l.B();
+        a.dinv = l.dinv;
+        a.n = (int64_t)l.n;
+        a.m = l.side;
+        return a;
+    };
+    auto norm_grid = [&](uint64_t nn) { return dim3(std::min<unsigned>(lanes4(nn), kNormBlocks), 1, S); };
+
+    // ---- gates, start of new solves, FP64 outer step, opening of inner solves
+    hipLaunchKernelGGL(plan_kernel, dim3(grid_for(S, 64)), dim3(64), 0, st, T, S);
+    hipLaunchKernelGGL(start_init_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n,
+                       top.side);
+    if (maxl > minl)
+        hipLaunchKernelGGL(start_zero_kernel, dim3(grid_for(h.lv[maxl - 1].n, kBlock), 1, S), blk, 0, st, T,
+                           (int)minl, (int)maxl);
+    hipLaunchKernelGGL(outer_accumulate_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
+    hipLaunchKernelGGL(outer_defect_kernel, dim3(std::min<unsigned>(grid_for((top.n + 1) / 2, kBlock), kNormBlocks), 1, S),
+                       blk, 0, st, h.B64(maxl), T, n, top.side, (double)p.tolerance, p.max_outer);
+    hipLaunchKernelGGL(to_inner_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
+    if (p.initial_zero)
+        hipLaunchKernelGGL((open_inner_kernel<1>), dim3(lanes4(top.n), 1, S), blk, 0, st, top.B(), T, (int)maxl, n,
+                           top.side);
+    else
+        hipLaunchKernelGGL((open_inner_kernel<0>), dim3(lanes4(top.n), 1, S), blk, 0, st, top.B(), T, (int)maxl, n,
+                           top.side);
+    hipLaunchKernelGGL((inner_norm_kernel<0>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n, p.tolerance,
+                       p.convergence_check, p.n_max_iter);
+
+    // ---- one V-cycle (multigrid.hh:1660-1905)
+    const uint64_t npre = p.n_pre_smooth;
+    auto sweeps = [&](uint64_t L, int & xc, int & xa, int frhs, uint64_t count) {
+        const LevelArgs a = largs(L);
+        for (uint64_t i = 0; i < count; ++i)
+        {
+            hipLaunchKernelGGL(v_sweep_kernel, dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, xc, xa, frhs);
+            std::swap(xc, xa);
+        }
+    };
+    auto move = [&](uint64_t L, int fsrc, int fdst, bool add) {
+        const int64_t nn = (int64_t)h.lv[L].n;
+        if (add)
+            hipLaunchKernelGGL((v_move_kernel<1>), dim3(lanes4(nn), 1, S), blk, 0, st, T, (int)L, nn, fsrc, fdst);
+        else
+            hipLaunchKernelGGL((v_move_kernel<0>), dim3(lanes4(nn), 1, S), blk, 0, st, T, (int)L, nn, fsrc, fdst);
+    };
+    auto defect = [&](uint64_t L) {
+        const LevelArgs a = largs(L);
+        hipLaunchKernelGGL(v_defect_kernel, dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L);
+    };
+
+    if (minl != maxl)
+    {
+        for (uint64_t L = maxl; L > minl; --L)
+        {
+            const LevelArgs a = largs(L);
+            // the finest level's defect is d; coarser levels' defect is the
+            // restricted one in rhs
+            const int dsrc = (L == maxl) ? F_D : F_RHS;
+            if (L == maxl)
+            {
+                // c = dinv*d, (npre-1) sweeps on c with rhs d, x += c
+                if (npre == 1)
+                    hipLaunchKernelGGL((v_zero1_kernel<1>), dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, a, T,
+                                       (int)L, dsrc, (int)F_X);
+                else if (npre == 2)
+                    hipLaunchKernelGGL((v_zero2_kernel<1>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
+                                       (int)F_X);
+                else
+                {
+                    hipLaunchKernelGGL((v_zero2_kernel<0>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
+                                       (int)F_C);
+                    int cc = F_C, ca = F_TMP;
+                    sweeps(L, cc, ca, dsrc, npre - 2);
+                    move(L, cc, F_X, true);
+                }
+            }
+            else
+            {
+                // x = dinv*d, (npre-1) sweeps on x with rhs d
+                if (npre == 1)
+                    hipLaunchKernelGGL((v_zero1_kernel<0>), dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, a, T,
+                                       (int)L, dsrc, (int)F_X);
+                else
+                {
+                    hipLaunchKernelGGL((v_zero2_kernel<0>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
+                                       (int)F_X);
+                    int xc = F_X, xa = F_TMP;
+                    sweeps(L, xc, xa, dsrc, npre - 2);
+                    if (xc != F_X)
+                        move(L, xc, F_X, false);
+                }
+            }
+            defect(L);
+            const int64_t Nc = h.lv[L - 1].side;
+            hipLaunchKernelGGL(v_restrict_kernel, dim3(grid_for(Nc, kBlock), (unsigned)Nc, S), blk, 0, st, T, (int)L,
+                               Nc, p.mask);
+        }
+    }
+    {
+        // coarse grid solve: ConjugateGradients<NONE>(A, d, x, FLT_EPSILON);
+        // with one level it sees the finest defect d
+        const LevelArgs a = largs(minl);
+        const int crhs = (minl == maxl) ? F_D : F_RHS;
+        hipLaunchKernelGGL(v_coarse_kernel, dim3(1, 1, S), blk, 0, st, a, T, (int)minl, crhs,
+                           (double)1.19209290e-07F, (uint64_t)(100 * a.n + 1000));
+        if (minl == maxl)
+        {
+            defect(minl);
+            hipLaunchKernelGGL(one_level_end_kernel, dim3(grid_for(S, 64)), dim3(64), 0, st, T, S);
+            HIP_CHECK(hipGetLastError());
+            return;
+        }
+    }
+    for (uint64_t L = minl + 1; L <= maxl; ++L)
+    {
+        const LevelArgs a = largs(L);
+        const int Nf = (int)h.lv[L].side;
+        const int Nc = (int)h.lv[L - 1].side;
+        const dim3 pg(grid_for((uint64_t)Nc, kBlock), (unsigned)Nf, S);
+        if (p.adapt != 0.0f)
+            hipLaunchKernelGGL((v_prolong_kernel<true>), pg, blk, 0, st, T, (int)L, Nf, p.adapt, p.mask, (int)F_X);
+        else
+        {
+            hipLaunchKernelGGL((v_prolong_kernel<false>), pg, blk, 0, st, T, (int)L, Nf, 0.0f, p.mask, (int)F_C);
+            hipLaunchKernelGGL(v_adapt_alpha_kernel, dim3(std::min<unsigned>(grid_for(a.n, kBlock), kNormBlocks), 1, S),
+                               blk, 0, st, a, T, (int)L);
+            hipLaunchKernelGGL(v_axpy_alpha_kernel, dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, T, (int)L, a.n);
+        }
+        // post-smoothing: n_pre_smooth sweeps (multigrid.hh:1851)
+        int xc = F_X, xa = F_TMP;
+        sweeps(L, xc, xa, F_RHS, npre);
+        if (xc != F_X)
+            move(L, xc, F_X, false);
+        if (L == maxl)
+            defect(L);
+    }
+    hipLaunchKernelGGL((inner_norm_kernel<1>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n, p.tolerance,
+                       p.convergence_check, p.n_max_iter);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace mg
+}  // namespace hsolve

@@ -74,16 +74,12 @@ namespace hsolve
             return;
         const int64_t m = (int64_t)std::sqrt((double)n);
         require((uint64_t)(m * m) == n, HSOLVE_ERR_SIZE, "Q1 size is not a square");
-        const unsigned grid = grid_for(n, kBlock);
         if (mode == 0)
-            hipLaunchKernelGGL((k::q1_kernel<DT, 0>), dim3(grid), dim3(kBlock), 0, st, B, x, y, rhs, dinv,
-                               (int64_t)n, m, (const int *)nullptr);
+            k::q1_launch<DT, 0>(B, x, y, rhs, dinv, (int64_t)n, m, nullptr, st);
         else if (mode == 1)
-            hipLaunchKernelGGL((k::q1_kernel<DT, 1>), dim3(grid), dim3(kBlock), 0, st, B, x, y, rhs, dinv,
-                               (int64_t)n, m, (const int *)nullptr);
+            k::q1_launch<DT, 1>(B, x, y, rhs, dinv, (int64_t)n, m, nullptr, st);
         else
-            hipLaunchKernelGGL((k::q1_kernel<DT, 2>), dim3(grid), dim3(kBlock), 0, st, B, x, y, rhs, dinv,
-                               (int64_t)n, m, (const int *)nullptr);
+            k::q1_launch<DT, 2>(B, x, y, rhs, dinv, (int64_t)n, m, nullptr, st);
         HIP_CHECK(hipGetLastError());
     }
 
@@ -141,6 +137,8 @@ namespace hsolve
         const int64_t M = (int64_t)std::sqrt((double)n);
         require(M >= 2 && (uint64_t)(M * M) == n && ((M - 1) & (M - 2)) == 0, HSOLVE_ERR_SIZE,
                 std::string(what) + ": size is not (2^L+1)^2");
+        // the transfer kernels put grid rows on gridDim.y (at most 65535)
+        require(M <= 32769, HSOLVE_ERR_SIZE, std::string(what) + ": grid side above 2^15+1");
         return M;
     }
 
@@ -150,7 +148,7 @@ namespace hsolve
     {
         const int64_t Mc = grid_side(nc, "restriction coarse"), Mf = grid_side(nf, "restriction fine");
         require(Mf == 2 * Mc - 1, HSOLVE_ERR_SIZE, "restriction: fine side != 2*coarse side - 1");
-        hipLaunchKernelGGL((k::restriction_kernel<DT>), dim3(grid_for(nc, kBlock)), dim3(kBlock), 0, st, coarse,
+        hipLaunchKernelGGL((k::restriction_kernel<DT>), dim3(grid_for(Mc, kBlock), Mc), dim3(kBlock), 0, st, coarse,
                            fine, Mc, mask_of(mask), (const int *)nullptr);
         HIP_CHECK(hipGetLastError());
     }
@@ -161,8 +159,8 @@ namespace hsolve
     {
         const int64_t Mc = grid_side(nc, "prolongation coarse"), Mf = grid_side(nf, "prolongation fine");
         require(Mf == 2 * Mc - 1, HSOLVE_ERR_SIZE, "prolongation: fine side != 2*coarse side - 1");
-        hipLaunchKernelGGL((k::prolongation_kernel<DT, AXPY>), dim3(grid_for(nf, kBlock)), dim3(kBlock), 0, st,
-                           fine, coarse, Mf, alpha, mask_of(mask), (const int *)nullptr);
+        hipLaunchKernelGGL((k::prolongation2_kernel<DT, AXPY>), dim3(grid_for(Mc, kBlock), Mf), dim3(kBlock), 0, st,
+                           fine, coarse, (int)Mf, alpha, mask_of(mask), (const int *)nullptr);
         HIP_CHECK(hipGetLastError());
     }
 
