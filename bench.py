@@ -134,6 +134,65 @@ def mg_case(hip, P, level, min_level, reps, tol=1e-8):
                 solves_per_s=float(1e3 / lat.mean()), device_bytes=mg.device_bytes), x
 
 
+def serve_trace(count, rate, seed):
+    """Served stream: side = nearest 2^k+1 to a log-uniform draw on [65, 2049]
+    (k = 6..11), Poisson arrivals at `rate` per second (None: all at t = 0),
+    one RHS seed per request."""
+    rng = np.random.default_rng(seed)
+    s = np.exp(rng.uniform(np.log(65.0), np.log(2049.0), count))
+    levels = np.clip(np.rint(np.log2(s - 1.0)), 6, 11).astype(np.uint64)
+    if rate is None:
+        arrivals = np.zeros(count)
+    else:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+        arrivals -= arrivals[0]
+    seeds = rng.integers(1, 1 << 62, count, dtype=np.uint64)
+    return arrivals, levels, seeds
+
+
+def serve_summary(res, arrivals, levels):
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res]) * 1e3
+    wait = np.array([r["t_admit"] - r["t_submit"] for r in res]) * 1e3
+    span = max(r["t_done"] for r in res) - float(np.min(arrivals))
+    rel = np.array([r["final_defect"] / r["initial_defect"] for r in res])
+    per_level = {}
+    for L in sorted(set(int(v) for v in levels)):
+        m = levels == L
+        per_level[f"{(1 << L) + 1}^2"] = dict(
+            requests=int(m.sum()), p50_ms=round(float(np.percentile(lat[m], 50)), 3),
+            p99_ms=round(float(np.percentile(lat[m], 99)), 3),
+            outer_mean=round(float(np.mean([res[i]["outer"] for i in np.nonzero(m)[0]])), 2),
+            inner_mean=round(float(np.mean([res[i]["inner"] for i in np.nonzero(m)[0]])), 2))
+    return dict(requests=len(res), solves_per_s=round(len(res) / span, 1), makespan_s=round(span, 4),
+                p50_ms=round(float(np.percentile(lat, 50)), 3), p99_ms=round(float(np.percentile(lat, 99)), 3),
+                admit_wait_p50_ms=round(float(np.percentile(wait, 50)), 3),
+                admit_wait_p99_ms=round(float(np.percentile(wait, 99)), 3),
+                max_rel_defect=float(rel.max()), all_converged=bool(all(r["status"] == 0 for r in res)),
+                per_level=per_level)
+
+
+def serve_case(hip, count, budget_gib=32, max_slots=64, load=0.7, seed=11):
+    """Config: a served stream of `count` Poisson Q1 MG-IR solves (sides 65..2049,
+    FP32 V-cycles + FP64 refinement to rel. residual 1e-8, device-generated
+    seeded U[-1,1] RHS).  First all requests at t = 0 (saturation
+    throughput), then Poisson arrivals at `load` x that rate (latency)."""
+    srv = hip.Server(hip.serve_params(min_level=2, min_served_level=6, max_served_level=11, n_max_iter=2,
+                                      tolerance=1e-8, memory_budget=budget_gib << 30, max_slots=max_slots,
+                                      use_graphs=True))
+    a, l, s = serve_trace(min(count, 256), None, seed + 1)
+    srv.run(a, l, s)                                           # warm-up: graph captures, slot caches
+    a, l, s = serve_trace(count, None, seed)
+    burst = serve_summary(srv.run(a, l, s), a, l)
+    rate = load * burst["solves_per_s"]
+    a, l, s = serve_trace(count, rate, seed + 2)
+    steady = serve_summary(srv.run(a, l, s), a, l)
+    st = srv.stats()
+    return dict(burst=burst, poisson_arrivals=dict(rate_per_s=round(rate, 1), **steady),
+                memory_budget_bytes=st["budget"], shared_bytes=st["bytes_shared"],
+                peak_running=st["peak_running"], max_slots_per_level=max_slots,
+                graph_captures=st["graph_captures"], ticks=st["ticks"])
+
+
 def cg_case(hip, P, side, max_iters, tol=1e-8):
     n = side * side
     A = hip.Q1Matrix(P.poisson_q1_bands(n))
@@ -180,7 +239,13 @@ def main():
                     help="also run a plain 2 GiB device copy kernel (for byte-counter calibration)")
     ap.add_argument("--mg-only", type=int, default=0, metavar="REPS",
                     help="only time REPS MG-IR solves on the 4097^2 Poisson problem (for profiling)")
+    ap.add_argument("--serve-only", type=int, default=0, metavar="REQUESTS",
+                    help="only run the served mixed-size stream with REQUESTS requests")
     args = ap.parse_args()
+    if args.serve_only:
+        from honei_amd import device as hip
+        print(json.dumps({"serve_mixed_poisson": serve_case(hip, args.serve_only)}))
+        return
     if args.mg_only:
         from honei_amd import device as hip
         from honei_amd import problems as P

@@ -119,7 +119,46 @@ _SIGS = {
     "hsolve_mg_destroy": [vp],
     "hsolve_mg_device_bytes": [vp, C.POINTER(u64)],
     "hsolve_mg_solve": [vp, vp, vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl), vp],
+    "hsolve_serve_create": [C.c_void_p, C.POINTER(vp)],
+    "hsolve_serve_destroy": [vp],
+    "hsolve_serve_request_bytes": [vp, u64, C.POINTER(u64)],
+    "hsolve_serve_submit": [vp, u64, vp, u64, vp, u64, dbl, C.POINTER(u64)],
+    "hsolve_serve_step": [vp, C.POINTER(u64)],
+    "hsolve_serve_poll": [vp, vp, u64, C.POINTER(u64)],
+    "hsolve_serve_now": [vp, C.POINTER(dbl)],
+    "hsolve_serve_get_stats": [vp, vp],
+    "hsolve_serve_run": [vp, u64, vp, vp, vp, vp, vp],
 }
+
+
+class ServeParams(C.Structure):
+    """hsolve_serve_params."""
+    _fields_ = [
+        ("min_level", u64), ("min_served_level", u64), ("max_served_level", u64),
+        ("n_max_iter", u64), ("initial_zero", i32), ("convergence_check", i32),
+        ("tolerance", dbl), ("n_pre_smooth", u64), ("adapt_correction_factor", dbl),
+        ("macro_border_mask", u64 * 8), ("max_outer", u64),
+        ("memory_budget", u64), ("max_slots", u64), ("use_graphs", i32),
+    ]
+
+
+class ServeResult(C.Structure):
+    """hsolve_serve_result."""
+    _fields_ = [
+        ("id", u64), ("tag", u64), ("level", u64), ("status", i32),
+        ("outer_iters", u64), ("inner_iters", u64), ("ticks", u64),
+        ("initial_defect", dbl), ("final_defect", dbl),
+        ("t_submit", dbl), ("t_admit", dbl), ("t_done", dbl),
+    ]
+
+
+class ServeStats(C.Structure):
+    """hsolve_serve_stats."""
+    _fields_ = [
+        ("bytes_in_use", u64), ("bytes_held", u64), ("bytes_shared", u64), ("budget", u64),
+        ("submitted", u64), ("admitted", u64), ("completed", u64), ("queued", u64),
+        ("running", u64), ("peak_running", u64), ("ticks", u64), ("graph_captures", u64),
+    ]
 
 
 def exported_names():

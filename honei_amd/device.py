@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import MGParams, check, lib, u64, vp
+from ._lib import MGParams, ServeParams, ServeResult, ServeStats, check, lib, u64, vp
 
 _DT = {np.dtype(np.float32): 0, np.dtype(np.float64): 1}
 
@@ -327,6 +327,89 @@ class MGSolver:
         try:
             if getattr(self, "handle", None):
                 lib().hsolve_mg_destroy(vp(self.handle))
+                self.handle = None
+        except Exception:
+            pass
+
+
+def serve_params(min_level=2, min_served_level=6, max_served_level=11, n_max_iter=2, tolerance=1e-8,
+                 n_pre_smooth=2, adapt=1.0, mask=(2,) * 8, initial_zero=True, convergence_check=False,
+                 max_outer=0, memory_budget=16 << 30, max_slots=64, use_graphs=True):
+    return ServeParams(min_level, min_served_level, max_served_level, n_max_iter, int(initial_zero),
+                       int(convergence_check), tolerance, n_pre_smooth, adapt, (u64 * 8)(*mask), max_outer,
+                       memory_budget, max_slots, int(use_graphs))
+
+
+def _result_dict(r):
+    return dict(id=r.id, tag=r.tag, level=r.level, status=r.status, outer=r.outer_iters, inner=r.inner_iters,
+                ticks=r.ticks, initial_defect=r.initial_defect, final_defect=r.final_defect,
+                t_submit=r.t_submit, t_admit=r.t_admit, t_done=r.t_done)
+
+
+class Server:
+    """Request server for Poisson Q1 MG-IR solves of mixed sizes (hsolve_serve_*)."""
+
+    def __init__(self, params: ServeParams):
+        self.params = params
+        h = vp()
+        check(lib().hsolve_serve_create(C.byref(params), C.byref(h)), "serve_create")
+        self.handle = h.value
+
+    def request_bytes(self, level):
+        b = u64()
+        check(lib().hsolve_serve_request_bytes(vp(self.handle), u64(level), C.byref(b)), "serve_request_bytes")
+        return b.value
+
+    def submit(self, level, b: DeviceVector = None, seed=0, x: DeviceVector = None, tag=0, t_submit=-1.0):
+        i = u64()
+        check(lib().hsolve_serve_submit(vp(self.handle), u64(level), vp(b.ptr if b is not None else None),
+                                        u64(seed), vp(x.ptr if x is not None else None), u64(tag),
+                                        C.c_double(t_submit), C.byref(i)), "serve_submit")
+        return i.value
+
+    def step(self):
+        busy = u64()
+        check(lib().hsolve_serve_step(vp(self.handle), C.byref(busy)), "serve_step")
+        return busy.value
+
+    def poll(self, max_results=256):
+        out = (ServeResult * max_results)()
+        n = u64()
+        check(lib().hsolve_serve_poll(vp(self.handle), out, u64(max_results), C.byref(n)), "serve_poll")
+        return [_result_dict(out[k]) for k in range(n.value)]
+
+    def stats(self):
+        s = ServeStats()
+        check(lib().hsolve_serve_get_stats(vp(self.handle), C.byref(s)), "serve_get_stats")
+        return {name: getattr(s, name) for name, _ in ServeStats._fields_}
+
+    def drain(self, expected):
+        """Steps until `expected` results have been polled; returns them by id."""
+        got = {}
+        while len(got) < expected:
+            self.step()
+            for r in self.poll():
+                got[r["id"]] = r
+        return got
+
+    def run(self, arrivals, levels, seeds, outputs=None):
+        """Replays an arrival trace in real time (hsolve_serve_run); returns one dict per request."""
+        n = len(arrivals)
+        arr = np.ascontiguousarray(arrivals, np.float64)
+        lv = np.ascontiguousarray(levels, np.uint64)
+        sd = np.ascontiguousarray(seeds, np.uint64)
+        xs = None
+        if outputs is not None:
+            xs = (vp * n)(*[o.ptr if o is not None else None for o in outputs])
+        res = (ServeResult * max(n, 1))()
+        check(lib().hsolve_serve_run(vp(self.handle), u64(n), arr.ctypes.data_as(vp), lv.ctypes.data_as(vp),
+                                     sd.ctypes.data_as(vp), xs, res), "serve_run")
+        return [_result_dict(res[k]) for k in range(n)]
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().hsolve_serve_destroy(vp(self.handle))
                 self.handle = None
         except Exception:
             pass

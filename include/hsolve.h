@@ -297,6 +297,71 @@ int hsolve_mg_solve(hsolve_mg mg, const void *b, void *x, uint64_t *outer_iters,
                     uint64_t *inner_iters, double *final_defect, double *initial_defect,
                     void *stream);
 
+/* Request server: Poisson Q1 solves (FillMatrix<POISSON, DIRICHLET_0>, side
+ * 2^L+1, L in [min_served_level, max_served_level]) by the MG-IR above with
+ * x0 = 0.  Requests are admitted into per-level slot tables as slots and the
+ * device-memory budget allow (first come, first served on memory); every
+ * level's admitted solves advance together, one V-cycle per tick, in the same
+ * kernel launches, and levels tick independently on their own streams, so a
+ * request that arrives while others are on the GPU joins at the next tick of
+ * its level.  A solve gives the same bits alone and in any mix.  No reference
+ * counterpart: HONEI solves one system per call (multigrid.hh:2245); this is
+ * the serving layer around that call. */
+typedef struct hsolve_serve_params_s {
+    uint64_t min_level;                 /* coarsest level of every hierarchy */
+    uint64_t min_served_level, max_served_level;
+    uint64_t n_max_iter;
+    int32_t initial_zero, convergence_check;
+    double tolerance;
+    uint64_t n_pre_smooth;
+    double adapt_correction_factor;
+    uint64_t macro_border_mask[8];
+    uint64_t max_outer;
+    uint64_t memory_budget;             /* bytes of request state held at once */
+    uint64_t max_slots;                 /* concurrent solves per level */
+    int32_t use_graphs;                 /* replay each level's tick as a captured graph */
+} hsolve_serve_params;
+typedef struct hsolve_serve_s *hsolve_serve;
+typedef struct hsolve_serve_result_s {
+    uint64_t id, tag, level;
+    int32_t status;                     /* HSOLVE_OK or HSOLVE_ERR_NOT_CONVERGED */
+    uint64_t outer_iters, inner_iters, ticks;
+    double initial_defect, final_defect;
+    double t_submit, t_admit, t_done;   /* seconds on the server clock */
+} hsolve_serve_result;
+typedef struct hsolve_serve_stats_s {
+    uint64_t bytes_in_use, bytes_held, bytes_shared, budget;
+    uint64_t submitted, admitted, completed, queued, running, peak_running;
+    uint64_t ticks, graph_captures;
+} hsolve_serve_stats;
+int hsolve_serve_create(const hsolve_serve_params *params, hsolve_serve *out);
+int hsolve_serve_destroy(hsolve_serve srv);
+/* Bytes one request of level L holds while admitted. */
+int hsolve_serve_request_bytes(hsolve_serve srv, uint64_t level, uint64_t *bytes);
+/* Queues a request.  b: FP64 device vector of (2^L+1)^2 entries, copied at
+ * admission (keep it valid until the request completes), or NULL to generate
+ * b = U[-1,1] from SplitMix64(rhs_seed, stream 1) with zero Dirichlet nodes.
+ * x: FP64 device output, or NULL to discard the solution.  t_submit: the
+ * arrival time on the server clock (< 0: now).  HSOLVE_ERR_BUDGET when one
+ * request alone exceeds the budget. */
+int hsolve_serve_submit(hsolve_serve srv, uint64_t level, const double *b, uint64_t rhs_seed,
+                        double *x, uint64_t tag, double t_submit, uint64_t *id);
+/* One scheduling pass: collects every level whose tick finished, admits,
+ * and launches the next tick of every level with work.  Never blocks on the
+ * GPU.  *busy = levels with a tick in flight. */
+int hsolve_serve_step(hsolve_serve srv, uint64_t *busy);
+/* Moves up to max finished requests into out. */
+int hsolve_serve_poll(hsolve_serve srv, hsolve_serve_result *out, uint64_t max, uint64_t *count);
+int hsolve_serve_now(hsolve_serve srv, double *seconds);
+int hsolve_serve_get_stats(hsolve_serve srv, hsolve_serve_stats *out);
+/* Replays an arrival trace in real time: request i (level levels[i], seed
+ * seeds[i], output x[i] or NULL) is submitted at arrival_s[i] seconds after
+ * the call starts; returns when every request is done, with results[i] for
+ * request i. */
+int hsolve_serve_run(hsolve_serve srv, uint64_t count, const double *arrival_s,
+                     const uint64_t *levels, const uint64_t *seeds, double *const *x,
+                     hsolve_serve_result *results);
+
 #ifdef __cplusplus
 }
 #endif
