@@ -138,7 +138,7 @@ class ServeParams(C.Structure):
         ("n_max_iter", u64), ("initial_zero", i32), ("convergence_check", i32),
         ("tolerance", dbl), ("n_pre_smooth", u64), ("adapt_correction_factor", dbl),
         ("macro_border_mask", u64 * 8), ("max_outer", u64),
-        ("memory_budget", u64), ("max_slots", u64), ("use_graphs", i32),
+        ("memory_budget", u64), ("max_slots", u64), ("rows_per_tick", u64), ("use_graphs", i32),
     ]
 
 

@@ -57,10 +57,10 @@ namespace
         std::vector<Request> req;            // request per table entry
         std::vector<char> used;
         unsigned extent = 0;                 // entries covered by the last launched tick
+        unsigned cap = 0;                    // table entries in use at most for this level
         unsigned running = 0;
         bool dirty = false;
-        hipGraphExec_t graph = nullptr;
-        unsigned graph_extent = 0;
+        hipGraphExec_t graphs[16] = {};      // captured tick per launch extent 2^k
         std::deque<Request> queue;
         std::vector<void *> cache;           // released slot allocations
     };
@@ -89,8 +89,9 @@ struct hsolve_serve_s
         {
             if (b.st)
                 (void)hipStreamSynchronize(b.st);
-            if (b.graph)
-                (void)hipGraphExecDestroy(b.graph);
+            for (hipGraphExec_t g : b.graphs)
+                if (g)
+                    (void)hipGraphExecDestroy(g);
             for (void * p : b.mem)
                 if (p)
                     (void)hipFree(p);
@@ -141,9 +142,9 @@ struct hsolve_serve_s
             if (r.id > barrier)
                 return;
             unsigned idx = 0;
-            while (idx < b.used.size() && b.used[idx])
+            while (idx < b.cap && b.used[idx])
                 ++idx;
-            if (idx == b.used.size())
+            if (idx == b.cap)
                 return;   // every table entry of this level is taken
             void * mem = nullptr;
             if (! b.cache.empty())
@@ -226,16 +227,19 @@ struct hsolve_serve_s
     void launch(Bucket & b)
     {
         unsigned hw = 0;
-        for (unsigned i = 0; i < b.used.size(); ++i)
+        for (unsigned i = 0; i < b.cap; ++i)
             if (b.used[i])
                 hw = i + 1;
         if (hw == 0)
             return;
         // launch extent: the high-water entry rounded up to a power of two
-        unsigned ext = 1;
+        // (entries past it are empty; kernels of empty entries return at once)
+        unsigned ext = 1, lg = 0;
         while (ext < hw)
+        {
             ext <<= 1;
-        ext = std::min<unsigned>(ext, (unsigned)b.used.size());
+            ++lg;
+        }
         if (b.dirty || ext != b.extent)
         {
             HIP_CHECK(hipMemcpyAsync(b.table, b.table_host, sizeof(mg::Slot) * ext, hipMemcpyHostToDevice, b.st));
@@ -244,23 +248,20 @@ struct hsolve_serve_s
         b.extent = ext;
         if (prm.use_graphs)
         {
-            if (b.graph == nullptr || b.graph_extent != ext)
+            hipGraphExec_t & gx = b.graphs[lg];
+            if (gx == nullptr)
             {
-                if (b.graph)
-                    HIP_CHECK(hipGraphExecDestroy(b.graph));
-                b.graph = nullptr;
                 hipGraph_t g = nullptr;
                 HIP_CHECK(hipStreamBeginCapture(b.st, hipStreamCaptureModeThreadLocal));
                 mg::enqueue_tick(h, b.L, b.table, ext, b.st);
                 mg::enqueue_report(b.table, ext, b.rep, b.st);
                 HIP_CHECK(hipStreamEndCapture(b.st, &g));
-                const hipError_t e = hipGraphInstantiate(&b.graph, g, nullptr, nullptr, 0);
+                const hipError_t e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 HIP_CHECK(e);
-                b.graph_extent = ext;
                 ++captures;
             }
-            HIP_CHECK(hipGraphLaunch(b.graph, b.st));
+            HIP_CHECK(hipGraphLaunch(gx, b.st));
         }
         else
         {
@@ -378,6 +379,9 @@ int hsolve_serve_create(const hsolve_serve_params * params, hsolve_serve * out)
         b.mem.assign(p.max_slots, nullptr);
         b.req.resize(p.max_slots);
         b.used.assign(p.max_slots, 0);
+        b.cap = (unsigned)p.max_slots;
+        if (p.rows_per_tick)
+            b.cap = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(p.max_slots, p.rows_per_tick / mg::size_of(L)));
     }
     srv->t0 = std::chrono::steady_clock::now();
     *out = srv.release();

@@ -334,10 +334,10 @@ class MGSolver:
 
 def serve_params(min_level=2, min_served_level=6, max_served_level=11, n_max_iter=2, tolerance=1e-8,
                  n_pre_smooth=2, adapt=1.0, mask=(2,) * 8, initial_zero=True, convergence_check=False,
-                 max_outer=0, memory_budget=16 << 30, max_slots=64, use_graphs=True):
+                 max_outer=0, memory_budget=16 << 30, max_slots=64, rows_per_tick=1 << 24, use_graphs=True):
     return ServeParams(min_level, min_served_level, max_served_level, n_max_iter, int(initial_zero),
                        int(convergence_check), tolerance, n_pre_smooth, adapt, (u64 * 8)(*mask), max_outer,
-                       memory_budget, max_slots, int(use_graphs))
+                       memory_budget, max_slots, rows_per_tick, int(use_graphs))
 
 
 def _result_dict(r):

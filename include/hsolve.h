@@ -319,6 +319,8 @@ typedef struct hsolve_serve_params_s {
     uint64_t max_outer;
     uint64_t memory_budget;             /* bytes of request state held at once */
     uint64_t max_slots;                 /* concurrent solves per level */
+    uint64_t rows_per_tick;             /* also caps a level's slots at rows_per_tick / (2^L+1)^2
+                                           (at least 1), which bounds one tick's work; 0: no cap */
     int32_t use_graphs;                 /* replay each level's tick as a captured graph */
 } hsolve_serve_params;
 typedef struct hsolve_serve_s *hsolve_serve;
