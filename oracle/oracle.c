@@ -253,3 +253,40 @@ int or_mg_mixed_q1(const double *const *A64, const double *b, double *x,
     free(result); free(ig); free(od); free(prod); free(x_outer);
     return status;
 }
+
+/* IRSolver<OuterTag, InnerTag, Norm<vnt_l_two, true, OuterTag>>::value
+ * (honei/math/ir.hh:30-79). */
+int or_ir_mgop(const or_mgop_params *p, const or_ell_f64 *A64, const double *b, double *x,
+               const or_ell_f32 *A, const float *const *P, const or_ell_f32 *prol,
+               const or_ell_f32 *res, float *x_inner, double eps, uint64_t max_iters,
+               uint64_t *used_iters, uint64_t *inner_used, uint64_t *coarse_used)
+{
+    const uint64_t n = A64->rows;
+    if (p->levels < 1 || A[p->levels - 1].rows != n)
+        return -1;
+    double *r = (double *)malloc(sizeof(double) * n), *c = (double *)malloc(sizeof(double) * n);
+    float *b_inner = (float *)malloc(sizeof(float) * n);
+    double rnorm_initial = 1e16, rnorm_current = 1e16;
+    int status = 0;
+    for (uint64_t i = 0; i < max_iters; ++i) {
+        or_ell_defect_f64(r, b, A64->Aj, A64->Ax, A64->Arl, x, n, A64->stride, A64->threads);
+        if (i == 0)
+            rnorm_initial = or_norm_l2_f64(r, n, 1);
+        else {
+            rnorm_current = or_norm_l2_f64(r, n, 1);
+            *used_iters = i;
+            if (rnorm_current < eps * rnorm_initial)
+                break;
+        }
+        or_convert_d2f(b_inner, r, n);
+        status = or_mgop_solve_f32(p, A, P, prol, res, b_inner, x_inner, inner_used, coarse_used, NULL);
+        if (status != 0)
+            break;
+        or_convert_f2d(c, x_inner, n);
+        or_sum_f64(x, c, n);
+    }
+    free(r);
+    free(c);
+    free(b_inner);
+    return status;
+}

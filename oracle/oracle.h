@@ -41,6 +41,18 @@ typedef struct or_mg_params {
     uint64_t max_outer;           /* safety cap: 0 = none (reference has none) */
 } or_mg_params;
 
+/* Operator-list multigrid (honei/math/mg.hh): MGCycleCreation shape, coarse
+ * solver and MGData configuration (MGUtil::configure, mg.hh:434-447). */
+typedef struct or_mgop_params {
+    int32_t cycle;                /* 0: CYCLE::V, 1: CYCLE::W, 2: CYCLE::F::V (all STATIC) */
+    int32_t coarse_var;           /* 0: CGSolver<NONE>, 1: CGSolver<VAR> with the level's P */
+    uint64_t levels;              /* max level; levels 1..levels (internal index 0..levels-1) */
+    uint64_t min_level;
+    uint64_t max_iters, max_iters_coarse;
+    uint64_t n_pre_smooth, n_post_smooth;
+    double eps_relative;          /* stored as the data type, like MGData::eps_relative */
+} or_mgop_params;
+
 /* ---- shape helpers (type independent) ---------------------------------- */
 /* SparseMatrixELL(SparseMatrix) shape: honei/la/sparse_matrix_ell-impl.hh:78-101 */
 void or_ell_shape(uint64_t rows, const uint64_t *rowptr, uint64_t threads,
@@ -97,7 +109,16 @@ int or_cg_var_ell_##S(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,     
                       uint64_t stride, uint64_t threads, const T *P, const T *b, T *x,   \
                       uint64_t n, uint64_t max_iters, T eps, uint64_t *used_iters);      \
 int or_cg_konv_q1_##S(const T *const *bands, const T *rhs, T *x, uint64_t n,            \
-                      double konv_rad, uint64_t max_iters, uint64_t *iters);
+                      double konv_rad, uint64_t max_iters, uint64_t *iters);             \
+typedef struct or_ell_##S {                                                              \
+    uint64_t rows, cols, stride, threads;                                                \
+    const uint64_t *Aj;                                                                  \
+    const T *Ax;                                                                         \
+    const uint64_t *Arl;                                                                 \
+} or_ell_##S;                                                                            \
+int or_mgop_solve_##S(const or_mgop_params *p, const or_ell_##S *A, const T *const *P,  \
+                      const or_ell_##S *prol, const or_ell_##S *res, const T *b, T *x,   \
+                      uint64_t *used_iters, uint64_t *used_iters_coarse, T *final_norm);
 
 OR_DECLARE(double, f64)
 OR_DECLARE(float, f32)
@@ -116,6 +137,16 @@ int or_mg_mixed_q1(const double *const *A64, const double *b, double *x,
                    const float *const *a32, const float *const *dinv32,
                    const or_mg_params *p, uint64_t *outer_iters, uint64_t *inner_iters,
                    double *final_defect, double *initial_defect);
+
+/* IRSolver<OuterTag, InnerTag, Norm<vnt_l_two, true>>::value (honei/math/ir.hh:
+ * 30-79): FP64 defect and update around MGSolver on FP32 operator-list data.
+ * x_inner is the inner finest iterate (MGData::x of the top level), which the
+ * reference keeps from one outer step to the next: in = its initial value,
+ * out = its last value.  used_iters as the reference sets it. */
+int or_ir_mgop(const or_mgop_params *p, const or_ell_f64 *A64, const double *b, double *x,
+               const or_ell_f32 *A, const float *const *P, const or_ell_f32 *prol,
+               const or_ell_f32 *res, float *x_inner, double eps, uint64_t max_iters,
+               uint64_t *used_iters, uint64_t *inner_used, uint64_t *coarse_used);
 
 /* Multigrid<..., CYCLE::V, FIXED> on one precision is not restated. */
 

@@ -609,6 +609,130 @@ int FN(or_cg_konv_q1)(const T *const *bands, const T *rhs, T *x, uint64_t n,
     return capped;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Operator-list multigrid: MGCycleCreation (honei/math/mg.hh:560-1060),     */
+/* MGSolver (mg.hh:1064-1100), RISmoother (ri.hh:96-134), PROLMAT transfers */
+/* (restriction.hh:174-185, prolongation.hh:141-155).                        */
+/* ------------------------------------------------------------------------ */
+typedef struct FN(or_mgop) {
+    const or_mgop_params *p;
+    const OR_CAT(or_ell, OR_S) *A, *prol, *res;
+    const T *const *P;
+    T **c, **d, **store, **t0, **t1;    /* per internal level */
+    uint64_t used_coarse;
+} FN(or_mgop);
+
+static void FN(mgop_product)(const OR_CAT(or_ell, OR_S) *M, T *y, const T *x)
+{
+    FN(or_ell_product)(y, M->Aj, M->Ax, M->Arl, x, 0, M->rows, M->stride, M->threads);
+}
+
+static void FN(mgop_defect)(const OR_CAT(or_ell, OR_S) *M, T *r, const T *b, const T *x)
+{
+    FN(or_ell_defect)(r, b, M->Aj, M->Ax, M->Arl, x, M->rows, M->stride, M->threads);
+}
+
+/* RISmoother::value: iters times { t0 = b - A x; t1 = P * t0; x += t1 } */
+static void FN(mgop_smooth)(FN(or_mgop) *g, uint64_t i, T *x, const T *b, uint64_t iters)
+{
+    const uint64_t n = g->A[i].rows;
+    for (uint64_t k = 0; k < iters; ++k) {
+        FN(mgop_defect)(&g->A[i], g->t0[i], b, x);
+        FN(or_element_product3)(g->t1[i], g->P[i], g->t0[i], n);
+        FN(or_sum)(x, g->t1[i], n);
+    }
+}
+
+/* SolverOperator on the coarsest level: CGSolver<NONE> or <VAR> */
+static void FN(mgop_coarse)(FN(or_mgop) *g, uint64_t i, T *x, const T *b)
+{
+    const OR_CAT(or_ell, OR_S) *M = &g->A[i];
+    uint64_t used = 0;
+    if (g->p->coarse_var)
+        FN(or_cg_var_ell)(M->Aj, M->Ax, M->Arl, M->stride, M->threads, g->P[i], b, x, M->rows,
+                          g->p->max_iters_coarse, (T)g->p->eps_relative, &used);
+    else
+        FN(or_cg_none_ell)(M->Aj, M->Ax, M->Arl, M->stride, M->threads, b, x, M->rows,
+                           g->p->max_iters_coarse, (T)g->p->eps_relative, &used);
+    g->used_coarse = used;
+}
+
+/* One cycle of shape `shape` (0 V, 1 W, 2 F) on base level `level` with
+ * iterate x and right-hand side b; coarser levels use c (iterate) and d. */
+static void FN(mgop_cycle)(FN(or_mgop) *g, int shape, uint64_t level, T *x, const T *b)
+{
+    const uint64_t i = level - 1;
+    if (level == g->p->min_level) {
+        FN(mgop_coarse)(g, i, x, b);
+        return;
+    }
+    FN(mgop_smooth)(g, i, x, b, g->p->n_pre_smooth);
+    FN(mgop_defect)(&g->A[i], g->store[i], b, x);
+    FN(mgop_product)(&g->res[i - 1], g->d[i - 1], g->store[i]);   /* Restriction<PROLMAT> */
+    memset(g->c[i - 1], 0, sizeof(T) * g->A[i - 1].rows);         /* FillOperator */
+    if (shape == 0) {
+        FN(mgop_cycle)(g, 0, level - 1, g->c[i - 1], g->d[i - 1]);
+    } else if (shape == 1) {
+        if (level - 1 != g->p->min_level)
+            FN(mgop_cycle)(g, 1, level - 1, g->c[i - 1], g->d[i - 1]);
+        FN(mgop_cycle)(g, 1, level - 1, g->c[i - 1], g->d[i - 1]);
+    } else {
+        FN(mgop_cycle)(g, 2, level - 1, g->c[i - 1], g->d[i - 1]);
+        FN(mgop_cycle)(g, 0, level - 1, g->c[i - 1], g->d[i - 1]);
+    }
+    FN(mgop_product)(&g->prol[i - 1], g->store[i], g->c[i - 1]);  /* Prolongation<PROLMAT> */
+    FN(or_sum)(x, g->store[i], g->A[i].rows);                     /* SumOperator */
+    FN(mgop_smooth)(g, i, x, b, g->p->n_post_smooth);
+}
+
+int FN(or_mgop_solve)(const or_mgop_params *p, const OR_CAT(or_ell, OR_S) *A, const T *const *P,
+                      const OR_CAT(or_ell, OR_S) *prol, const OR_CAT(or_ell, OR_S) *res, const T *b, T *x,
+                      uint64_t *used_iters, uint64_t *used_iters_coarse, T *final_norm)
+{
+    if (p->levels < 1 || p->min_level < 1 || p->min_level > p->levels || p->cycle < 0 || p->cycle > 2)
+        return -1;
+    const uint64_t L = p->levels, top = L - 1, n = A[top].rows;
+    FN(or_mgop) g;
+    g.p = p;
+    g.A = A;
+    g.P = P;
+    g.prol = prol;
+    g.res = res;
+    g.used_coarse = 0;
+    T **vec[5] = {NULL, NULL, NULL, NULL, NULL};
+    for (int v = 0; v < 5; ++v) {
+        vec[v] = (T **)calloc(L, sizeof(T *));
+        for (uint64_t i = 0; i < L; ++i)
+            vec[v][i] = (T *)calloc(A[i].rows ? A[i].rows : 1, sizeof(T));
+    }
+    g.c = vec[0]; g.d = vec[1]; g.store = vec[2]; g.t0 = vec[3]; g.t1 = vec[4];
+
+    T *r = (T *)malloc(sizeof(T) * n);
+    FN(mgop_defect)(&A[top], r, b, x);
+    const T rnorm_initial = FN(or_norm_l2)(r, n, 1);
+    T rnorm_current = (T)1e16;
+    *used_iters = 0;
+    for (uint64_t it = 0; it < p->max_iters; ++it) {
+        FN(mgop_cycle)(&g, p->cycle, L, x, b);
+        FN(mgop_defect)(&A[top], r, b, x);
+        rnorm_current = FN(or_norm_l2)(r, n, 1);
+        *used_iters = it + 1;
+        if (rnorm_current < (T)p->eps_relative * rnorm_initial)
+            break;
+    }
+    if (used_iters_coarse)
+        *used_iters_coarse = g.used_coarse;
+    if (final_norm)
+        *final_norm = rnorm_current;
+    free(r);
+    for (int v = 0; v < 5; ++v) {
+        for (uint64_t i = 0; i < L; ++i)
+            free(vec[v][i]);
+        free(vec[v]);
+    }
+    return 0;
+}
+
 #undef OR_EPS
 #undef T
 #undef FN

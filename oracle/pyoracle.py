@@ -43,6 +43,32 @@ class MGParams(C.Structure):
     ]
 
 
+class MGOpParams(C.Structure):
+    """or_mgop_params: operator-list MG configuration (mg.hh MGUtil::configure)."""
+    _fields_ = [
+        ("cycle", C.c_int32), ("coarse_var", C.c_int32), ("levels", u64), ("min_level", u64),
+        ("max_iters", u64), ("max_iters_coarse", u64), ("n_pre_smooth", u64), ("n_post_smooth", u64),
+        ("eps_relative", C.c_double),
+    ]
+
+
+class OrEll(C.Structure):
+    """or_ell_f64 / or_ell_f32 (same layout; Ax points at the value type)."""
+    _fields_ = [("rows", u64), ("cols", u64), ("stride", u64), ("threads", u64),
+                ("Aj", vp), ("Ax", vp), ("Arl", vp)]
+
+
+def _ells(mats):
+    arr = (OrEll * max(len(mats), 1))()
+    for i, E in enumerate(mats):
+        arr[i] = OrEll(E["rows"], E.get("cols", E["rows"]), E["stride"], E["threads"],
+                       E["Aj"].ctypes.data, E["Ax"].ctypes.data, E["Arl"].ctypes.data)
+    return arr
+
+
+CYCLES = {"V": 0, "W": 1, "F": 2}
+
+
 class Oracle:
     """The C restatement of HONEI's tags::CPU path."""
 
@@ -72,6 +98,10 @@ class Oracle:
         L.or_rng_uniform.restype = C.c_double
         L.or_rng_uniform.argtypes = [u64, u64, u64, C.c_double, C.c_double]
         L.or_mg_mixed_q1.argtypes = [vp, vp, vp, vp, vp, C.POINTER(MGParams), vp, vp, vp, vp]
+        for s in ("f64", "f32"):
+            getattr(L, f"or_mgop_solve_{s}").argtypes = [C.POINTER(MGOpParams), vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.or_ir_mgop.argtypes = [C.POINTER(MGOpParams), vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, u64,
+                                 vp, vp, vp]
 
     # -- ELL ---------------------------------------------------------------
     def ell_from_csr(self, rows, rowptr, col, val, threads=1):
@@ -284,6 +314,38 @@ class Oracle:
         return dict(x=x, outer=outer.value, inner=inner.value, final_defect=fd.value,
                     initial_defect=idf.value, status=st)
 
+    # -- operator-list MG (mg.hh, ri.hh, ir.hh) --------------------------------
+    def mgop_solve(self, data, b, x0, *, cycle="V", coarse_var=False, min_level=1, max_iters=100,
+                   max_iters_coarse=100, n_pre=4, n_post=4, eps=1e-8):
+        """MGSolver over MGCycleCreation<cycle, CGSolver<NONE|VAR>, RISmoother, PROLMAT>.
+        data: dict with lists A, P, prol, res (see tests/honei_fixtures.load_mg_data)."""
+        dt = np.dtype(b.dtype)
+        p = MGOpParams(CYCLES[cycle], int(coarse_var), len(data["A"]), min_level, max_iters, max_iters_coarse,
+                       n_pre, n_post, eps)
+        x = np.ascontiguousarray(x0, dt).copy()
+        used, used_c = u64(), u64()
+        fn = C.c_double() if dt == np.float64 else C.c_float()
+        st = getattr(self.lib, f"or_mgop_solve_{_sfx(dt)}")(
+            C.byref(p), _ells(data["A"]), _pp(data["P"]), _ells(data["prol"]), _ells(data["res"]),
+            _p(b), _p(x), C.byref(used), C.byref(used_c), C.byref(fn))
+        assert st == 0, st
+        return dict(x=x, used=used.value, used_coarse=used_c.value, final_norm=fn.value)
+
+    def ir_mgop(self, A64, b, x0, data32, x_inner0, *, min_level=1, max_iters_inner=8,
+                max_iters_coarse=100, n_pre=4, n_post=4, eps_inner=1e-8, eps=1e-8, max_iters=100):
+        """IRSolver<Outer, Inner, Norm<l2, true>> with the IR test's inner set-up
+        (V cycle, CGSolver<VAR> coarse, RISmoother, PROLMAT)."""
+        p = MGOpParams(0, 1, len(data32["A"]), min_level, max_iters_inner, max_iters_coarse, n_pre, n_post,
+                       float(np.float32(eps_inner)))
+        x = np.ascontiguousarray(x0, np.float64).copy()
+        xi = np.ascontiguousarray(x_inner0, np.float32).copy()
+        used, iu, cu = u64(), u64(), u64()
+        st = self.lib.or_ir_mgop(C.byref(p), _ells([A64]), _p(b), _p(x), _ells(data32["A"]), _pp(data32["P"]),
+                                 _ells(data32["prol"]), _ells(data32["res"]), _p(xi), C.c_double(eps),
+                                 u64(max_iters), C.byref(used), C.byref(iu), C.byref(cu))
+        assert st == 0, st
+        return dict(x=x, x_inner=xi, used=used.value, inner_used=iu.value, coarse_used=cu.value)
+
     # -- RNG -----------------------------------------------------------------
     def rng_uniform(self, seed, stream, index, lo, hi):
         return self.lib.or_rng_uniform(u64(seed), u64(stream), u64(index), lo, hi)
@@ -309,6 +371,10 @@ class Reference:
             getattr(L, f"ref_cg_konv_q1_{s}").argtypes = [vp, vp, vp, u64, C.c_double]
         L.ref_mg_mixed_q1.argtypes = [vp, vp, vp, vp, u64, u64, u64, C.c_double, u64, C.c_double,
                                       vp, vp, vp]
+        L.ref_mgop_solve_f64.argtypes = [C.c_char_p, C.c_int, C.c_int, u64, u64, u64, u64, u64, u64,
+                                         C.c_double, C.c_double, vp, vp, vp]
+        L.ref_ir_mgop.argtypes = [C.c_char_p, u64, u64, u64, u64, u64, u64, C.c_double, C.c_double,
+                                  C.c_double, u64, vp, vp, vp, vp]
 
     def ell_from_csr(self, rows, cols, rowptr, col, val, threads=1):
         rowptr = np.ascontiguousarray(rowptr, np.uint64)
@@ -422,6 +488,34 @@ class Reference:
                                  u64(n_max_iter), C.c_double(tolerance), u64(n_pre_smooth),
                                  C.c_double(adapt), _p(m), C.byref(outer), C.byref(inner))
         return dict(x=x, outer=outer.value, inner=inner.value)
+
+
+def _ref_mgop_solve(self, base, n, *, cycle="V", coarse_var=False, levels=6, min_level=1, max_iters=100,
+                    max_iters_coarse=100, n_pre=4, n_post=4, eps=1e-8, damping=0.7):
+    """The reference's MGUtil::load_data(base, levels, damping, "jac") + MGSolver."""
+    base = base if base.endswith("/") else base + "/"
+    x = np.zeros(n, np.float64)
+    used, used_c = u64(), u64()
+    st = self.lib.ref_mgop_solve_f64(base.encode(), CYCLES[cycle], int(coarse_var), levels, min_level, max_iters,
+                                     max_iters_coarse, n_pre, n_post, eps, damping, _p(x), C.byref(used),
+                                     C.byref(used_c))
+    assert st == 0
+    return dict(x=x, used=used.value, used_coarse=used_c.value)
+
+
+def _ref_ir_mgop(self, base, n, *, levels=4, min_level=1, max_iters_inner=8, max_iters_coarse=100, n_pre=4,
+                 n_post=4, eps_inner=1e-8, damping=0.7, eps=1e-8, max_iters=100):
+    """The IRSolver test's set-up run by the reference (ir_TEST.cc:30-80)."""
+    base = base if base.endswith("/") else base + "/"
+    x = np.zeros(n, np.float64)
+    used, iu, cu = u64(), u64(), u64()
+    self.lib.ref_ir_mgop(base.encode(), levels, min_level, max_iters_inner, max_iters_coarse, n_pre, n_post,
+                         eps_inner, damping, eps, max_iters, _p(x), C.byref(used), C.byref(iu), C.byref(cu))
+    return dict(x=x, used=used.value, inner_used=iu.value, coarse_used=cu.value)
+
+
+Reference.mgop_solve = _ref_mgop_solve
+Reference.ir_mgop = _ref_ir_mgop
 
 
 def have_reference() -> bool:

@@ -25,6 +25,11 @@
 #include <honei/math/restriction.hh>
 #include <honei/math/prolongation.hh>
 #include <honei/math/fill_matrix.hh>
+#include <honei/math/mg.hh>
+#include <honei/math/ri.hh>
+#include <honei/math/ir.hh>
+#include <honei/math/matrix_io.hh>
+#include <honei/math/vector_io.hh>
 #include <honei/util/configuration.hh>
 
 #include <cstring>
@@ -314,6 +319,106 @@ int ref_mg_mixed_q1(const double * const * A64, const double * b, double * x,
             *inner = std::strtoul(line.c_str() + 19, 0, 10);
     }
     return (int)*outer;
+}
+
+}  // extern "C"
+
+/* Operator-list MG from the reference's own loader and templates:
+ * MGUtil::load_data(base, levels, damping, "jac") + configure, the cycle of
+ * `cycle` (0 V, 1 W, 2 F::V) with CGSolver<NONE> (coarse_var 0) or <VAR>
+ * (1), RISmoother and PROLMAT transfers, then MGSolver with the l2 norm.
+ * x receives the finest iterate. */
+template <typename DT_, typename Cycle_, typename Coarse_>
+static void mgop_run(const char * base, unsigned long levels, unsigned long min_level, unsigned long max_iters,
+        unsigned long max_iters_coarse, unsigned long npre, unsigned long npost, double eps, double damping,
+        DT_ * x, unsigned long * used, unsigned long * used_coarse)
+{
+    typedef MGUtil<tags::CPU, SparseMatrixELL<DT_>, DenseVector<DT_>, SparseMatrixELL<DT_>, DenseVector<DT_>,
+            MatrixIO<io_formats::ELL>, VectorIO<io_formats::EXP>, DT_> Util;
+    MGData<SparseMatrixELL<DT_>, DenseVector<DT_>, SparseMatrixELL<DT_>, DenseVector<DT_>, DT_> data(
+            Util::load_data(base, levels, DT_(damping), "jac"));
+    Util::configure(data, max_iters, max_iters_coarse, npre, npost, min_level, DT_(eps));
+    OperatorList ol(MGCycleCreation<tags::CPU, Cycle_, Coarse_, RISmoother<tags::CPU>,
+            Restriction<tags::CPU, methods::PROLMAT>, Prolongation<tags::CPU, methods::PROLMAT>, DT_>::value(data));
+    std::stringstream captured;
+    std::streambuf * old(std::cout.rdbuf(captured.rdbuf()));
+    MGSolver<tags::CPU, Norm<vnt_l_two, true, tags::CPU> >::value(data, ol);
+    std::cout.rdbuf(old);
+    const DenseVector<DT_> & xr(data.x.at(data.x.size() - 1));
+    for (unsigned long i(0) ; i < xr.size() ; ++i)
+        x[i] = xr[i];
+    *used = data.used_iters;
+    *used_coarse = data.used_iters_coarse;
+}
+
+template <typename DT_, typename Coarse_>
+static int mgop_by_cycle(int cycle, const char * base, unsigned long levels, unsigned long min_level,
+        unsigned long max_iters, unsigned long max_iters_coarse, unsigned long npre, unsigned long npost, double eps,
+        double damping, DT_ * x, unsigned long * used, unsigned long * used_coarse)
+{
+    if (cycle == 0)
+        mgop_run<DT_, methods::CYCLE::V::STATIC, Coarse_>(base, levels, min_level, max_iters, max_iters_coarse,
+                npre, npost, eps, damping, x, used, used_coarse);
+    else if (cycle == 1)
+        mgop_run<DT_, methods::CYCLE::W::STATIC, Coarse_>(base, levels, min_level, max_iters, max_iters_coarse,
+                npre, npost, eps, damping, x, used, used_coarse);
+    else if (cycle == 2)
+        mgop_run<DT_, methods::CYCLE::F::V::STATIC, Coarse_>(base, levels, min_level, max_iters, max_iters_coarse,
+                npre, npost, eps, damping, x, used, used_coarse);
+    else
+        return -1;
+    return 0;
+}
+
+extern "C" {
+
+int ref_mgop_solve_f64(const char * base, int cycle, int coarse_var, uint64_t levels, uint64_t min_level,
+        uint64_t max_iters, uint64_t max_iters_coarse, uint64_t npre, uint64_t npost, double eps, double damping,
+        double * x, uint64_t * used, uint64_t * used_coarse)
+{
+    unsigned long u(0), uc(0);
+    int st = coarse_var ?
+        mgop_by_cycle<double, CGSolver<tags::CPU, methods::VAR> >(cycle, base, levels, min_level, max_iters,
+                max_iters_coarse, npre, npost, eps, damping, x, &u, &uc) :
+        mgop_by_cycle<double, CGSolver<tags::CPU, methods::NONE> >(cycle, base, levels, min_level, max_iters,
+                max_iters_coarse, npre, npost, eps, damping, x, &u, &uc);
+    *used = u;
+    *used_coarse = uc;
+    return st;
+}
+
+/* The IRSolver test's set-up (ir_TEST.cc:30-80): FP32 operator-list data
+ * from `base` (V cycle, CGSolver<VAR> coarse, RISmoother, jac damping),
+ * FP64 A_levels, rhs_levels, init_levels, then IRSolver. */
+int ref_ir_mgop(const char * base, uint64_t levels, uint64_t min_level, uint64_t max_iters_inner,
+        uint64_t max_iters_coarse, uint64_t npre, uint64_t npost, double eps_inner, double damping,
+        double eps, uint64_t max_iters, double * x_out, uint64_t * used, uint64_t * inner_used,
+        uint64_t * coarse_used)
+{
+    typedef MGUtil<tags::CPU, SparseMatrixELL<float>, DenseVector<float>, SparseMatrixELL<float>, DenseVector<float>,
+            MatrixIO<io_formats::ELL>, VectorIO<io_formats::EXP>, float> Util;
+    MGData<SparseMatrixELL<float>, DenseVector<float>, SparseMatrixELL<float>, DenseVector<float>, float> data(
+            Util::load_data(base, levels, float(damping), "jac"));
+    Util::configure(data, max_iters_inner, max_iters_coarse, npre, npost, min_level, float(eps_inner));
+    OperatorList ol(MGCycleCreation<tags::CPU, methods::CYCLE::V::STATIC, CGSolver<tags::CPU, methods::VAR>,
+            RISmoother<tags::CPU>, Restriction<tags::CPU, methods::PROLMAT>, Prolongation<tags::CPU, methods::PROLMAT>,
+            float>::value(data));
+    std::string b(base);
+    SparseMatrixELL<double> A(MatrixIO<io_formats::ELL>::read_matrix(b + "A_" + stringify(levels) + ".ell", double(0)));
+    DenseVector<double> rhs(VectorIO<io_formats::EXP>::read_vector(b + "rhs_" + stringify(levels), double(0)));
+    DenseVector<double> x(VectorIO<io_formats::EXP>::read_vector(b + "init_" + stringify(levels), double(0)));
+    unsigned long u(0);
+    std::stringstream captured;
+    std::streambuf * old(std::cout.rdbuf(captured.rdbuf()));
+    IRSolver<tags::CPU, tags::CPU, Norm<vnt_l_two, true, tags::CPU> >::value(A, rhs, x, data, ol, eps,
+            (unsigned long)max_iters, u);
+    std::cout.rdbuf(old);
+    for (unsigned long i(0) ; i < x.size() ; ++i)
+        x_out[i] = x[i];
+    *used = u;
+    *inner_used = data.used_iters;
+    *coarse_used = data.used_iters_coarse;
+    return 0;
 }
 
 /* The ell::threads value the reference's ELL constructor reads. */

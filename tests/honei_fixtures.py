@@ -132,3 +132,81 @@ def q1_csr(bands, n, drop_zeros=True):
     col = J.T[keep.T].astype(np.uint64)
     val = B.T[keep.T]
     return rowptr, col, val
+
+
+# --------------------------------------------------------------------------
+# Operator-list multigrid data as MGUtil::load_data builds it
+# (honei/math/mg.hh:449-545): A_i.ell, the "jac" preconditioner
+# P = damping / diag(A) (PreconFill, mg.hh:254-283), prol_i.ell and
+# res = Transposition(prol) (transposition.hh:30-43), rhs / init vectors.
+# --------------------------------------------------------------------------
+def ell_from_file(path, dtype, oracle):
+    """MatrixIO<ELL>::read_matrix(path, DT): values converted to DT, row
+    lengths derived by the raw constructor's first-zero scan."""
+    E = load_ell(path)
+    Ax = E["Ax"].astype(dtype)
+    Arl = oracle.ell_row_length(E["rows"], E["stride"], 1, Ax)
+    return dict(rows=E["rows"], cols=E["cols"], stride=E["stride"], ncpr=E["ncpr"], threads=1,
+                Aj=E["Aj"], Ax=Ax, Arl=Arl)
+
+
+def ell_entries(E):
+    """(row, col, val) of the stored entries, row by row in slot order."""
+    rows = np.arange(E["rows"])
+    rr, cc, vv = [], [], []
+    for k in range(E["ncpr"]):
+        m = k < E["Arl"]
+        idx = rows[m] + k * E["stride"]
+        rr.append(rows[m])
+        cc.append(E["Aj"][idx].astype(np.int64))
+        vv.append(E["Ax"][idx])
+    return np.concatenate(rr), np.concatenate(cc), np.concatenate(vv)
+
+
+def ell_diag(E):
+    r, c, v = ell_entries(E)
+    d = np.zeros(E["rows"], E["Ax"].dtype)
+    m = r == c
+    d[r[m]] = v[m]
+    return d
+
+
+def ell_transpose(E, oracle):
+    """SparseMatrixELL of the transpose, each row's entries in ascending column order."""
+    r, c, v = ell_entries(E)
+    order = np.lexsort((r, c))
+    nrows = E["cols"]
+    rowptr = np.zeros(nrows + 1, np.uint64)
+    np.add.at(rowptr, c[order] + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.uint64)
+    T = oracle.ell_from_csr(nrows, rowptr, r[order].astype(np.uint64), v[order])
+    T["cols"] = E["rows"]
+    return T
+
+
+def load_mg_data(base, levels, dtype, oracle, damping=0.7):
+    """A[i], P[i] for internal levels i < levels; prol[i], res[i] (level i -> i+1)
+    for i < levels - 1; b = rhs_<levels>, x = init_<levels>."""
+    base = base if os.path.isabs(base) else os.path.join(TESTDATA, base)
+    A, P, prol, res = [], [], [], []
+    for i in range(levels):
+        Ai = ell_from_file(os.path.join(base, f"A_{i + 1}.ell"), dtype, oracle)
+        A.append(Ai)
+        P.append((np.full(Ai["rows"], dtype(damping), dtype) / ell_diag(Ai)).astype(dtype))
+        if i < levels - 1:
+            Pi = ell_from_file(os.path.join(base, f"prol_{i + 2}.ell"), dtype, oracle)
+            prol.append(Pi)
+            res.append(ell_transpose(Pi, oracle))
+    rhs, init = os.path.join(base, f"rhs_{levels}"), os.path.join(base, f"init_{levels}")
+    b = load_exp(rhs).astype(dtype) if os.path.exists(rhs) else None
+    x = load_exp(init).astype(dtype) if os.path.exists(init) else None
+    return dict(A=A, P=P, prol=prol, res=res, b=b, x=x, base=base)
+
+
+def honei_eps_8e3():
+    """The tolerance of mg_TEST.cc / ir_TEST.cc: 8 * (m * sizeof(double) + b)."""
+    base_eps = 1 / 10 ** 1
+    add_eps = base_eps / 10 ** 2
+    m = (add_eps - base_eps) / 4.0
+    b = base_eps - 4.0 * m
+    return (m * 8 + b) * 8.0
