@@ -15,15 +15,29 @@ namespace honei
         {
             struct JAC;
         };
+        // Multigrid<..., CYCLE::V, ...> takes the shape itself; the operator-list
+        // MGCycleCreation takes the ::STATIC member
         struct CYCLE
         {
             struct F
             {
-                struct V;
-                struct W;
+                struct V
+                {
+                    struct STATIC;
+                };
+                struct W
+                {
+                    struct STATIC;
+                };
             };
-            struct W;
-            struct V;
+            struct W
+            {
+                struct STATIC;
+            };
+            struct V
+            {
+                struct STATIC;
+            };
         };
         struct FIXED;
         struct MIXED;

@@ -25,6 +25,9 @@
 #include <honei/math/matrix_io.hh>
 #include <honei/math/vector_io.hh>
 #include <honei/math/fill_matrix.hh>
+#include <honei/math/mg.hh>
+#include <honei/math/ir.hh>
+#include <honei/math/ri.hh>
 
 #include <oracle.h>
 
@@ -449,6 +452,168 @@ static void test_lock_semantics()
     TEST_CHECK(v[0] == -1.0);
 }
 
+// ------------------------------------------------------------------------
+// Operator-list multigrid (mg.hh) and IRSolver (ir.hh) on tags::GPU::MI355X
+// ------------------------------------------------------------------------
+static double honei_eps_8e3()
+{
+    const double base_eps = 0.1, add_eps = base_eps / 100.0;
+    const double m = (add_eps - base_eps) / 4.0, b = base_eps - 4.0 * m;
+    return (m * sizeof(double) + b) * 8.0;
+}
+
+template <typename DT> static std::vector<DT> host_copy(const DenseVector<DT> & v)
+{
+    return std::vector<DT>(v.read_elements(), v.read_elements() + v.size());
+}
+
+template <typename OE, typename DT> static OE or_view(const SparseMatrixELL<DT> & m)
+{
+    OE e;
+    e.rows = m.rows();
+    e.cols = m.columns();
+    e.stride = m.stride();
+    e.threads = m.threads();
+    e.Aj = (const uint64_t *)m.Aj().read_elements();
+    e.Ax = m.Ax().read_elements();
+    e.Arl = (const uint64_t *)m.Arl().read_elements();
+    return e;
+}
+
+template <typename OE, typename Data> struct OracleLevels
+{
+    std::vector<OE> A, prol, res;
+    std::vector<const typename std::remove_pointer<decltype(OE().Ax)>::type *> P;
+    explicit OracleLevels(Data & d)
+    {
+        for (auto & m : d.A) A.push_back(or_view<OE>(m));
+        for (auto & m : d.prolmat) prol.push_back(or_view<OE>(m));
+        for (auto & m : d.resmat) res.push_back(or_view<OE>(m));
+        for (auto & v : d.P) P.push_back(v.read_elements());
+    }
+};
+
+static void test_mgop_cycles()
+{
+    typedef MGUtil<Tag, SparseMatrixELL<double>, DenseVector<double>, SparseMatrixELL<double>, DenseVector<double>,
+                   MatrixIO<io_formats::ELL>, VectorIO<io_formats::EXP>, double> Util;
+    const std::string base = g_dir + "/poisson/";
+    const std::vector<double> sol = host_copy(VectorIO<io_formats::EXP>::read_vector(base + "sol_6", double(0)));
+    for (int shape = 0; shape < 3; ++shape)
+        for (int var = 0; var < 2; ++var)
+        {
+            auto data = Util::load_data(base, 6, 0.7, "jac");
+            Util::configure(data, 100, 100, 4, 4, 1, 1e-8);
+            const std::vector<double> x0 = host_copy(data.x.back());
+            OperatorList ol;
+            if (shape == 0 && var == 0)
+                ol = MGCycleCreation<Tag, methods::CYCLE::V::STATIC, CGSolver<Tag, methods::NONE>, RISmoother<Tag>,
+                        Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>, double>::value(data);
+            else if (shape == 0)
+                ol = MGCycleCreation<Tag, methods::CYCLE::V::STATIC, CGSolver<Tag, methods::VAR>, RISmoother<Tag>,
+                        Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>, double>::value(data);
+            else if (shape == 1 && var == 0)
+                ol = MGCycleCreation<Tag, methods::CYCLE::W::STATIC, CGSolver<Tag, methods::NONE>, RISmoother<Tag>,
+                        Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>, double>::value(data);
+            else if (shape == 1)
+                ol = MGCycleCreation<Tag, methods::CYCLE::W::STATIC, CGSolver<Tag, methods::VAR>, RISmoother<Tag>,
+                        Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>, double>::value(data);
+            else if (var == 0)
+                ol = MGCycleCreation<Tag, methods::CYCLE::F::V::STATIC, CGSolver<Tag, methods::NONE>,
+                        RISmoother<Tag>, Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>,
+                        double>::value(data);
+            else
+                ol = MGCycleCreation<Tag, methods::CYCLE::F::V::STATIC, CGSolver<Tag, methods::VAR>,
+                        RISmoother<Tag>, Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>,
+                        double>::value(data);
+            MGSolver<Tag, Norm<vnt_l_two, true, Tag>>::value(data, ol);
+            const std::vector<double> x = host_copy(data.x.back());
+            // the oracle on the same level data
+            OracleLevels<or_ell_f64, decltype(data)> L(data);
+            or_mgop_params p;
+            p.cycle = shape;
+            p.coarse_var = var;
+            p.levels = 6;
+            p.min_level = 1;
+            p.max_iters = 100;
+            p.max_iters_coarse = 100;
+            p.n_pre_smooth = 4;
+            p.n_post_smooth = 4;
+            p.eps_relative = 1e-8;
+            std::vector<double> xr(x0);
+            uint64_t used = 0, used_c = 0;
+            double fin = 0;
+            or_mgop_solve_f64(&p, L.A.data(), L.P.data(), L.prol.data(), L.res.data(),
+                              data.b.back().read_elements(), xr.data(), &used, &used_c, &fin);
+            std::printf("  cycle %d coarse %s: %lu cycles (oracle %lu), coarse CG %lu (oracle %lu)\n", shape,
+                        var ? "VAR" : "NONE", data.used_iters, (unsigned long)used, data.used_iters_coarse,
+                        (unsigned long)used_c);
+            TEST_CHECK((long)data.used_iters - (long)used <= 1 && (long)used - (long)data.used_iters <= 1);
+            double num = 0, den = 0, maxdev = 0;
+            for (size_t i = 0; i < x.size(); ++i)
+            {
+                num += (x[i] - xr[i]) * (x[i] - xr[i]);
+                den += xr[i] * xr[i];
+                maxdev = std::max(maxdev, std::fabs(x[i] - sol[i]));
+            }
+            TEST_CHECK(std::sqrt(num / den) <= 1e-6);
+            if (shape == 1 && var == 0)   // mg_TEST.cc:331-420's configuration
+                TEST_CHECK(maxdev <= honei_eps_8e3());
+        }
+}
+
+static void test_ir_mgop()
+{
+    typedef MGUtil<Tag, SparseMatrixELL<float>, DenseVector<float>, SparseMatrixELL<float>, DenseVector<float>,
+                   MatrixIO<io_formats::ELL>, VectorIO<io_formats::EXP>, float> Util;
+    const std::string base = g_dir + "/poisson_advanced/sort_0/";
+    auto data = Util::load_data(base, 4, 0.7f, "jac");
+    Util::configure(data, 8, 100, 4, 4, 1, 1e-8f);
+    const std::vector<float> xi0 = host_copy(data.x.back());
+    OperatorList ol = MGCycleCreation<Tag, methods::CYCLE::V::STATIC, CGSolver<Tag, methods::VAR>, RISmoother<Tag>,
+            Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>, float>::value(data);
+    SparseMatrixELL<double> A(MatrixIO<io_formats::ELL>::read_matrix(base + "A_4.ell", double(0)));
+    DenseVector<double> b(VectorIO<io_formats::EXP>::read_vector(base + "rhs_4", double(0)));
+    DenseVector<double> x(VectorIO<io_formats::EXP>::read_vector(base + "init_4", double(0)));
+    const std::vector<double> x0 = host_copy(x);
+    unsigned long used = 0;
+    IRSolver<Tag, Tag, Norm<vnt_l_two, true, Tag>>::value(A, b, x, data, ol, 1e-8, 100ul, used);
+    const std::vector<double> xs = host_copy(x);
+    const std::vector<double> sol = host_copy(VectorIO<io_formats::EXP>::read_vector(base + "sol_4", double(0)));
+    double maxdev = 0;
+    for (size_t i = 0; i < xs.size(); ++i)
+        maxdev = std::max(maxdev, std::fabs(xs[i] - sol[i]));
+    TEST_CHECK(maxdev <= honei_eps_8e3());
+    // the oracle on the same data
+    OracleLevels<or_ell_f32, decltype(data)> L(data);
+    or_mgop_params p;
+    p.cycle = 0;
+    p.coarse_var = 1;
+    p.levels = 4;
+    p.min_level = 1;
+    p.max_iters = 8;
+    p.max_iters_coarse = 100;
+    p.n_pre_smooth = 4;
+    p.n_post_smooth = 4;
+    p.eps_relative = (double)1e-8f;
+    or_ell_f64 A64 = or_view<or_ell_f64>(A);
+    std::vector<double> xr(x0);
+    std::vector<float> xir(xi0);
+    uint64_t u = 0, iu = 0, cu = 0;
+    or_ir_mgop(&p, &A64, b.read_elements(), xr.data(), L.A.data(), L.P.data(), L.prol.data(), L.res.data(),
+               xir.data(), 1e-8, 100, &u, &iu, &cu);
+    std::printf("  IR outer %lu (oracle %lu), last inner %lu (oracle %lu)\n", used, (unsigned long)u,
+                data.used_iters, (unsigned long)iu);
+    TEST_CHECK((long)used - (long)u <= 1 && (long)u - (long)used <= 1);
+    double num = 0, den = 0;
+    for (size_t i = 0; i < xs.size(); ++i)
+    {
+        num += (xs[i] - xr[i]) * (xs[i] - xr[i]);
+        den += xr[i] * xr[i];
+    }
+    TEST_CHECK(std::sqrt(num / den) <= 1e-6);
+}
+
 int main(int argc, char ** argv)
 {
     g_dir = argc > 1 ? argv[1] : "tests/golden/honei_testdata";
@@ -465,6 +630,8 @@ int main(int argc, char ** argv)
         {"cg", test_cg},
         {"multigrid_mixed", test_multigrid_mixed},
         {"lock_semantics", test_lock_semantics},
+        {"mgop_cycles", test_mgop_cycles},
+        {"ir_mgop", test_ir_mgop},
     };
     for (auto & c : cases)
     {
