@@ -79,123 +79,35 @@ void or_convert_f2d(double *dst, const float *src, uint64_t n)
         dst[i] = (double)src[i];
 }
 
-/* ------------------------------------------------------------------------ */
-/* Multigrid<Tag, OuterTag, NONE, JAC, CYCLE::V, MIXED> on Q1 bands.         */
-/* ------------------------------------------------------------------------ */
-
-typedef struct or_mg_level {
-    uint64_t n;
-    const float *const *a;
-    const float *dinv;
-    float *c, *d, *rhs, *x, *tmp;
-} or_mg_level;
-
-/* _multigrid_kernel<float>: honei/math/multigrid.hh:1620-1933 (V-cycle only;
- * the local W-cycle branch of the prolongation loop is unreachable for V). */
-static void or_mg_kernel(or_mg_level *lv, const or_mg_params *p, const float tol,
-                         const float adapt, uint64_t *inner_iterations)
-{
-    const uint64_t minl = p->min_level, maxl = p->max_level;
-    or_mg_level *top = &lv[maxl];
-    float defect, initial_defect;
-
-    if (p->initial_zero)
-        memcpy(top->d, top->rhs, sizeof(float) * top->n);
-    else
-        or_q1_defect_f32(top->d, top->rhs, top->a, top->x, top->n);
-
-    defect = or_norm_l2_f32(top->d, top->n, 1);
-    initial_defect = defect;
-
-    if (p->convergence_check && defect <= tol)
-        return;
-
-    uint64_t iter;
-    for (iter = 1; iter <= p->n_max_iter; ++iter) {
-        uint64_t L = maxl;
-        int restriction_started = 1;
-        /* restriction loop */
-        while (minl != maxl) {
-            or_mg_level *l = &lv[L];
-            if (restriction_started) {
-                or_element_product3_f32(l->c, l->dinv, l->d, l->n);
-                or_jacobi_q1_f32(l->c, l->a, l->d, l->dinv, l->n, p->n_pre_smooth - 1, l->tmp);
-                or_sum_f32(l->x, l->c, l->n);
-            } else {
-                or_element_product3_f32(l->x, l->dinv, l->d, l->n);
-                or_jacobi_q1_f32(l->x, l->a, l->d, l->dinv, l->n, p->n_pre_smooth - 1, l->tmp);
-            }
-            or_q1_defect_f32(l->d, l->rhs, l->a, l->x, l->n);
-            --L;
-            or_restriction_f32(lv[L].d, lv[L + 1].d, lv[L + 1].n, p->macro_border_mask);
-            memcpy(lv[L].rhs, lv[L].d, sizeof(float) * lv[L].n);
-            if (L == minl)
-                break;
-            restriction_started = 0;
-        }
-        /* coarse grid correction */
-        {
-            uint64_t cg_iters;
-            or_cg_konv_q1_f32(lv[L].a, lv[L].d, lv[L].x, lv[L].n, FLT_EPSILON, 0, &cg_iters);
-        }
-        if (minl == maxl) {
-            or_q1_defect_f32(lv[L].d, lv[L].rhs, lv[L].a, lv[L].x, lv[L].n);
-            break;
-        }
-        /* prolongation loop */
-        while (1) {
-            ++L;
-            or_mg_level *l = &lv[L];
-            or_prolongation_f32(l->c, lv[L - 1].x, l->n, p->macro_border_mask);
-            float alpha;
-            if (adapt == 0.0f) {
-                float d1 = or_dot_f32(l->d, l->c, l->n);
-                or_q1_product_f32(l->tmp, l->a, l->c, l->n);
-                float d2 = or_dot_f32(l->tmp, l->c, l->n);
-                alpha = d1 / d2;
-            } else {
-                alpha = adapt;
-            }
-            or_scaled_sum_f32(l->x, l->c, alpha, l->n);
-            or_jacobi_q1_f32(l->x, l->a, l->rhs, l->dinv, l->n, p->n_pre_smooth, l->tmp);
-            or_q1_defect_f32(l->d, l->rhs, l->a, l->x, l->n);
-            if (L == maxl)
-                break;
-        }
-        defect = or_norm_l2_f32(top->d, top->n, 1);
-        if (defect <= initial_defect * tol)
-            break;
-    }
-    *inner_iterations = (iter > p->n_max_iter) ? iter - 1 : iter;
-}
-
-/* Multigrid<...MIXED>::value(BandedMatrixQx<Q1Type,double>&, b, x, ...):
- * honei/math/multigrid.hh:2253-2393. */
-int or_mg_mixed_q1(const double *const *A64, const double *b, double *x,
-                   const float *const *a32, const float *const *dinv32,
-                   const or_mg_params *p, uint64_t *outer_iters, uint64_t *inner_iters,
-                   double *final_defect, double *initial_defect_out)
+/* Multigrid<...MIXED>::value: honei/math/multigrid.hh:2253-2393 (Q1) and
+ * 2400-2543 (ELL), the same outer loop: x0 with b injected on the first row,
+ * the last row and the first column; result = 0; d = b - A x0 (product then
+ * difference); loop { rhs32 = (float)d; inner FP32 solve; result += scale *
+ * x32; d = b - A result; stop at |d| < eps |d0|; scale = |d|; d /= scale }.
+ * The inner FP32 iterate x32 carries over from one outer step to the next. */
+int or_mg_mixed(int kind, const double *const *A64_q1, const or_ell_f64 *A64_ell, const double *b,
+                double *x, const float *const *a32_q1, const or_ell_f32 *a32_ell,
+                const float *const *dinv32, const or_ell_f32 *prol32, const or_ell_f32 *res32,
+                const or_mg_params *p, uint64_t *outer_iters, uint64_t *inner_iters,
+                double *final_defect, double *initial_defect_out)
 {
     if (p->max_level < p->min_level || p->max_level > 14 || p->n_pre_smooth < 1)
         return -1;
     const uint64_t maxl = p->max_level;
-    const uint64_t n = ((1ull << maxl) + 1) * ((1ull << maxl) + 1);
     /* MGInfo<float, ...> stores tolerance and the correction factor as float */
     const float tol = (float)p->tolerance;
     const float adapt = (float)p->adapt_correction_factor;
-
-    or_mg_level lv[15];
-    memset(lv, 0, sizeof(lv));
-    for (uint64_t L = p->min_level; L <= maxl; ++L) {
-        uint64_t nl = ((1ull << L) + 1) * ((1ull << L) + 1);
-        lv[L].n = nl;
-        lv[L].a = a32 + 9 * L;
-        lv[L].dinv = dinv32[L];
-        lv[L].c = (float *)calloc(nl, sizeof(float));
-        lv[L].d = (float *)calloc(nl, sizeof(float));
-        lv[L].rhs = (float *)calloc(nl, sizeof(float));
-        lv[L].x = (float *)calloc(nl, sizeof(float));
-        lv[L].tmp = (float *)calloc(nl, sizeof(float));
+    or_mgl_f32 lv[15];
+    or_mg_levels_f32(lv, kind, a32_q1, a32_ell, dinv32, prol32, res32, p);
+    or_mgl_f32 *top = &lv[maxl];
+    const uint64_t n = top->n;
+    or_matrix_f64 A64;
+    if (kind == 0) {
+        or_matrix_f64 m = {0, A64_q1, NULL, NULL, NULL, 0, 0};
+        A64 = m;
+    } else {
+        or_matrix_f64 m = {1, NULL, A64_ell->Aj, A64_ell->Ax, A64_ell->Arl, A64_ell->stride, A64_ell->threads};
+        A64 = m;
     }
 
     double *result = (double *)calloc(n, sizeof(double));
@@ -209,11 +121,11 @@ int or_mg_mixed_q1(const double *const *A64, const double *b, double *x,
     for (uint64_t i = 0; i < sqrt_N; ++i) ig[i] = b[i];
     for (uint64_t i = n - sqrt_N; i < n; ++i) ig[i] = b[i];
     for (uint64_t i = 0; i < n; i += sqrt_N) ig[i] = b[i];
-    or_convert_d2f(lv[maxl].x, ig, n);
+    or_convert_d2f(top->x, ig, n);
 
     uint64_t inner_total = 0, outer = 0;
     double scale_factor = 1.0;
-    or_q1_product_f64(prod, A64, ig, n);
+    or_apply_f64(&A64, prod, ig, n);
     memcpy(od, b, sizeof(double) * n);
     or_difference_f64(od, prod, n);
     double initial_defect = or_norm_l2_f64(od, n, 1);
@@ -226,14 +138,13 @@ int or_mg_mixed_q1(const double *const *A64, const double *b, double *x,
             status = 1;
             break;
         }
-        or_convert_d2f(lv[maxl].rhs, od, n);
-        uint64_t inner = 0;
-        or_mg_kernel(lv, p, tol, adapt, &inner);
+        or_convert_d2f(top->rhs, od, n);
+        uint64_t inner = or_mg_inner_f32(lv, p, tol, adapt, NULL, NULL);
         ++outer;
         inner_total += inner;
-        or_convert_f2d(x_outer, lv[maxl].x, n);
+        or_convert_f2d(x_outer, top->x, n);
         or_scaled_sum_f64(result, x_outer, scale_factor, n);
-        or_q1_defect_f64(od, b, A64, result, n);
+        or_defect_any_f64(&A64, od, b, result, n);
         def_norm = or_norm_l2_f64(od, n, 1);
         if (def_norm < outer_eps * initial_defect)
             break;
@@ -246,12 +157,18 @@ int or_mg_mixed_q1(const double *const *A64, const double *b, double *x,
     *inner_iters = inner_total;
     if (final_defect) *final_defect = def_norm;
     if (initial_defect_out) *initial_defect_out = initial_defect;
-
-    for (uint64_t L = p->min_level; L <= maxl; ++L) {
-        free(lv[L].c); free(lv[L].d); free(lv[L].rhs); free(lv[L].x); free(lv[L].tmp);
-    }
+    or_mg_free_f32(lv, p);
     free(result); free(ig); free(od); free(prod); free(x_outer);
     return status;
+}
+
+int or_mg_mixed_q1(const double *const *A64, const double *b, double *x,
+                   const float *const *a32, const float *const *dinv32,
+                   const or_mg_params *p, uint64_t *outer_iters, uint64_t *inner_iters,
+                   double *final_defect, double *initial_defect_out)
+{
+    return or_mg_mixed(0, A64, NULL, b, x, a32, NULL, dinv32, NULL, NULL, p, outer_iters, inner_iters,
+                       final_defect, initial_defect_out);
 }
 
 /* IRSolver<OuterTag, InnerTag, Norm<vnt_l_two, true, OuterTag>>::value

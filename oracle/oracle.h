@@ -62,6 +62,16 @@ void or_ell_shape(uint64_t rows, const uint64_t *rowptr, uint64_t threads,
 uint64_t or_rng_bits(uint64_t seed, uint64_t stream, uint64_t index);
 double or_rng_uniform(uint64_t seed, uint64_t stream, uint64_t index, double lo, double hi);
 
+#define OR_ELL_TYPE(T, S)                                                                 \
+typedef struct or_ell_##S {                                                              \
+    uint64_t rows, cols, stride, threads;                                                \
+    const uint64_t *Aj;                                                                  \
+    const T *Ax;                                                                         \
+    const uint64_t *Arl;                                                                 \
+} or_ell_##S;
+OR_ELL_TYPE(double, f64)
+OR_ELL_TYPE(float, f32)
+
 #define OR_DECLARE(T, S)                                                                  \
 void or_ell_from_csr_##S(uint64_t rows, const uint64_t *rowptr, const uint64_t *col,     \
                          const T *val, uint64_t threads, uint64_t stride,                \
@@ -110,12 +120,12 @@ int or_cg_var_ell_##S(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,     
                       uint64_t n, uint64_t max_iters, T eps, uint64_t *used_iters);      \
 int or_cg_konv_q1_##S(const T *const *bands, const T *rhs, T *x, uint64_t n,            \
                       double konv_rad, uint64_t max_iters, uint64_t *iters);             \
-typedef struct or_ell_##S {                                                              \
-    uint64_t rows, cols, stride, threads;                                                \
-    const uint64_t *Aj;                                                                  \
-    const T *Ax;                                                                         \
-    const uint64_t *Arl;                                                                 \
-} or_ell_##S;                                                                            \
+int or_cg_konv_ell_##S(const or_ell_##S *M, const T *rhs, T *x, double konv_rad,          \
+                       uint64_t max_iters, uint64_t *iters);                             \
+int or_mg_fixed_##S(int kind, const T *const *a_q1, const or_ell_##S *a_ell,             \
+                    const T *const *dinv, const or_ell_##S *prol, const or_ell_##S *res, \
+                    const T *b, T *x, const or_mg_params *p, uint64_t *iters,            \
+                    T *final_defect, T *initial_defect);                                 \
 int or_mgop_solve_##S(const or_mgop_params *p, const or_ell_##S *A, const T *const *P,  \
                       const or_ell_##S *prol, const or_ell_##S *res, const T *b, T *x,   \
                       uint64_t *used_iters, uint64_t *used_iters_coarse, T *final_norm);
@@ -148,7 +158,16 @@ int or_ir_mgop(const or_mgop_params *p, const or_ell_f64 *A64, const double *b, 
                const or_ell_f32 *res, float *x_inner, double eps, uint64_t max_iters,
                uint64_t *used_iters, uint64_t *inner_used, uint64_t *coarse_used);
 
-/* Multigrid<..., CYCLE::V, FIXED> on one precision is not restated. */
+/* Multigrid<..., ProlType, JAC, CYCLE::V, MIXED> on Q1 bands (kind 0:
+ * A64_q1 finest FP64 bands, a32_q1[9 L + b]) or ELL matrices (kind 1:
+ * A64_ell finest, a32_ell[L]); dinv32[L]; prol32[L] / res32[L] for
+ * Prolongation/Restriction<PROLMAT> (NULL: structured transfers with the
+ * border mask).  Outer loop: multigrid.hh:2253-2393 (Q1), 2400-2543 (ELL). */
+int or_mg_mixed(int kind, const double *const *A64_q1, const or_ell_f64 *A64_ell, const double *b,
+                double *x, const float *const *a32_q1, const or_ell_f32 *a32_ell,
+                const float *const *dinv32, const or_ell_f32 *prol32, const or_ell_f32 *res32,
+                const or_mg_params *p, uint64_t *outer_iters, uint64_t *inner_iters,
+                double *final_defect, double *initial_defect);
 
 #ifdef __cplusplus
 }

@@ -558,8 +558,8 @@ int FN(or_cg_var_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
 /* ConjugateGradients<Tag, NONE>::value(Q1, rhs, x, konv_rad):
  * honei/math/conjugate_gradients.hh:490-526 with cg_kernel :108-151.
  * max_iters is a safety cap the reference does not have (0 = none). */
-int FN(or_cg_konv_q1)(const T *const *bands, const T *rhs, T *x, uint64_t n,
-                      double konv_rad, uint64_t max_iters, uint64_t *iters)
+static int FN(or_cg_konv_any)(const FN(or_matrix) *A, const T *rhs, T *x, uint64_t n,
+                              double konv_rad, uint64_t max_iters, uint64_t *iters)
 {
     T *g = (T *)malloc(sizeof(T) * n), *u = (T *)malloc(sizeof(T) * n);
     T *energy = (T *)malloc(sizeof(T) * n), *ng = (T *)malloc(sizeof(T) * n);
@@ -568,7 +568,7 @@ int FN(or_cg_konv_q1)(const T *const *bands, const T *rhs, T *x, uint64_t n,
 
     for (uint64_t i = 0; i < n; ++i)
         x[i] = 0;
-    FN(or_q1_product)(g, bands, x, n);
+    FN(or_apply)(A, g, x, n);
     FN(or_difference)(g, rhs, n);
     memcpy(u, g, sizeof(T) * n);
     FN(or_scale)(u, (T)-1., n);
@@ -581,7 +581,7 @@ int FN(or_cg_konv_q1)(const T *const *bands, const T *rhs, T *x, uint64_t n,
         }
         /* cg_kernel */
         T upper = FN(or_norm_l2)(g, n, 0);
-        FN(or_q1_product)(energy, bands, u, n);
+        FN(or_apply)(A, energy, u, n);
         T lower = FN(or_dot)(energy, u, n);
         memcpy(energy, u, sizeof(T) * n); /* u_c = copy(u); energy = u_c */
         if (fabs(lower) >= OR_EPS)
@@ -590,7 +590,7 @@ int FN(or_cg_konv_q1)(const T *const *bands, const T *rhs, T *x, uint64_t n,
             FN(or_scale)(energy, (T)(upper / (T)OR_EPS), n);
         FN(or_sum)(energy, x, n);
         lower = upper;
-        FN(or_q1_product)(ng, bands, energy, n);
+        FN(or_apply)(A, ng, energy, n);
         FN(or_difference)(ng, rhs, n);
         upper = FN(or_norm_l2)(ng, n, 0);
         if (fabs(lower) >= OR_EPS)
@@ -607,6 +607,23 @@ int FN(or_cg_konv_q1)(const T *const *bands, const T *rhs, T *x, uint64_t n,
     *iters = it;
     free(g); free(u); free(energy); free(ng);
     return capped;
+}
+
+/* ConjugateGradients<Tag, NONE>::value(A, rhs, x, konv_rad): Q1
+ * conjugate_gradients.hh:490-526 with kernel :108-151; ELL :588-620 with
+ * kernel :153-200 (the same steps on ELL products). */
+int FN(or_cg_konv_q1)(const T *const *bands, const T *rhs, T *x, uint64_t n,
+                      double konv_rad, uint64_t max_iters, uint64_t *iters)
+{
+    FN(or_matrix) A = {0, bands, NULL, NULL, NULL, 0, 0};
+    return FN(or_cg_konv_any)(&A, rhs, x, n, konv_rad, max_iters, iters);
+}
+
+int FN(or_cg_konv_ell)(const OR_CAT(or_ell, OR_S) *M, const T *rhs, T *x, double konv_rad,
+                       uint64_t max_iters, uint64_t *iters)
+{
+    FN(or_matrix) A = {1, NULL, M->Aj, M->Ax, M->Arl, M->stride, M->threads};
+    return FN(or_cg_konv_any)(&A, rhs, x, M->rows, konv_rad, max_iters, iters);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -730,6 +747,181 @@ int FN(or_mgop_solve)(const or_mgop_params *p, const OR_CAT(or_ell, OR_S) *A, co
             free(vec[v][i]);
         free(vec[v]);
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multigrid<Tag, OuterTag, ProlType, JAC, CYCLE::V, FIXED|MIXED>: the      */
+/* V-cycle kernel _multigrid_kernel (multigrid.hh:160-466 Q1 and 469-820 ELL */
+/* FIXED, 1620-1933 Q1 and 1936-2335 ELL MIXED: the same steps on either    */
+/* matrix kind), with structured (NONE) or matrix (PROLMAT) transfers.       */
+/* ------------------------------------------------------------------------ */
+typedef struct FN(or_mgl) {
+    uint64_t n;
+    FN(or_matrix) A;
+    const T *dinv;
+    const OR_CAT(or_ell, OR_S) *prol;   /* level-1 -> level (PROLMAT), or NULL */
+    const OR_CAT(or_ell, OR_S) *res;    /* level -> level-1 (PROLMAT), or NULL */
+    T *c, *d, *rhs, *x, *tmp;
+} FN(or_mgl);
+
+static void FN(or_mgl_jacobi)(FN(or_mgl) *l, T *x, const T *rhs, uint64_t iters)
+{
+    for (uint64_t it = 0; it < iters; ++it) {
+        FN(or_defect_any)(&l->A, l->tmp, rhs, x, l->n);
+        FN(or_scaled_sum_vv)(x, l->tmp, l->dinv, l->n);
+    }
+}
+
+static void FN(or_mgl_product)(const OR_CAT(or_ell, OR_S) *M, T *y, const T *x)
+{
+    FN(or_ell_product)(y, M->Aj, M->Ax, M->Arl, x, 0, M->rows, M->stride, M->threads);
+}
+
+/* lv indexed by level; returns the V-cycles run (MGInfo::inner_iterations) */
+static uint64_t FN(or_mg_inner)(FN(or_mgl) *lv, const or_mg_params *p, T tol, T adapt, T *defect_out,
+                                T *initial_out)
+{
+    const uint64_t minl = p->min_level, maxl = p->max_level;
+    FN(or_mgl) *top = &lv[maxl];
+    if (p->initial_zero)
+        memcpy(top->d, top->rhs, sizeof(T) * top->n);
+    else
+        FN(or_defect_any)(&top->A, top->d, top->rhs, top->x, top->n);
+    T defect = FN(or_norm_l2)(top->d, top->n, 1);
+    const T initial_defect = defect;
+    if (initial_out) *initial_out = initial_defect;
+    if (defect_out) *defect_out = defect;
+    if (p->convergence_check && defect <= tol)
+        return 0;
+    uint64_t iter;
+    for (iter = 1; iter <= p->n_max_iter; ++iter) {
+        uint64_t L = maxl;
+        int restriction_started = 1;
+        while (minl != maxl) {
+            FN(or_mgl) *l = &lv[L];
+            if (restriction_started) {
+                FN(or_element_product3)(l->c, l->dinv, l->d, l->n);
+                FN(or_mgl_jacobi)(l, l->c, l->d, p->n_pre_smooth - 1);
+                FN(or_sum)(l->x, l->c, l->n);
+            } else {
+                FN(or_element_product3)(l->x, l->dinv, l->d, l->n);
+                FN(or_mgl_jacobi)(l, l->x, l->d, p->n_pre_smooth - 1);
+            }
+            FN(or_defect_any)(&l->A, l->d, l->rhs, l->x, l->n);
+            --L;
+            if (lv[L + 1].res)
+                FN(or_mgl_product)(lv[L + 1].res, lv[L].d, lv[L + 1].d);
+            else
+                FN(or_restriction)(lv[L].d, lv[L + 1].d, lv[L + 1].n, p->macro_border_mask);
+            memcpy(lv[L].rhs, lv[L].d, sizeof(T) * lv[L].n);
+            if (L == minl)
+                break;
+            restriction_started = 0;
+        }
+        {
+            uint64_t cg_iters;
+            FN(or_cg_konv_any)(&lv[L].A, lv[L].d, lv[L].x, lv[L].n, OR_EPS, 0, &cg_iters);
+        }
+        if (minl == maxl) {
+            FN(or_defect_any)(&lv[L].A, lv[L].d, lv[L].rhs, lv[L].x, lv[L].n);
+            break;
+        }
+        while (1) {
+            ++L;
+            FN(or_mgl) *l = &lv[L];
+            if (l->prol)
+                FN(or_mgl_product)(l->prol, l->c, lv[L - 1].x);
+            else
+                FN(or_prolongation)(l->c, lv[L - 1].x, l->n, p->macro_border_mask);
+            T alpha;
+            if (adapt == (T)0) {
+                T d1 = FN(or_dot)(l->d, l->c, l->n);
+                FN(or_apply)(&l->A, l->tmp, l->c, l->n);
+                T d2 = FN(or_dot)(l->tmp, l->c, l->n);
+                alpha = d1 / d2;
+            } else {
+                alpha = adapt;
+            }
+            FN(or_scaled_sum)(l->x, l->c, alpha, l->n);
+            FN(or_mgl_jacobi)(l, l->x, l->rhs, p->n_pre_smooth);
+            FN(or_defect_any)(&l->A, l->d, l->rhs, l->x, l->n);
+            if (L == maxl)
+                break;
+        }
+        defect = FN(or_norm_l2)(top->d, top->n, 1);
+        if (defect_out) *defect_out = defect;
+        if (defect <= initial_defect * tol)
+            break;
+    }
+    return (iter > p->n_max_iter) ? iter - 1 : iter;
+}
+
+/* Levels min..max of either kind (kind 0: a_q1[9 L + b]; kind 1: a_ell[L]),
+ * dinv[L], PROLMAT matrices prol[L] / res[L] for L in (min, max] or NULL. */
+static int FN(or_mg_levels)(FN(or_mgl) *lv, int kind, const T *const *a_q1, const OR_CAT(or_ell, OR_S) *a_ell,
+                            const T *const *dinv, const OR_CAT(or_ell, OR_S) *prol,
+                            const OR_CAT(or_ell, OR_S) *res, const or_mg_params *p)
+{
+    memset(lv, 0, sizeof(FN(or_mgl)) * 15);
+    for (uint64_t L = p->min_level; L <= p->max_level; ++L) {
+        FN(or_mgl) *l = &lv[L];
+        if (kind == 0) {
+            l->n = ((1ull << L) + 1) * ((1ull << L) + 1);
+            FN(or_matrix) A = {0, a_q1 + 9 * L, NULL, NULL, NULL, 0, 0};
+            l->A = A;
+        } else {
+            const OR_CAT(or_ell, OR_S) *M = &a_ell[L];
+            l->n = M->rows;
+            FN(or_matrix) A = {1, NULL, M->Aj, M->Ax, M->Arl, M->stride, M->threads};
+            l->A = A;
+        }
+        l->dinv = dinv[L];
+        if (L > p->min_level && prol && res) {
+            l->prol = &prol[L];
+            l->res = &res[L];
+        }
+        l->c = (T *)calloc(l->n, sizeof(T));
+        l->d = (T *)calloc(l->n, sizeof(T));
+        l->rhs = (T *)calloc(l->n, sizeof(T));
+        l->x = (T *)calloc(l->n, sizeof(T));
+        l->tmp = (T *)calloc(l->n, sizeof(T));
+    }
+    return 0;
+}
+
+static void FN(or_mg_free)(FN(or_mgl) *lv, const or_mg_params *p)
+{
+    for (uint64_t L = p->min_level; L <= p->max_level; ++L) {
+        free(lv[L].c); free(lv[L].d); free(lv[L].rhs); free(lv[L].x); free(lv[L].tmp);
+    }
+}
+
+/* Multigrid<..., FIXED>::value (multigrid.hh:790-872): x0 with b injected on
+ * the first row, the last row and the first column, then one inner solve
+ * in this precision. */
+int FN(or_mg_fixed)(int kind, const T *const *a_q1, const OR_CAT(or_ell, OR_S) *a_ell, const T *const *dinv,
+                    const OR_CAT(or_ell, OR_S) *prol, const OR_CAT(or_ell, OR_S) *res, const T *b, T *x,
+                    const or_mg_params *p, uint64_t *iters, T *final_defect, T *initial_defect)
+{
+    if (p->max_level < p->min_level || p->max_level > 14 || p->n_pre_smooth < 1)
+        return -1;
+    FN(or_mgl) lv[15];
+    FN(or_mg_levels)(lv, kind, a_q1, a_ell, dinv, prol, res, p);
+    FN(or_mgl) *top = &lv[p->max_level];
+    const uint64_t n = top->n;
+    const uint64_t sq = (uint64_t)sqrt((double)n);
+    memcpy(top->x, x, sizeof(T) * n);
+    for (uint64_t i = 0; i < sq; ++i) top->x[i] = b[i];
+    for (uint64_t i = n - sq; i < n; ++i) top->x[i] = b[i];
+    for (uint64_t i = 0; i < n; i += sq) top->x[i] = b[i];
+    memcpy(top->rhs, b, sizeof(T) * n);
+    T fd = 0, id = 0;
+    *iters = FN(or_mg_inner)(lv, p, (T)p->tolerance, (T)p->adapt_correction_factor, &fd, &id);
+    memcpy(x, top->x, sizeof(T) * n);
+    if (final_defect) *final_defect = fd;
+    if (initial_defect) *initial_defect = id;
+    FN(or_mg_free)(lv, p);
     return 0;
 }
 

@@ -520,3 +520,130 @@ Reference.ir_mgop = _ref_ir_mgop
 
 def have_reference() -> bool:
     return os.path.exists(REF_SO)
+
+
+# -- Multigrid<..., FIXED | MIXED> on Q1 bands or ELL matrices ---------------
+class RefEll(C.Structure):
+    """ref_ell of oracle/ref/refdrv.cc."""
+    _fields_ = [("rows", u64), ("cols", u64), ("stride", u64), ("ncpr", u64), ("Aj", vp), ("Ax", vp)]
+
+
+def _level_ells(struct, mats, max_level):
+    """Array indexed by level (entries below the lowest given level are empty)."""
+    arr = (struct * (max_level + 1))()
+    for L, E in mats.items():
+        if struct is OrEll:
+            arr[L] = OrEll(E["rows"], E.get("cols", E["rows"]), E["stride"], E["threads"], E["Aj"].ctypes.data,
+                           E["Ax"].ctypes.data, E["Arl"].ctypes.data)
+        else:
+            arr[L] = RefEll(E["rows"], E.get("cols", E["rows"]), E["stride"], E["ncpr"], E["Aj"].ctypes.data,
+                            E["Ax"].ctypes.data)
+    return arr
+
+
+def _level_ptrs(arrs, max_level, per=1):
+    """vp array [L * per + k] from a dict level -> array (per == 1) or list of arrays."""
+    dummy = np.zeros(16, np.float64)
+    out = (vp * ((max_level + 1) * per))()
+    for L in range(max_level + 1):
+        for k in range(per):
+            a = arrs.get(L) if per == 1 else (arrs[L][k] if L in arrs else None)
+            out[L * per + k] = (a if a is not None else dummy).ctypes.data
+    return out
+
+
+def _mg_params(min_level, max_level, n_max_iter, tolerance, n_pre, adapt, mask, initial_zero, convergence_check,
+               max_outer=0):
+    return MGParams(min_level, max_level, n_max_iter, int(initial_zero), int(convergence_check), tolerance, n_pre,
+                    adapt, (u64 * 8)(*mask), max_outer)
+
+
+def _oracle_mg_mixed(self, *, kind, b, x0, A64, a32, dinv32, prol32=None, res32=None, min_level, max_level,
+                     n_max_iter=2, tolerance=1e-8, n_pre=2, adapt=1.0, mask=(2,) * 8, initial_zero=True,
+                     convergence_check=False):
+    """kind "q1": A64 = 9 FP64 bands, a32 = {L: 9 bands}; kind "ell": A64 = ELL, a32 = {L: ELL}.
+    prol32 / res32 = {L: ELL} for PROLMAT transfers, else structured."""
+    p = _mg_params(min_level, max_level, n_max_iter, tolerance, n_pre, adapt, mask, initial_zero, convergence_check)
+    x = np.ascontiguousarray(x0, np.float64).copy()
+    o, i, fd, idf = u64(), u64(), C.c_double(), C.c_double()
+    L = self.lib
+    L.or_mg_mixed.argtypes = [C.c_int] + [vp] * 9 + [C.POINTER(MGParams), vp, vp, vp, vp]
+    if kind == "q1":
+        args = (0, _pp(A64), None, _p(b), _p(x), _level_ptrs(a32, max_level, 9), None)
+    else:
+        args = (1, None, _ells([A64]), _p(b), _p(x), None, _level_ells(OrEll, a32, max_level))
+    pr = _level_ells(OrEll, prol32, max_level) if prol32 else None
+    rs = _level_ells(OrEll, res32, max_level) if res32 else None
+    st = L.or_mg_mixed(*args, _level_ptrs(dinv32, max_level), pr, rs, C.byref(p), C.byref(o), C.byref(i),
+                       C.byref(fd), C.byref(idf))
+    return dict(x=x, outer=o.value, inner=i.value, final_defect=fd.value, initial_defect=idf.value, status=st)
+
+
+def _oracle_mg_fixed(self, *, kind, b, x0, a, dinv, prol=None, res=None, min_level, max_level, n_max_iter=16,
+                     tolerance=1e-8, n_pre=2, adapt=1.0, mask=(2,) * 8, initial_zero=True, convergence_check=False):
+    dt = np.dtype(b.dtype)
+    s = _sfx(dt)
+    p = _mg_params(min_level, max_level, n_max_iter, tolerance, n_pre, adapt, mask, initial_zero, convergence_check)
+    x = np.ascontiguousarray(x0, dt).copy()
+    it = u64()
+    fd = C.c_double() if dt == np.float64 else C.c_float()
+    idf = C.c_double() if dt == np.float64 else C.c_float()
+    f = getattr(self.lib, f"or_mg_fixed_{s}")
+    f.argtypes = [C.c_int] + [vp] * 7 + [C.POINTER(MGParams), vp, vp, vp]
+    if kind == "q1":
+        aq, ae = _level_ptrs(a, max_level, 9), None
+    else:
+        aq, ae = None, _level_ells(OrEll, a, max_level)
+    pr = _level_ells(OrEll, prol, max_level) if prol else None
+    rs = _level_ells(OrEll, res, max_level) if res else None
+    st = f(0 if kind == "q1" else 1, aq, ae, _level_ptrs(dinv, max_level), pr, rs, _p(b), _p(x), C.byref(p),
+           C.byref(it), C.byref(fd), C.byref(idf))
+    return dict(x=x, iters=it.value, final_defect=fd.value, initial_defect=idf.value, status=st)
+
+
+Oracle.mg_mixed = _oracle_mg_mixed
+Oracle.mg_fixed = _oracle_mg_fixed
+
+
+def _ref_mg_mixed(self, *, kind, b, x0, A64, a32, dinv32, prol32=None, res32=None, min_level, max_level,
+                  n_max_iter=2, tolerance=1e-8, n_pre=2, adapt=1.0, mask=(2,) * 8, initial_zero=True,
+                  convergence_check=False):
+    x = np.ascontiguousarray(x0, np.float64).copy()
+    o, i = u64(), u64()
+    L = self.lib
+    L.ref_mg_mixed.argtypes = [C.c_int, C.c_int] + [vp] * 9 + [u64, u64, u64, C.c_double, u64, C.c_double, vp,
+                                                             C.c_int, C.c_int, vp, vp]
+    m = np.asarray(mask, np.uint64)
+    if kind == "q1":
+        a64q, a64e, a32q, a32e = _pp(A64), None, _level_ptrs(a32, max_level, 9), None
+    else:
+        a64q, a64e = None, _level_ells(RefEll, {0: A64}, 0)
+        a32q, a32e = None, _level_ells(RefEll, a32, max_level)
+    pr = _level_ells(RefEll, prol32, max_level) if prol32 else None
+    rs = _level_ells(RefEll, res32, max_level) if res32 else None
+    L.ref_mg_mixed(0 if kind == "q1" else 1, int(bool(prol32)), a64q, a64e, _p(b), _p(x), a32q, a32e, pr, rs,
+                   _level_ptrs(dinv32, max_level), min_level, max_level, n_max_iter, tolerance, n_pre, adapt, _p(m),
+                   int(initial_zero), int(convergence_check), C.byref(o), C.byref(i))
+    return dict(x=x, outer=o.value, inner=i.value)
+
+
+def _ref_mg_fixed(self, *, kind, b, x0, a, dinv, prol=None, res=None, min_level, max_level, n_max_iter=16,
+                  tolerance=1e-8, n_pre=2, adapt=1.0, mask=(2,) * 8, initial_zero=True, convergence_check=False):
+    dt = np.dtype(b.dtype)
+    x = np.ascontiguousarray(x0, dt).copy()
+    f = getattr(self.lib, f"ref_mg_fixed_{_sfx(dt)}")
+    f.argtypes = [C.c_int, C.c_int] + [vp] * 7 + [u64, u64, u64, C.c_double, u64, C.c_double, vp, C.c_int, C.c_int]
+    m = np.asarray(mask, np.uint64)
+    if kind == "q1":
+        aq, ae = _level_ptrs(a, max_level, 9), None
+    else:
+        aq, ae = None, _level_ells(RefEll, a, max_level)
+    pr = _level_ells(RefEll, prol, max_level) if prol else None
+    rs = _level_ells(RefEll, res, max_level) if res else None
+    f(0 if kind == "q1" else 1, int(bool(prol)), aq, ae, pr, rs, _level_ptrs(dinv, max_level), _p(b), _p(x),
+      min_level, max_level, n_max_iter, tolerance, n_pre, adapt, _p(m), int(initial_zero), int(convergence_check))
+    return dict(x=x)
+
+
+Reference.mg_mixed = _ref_mg_mixed
+Reference.mg_fixed = _ref_mg_fixed

@@ -421,6 +421,221 @@ int ref_ir_mgop(const char * base, uint64_t levels, uint64_t min_level, uint64_t
     return 0;
 }
 
+}  // extern "C"
+
+/* ---- Multigrid<CPU, CPU, NONE|PROLMAT, JAC, CYCLE::V, FIXED|MIXED> on Q1 bands
+ * or ELL matrices, with the MGInfo set-up of the reference's tests
+ * (poisson_mg_*_TEST.cc): levels below min_level hold dummies, dinv is given. */
+struct ref_ell
+{
+    uint64_t rows, cols, stride, ncpr;
+    const uint64_t * Aj;
+    const void * Ax;
+};
+
+namespace
+{
+    template <typename DT_> BandedMatrixQx<Q1Type, DT_> q1_dummy(unsigned long size)
+    {
+        DenseVector<DT_> z(size, DT_(0));
+        return BandedMatrixQx<Q1Type, DT_>(size, z, z, z, z, z, z, z, z, z);
+    }
+
+    template <typename DT_> SparseMatrixELL<DT_> ell_of(const ref_ell & e)
+    {
+        return ell<DT_>(e.rows, e.cols, e.stride, e.ncpr, 1, e.Aj, (const DT_ *)e.Ax);
+    }
+
+    template <typename DT_> SparseMatrixELL<DT_> ell_dummy()
+    {
+        SparseMatrix<DT_> s(1, 1);
+        s(0, 0, DT_(1));
+        return SparseMatrixELL<DT_>(s);
+    }
+
+    template <typename Layout_> struct MGMats;
+    template <typename DT_> struct MGMats<BandedMatrixQx<Q1Type, DT_> >
+    {
+        static BandedMatrixQx<Q1Type, DT_> level(const void * const * a_q1, const ref_ell *, unsigned long L,
+                                                 unsigned long n)
+        {
+            return q1((const DT_ * const *)a_q1 + 9 * L, n);
+        }
+        static BandedMatrixQx<Q1Type, DT_> dummy(unsigned long n) { return q1_dummy<DT_>(n); }
+        static BandedMatrixQx<Q1Type, DT_> transfer(const ref_ell *, unsigned long, unsigned long n)
+        {
+            return q1_dummy<DT_>(n);
+        }
+    };
+    template <typename DT_> struct MGMats<SparseMatrixELL<DT_> >
+    {
+        static SparseMatrixELL<DT_> level(const void * const *, const ref_ell * a_ell, unsigned long L, unsigned long)
+        {
+            return ell_of<DT_>(a_ell[L]);
+        }
+        static SparseMatrixELL<DT_> dummy(unsigned long) { return ell_dummy<DT_>(); }
+        static SparseMatrixELL<DT_> transfer(const ref_ell * t, unsigned long L, unsigned long)
+        {
+            return t ? ell_of<DT_>(t[L]) : ell_dummy<DT_>();
+        }
+    };
+
+    template <typename DT_, typename Layout_>
+    void mg_info(MGInfo<DT_, Layout_> & info, const void * const * a_q1, const ref_ell * a_ell,
+                 const ref_ell * prol, const ref_ell * res, const void * const * dinv, uint64_t min_level,
+                 uint64_t max_level, uint64_t n_max_iter, double tolerance, uint64_t n_pre, double adapt,
+                 const uint64_t * mask, int initial_zero, int convergence_check)
+    {
+        info.is_smoother = false;
+        info.macro_border_mask = DenseVector<unsigned long>(8);
+        for (int i(0) ; i < 8 ; ++i)
+            info.macro_border_mask[i] = mask[i];
+        info.min_level = min_level;
+        info.max_level = max_level;
+        info.n_max_iter = n_max_iter;
+        info.initial_zero = initial_zero != 0;
+        info.tolerance = tolerance;
+        info.convergence_check = convergence_check != 0;
+        info.n_pre_smooth = n_pre;
+        info.n_post_smooth = n_pre;
+        info.n_max_iter_coarse = 0;
+        info.tolerance_coarse = 1e-2;
+        info.adapt_correction_factor = adapt;
+        for (unsigned long L(0) ; L <= max_level ; ++L)
+        {
+            const bool real = L >= min_level;
+            unsigned long size = ((1ul << L) + 1) * ((1ul << L) + 1);
+            if (real && a_ell)
+                size = a_ell[L].rows;
+            info.a.push_back(real ? MGMats<Layout_>::level(a_q1, a_ell, L, size) : MGMats<Layout_>::dummy(size));
+            info.prolmats.push_back(real && L > min_level ? MGMats<Layout_>::transfer(prol, L, size)
+                                                          : MGMats<Layout_>::dummy(size));
+            info.resmats.push_back(real && L > min_level ? MGMats<Layout_>::transfer(res, L, size)
+                                                         : MGMats<Layout_>::dummy(size));
+            info.c.push_back(DenseVector<DT_>(size, DT_(0)));
+            info.d.push_back(DenseVector<DT_>(size, DT_(0)));
+            info.rhs.push_back(DenseVector<DT_>(size, DT_(0)));
+            info.x.push_back(DenseVector<DT_>(size, DT_(0)));
+            info.diags_inverted.push_back(real ? dv((const DT_ *)dinv[L], size) : DenseVector<DT_>(size, DT_(0)));
+        }
+    }
+
+    template <typename Prol_, typename Layout_, typename Outer_>
+    void mg_mixed_run(const Outer_ & A, const double * b, double * x, MGInfo<float, Layout_> & info,
+                      uint64_t n, uint64_t * outer, uint64_t * inner)
+    {
+        DenseVector<double> bv(dv(b, n)), xv(dv(x, n));
+        Outer_ Ac(A);
+        std::stringstream captured;
+        std::streambuf * old(std::cout.rdbuf(captured.rdbuf()));
+        Multigrid<tags::CPU, tags::CPU, Prol_, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
+                Ac, bv, xv, (unsigned long)11, std::numeric_limits<double>::epsilon(), info);
+        std::cout.rdbuf(old);
+        out(x, xv);
+        std::string line;
+        *outer = 0;
+        *inner = 0;
+        while (std::getline(captured, line))
+        {
+            if (line.find("TN of outer iters: ") == 0)
+                *outer = std::strtoul(line.c_str() + 19, 0, 10);
+            if (line.find("TN of inner iters: ") == 0)
+                *inner = std::strtoul(line.c_str() + 19, 0, 10);
+        }
+    }
+
+    template <typename Prol_, typename DT_, typename Layout_>
+    void mg_fixed_run(Layout_ & A, const DT_ * b, DT_ * x, MGInfo<DT_, Layout_> & info, uint64_t n)
+    {
+        DenseVector<DT_> bv(dv(b, n)), xv(dv(x, n));
+        info.rhs[info.max_level] = bv;
+        std::stringstream captured;
+        std::streambuf * old(std::cout.rdbuf(captured.rdbuf()));
+        Multigrid<tags::CPU, tags::CPU, Prol_, methods::JAC, methods::CYCLE::V, methods::FIXED>::value(
+                A, bv, xv, (unsigned long)11, std::numeric_limits<DT_>::epsilon(), info);
+        std::cout.rdbuf(old);
+        out(x, xv);
+    }
+
+    template <typename DT_, typename Layout_>
+    int mg_fixed_any(int prolmat, const void * const * a_q1, const ref_ell * a_ell, const ref_ell * prol,
+                     const ref_ell * res, const void * const * dinv, const DT_ * b, DT_ * x, uint64_t min_level,
+                     uint64_t max_level, uint64_t n_max_iter, double tolerance, uint64_t n_pre, double adapt,
+                     const uint64_t * mask, int initial_zero, int convergence_check)
+    {
+        MGInfo<DT_, Layout_> info;
+        mg_info(info, a_q1, a_ell, prol, res, dinv, min_level, max_level, n_max_iter, tolerance, n_pre, adapt, mask,
+                initial_zero, convergence_check);
+        const uint64_t n = info.a[max_level].rows();
+        Layout_ A(info.a[max_level]);
+        if (prolmat)
+            mg_fixed_run<methods::PROLMAT>(A, b, x, info, n);
+        else
+            mg_fixed_run<methods::NONE>(A, b, x, info, n);
+        return 0;
+    }
+}
+
+extern "C" {
+
+/* kind 0: Q1 (a32_q1 bands [9 L + b], A64_q1 finest FP64 bands); kind 1: ELL
+ * (a32_ell[L], A64_ell).  prol/res: PROLMAT matrices per level, or NULL. */
+int ref_mg_mixed(int kind, int prolmat, const void * const * A64_q1, const ref_ell * A64_ell, const double * b,
+                 double * x, const void * const * a32_q1, const ref_ell * a32_ell, const ref_ell * prol32,
+                 const ref_ell * res32, const void * const * dinv32, uint64_t min_level, uint64_t max_level,
+                 uint64_t n_max_iter, double tolerance, uint64_t n_pre, double adapt, const uint64_t * mask,
+                 int initial_zero, int convergence_check, uint64_t * outer, uint64_t * inner)
+{
+    if (kind == 0)
+    {
+        MGInfo<float, BandedMatrixQx<Q1Type, float> > info;
+        mg_info(info, a32_q1, (const ref_ell *)0, prol32, res32, dinv32, min_level, max_level, n_max_iter, tolerance,
+                n_pre, adapt, mask, initial_zero, convergence_check);
+        const unsigned long n = ((1ul << max_level) + 1) * ((1ul << max_level) + 1);
+        BandedMatrixQx<Q1Type, double> A(q1((const double * const *)A64_q1, n));
+        if (prolmat)
+            mg_mixed_run<methods::PROLMAT>(A, b, x, info, n, outer, inner);
+        else
+            mg_mixed_run<methods::NONE>(A, b, x, info, n, outer, inner);
+    }
+    else
+    {
+        MGInfo<float, SparseMatrixELL<float> > info;
+        mg_info(info, (const void * const *)0, a32_ell, prol32, res32, dinv32, min_level, max_level, n_max_iter,
+                tolerance, n_pre, adapt, mask, initial_zero, convergence_check);
+        SparseMatrixELL<double> A(ell_of<double>(*A64_ell));
+        if (prolmat)
+            mg_mixed_run<methods::PROLMAT>(A, b, x, info, A64_ell->rows, outer, inner);
+        else
+            mg_mixed_run<methods::NONE>(A, b, x, info, A64_ell->rows, outer, inner);
+    }
+    return 0;
+}
+
+int ref_mg_fixed_f32(int kind, int prolmat, const void * const * a_q1, const ref_ell * a_ell, const ref_ell * prol,
+                     const ref_ell * res, const void * const * dinv, const float * b, float * x, uint64_t min_level,
+                     uint64_t max_level, uint64_t n_max_iter, double tolerance, uint64_t n_pre, double adapt,
+                     const uint64_t * mask, int initial_zero, int convergence_check)
+{
+    if (kind == 0)
+        return mg_fixed_any<float, BandedMatrixQx<Q1Type, float> >(prolmat, a_q1, a_ell, prol, res, dinv, b, x,
+                min_level, max_level, n_max_iter, tolerance, n_pre, adapt, mask, initial_zero, convergence_check);
+    return mg_fixed_any<float, SparseMatrixELL<float> >(prolmat, a_q1, a_ell, prol, res, dinv, b, x, min_level,
+            max_level, n_max_iter, tolerance, n_pre, adapt, mask, initial_zero, convergence_check);
+}
+
+int ref_mg_fixed_f64(int kind, int prolmat, const void * const * a_q1, const ref_ell * a_ell, const ref_ell * prol,
+                     const ref_ell * res, const void * const * dinv, const double * b, double * x, uint64_t min_level,
+                     uint64_t max_level, uint64_t n_max_iter, double tolerance, uint64_t n_pre, double adapt,
+                     const uint64_t * mask, int initial_zero, int convergence_check)
+{
+    if (kind == 0)
+        return mg_fixed_any<double, BandedMatrixQx<Q1Type, double> >(prolmat, a_q1, a_ell, prol, res, dinv, b, x,
+                min_level, max_level, n_max_iter, tolerance, n_pre, adapt, mask, initial_zero, convergence_check);
+    return mg_fixed_any<double, SparseMatrixELL<double> >(prolmat, a_q1, a_ell, prol, res, dinv, b, x, min_level,
+            max_level, n_max_iter, tolerance, n_pre, adapt, mask, initial_zero, convergence_check);
+}
+
 /* The ell::threads value the reference's ELL constructor reads. */
 void ref_set_ell_threads(int threads)
 {

@@ -210,3 +210,39 @@ def honei_eps_8e3():
     m = (add_eps - base_eps) / 4.0
     b = base_eps - 4.0 * m
     return (m * 8 + b) * 8.0
+
+
+def bilinear_prolmat_csr(L, dirichlet=True):
+    """Natural-order bilinear prolongation (level L-1 -> L) as CSR with
+    ascending columns: fine node (R, S) takes h(R - 2i) h(S - 2j) from coarse
+    node (i, j), h(0) = 1, h(+-1) = 1/2; with `dirichlet` the rows of
+    boundary fine nodes are empty."""
+    sf, sc = (1 << L) + 1, (1 << (L - 1)) + 1
+    h = {0: 1.0, 1: 0.5, -1: 0.5}
+    rowptr, col, val = [0], [], []
+    for R in range(sf):
+        for S in range(sf):
+            if not (dirichlet and (R in (0, sf - 1) or S in (0, sf - 1))):
+                for i in range(max(0, (R - 1) // 2), min(sc, R // 2 + 2)):
+                    for j in range(max(0, (S - 1) // 2), min(sc, S // 2 + 2)):
+                        dr, ds = R - 2 * i, S - 2 * j
+                        if dr in h and ds in h:
+                            col.append(i * sc + j)
+                            val.append(h[dr] * h[ds])
+            rowptr.append(len(col))
+    return (sf * sf, sc * sc, np.asarray(rowptr, np.uint64), np.asarray(col, np.uint64),
+            np.asarray(val, np.float64))
+
+
+def csr_transpose(rows, cols, rowptr, col, val):
+    r = np.repeat(np.arange(rows, dtype=np.uint64), np.diff(rowptr.astype(np.int64)))
+    order = np.lexsort((r, col))
+    tptr = np.zeros(cols + 1, np.uint64)
+    np.add.at(tptr, col.astype(np.int64) + 1, 1)
+    return cols, rows, np.cumsum(tptr).astype(np.uint64), r[order], val[order]
+
+
+def ell_of_csr(oracle, rows, cols, rowptr, col, val, dtype):
+    E = oracle.ell_from_csr(rows, rowptr, col, np.asarray(val, dtype))
+    E["cols"] = cols
+    return E
