@@ -111,6 +111,7 @@ _SIGS = {
     "hsolve_ell_set_variant": [vp, C.c_int],
     "hsolve_ell_spmv": [vp, vp, vp, vp, vp],
     "hsolve_ell_jacobi": [vp, vp, vp, vp, vp, vp],
+    "hsolve_cg_konv_ell": [vp, vp, vp, dbl, u64, vp, vp],
     "hsolve_cg_q1": [C.c_int, vp, u64, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_cg_ell": [vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_pcg_ell": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
@@ -119,6 +120,9 @@ _SIGS = {
     "hsolve_mg_destroy": [vp],
     "hsolve_mg_device_bytes": [vp, C.POINTER(u64)],
     "hsolve_mg_solve": [vp, vp, vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl), vp],
+    "hsolve_mg_create_ell": [C.POINTER(MGParams), vp, vp, vp, vp, vp, u64, C.POINTER(vp)],
+    "hsolve_mg_slots": [vp, C.POINTER(u64)],
+    "hsolve_mg_solve_batch": [vp, u64, vp, vp, vp, vp, vp, vp, vp],
     "hsolve_serve_create": [C.c_void_p, C.POINTER(vp)],
     "hsolve_serve_destroy": [vp],
     "hsolve_serve_request_bytes": [vp, u64, C.POINTER(u64)],
@@ -129,6 +133,12 @@ _SIGS = {
     "hsolve_serve_get_stats": [vp, vp],
     "hsolve_serve_run": [vp, u64, vp, vp, vp, vp, vp],
 }
+
+
+class EllHost(C.Structure):
+    """hsolve_ell_host: an ELL matrix in HONEI's host layout (one lane per row)."""
+    _fields_ = [("rows", u64), ("cols", u64), ("stride", u64), ("num_cols_per_row", u64),
+                ("Aj", vp), ("Ax", vp), ("Arl", vp)]
 
 
 class ServeParams(C.Structure):

@@ -11,6 +11,7 @@
 // 1 instead of 8 bytes per row length.  Which products are summed, and in
 // which order, does not change.
 #include "ell_handle.hh"
+#include "kernels_mg.hh"
 
 #include <algorithm>
 #include <cmath>
@@ -20,6 +21,15 @@
 namespace hsolve
 {
     void * stream_work(hipStream_t st, size_t bytes);
+
+    // ConjugateGradients<NONE>(ELL, rhs, x, konv_rad) in one workgroup
+    template <typename DT, typename IT, typename LT>
+    __global__ __launch_bounds__(256) void cg_konv_ell_kernel(EllRowOp<DT, IT, LT> A, const DT * __restrict__ rhs,
+            DT * __restrict__ x, DT * __restrict__ work, int64_t n, double konv_rad, uint64_t max_iters,
+            uint64_t * iters_out)
+    {
+        k::cg_konv_body_op<DT>(A, rhs, x, work, n, konv_rad, max_iters, iters_out);
+    }
 }
 
 using namespace hsolve;
@@ -299,6 +309,40 @@ int hsolve_ell_jacobi(hsolve_ell m, const void * rhs, const void * dinv, const v
         ell_launch<double, 2>(m, (const double *)x_old, (double *)x_new, (const double *)rhs, (const double *)dinv, st);
     else
         ell_launch<float, 2>(m, (const float *)x_old, (float *)x_new, (const float *)rhs, (const float *)dinv, st);
+    HSOLVE_API_END
+}
+
+int hsolve_cg_konv_ell(hsolve_ell m, const void * rhs, void * x, double konv_rad, uint64_t max_iters,
+                       void * iters_dev, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(m && rhs && x, HSOLVE_ERR_INVALID, "cg_konv_ell: null pointer");
+    require(m->rows == m->cols, HSOLVE_ERR_SIZE, "cg_konv_ell: matrix is not square");
+    require(m->rows <= (1u << 20), HSOLVE_ERR_SIZE, "cg_konv_ell: more than 2^20 rows for one workgroup");
+    hipStream_t st = as_stream(stream);
+    const uint64_t n = m->rows;
+    const uint64_t cap = max_iters ? max_iters : 100 * n + 1000;
+    const size_t es = m->dtype ? sizeof(double) : sizeof(float);
+    void * work = stream_work(st, es * 4 * n + 256);
+    with_ell_types(m, [&](auto Aj, auto Arl) {
+        using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
+        using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
+        if (m->dtype)
+        {
+            EllRowOp<double, IT, LT> A{Aj, (const double *)m->Ax, Arl, m->stride, (uint32_t)m->threads};
+            hipLaunchKernelGGL((cg_konv_ell_kernel<double, IT, LT>), dim3(1), dim3(kBlock), 0, st, A,
+                               (const double *)rhs, (double *)x, (double *)work, (int64_t)n, konv_rad, cap,
+                               (uint64_t *)iters_dev);
+        }
+        else
+        {
+            EllRowOp<float, IT, LT> A{Aj, (const float *)m->Ax, Arl, m->stride, (uint32_t)m->threads};
+            hipLaunchKernelGGL((cg_konv_ell_kernel<float, IT, LT>), dim3(1), dim3(kBlock), 0, st, A,
+                               (const float *)rhs, (float *)x, (float *)work, (int64_t)n, konv_rad, cap,
+                               (uint64_t *)iters_dev);
+        }
+    });
+    HIP_CHECK(hipGetLastError());
     HSOLVE_API_END
 }
 

@@ -325,15 +325,41 @@ namespace k
     template <> struct Eps<float> { static constexpr float v = 1.19209290e-07F; };
     template <> struct Eps<double> { static constexpr double v = 2.2204460492503131e-16; };
 
-    // ConjugateGradients<Tag, NONE>::value(Q1, rhs, x, konv_rad) in one
-    // workgroup (conjugate_gradients.hh:490-526, kernel :108-151).  Vectors
-    // g, u, energy, new_gradient live in `work` (4n); reductions are the
-    // sequential CPU sums, so the iteration is bit-identical to tags::CPU.
-    // `active` (may be null): when it points to 0 the kernel does nothing.
+    // A row of a Q1 banded matrix for the generic one-workgroup kernels.
+    template <typename DT> struct Q1RowOp
+    {
+        Bands B;
+        int64_t n, m;
+        __device__ __forceinline__ DT row(const DT * __restrict__ x, int64_t i) const
+        {
+            return q1_row<DT>(B, x, i, n, m);
+        }
+    };
+
+    // ConjugateGradients<Tag, NONE>::value(A, rhs, x, konv_rad) in one
+    // workgroup (Q1: conjugate_gradients.hh:490-526, kernel :108-151; ELL:
+    // :588-620, kernel :153-200 -- the same steps).  A.row(x, i) is row i of
+    // A x.  Vectors g, u, energy, new_gradient live in `work` (4n);
+    // reductions are the sequential CPU sums, so the iteration is
+    // bit-identical to tags::CPU.
+    template <typename DT, typename Op>
+    __device__ __forceinline__ void cg_konv_body_op(const Op & A, const DT * __restrict__ rhs,
+            DT * __restrict__ x, DT * __restrict__ work, int64_t n, double konv_rad, uint64_t max_iters,
+            uint64_t * iters_out);
+
     template <typename DT>
     __device__ __forceinline__ void cg_konv_body(const Bands & B, const DT * __restrict__ rhs,
             DT * __restrict__ x, DT * __restrict__ work, int64_t n, int64_t m, double konv_rad,
             uint64_t max_iters, uint64_t * iters_out)
+    {
+        Q1RowOp<DT> A{B, n, m};
+        cg_konv_body_op<DT>(A, rhs, x, work, n, konv_rad, max_iters, iters_out);
+    }
+
+    template <typename DT, typename Op>
+    __device__ __forceinline__ void cg_konv_body_op(const Op & A, const DT * __restrict__ rhs,
+            DT * __restrict__ x, DT * __restrict__ work, int64_t n, double konv_rad, uint64_t max_iters,
+            uint64_t * iters_out)
     {
         __shared__ DT sh[4];
         DT * g = work;
@@ -346,7 +372,7 @@ namespace k
         __syncthreads();
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
         {
-            const DT p = q1_row<DT>(B, x, i, n, m);
+            const DT p = A.row(x, i);
             g[i] = p - rhs[i];
         }
         __syncthreads();
@@ -361,7 +387,7 @@ namespace k
             seq_dot<DT>(g, g, n, &sh[1]);
             DT upper = sh[1];
             for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
-                en[i] = q1_row<DT>(B, u, i, n, m);
+                en[i] = A.row(u, i);
             __syncthreads();
             seq_dot<DT>(en, u, n, &sh[2]);
             DT lower = sh[2];
@@ -375,7 +401,7 @@ namespace k
             lower = upper;
             for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
             {
-                const DT p = q1_row<DT>(B, en, i, n, m);
+                const DT p = A.row(en, i);
                 ng[i] = p - rhs[i];
             }
             __syncthreads();

@@ -103,12 +103,27 @@ namespace mg
         uint64_t max_outer;
     };
 
+    // An ELL matrix owned by a hierarchy: HONEI's one-lane slot order (slot k
+    // of row r at r + k * stride), 32-bit absolute columns, 32-bit row lengths.
+    template <typename DT> struct EllMat
+    {
+        const uint32_t * col = nullptr;
+        const DT * val = nullptr;
+        const uint32_t * len = nullptr;
+        uint64_t stride = 0, rows = 0, cols = 0;
+        uint32_t ncpr = 0;
+    };
+    typedef EllMat<float> EllMat32;
+    typedef EllMat<double> EllMat64;
+
     struct HierLevel
     {
         uint64_t n = 0;
         uint64_t bs = 0;          // band stride (n rounded up to 64)
         int64_t side = 0;
-        float * bands = nullptr;  // 9 * bs
+        float * bands = nullptr;  // 9 * bs (Q1 hierarchies)
+        EllMat32 ell;             // the level matrix (ELL hierarchies)
+        EllMat32 prol, res;       // PROLMAT transfers to and from level - 1
         float * dinv = nullptr;
         k::Bands B() const
         {
@@ -123,10 +138,13 @@ namespace mg
     struct Hierarchy
     {
         Params p;
+        int kind = 0;                        // 0: Q1 bands, 1: ELL matrices
+        int prolmat = 0;                     // 1: transfers by the prol/res matrices
         uint64_t max_level = 0;
         HierLevel lv[kMaxLevels];
         double * bands64[kMaxLevels] = {};   // FP64 bands of each finest level solved on
         uint64_t bs64[kMaxLevels] = {};
+        EllMat64 ell64[kMaxLevels];          // FP64 finest ELL matrix (ELL hierarchies)
         std::vector<void *> allocations;
         uint64_t bytes = 0;
 
@@ -154,6 +172,23 @@ namespace mg
     void hierarchy_fill_poisson(Hierarchy & h, hipStream_t st);
     // dinv = (float)((double)(1/dd) * 0.7) on level L
     void hierarchy_dinv(Hierarchy & h, uint64_t L, hipStream_t st);
+
+    // An ELL matrix in HONEI's host layout (threads == 1: slot k of row r
+    // at r + k * stride; uint64 columns and row lengths).
+    struct EllHost
+    {
+        uint64_t rows, cols, stride, ncpr;
+        const uint64_t * Aj;
+        const void * Ax;
+        const uint64_t * Arl;
+    };
+    // ELL hierarchy: a32[L] for L in [min_level, max_level], A64 the FP64
+    // finest matrix, dinv32[L] (or null: (float)((double)(1/dd) * 0.7)),
+    // prol32 / res32 [L] for L in (min_level, max_level] or null for the
+    // structured transfers (then every level must be (2^L+1)^2).
+    void hierarchy_alloc_ell(Hierarchy & h, const Params & p, uint64_t max_level, const EllHost * a32,
+                             const EllHost & A64, const float * const * dinv32, const EllHost * prol32,
+                             const EllHost * res32);
 
     // Bytes of one slot whose finest level is L (own_b / own_result: the
     // slot also holds the right-hand side / the result array).

@@ -17,7 +17,9 @@
 //          the finest level.
 #include "mg_engine.hh"
 
+#include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace hsolve
 {
@@ -26,6 +28,58 @@ namespace mg
 namespace
 {
     __device__ __forceinline__ bool live(const Slot & sl) { return sl.s != nullptr; }
+
+    // Row i of y = A x for an engine ELL matrix, in the reference's order
+    // (slots ascending, summed from zero: honei/la/product.hh:148-206).
+    template <typename DT>
+    __device__ __forceinline__ DT ell_row(const EllMat<DT> & E, const DT * __restrict__ x, int64_t i)
+    {
+        const uint32_t l = E.len[i];
+        DT s = DT(0);
+        uint64_t j = (uint64_t)i;
+        for (uint32_t k = 0; k < l; ++k, j += E.stride)
+            s += E.val[j] * x[E.col[j]];
+        return s;
+    }
+
+    struct LevelSizes
+    {
+        int64_t n[kMaxLevels];
+    };
+
+    // One level's matrix and smoother data for the kernels: Q1 bands (B, m =
+    // grid side) or an ELL matrix (E), chosen by the kernels' MK argument.
+    struct LevelArgs
+    {
+        k::Bands B;
+        EllMat32 E;
+        const float * dinv;
+        int64_t n, m;
+    };
+
+    template <int MK>
+    __device__ __forceinline__ float level_row(const LevelArgs & a, const float * __restrict__ x, int64_t i)
+    {
+        if (MK == 0)
+            return k::q1_row<float>(a.B, x, i, a.n, a.m);
+        return ell_row<float>(a.E, x, i);
+    }
+
+    // rows per lane of the row kernels: 4 (16-byte vectors) on Q1 bands, 1 on ELL
+    template <int MK> __device__ __forceinline__ int64_t lane_id()
+    {
+        return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    }
+
+    // the level operator for the one-workgroup coarse CG
+    template <int MK> struct LevelRowOp
+    {
+        LevelArgs a;
+        __device__ __forceinline__ float row(const float * __restrict__ x, int64_t i) const
+        {
+            return level_row<MK>(a, x, i);
+        }
+    };
 
     // ------------------------------------------------------------ tick gates
     __global__ void plan_kernel(const Slot * T, unsigned nslots)
@@ -64,14 +118,14 @@ namespace
     }
 
     // iterates of the coarser levels start at zero
-    __global__ __launch_bounds__(256) void start_zero_kernel(const Slot * T, int minl, int L)
+    __global__ __launch_bounds__(256) void start_zero_kernel(const Slot * T, int minl, int L, LevelSizes sz)
     {
         const Slot & sl = T[blockIdx.z];
         if (! live(sl) || sl.s->do_start == 0)
             return;
         const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         for (int l = minl; l < L; ++l)
-            if (i < (int64_t)size_of(l))
+            if (i < sz.n[l])
                 sl.v[l][F_X][i] = 0.0f;
     }
 
@@ -90,9 +144,11 @@ namespace
     }
 
     // od = b - A64 result and its norm; opens the next inner solve or ends
-    // the outer loop.  On a start: the initial defect.
-    __global__ __launch_bounds__(256) void outer_defect_kernel(k::Bands B, const Slot * T, int64_t n, int64_t m,
-            double outer_eps, uint64_t max_outer)
+    // the outer loop.  On a start: the initial defect.  MK 0: Q1 bands B
+    // (two rows per lane, 16-byte loads inside); MK 1: the ELL matrix E.
+    template <int MK>
+    __global__ __launch_bounds__(256) void outer_defect_kernel(k::Bands B, EllMat64 E, const Slot * T, int64_t n,
+            int64_t m, double outer_eps, uint64_t max_outer)
     {
         const Slot & sl = T[blockIdx.z];
         if (! live(sl))
@@ -104,12 +160,21 @@ namespace
         const double * __restrict__ b = sl.b;
         const double * __restrict__ x = sl.result;
         double * __restrict__ od = sl.od;
-        const bool vec = ((uintptr_t)b & 15u) == 0;
+        const bool vec = MK == 0 && ((uintptr_t)b & 15u) == 0;
         double acc = 0.0;
         const int64_t stride = (int64_t)gridDim.x * blockDim.x * 2;
         for (int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i0 < n; i0 += stride)
         {
-            if (vec && i0 - m - 1 >= 0 && i0 + 2 + m < n)
+            if (MK == 1)
+            {
+                for (int64_t i = i0; i < i0 + 2 && i < n; ++i)
+                {
+                    const double di = b[i] - ell_row<double>(E, x, i);
+                    od[i] = di;
+                    acc += di * di;
+                }
+            }
+            else if (vec && i0 - m - 1 >= 0 && i0 + 2 + m < n)
             {
                 double prod[2];
                 k::q1_rows_v<double>(B, x, i0, m, prod);
@@ -192,12 +257,13 @@ namespace
 
     // Opens the inner solve: its initial defect d is rhs itself when
     // initial_zero (multigrid.hh:1626-1628), else rhs - A x.
-    template <int INITIAL_ZERO>
-    __global__ __launch_bounds__(256) void open_inner_kernel(k::Bands B, const Slot * T, int L, int64_t n, int64_t m)
+    template <int INITIAL_ZERO, int MK>
+    __global__ __launch_bounds__(256) void open_inner_kernel(LevelArgs a, const Slot * T, int L)
     {
         const Slot & sl = T[blockIdx.z];
         if (! live(sl) || sl.s->open == 0)
             return;
+        const int64_t n = a.n;
         const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         const float * rhs = sl.v[L][F_RHS];
         float * d = sl.v[L][F_D];
@@ -211,8 +277,14 @@ namespace
                 for (int64_t i = i0; i < n; ++i)
                     d[i] = rhs[i];
         }
+        else if (MK == 0)
+            k::q1_body_v<float, 1>(a.B, sl.v[L][F_X], d, rhs, nullptr, n, a.m, lane);
         else
-            k::q1_body_v<float, 1>(B, sl.v[L][F_X], d, rhs, nullptr, n, m, lane);
+        {
+            const float * x = sl.v[L][F_X];
+            for (int64_t i = lane * 4; i < lane * 4 + 4 && i < n; ++i)
+                d[i] = rhs[i] - ell_row<float>(a.E, x, i);
+        }
         if (blockIdx.x == 0 && threadIdx.x == 0)
         {
             sl.s->run = 1;
@@ -312,13 +384,6 @@ namespace
     }
 
     // --------------------------------------------------------------- V-cycle
-    struct LevelArgs
-    {
-        k::Bands B;
-        const float * dinv;
-        int64_t n, m;
-    };
-
 #define HS_GATE_RUN                                     \
     const Slot & sl = T[blockIdx.z];                    \
     if (! live(sl) || sl.s->run == 0)                   \
@@ -338,28 +403,101 @@ namespace
         out[i] = ACC ? out[i] + c : c;
     }
 
-    template <int ACC>
+    // The row kernels below cover 4 rows per lane: on Q1 bands as 16-byte
+    // vectors (kernels_la.hh / kernels_mg.hh bodies), on ELL one row at a time.
+
+    // the first two sweeps from zero (c1 = dinv*d, c2 = c1 + dinv*(d - A c1));
+    // ACC: out += c2, else out = c2
+    template <int ACC, int MK>
     __global__ __launch_bounds__(256) void v_zero2_kernel(LevelArgs a, const Slot * T, int L, int fsrc, int fout)
     {
         HS_GATE_RUN
-        k::jacobi_zero2_body<float, ACC>(a.B, sl.v[L][fsrc], a.dinv, sl.v[L][fout], a.n, a.m,
-                                         (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const float * d = sl.v[L][fsrc];
+        float * out = sl.v[L][fout];
+        if (MK == 0)
+        {
+            k::jacobi_zero2_body<float, ACC>(a.B, d, a.dinv, out, a.n, a.m, lane);
+            return;
+        }
+        for (int64_t i = lane * 4; i < lane * 4 + 4 && i < a.n; ++i)
+        {
+            const uint32_t l = a.E.len[i];
+            float s = 0.0f;
+            uint64_t j = (uint64_t)i;
+            for (uint32_t kk = 0; kk < l; ++kk, j += a.E.stride)
+            {
+                const uint32_t c = a.E.col[j];
+                s += a.E.val[j] * (a.dinv[c] * d[c]);
+            }
+            const float di = a.dinv[i];
+            const float c1 = di * d[i];
+            const float t = d[i] - s;
+            const float c2 = c1 + t * di;
+            out[i] = ACC ? out[i] + c2 : c2;
+        }
     }
 
     // y = x + dinv * (rhs - A x)
+    template <int MK>
     __global__ __launch_bounds__(256) void v_sweep_kernel(LevelArgs a, const Slot * T, int L, int fx, int fy, int frhs)
     {
         HS_GATE_RUN
-        k::q1_body_v<float, 2>(a.B, sl.v[L][fx], sl.v[L][fy], sl.v[L][frhs], a.dinv, a.n, a.m,
-                               (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const float * x = sl.v[L][fx];
+        float * y = sl.v[L][fy];
+        const float * rhs = sl.v[L][frhs];
+        if (MK == 0)
+        {
+            k::q1_body_v<float, 2>(a.B, x, y, rhs, a.dinv, a.n, a.m, lane);
+            return;
+        }
+        for (int64_t i = lane * 4; i < lane * 4 + 4 && i < a.n; ++i)
+        {
+            const float t = rhs[i] - ell_row<float>(a.E, x, i);
+            y[i] = x[i] + t * a.dinv[i];
+        }
     }
 
     // d = rhs - A x
+    template <int MK>
     __global__ __launch_bounds__(256) void v_defect_kernel(LevelArgs a, const Slot * T, int L)
     {
         HS_GATE_RUN
-        k::q1_body_v<float, 1>(a.B, sl.v[L][F_X], sl.v[L][F_D], sl.v[L][F_RHS], nullptr, a.n, a.m,
-                               (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const float * x = sl.v[L][F_X];
+        float * d = sl.v[L][F_D];
+        const float * rhs = sl.v[L][F_RHS];
+        if (MK == 0)
+        {
+            k::q1_body_v<float, 1>(a.B, x, d, rhs, nullptr, a.n, a.m, lane);
+            return;
+        }
+        for (int64_t i = lane * 4; i < lane * 4 + 4 && i < a.n; ++i)
+            d[i] = rhs[i] - ell_row<float>(a.E, x, i);
+    }
+
+    // Product<PROLMAT>: level Ldst's fdst = E * (level Lsrc's fsrc)
+    __global__ __launch_bounds__(256) void v_apply_ell_kernel(EllMat32 E, const Slot * T, int Lsrc, int fsrc, int Ldst,
+                                                              int fdst)
+    {
+        HS_GATE_RUN
+        const float * x = sl.v[Lsrc][fsrc];
+        float * y = sl.v[Ldst][fdst];
+        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        for (int64_t i = lane * 4; i < lane * 4 + 4 && i < (int64_t)E.rows; ++i)
+            y[i] = ell_row<float>(E, x, i);
+    }
+
+    // x += alpha * c (ScaledSum with the constant correction factor)
+    __global__ __launch_bounds__(256) void v_axpy_const_kernel(const Slot * T, int L, int64_t n, float alpha)
+    {
+        HS_GATE_RUN
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n)
+            return;
+        float * x = sl.v[L][F_X];
+        x[i] = x[i] + alpha * sl.v[L][F_C][i];
     }
 
     // dst = src (ADD 0) or dst += src (ADD 1), four entries per lane
@@ -400,12 +538,14 @@ namespace
     }
 
     // ConjugateGradients<NONE> (konv_rad form) on the coarsest level
+    template <int MK>
     __global__ __launch_bounds__(256) void v_coarse_kernel(LevelArgs a, const Slot * T, int L, int frhs,
                                                            double konv_rad, uint64_t max_iters)
     {
         HS_GATE_RUN
-        k::cg_konv_body<float>(a.B, sl.v[L][frhs], sl.v[L][F_X], sl.cg_work, a.n, a.m, konv_rad, max_iters,
-                               &sl.s->cg_iters);
+        LevelRowOp<MK> A{a};
+        k::cg_konv_body_op<float>(A, sl.v[L][frhs], sl.v[L][F_X], sl.cg_work, a.n, konv_rad, max_iters,
+                                  &sl.s->cg_iters);
     }
 
     // level L's fout (+)= alpha * P (x of level L-1)
@@ -419,6 +559,7 @@ namespace
     }
 
     // adapt_correction_factor == 0: alpha = (d.c) / ((A c).c)
+    template <int MK>
     __global__ __launch_bounds__(256) void v_adapt_alpha_kernel(LevelArgs a, const Slot * T, int L)
     {
         HS_GATE_RUN
@@ -428,7 +569,7 @@ namespace
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride)
         {
-            const float t = k::q1_row<float>(a.B, c, i, a.n, a.m);
+            const float t = level_row<MK>(a, c, i);
             a1 += (double)d[i] * (double)c[i];
             a2 += (double)t * (double)c[i];
         }
@@ -619,18 +760,121 @@ void hierarchy_fill_poisson(Hierarchy & h, hipStream_t st)
     HIP_CHECK(hipGetLastError());
 }
 
+namespace
+{
+    // HONEI-layout host ELL (threads == 1) -> engine ELL in device memory
+    template <typename DT>
+    EllMat<DT> upload_ell(Hierarchy & h, const EllHost & e)
+    {
+        require(e.Aj && e.Ax && e.Arl, HSOLVE_ERR_INVALID, "mg ELL level: null array");
+        require(e.rows < (1ull << 32) && e.cols < (1ull << 32), HSOLVE_ERR_SIZE, "mg ELL level: too many rows");
+        require(e.stride >= e.rows, HSOLVE_ERR_INVALID, "mg ELL level: stride below rows");
+        const uint64_t size = e.stride * e.ncpr;
+        std::vector<uint32_t> col(size), len(e.rows);
+        for (uint64_t i = 0; i < size; ++i)
+        {
+            require(e.Aj[i] < (e.cols ? e.cols : 1), HSOLVE_ERR_INVALID, "mg ELL level: column out of range");
+            col[i] = (uint32_t)e.Aj[i];
+        }
+        for (uint64_t r = 0; r < e.rows; ++r)
+        {
+            require(e.Arl[r] <= e.ncpr, HSOLVE_ERR_INVALID, "mg ELL level: row length above num_cols_per_row");
+            len[r] = (uint32_t)e.Arl[r];
+        }
+        EllMat<DT> m;
+        uint32_t * c = (uint32_t *)h.alloc(sizeof(uint32_t) * (size ? size : 1));
+        DT * v = (DT *)h.alloc(sizeof(DT) * (size ? size : 1));
+        uint32_t * l = (uint32_t *)h.alloc(sizeof(uint32_t) * (e.rows ? e.rows : 1));
+        HIP_CHECK(hipMemcpy(c, col.data(), sizeof(uint32_t) * size, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(v, e.Ax, sizeof(DT) * size, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(l, len.data(), sizeof(uint32_t) * e.rows, hipMemcpyHostToDevice));
+        m.col = c;
+        m.val = v;
+        m.len = l;
+        m.stride = e.stride;
+        m.rows = e.rows;
+        m.cols = e.cols;
+        m.ncpr = (uint32_t)e.ncpr;
+        return m;
+    }
+}
+
+void hierarchy_alloc_ell(Hierarchy & h, const Params & p, uint64_t max_level, const EllHost * a32,
+                         const EllHost & A64, const float * const * dinv32, const EllHost * prol32,
+                         const EllHost * res32)
+{
+    require(p.min_level >= 1 && p.min_level <= max_level && max_level < (uint64_t)kMaxLevels, HSOLVE_ERR_INVALID,
+            "mg: need 1 <= min_level <= max_level <= 14");
+    require(p.n_pre_smooth >= 1, HSOLVE_ERR_INVALID, "mg: n_pre_smooth must be >= 1");
+    require(p.n_max_iter >= 1, HSOLVE_ERR_INVALID, "mg: n_max_iter must be >= 1");
+    require(a32 != nullptr, HSOLVE_ERR_INVALID, "mg: null level matrices");
+    require((prol32 == nullptr) == (res32 == nullptr), HSOLVE_ERR_INVALID, "mg: give both prol and res or neither");
+    h.p = p;
+    h.kind = 1;
+    h.prolmat = prol32 ? 1 : 0;
+    h.max_level = max_level;
+    for (uint64_t L = p.min_level; L <= max_level; ++L)
+    {
+        const EllHost & e = a32[L];
+        require(e.rows == e.cols, HSOLVE_ERR_SIZE, "mg: level matrix is not square");
+        HierLevel & l = h.lv[L];
+        l.n = e.rows;
+        l.side = (int64_t)std::sqrt((double)e.rows);
+        if (! h.prolmat)
+            require(e.rows == size_of(L), HSOLVE_ERR_SIZE,
+                    "mg: structured transfers need level L of size (2^L+1)^2");
+        l.ell = upload_ell<float>(h, e);
+        l.dinv = (float *)h.alloc(sizeof(float) * (l.n ? l.n : 1));
+        std::vector<float> dv(l.n);
+        if (dinv32 && dinv32[L])
+            std::memcpy(dv.data(), dinv32[L], sizeof(float) * l.n);
+        else
+        {
+            // diags_inverted as the reference tests build it: ElementInverse, then
+            // Scale by 0.7 in double
+            const float * ax = (const float *)e.Ax;
+            for (uint64_t r = 0; r < l.n; ++r)
+            {
+                float dd = 0.0f;
+                for (uint64_t k = 0; k < e.Arl[r]; ++k)
+                    if (e.Aj[r + k * e.stride] == r)
+                    {
+                        dd = ax[r + k * e.stride];
+                        break;
+                    }
+                const float inv = dd != 0.0f ? 1.0f / dd : 0.0f;
+                dv[r] = (float)((double)inv * 0.7);
+            }
+        }
+        HIP_CHECK(hipMemcpy(l.dinv, dv.data(), sizeof(float) * l.n, hipMemcpyHostToDevice));
+        if (h.prolmat && L > p.min_level)
+        {
+            require(prol32[L].rows == l.n && prol32[L].cols == a32[L - 1].rows, HSOLVE_ERR_SIZE,
+                    "mg: prolongation matrix shape");
+            require(res32[L].rows == a32[L - 1].rows && res32[L].cols == l.n, HSOLVE_ERR_SIZE,
+                    "mg: restriction matrix shape");
+            l.prol = upload_ell<float>(h, prol32[L]);
+            l.res = upload_ell<float>(h, res32[L]);
+        }
+    }
+    require(h.lv[p.min_level].n <= 1048576, HSOLVE_ERR_INVALID, "mg: coarsest level above 2^20 rows");
+    require(A64.rows == h.lv[max_level].n && A64.cols == A64.rows, HSOLVE_ERR_SIZE,
+            "mg: FP64 matrix size differs from the finest level");
+    h.ell64[max_level] = upload_ell<double>(h, A64);
+}
+
 uint64_t slot_bytes(const Hierarchy & h, uint64_t L, bool own_b, bool own_result)
 {
-    const uint64_t n = size_of(L);
+    const uint64_t n = h.lv[L].n;
     uint64_t b = align256(sizeof(Scalars)) + align256(sizeof(double) * 2 * kNormBlocks);
     b += align256(sizeof(double) * n);                                  // od
     if (own_b)
         b += align256(sizeof(double) * n);
     if (own_result)
         b += align256(sizeof(double) * n);
-    b += align256(sizeof(float) * 4 * size_of(h.p.min_level));          // coarse CG
+    b += align256(sizeof(float) * 4 * h.lv[h.p.min_level].n);           // coarse CG
     for (uint64_t l = h.p.min_level; l <= L; ++l)
-        b += 5 * align256(sizeof(float) * size_of(l));
+        b += 5 * align256(sizeof(float) * h.lv[l].n);
     return b;
 }
 
@@ -638,7 +882,7 @@ Slot slot_carve(const Hierarchy & h, uint64_t L, char * base, bool own_b, bool o
 {
     Slot sl;
     std::memset(&sl, 0, sizeof(sl));
-    const uint64_t n = size_of(L);
+    const uint64_t n = h.lv[L].n;
     char * c = base;
     auto take = [&](uint64_t bytes) {
         char * r = c;
@@ -652,10 +896,10 @@ Slot slot_carve(const Hierarchy & h, uint64_t L, char * base, bool own_b, bool o
         sl.b = (const double *)take(sizeof(double) * n);
     if (own_result)
         sl.result = (double *)take(sizeof(double) * n);
-    sl.cg_work = (float *)take(sizeof(float) * 4 * size_of(h.p.min_level));
+    sl.cg_work = (float *)take(sizeof(float) * 4 * h.lv[h.p.min_level].n);
     for (uint64_t l = h.p.min_level; l <= L; ++l)
         for (int f = 0; f < 5; ++f)
-            sl.v[l][f] = (float *)take(sizeof(float) * size_of(l));
+            sl.v[l][f] = (float *)take(sizeof(float) * h.lv[l].n);
     return sl;
 }
 
@@ -684,11 +928,15 @@ void enqueue_report(const Slot * table, unsigned nslots, Report * reports, hipSt
     HIP_CHECK(hipGetLastError());
 }
 
-void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, hipStream_t st)
+namespace
+{
+template <int MK>
+void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, hipStream_t st)
 {
     const Params & p = h.p;
     const uint64_t minl = p.min_level;
-    require(maxl >= minl && maxl <= h.max_level && h.bands64[maxl] != nullptr, HSOLVE_ERR_INVALID,
+    require(maxl >= minl && maxl <= h.max_level && (MK == 1 ? h.ell64[maxl].val != nullptr : h.bands64[maxl] != nullptr),
+            HSOLVE_ERR_INVALID,
             "mg tick: level not prepared");
     const HierLevel & top = h.lv[maxl];
     const int64_t n = (int64_t)top.n;
@@ -697,6 +945,7 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
         const HierLevel & l = h.lv[L];
         LevelArgs a;
         a.B = l.B();
+        a.E = l.ell;
         a.dinv = l.dinv;
         a.n = (int64_t)l.n;
         a.m = l.side;
@@ -709,18 +958,25 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
     hipLaunchKernelGGL(start_init_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n,
                        top.side);
     if (maxl > minl)
+    {
+        LevelSizes sz;
+        for (int l = 0; l < kMaxLevels; ++l)
+            sz.n[l] = (int64_t)h.lv[l].n;
         hipLaunchKernelGGL(start_zero_kernel, dim3(grid_for(h.lv[maxl - 1].n, kBlock), 1, S), blk, 0, st, T,
-                           (int)minl, (int)maxl);
+                           (int)minl, (int)maxl, sz);
+    }
     hipLaunchKernelGGL(outer_accumulate_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
-    hipLaunchKernelGGL(outer_defect_kernel, dim3(std::min<unsigned>(grid_for((top.n + 1) / 2, kBlock), kNormBlocks), 1, S),
-                       blk, 0, st, h.B64(maxl), T, n, top.side, (double)p.tolerance, p.max_outer);
+    hipLaunchKernelGGL((outer_defect_kernel<MK>),
+                       dim3(std::min<unsigned>(grid_for((top.n + 1) / 2, kBlock), kNormBlocks), 1, S), blk, 0, st,
+                       MK == 0 ? h.B64(maxl) : k::Bands{}, h.ell64[maxl], T, n, top.side, (double)p.tolerance,
+                       p.max_outer);
     hipLaunchKernelGGL(to_inner_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
     if (p.initial_zero)
-        hipLaunchKernelGGL((open_inner_kernel<1>), dim3(lanes4(top.n), 1, S), blk, 0, st, top.B(), T, (int)maxl, n,
-                           top.side);
+        hipLaunchKernelGGL((open_inner_kernel<1, MK>), dim3(lanes4(top.n), 1, S), blk, 0, st, largs(maxl), T,
+                           (int)maxl);
     else
-        hipLaunchKernelGGL((open_inner_kernel<0>), dim3(lanes4(top.n), 1, S), blk, 0, st, top.B(), T, (int)maxl, n,
-                           top.side);
+        hipLaunchKernelGGL((open_inner_kernel<0, MK>), dim3(lanes4(top.n), 1, S), blk, 0, st, largs(maxl), T,
+                           (int)maxl);
     hipLaunchKernelGGL((inner_norm_kernel<0>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n, p.tolerance,
                        p.convergence_check, p.n_max_iter);
 
@@ -730,7 +986,7 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
         const LevelArgs a = largs(L);
         for (uint64_t i = 0; i < count; ++i)
         {
-            hipLaunchKernelGGL(v_sweep_kernel, dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, xc, xa, frhs);
+            hipLaunchKernelGGL((v_sweep_kernel<MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, xc, xa, frhs);
             std::swap(xc, xa);
         }
     };
@@ -743,7 +999,7 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
     };
     auto defect = [&](uint64_t L) {
         const LevelArgs a = largs(L);
-        hipLaunchKernelGGL(v_defect_kernel, dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L);
+        hipLaunchKernelGGL((v_defect_kernel<MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L);
     };
 
     if (minl != maxl)
@@ -761,11 +1017,11 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
                     hipLaunchKernelGGL((v_zero1_kernel<1>), dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, a, T,
                                        (int)L, dsrc, (int)F_X);
                 else if (npre == 2)
-                    hipLaunchKernelGGL((v_zero2_kernel<1>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
+                    hipLaunchKernelGGL((v_zero2_kernel<1, MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
                                        (int)F_X);
                 else
                 {
-                    hipLaunchKernelGGL((v_zero2_kernel<0>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
+                    hipLaunchKernelGGL((v_zero2_kernel<0, MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
                                        (int)F_C);
                     int cc = F_C, ca = F_TMP;
                     sweeps(L, cc, ca, dsrc, npre - 2);
@@ -780,7 +1036,7 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
                                        (int)L, dsrc, (int)F_X);
                 else
                 {
-                    hipLaunchKernelGGL((v_zero2_kernel<0>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
+                    hipLaunchKernelGGL((v_zero2_kernel<0, MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
                                        (int)F_X);
                     int xc = F_X, xa = F_TMP;
                     sweeps(L, xc, xa, dsrc, npre - 2);
@@ -789,9 +1045,15 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
                 }
             }
             defect(L);
-            const int64_t Nc = h.lv[L - 1].side;
-            hipLaunchKernelGGL(v_restrict_kernel, dim3(grid_for(Nc, kBlock), (unsigned)Nc, S), blk, 0, st, T, (int)L,
-                               Nc, p.mask);
+            if (h.prolmat)
+                hipLaunchKernelGGL(v_apply_ell_kernel, dim3(lanes4(h.lv[L].res.rows), 1, S), blk, 0, st, h.lv[L].res,
+                                   T, (int)L, (int)F_D, (int)(L - 1), (int)F_RHS);
+            else
+            {
+                const int64_t Nc = h.lv[L - 1].side;
+                hipLaunchKernelGGL(v_restrict_kernel, dim3(grid_for(Nc, kBlock), (unsigned)Nc, S), blk, 0, st, T,
+                                   (int)L, Nc, p.mask);
+            }
         }
     }
     {
@@ -799,7 +1061,7 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
         // with one level it sees the finest defect d
         const LevelArgs a = largs(minl);
         const int crhs = (minl == maxl) ? F_D : F_RHS;
-        hipLaunchKernelGGL(v_coarse_kernel, dim3(1, 1, S), blk, 0, st, a, T, (int)minl, crhs,
+        hipLaunchKernelGGL((v_coarse_kernel<MK>), dim3(1, 1, S), blk, 0, st, a, T, (int)minl, crhs,
                            (double)1.19209290e-07F, (uint64_t)(100 * a.n + 1000));
         if (minl == maxl)
         {
@@ -815,12 +1077,22 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
         const int Nf = (int)h.lv[L].side;
         const int Nc = (int)h.lv[L - 1].side;
         const dim3 pg(grid_for((uint64_t)Nc, kBlock), (unsigned)Nf, S);
-        if (p.adapt != 0.0f)
+        if (h.prolmat)
+        {
+            // c = P x_coarse, then x += alpha c
+            hipLaunchKernelGGL(v_apply_ell_kernel, dim3(lanes4(h.lv[L].prol.rows), 1, S), blk, 0, st, h.lv[L].prol, T,
+                               (int)(L - 1), (int)F_X, (int)L, (int)F_C);
+            if (p.adapt != 0.0f)
+                hipLaunchKernelGGL(v_axpy_const_kernel, dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, T, (int)L, a.n,
+                                   p.adapt);
+        }
+        else if (p.adapt != 0.0f)
             hipLaunchKernelGGL((v_prolong_kernel<true>), pg, blk, 0, st, T, (int)L, Nf, p.adapt, p.mask, (int)F_X);
         else
-        {
             hipLaunchKernelGGL((v_prolong_kernel<false>), pg, blk, 0, st, T, (int)L, Nf, 0.0f, p.mask, (int)F_C);
-            hipLaunchKernelGGL(v_adapt_alpha_kernel, dim3(std::min<unsigned>(grid_for(a.n, kBlock), kNormBlocks), 1, S),
+        if (p.adapt == 0.0f)
+        {
+            hipLaunchKernelGGL((v_adapt_alpha_kernel<MK>), dim3(std::min<unsigned>(grid_for(a.n, kBlock), kNormBlocks), 1, S),
                                blk, 0, st, a, T, (int)L);
             hipLaunchKernelGGL(v_axpy_alpha_kernel, dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, T, (int)L, a.n);
         }
@@ -835,6 +1107,15 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
     hipLaunchKernelGGL((inner_norm_kernel<1>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n, p.tolerance,
                        p.convergence_check, p.n_max_iter);
     HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, hipStream_t st)
+{
+    if (h.kind == 1)
+        tick_impl<1>(h, maxl, T, S, st);
+    else
+        tick_impl<0>(h, maxl, T, S, st);
 }
 
 }  // namespace mg

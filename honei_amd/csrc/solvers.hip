@@ -37,26 +37,7 @@ namespace
         }
     };
 
-    template <typename DT, typename IT, typename LT> struct EllOp
-    {
-        const IT * Aj;
-        const DT * Ax;
-        const LT * Arl;
-        uint64_t stride;
-        uint32_t threads;
-        __device__ __forceinline__ DT row(const DT * __restrict__ x, int64_t i) const
-        {
-            const uint32_t len = (uint32_t)Arl[i];
-            if (threads == 1)
-                return k::ell_row_sum<DT, IT, 0>(x, Aj + i, Ax + i, len, (uint64_t)i, stride);
-            DT sum = DT(0);
-            uint64_t j = (uint64_t)i * threads;
-            for (uint32_t k = 0; k < len; ++k, j += stride)
-                for (uint32_t t = 0; t < threads; ++t)
-                    sum += Ax[j + t] * x[k::col_of<IT>(Aj[j + t], (uint64_t)i)];
-            return sum;
-        }
-    };
+    template <typename DT, typename IT, typename LT> using EllOp = EllRowOp<DT, IT, LT>;
 
     using k::finish2;
 

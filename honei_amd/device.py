@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import MGParams, ServeParams, ServeResult, ServeStats, check, lib, u64, vp
+from ._lib import EllHost, MGParams, ServeParams, ServeResult, ServeStats, check, lib, u64, vp
 
 _DT = {np.dtype(np.float32): 0, np.dtype(np.float64): 1}
 
@@ -314,6 +314,59 @@ class MGSolver:
         nb = u64()
         check(lib().hsolve_mg_device_bytes(vp(self.handle), C.byref(nb)), "mg_device_bytes")
         self.device_bytes = nb.value
+
+    @classmethod
+    def from_ell(cls, params: MGParams, A64, a32, dinv32=None, prol32=None, res32=None, slots=1):
+        """Multigrid<..., MIXED> on ELL matrices (hsolve_mg_create_ell).  A64: FP64 ELL dict
+        (rows, cols, stride, ncpr, Aj, Ax, Arl); a32 / dinv32 / prol32 / res32: dicts level -> ..."""
+        self = cls.__new__(cls)
+        self.params = params
+        keep = []
+        maxl = params.max_level
+
+        def host(E, dtype):
+            Aj = np.ascontiguousarray(E["Aj"], np.uint64)
+            Ax = np.ascontiguousarray(E["Ax"], dtype)
+            Arl = np.ascontiguousarray(E["Arl"], np.uint64)
+            keep.extend([Aj, Ax, Arl])
+            return EllHost(E["rows"], E.get("cols", E["rows"]), E["stride"], E["ncpr"], Aj.ctypes.data,
+                           Ax.ctypes.data, Arl.ctypes.data)
+
+        def levels(d):
+            arr = (EllHost * (maxl + 1))()
+            for L, E in d.items():
+                arr[L] = host(E, np.float32)
+            return arr
+
+        a64 = host(A64, np.float64)
+        dv = None
+        if dinv32 is not None:
+            dv = (vp * (maxl + 1))()
+            for L, d in dinv32.items():
+                d = np.ascontiguousarray(d, np.float32)
+                keep.append(d)
+                dv[L] = d.ctypes.data
+        h = vp()
+        check(lib().hsolve_mg_create_ell(C.byref(params), C.byref(a64), levels(a32), dv,
+                                         levels(prol32) if prol32 else None, levels(res32) if res32 else None,
+                                         u64(slots), C.byref(h)), "mg_create_ell")
+        self.handle = h.value
+        nb = u64()
+        check(lib().hsolve_mg_device_bytes(vp(self.handle), C.byref(nb)), "mg_device_bytes")
+        self.device_bytes = nb.value
+        return self
+
+    def solve_batch(self, bs, xs, stream=None, raise_on_cap=True):
+        """Solves bs[i] into xs[i] in the same launches; returns one dict per solve."""
+        k = len(bs)
+        b = (vp * k)(*[v.ptr for v in bs])
+        x = (vp * k)(*[v.ptr for v in xs])
+        o, i = (u64 * k)(), (u64 * k)()
+        fd, idf = (C.c_double * k)(), (C.c_double * k)()
+        rc = lib().hsolve_mg_solve_batch(vp(self.handle), u64(k), b, x, o, i, fd, idf, vp(stream))
+        if rc != 5 or raise_on_cap:
+            check(rc, "mg_solve_batch")
+        return [dict(outer=o[j], inner=i[j], final_defect=fd[j], initial_defect=idf[j], status=rc) for j in range(k)]
 
     def solve(self, b: DeviceVector, x: DeviceVector, stream=None, raise_on_cap=True):
         o, i, fd, idf = u64(), u64(), C.c_double(), C.c_double()

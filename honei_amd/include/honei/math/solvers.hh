@@ -290,6 +290,18 @@ namespace honei
             A.device_bands(bands);
             check(abi::cg_konv_q1(bands, rd(rhs), wr(x), A.size(), konv_rad, 0), "ConjugateGradients");
         }
+
+        /// The same on an ELL matrix (conjugate_gradients.hh:588-620, kernel :153-200)
+        template <typename DT>
+        static void value(const SparseMatrixELL<DT> & A, const DenseVector<DT> & rhs, DenseVector<DT> & x,
+                          double konv_rad)
+        {
+            using namespace mi355x;
+            same_size(rhs.size(), A.rows());
+            same_size(x.size(), A.rows());
+            check(hsolve_cg_konv_ell(A.device_handle(), rd(rhs), wr(x), konv_rad, 0, nullptr, stream()),
+                  "ConjugateGradients ELL");
+        }
     };
 
     /// Configuration and level data of the multigrid solvers (multigrid.hh:65-149).
@@ -329,10 +341,26 @@ namespace honei
               typename Mode_>
     struct Multigrid;
 
-    /// FP32 V-cycles inside FP64 iterative refinement on Q1 bands (multigrid.hh:1616-2393).
+    namespace mg_kernel_detail
+    {
+        // host-driven MIXED solve on ELL matrices (multigrid_kernel.hh)
+        template <typename ProlType_>
+        void mixed_ell(const SparseMatrixELL<double> & system, const DenseVector<double> & rhs, DenseVector<double> & x,
+                       MGInfo<float, SparseMatrixELL<float>> & info);
+    }
+
+    /// FP32 V-cycles inside FP64 iterative refinement on Q1 bands (multigrid.hh:1616-2393)
+    /// on the device-resident engine; on ELL matrices (multigrid.hh:1936-2543) host-driven.
     template <> struct Multigrid<tags::GPU::MI355X, tags::GPU::MI355X, methods::NONE, methods::JAC, methods::CYCLE::V,
                                  methods::MIXED>
     {
+        static void value(const SparseMatrixELL<double> & system, const DenseVector<double> & right_hand_side,
+                          DenseVector<double> & x, unsigned long /*max_levels*/, double /*conv_rad*/,
+                          MGInfo<float, SparseMatrixELL<float>> & info)
+        {
+            mg_kernel_detail::mixed_ell<methods::NONE>(system, right_hand_side, x, info);
+        }
+
         static void value(const BandedMatrixQx<Q1Type, double> & system, const DenseVector<double> & right_hand_side,
                           DenseVector<double> & x, unsigned long /*max_levels*/, double /*conv_rad*/,
                           MGInfo<float, BandedMatrixQx<Q1Type, float>> & info)
@@ -383,3 +411,6 @@ namespace honei
         }
     };
 }
+
+// the host-driven FIXED and ELL MIXED solvers
+#include <honei/math/multigrid_kernel.hh>

@@ -245,6 +245,13 @@ int hsolve_ell_info(hsolve_ell m, uint64_t *rows, uint64_t *columns, uint64_t *s
 int hsolve_ell_set_variant(hsolve_ell m, int variant);
 /* y = A x, or y = rhs - A x when rhs != NULL */
 int hsolve_ell_spmv(hsolve_ell m, const void *x, void *y, const void *rhs, void *stream);
+/* ConjugateGradients<Tag, NONE>::value(ELL, rhs, x, konv_rad) in one
+ * workgroup (honei/math/conjugate_gradients.hh:588-620, kernel :153-200):
+ * x = 0, then CG steps while ||x||^2 grows by more than konv_rad; rows <=
+ * 2^20.  iters_dev (uint64 device scalar, may be NULL) receives the count;
+ * max_iters (0: 100 n + 1000) is a termination guarantee the reference lacks. */
+int hsolve_cg_konv_ell(hsolve_ell m, const void *rhs, void *x, double konv_rad, uint64_t max_iters,
+                       void *iters_dev, void *stream);
 /* x_new = x_old + dinv * (rhs - A x_old) */
 int hsolve_ell_jacobi(hsolve_ell m, const void *rhs, const void *dinv, const void *x_old,
                       void *x_new, void *stream);
@@ -296,6 +303,32 @@ int hsolve_mg_device_bytes(hsolve_mg mg, uint64_t *bytes);
 int hsolve_mg_solve(hsolve_mg mg, const void *b, void *x, uint64_t *outer_iters,
                     uint64_t *inner_iters, double *final_defect, double *initial_defect,
                     void *stream);
+/* An ELL matrix in HONEI's host layout with one lane per row (slot k of row
+ * r at r + k*stride; uint64 columns and row lengths; values float or double
+ * as the context says). */
+typedef struct hsolve_ell_host_s {
+    uint64_t rows, cols, stride, num_cols_per_row;
+    const uint64_t *Aj;
+    const void *Ax;
+    const uint64_t *Arl;
+} hsolve_ell_host;
+/* Multigrid<..., NONE | PROLMAT, JAC, CYCLE::V, MIXED> on ELL matrices
+ * (multigrid.hh:1936-2543): a32[L] FP32 level matrices for L in [min_level,
+ * max_level] (entries below min_level ignored), A64 the FP64 finest matrix,
+ * dinv32[L] or NULL (then (float)((double)(1/diag) * 0.7)), prol32[L] /
+ * res32[L] (L in (min_level, max_level]) for matrix transfers or both NULL
+ * for the structured Q1 transfers (then level L has (2^L+1)^2 rows).
+ * `slots` right-hand sides can be solved together (hsolve_mg_solve_batch). */
+int hsolve_mg_create_ell(const hsolve_mg_params *params, const hsolve_ell_host *A64,
+                         const hsolve_ell_host *a32, const float *const *dinv32,
+                         const hsolve_ell_host *prol32, const hsolve_ell_host *res32,
+                         uint64_t slots, hsolve_mg *out);
+int hsolve_mg_slots(hsolve_mg mg, uint64_t *slots);
+/* count <= slots solves in the same launches: b[i] into x[i] (x[i] in:
+ * initial guess); per-solve counts and defects into the arrays (may be NULL). */
+int hsolve_mg_solve_batch(hsolve_mg mg, uint64_t count, const void *const *b, void *const *x,
+                          uint64_t *outer_iters, uint64_t *inner_iters, double *final_defect,
+                          double *initial_defect, void *stream);
 
 /* Request server: Poisson Q1 solves (FillMatrix<POISSON, DIRICHLET_0>, side
  * 2^L+1, L in [min_served_level, max_served_level]) by the MG-IR above with

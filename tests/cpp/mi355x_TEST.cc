@@ -129,6 +129,9 @@ template <typename DT> static void bands_host(const BandedMatrixQx<Q1Type, DT> &
         out[k] = A.band(k).read_elements();
 }
 
+// an oracle ELL view (or_ell_f64 / or_ell_f32) of a SparseMatrixELL's host arrays
+template <typename OE, typename DT> static OE or_view(const SparseMatrixELL<DT> & m);
+
 // ------------------------------------------------------------------ tests
 template <typename DT> static void test_ell_product_defect()
 {
@@ -436,6 +439,238 @@ static void test_multigrid_mixed()
     TEST_CHECK(bit_equal(result2, result.read_elements()));
 }
 
+// Multigrid<..., FIXED> on the .bin hierarchies: poisson_mg_float_banded_TEST.cc
+// (81.bin, FP32, 4+4 sweeps, tol 1e-2 with convergence check, bound 1e-2) and
+// poisson_mg_double_banded_TEST.cc (1089.bin, FP64, 2+2, tol 1e-8, bound 1e-4)
+template <typename DT>
+static void test_multigrid_fixed_q1(unsigned long n, unsigned long maxl, unsigned long npre, DT tol, bool conv,
+                                    double bound)
+{
+    DenseVector<double> b(1), ref_sol(1);
+    read_bin(n, b, ref_sol);
+    MGInfo<DT, BandedMatrixQx<Q1Type, DT>> info;
+    info.min_level = 1;
+    info.max_level = maxl;
+    info.n_max_iter = 16;
+    info.initial_zero = true;
+    info.tolerance = tol;
+    info.convergence_check = conv;
+    info.n_pre_smooth = npre;
+    info.n_post_smooth = npre;
+    info.adapt_correction_factor = DT(1);
+    std::vector<const DT *> a_host(9 * (maxl + 1), nullptr), d_host(maxl + 1, nullptr);
+    for (unsigned long L = 0; L <= maxl; ++L)
+    {
+        const unsigned long nl = L == 0 ? 9 : ((1ul << L) + 1) * ((1ul << L) + 1);
+        BandedMatrixQx<Q1Type, DT> am(nl);
+        DenseVector<DT> dinv(nl, DT(0));
+        if (L >= 1)
+        {
+            DenseVector<double> bb(1), rs(1);
+            BandedMatrixQx<Q1Type, double> al(read_bin(nl, bb, rs));
+            for (int k = 0; k < 9; ++k)
+                for (unsigned long i = 0; i < nl; ++i)
+                    am.band(k)[i] = (DT)al.band(k)[i];
+            for (unsigned long i = 0; i < nl; ++i)
+            {
+                const DT dd = am.band(DD)[i];
+                const DT inv = dd != DT(0) ? DT(1) / dd : DT(0);
+                dinv[i] = (DT)((double)inv * 0.7);
+            }
+        }
+        info.a.push_back(am);
+        info.resmats.push_back(am);
+        info.prolmats.push_back(am);
+        info.diags_inverted.push_back(dinv);
+        info.c.push_back(DenseVector<DT>(nl, DT(0)));
+        info.d.push_back(DenseVector<DT>(nl, DT(0)));
+        info.rhs.push_back(DenseVector<DT>(nl, DT(0)));
+        info.x.push_back(DenseVector<DT>(nl, DT(0)));
+    }
+    for (unsigned long L = 1; L <= maxl; ++L)
+    {
+        for (int k = 0; k < 9; ++k)
+            a_host[9 * L + k] = info.a[L].band(k).read_elements();
+        d_host[L] = info.diags_inverted[L].read_elements();
+    }
+    DenseVector<DT> rhs(n), x(n, DT(0));
+    for (unsigned long i = 0; i < n; ++i)
+        rhs[i] = (DT)b[i];
+    info.rhs[maxl] = rhs;
+    Multigrid<Tag, Tag, methods::NONE, methods::JAC, methods::CYCLE::V, methods::FIXED>::value(
+            info.a[maxl], rhs, x, 11ul, std::numeric_limits<DT>::epsilon(), info);
+    double maxdev = 0;
+    for (unsigned long i = 0; i < n; ++i)
+        maxdev = std::max(maxdev, std::fabs((double)x[i] - ref_sol[i]));
+    TEST_CHECK(maxdev <= bound);
+    // the oracle on the same hierarchy
+    or_mg_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.min_level = 1; p.max_level = maxl; p.n_max_iter = 16; p.initial_zero = 1; p.convergence_check = conv;
+    p.tolerance = (double)tol; p.n_pre_smooth = npre; p.adapt_correction_factor = 1.0;
+    for (int i = 0; i < 8; ++i) p.macro_border_mask[i] = 2;
+    std::vector<DT> xr(n, DT(0));
+    uint64_t iters = 0;
+    int st;
+    if (std::is_same<DT, double>::value)
+        st = or_mg_fixed_f64(0, (const double * const *)a_host.data(), nullptr, (const double * const *)d_host.data(),
+                             nullptr, nullptr, (const double *)rhs.read_elements(), (double *)xr.data(), &p, &iters,
+                             nullptr, nullptr);
+    else
+        st = or_mg_fixed_f32(0, (const float * const *)a_host.data(), nullptr, (const float * const *)d_host.data(),
+                             nullptr, nullptr, (const float *)rhs.read_elements(), (float *)xr.data(), &p, &iters,
+                             nullptr, nullptr);
+    TEST_CHECK(st == 0);
+    std::printf("  FIXED %lu: %lu cycles (oracle %lu), |x - ref_sol| = %.3g\n", n, info.inner_iterations,
+                (unsigned long)iters, maxdev);
+    TEST_CHECK((long)info.inner_iterations - (long)iters <= 1 && (long)iters - (long)info.inner_iterations <= 1);
+    double num = 0, den = 0;
+    for (unsigned long i = 0; i < n; ++i)
+    {
+        num += ((double)x[i] - (double)xr[i]) * ((double)x[i] - (double)xr[i]);
+        den += (double)xr[i] * (double)xr[i];
+    }
+    TEST_CHECK(std::sqrt(num / den) <= (std::is_same<DT, double>::value ? 1e-6 : 1e-4));
+}
+
+static void test_multigrid_fixed()
+{
+    test_multigrid_fixed_q1<float>(81, 3, 4, 1e-2f, true, 1e-2);
+    test_multigrid_fixed_q1<double>(1089, 5, 2, 1e-8, false, 1e-4);
+}
+
+// SparseMatrixELL of FillMatrix<POISSON, DIRICHLET_0> on level L (ascending
+// columns, zeros dropped, as SparseMatrixELL(SparseMatrix) builds it)
+template <typename DT> static SparseMatrixELL<DT> poisson_ell(unsigned long L)
+{
+    const unsigned long n = ((1ul << L) + 1) * ((1ul << L) + 1);
+    BandedMatrixQx<Q1Type, DT> q(n);
+    FillMatrix<Tag, applications::POISSON, boundary_types::DIRICHLET::DIRICHLET_0>::value(q);
+    const long m = (long)std::sqrt((double)n);
+    const long off[9] = {-m - 1, -m, -m + 1, -1, 0, 1, m - 1, m, m + 1};
+    SparseMatrix<DT> s(n, n);
+    for (long i = 0; i < (long)n; ++i)
+        for (int k = 0; k < 9; ++k)
+        {
+            const long j = i + off[k];
+            if (j >= 0 && j < (long)n && q.band(k)[i] != DT(0))
+                s(i, j, q.band(k)[i]);
+        }
+    return SparseMatrixELL<DT>(s);
+}
+
+// natural-order bilinear prolongation level L-1 -> L, empty boundary rows
+template <typename DT> static SparseMatrixELL<DT> bilinear_prol(unsigned long L)
+{
+    const long sf = (1l << L) + 1, sc = (1l << (L - 1)) + 1;
+    SparseMatrix<DT> s(sf * sf, sc * sc);
+    for (long R = 1; R < sf - 1; ++R)
+        for (long S = 1; S < sf - 1; ++S)
+            for (long i = std::max(0l, (R - 1) / 2); i < std::min(sc, R / 2 + 2); ++i)
+                for (long j = std::max(0l, (S - 1) / 2); j < std::min(sc, S / 2 + 2); ++j)
+                {
+                    const long dr = R - 2 * i, ds = S - 2 * j;
+                    if (std::labs(dr) <= 1 && std::labs(ds) <= 1)
+                        s(R * sf + S, i * sc + j, DT((dr == 0 ? 1.0 : 0.5) * (ds == 0 ? 1.0 : 0.5)));
+                }
+    return SparseMatrixELL<DT>(s);
+}
+
+template <typename DT> static or_ell_f32 view32(const SparseMatrixELL<DT> & m) { return or_view<or_ell_f32>(m); }
+
+static void test_multigrid_ell()
+{
+    const unsigned long maxl = 5, minl = 1, n = 33 * 33;
+    for (int prolmat = 0; prolmat < 2; ++prolmat)
+    {
+        MGInfo<float, SparseMatrixELL<float>> info;
+        info.min_level = minl;
+        info.max_level = maxl;
+        info.n_max_iter = 2;
+        info.initial_zero = true;
+        info.tolerance = 1e-8f;
+        info.convergence_check = false;
+        info.n_pre_smooth = 2;
+        info.n_post_smooth = 2;
+        info.adapt_correction_factor = 1.f;
+        std::vector<or_ell_f32> a32(maxl + 1), pr(maxl + 1), rs(maxl + 1);
+        std::vector<const float *> d32(maxl + 1, nullptr);
+        for (unsigned long L = 0; L <= maxl; ++L)
+        {
+            const unsigned long nl = ((1ul << L) + 1) * ((1ul << L) + 1);
+            SparseMatrixELL<float> A(poisson_ell<float>(std::max(L, 1ul)));
+            if (L == 0)
+            {
+                SparseMatrix<float> one(nl, nl);
+                for (unsigned long i = 0; i < nl; ++i)
+                    one(i, i, 1.f);
+                A = SparseMatrixELL<float>(one);
+            }
+            DenseVector<float> dinv(nl, 0.f);
+            for (unsigned long i = 0; i < nl; ++i)
+            {
+                const float dd = A(i, i);
+                dinv[i] = (float)((double)(dd != 0.f ? 1.f / dd : 0.f) * 0.7);
+            }
+            info.a.push_back(A);
+            info.diags_inverted.push_back(dinv);
+            if (L > minl)
+            {
+                SparseMatrixELL<float> P(bilinear_prol<float>(L));
+                info.prolmats.push_back(P);
+                info.resmats.push_back(Transposition<Tag>::value(P));
+            }
+            else
+            {
+                info.prolmats.push_back(A);
+                info.resmats.push_back(A);
+            }
+            info.c.push_back(DenseVector<float>(nl, 0.f));
+            info.d.push_back(DenseVector<float>(nl, 0.f));
+            info.rhs.push_back(DenseVector<float>(nl, 0.f));
+            info.x.push_back(DenseVector<float>(nl, 0.f));
+        }
+        for (unsigned long L = minl; L <= maxl; ++L)
+        {
+            a32[L] = view32(info.a[L]);
+            pr[L] = view32(info.prolmats[L]);
+            rs[L] = view32(info.resmats[L]);
+            d32[L] = info.diags_inverted[L].read_elements();
+        }
+        SparseMatrixELL<double> A64(poisson_ell<double>(maxl));
+        DenseVector<double> b(n, 0.0), x(n, 0.0);
+        for (long r = 1; r < 32; ++r)
+            for (long s = 1; s < 32; ++s)
+                b[r * 33 + s] = std::sin(0.1 * (double)(r * 33 + s));
+        if (prolmat)
+            Multigrid<Tag, Tag, methods::PROLMAT, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
+                    A64, b, x, 11ul, std::numeric_limits<double>::epsilon(), info);
+        else
+            Multigrid<Tag, Tag, methods::NONE, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
+                    A64, b, x, 11ul, std::numeric_limits<double>::epsilon(), info);
+        or_mg_params p;
+        std::memset(&p, 0, sizeof(p));
+        p.min_level = minl; p.max_level = maxl; p.n_max_iter = 2; p.initial_zero = 1; p.tolerance = 1e-8;
+        p.n_pre_smooth = 2; p.adapt_correction_factor = 1.0;
+        for (int i = 0; i < 8; ++i) p.macro_border_mask[i] = 2;
+        or_ell_f64 a64 = or_view<or_ell_f64>(A64);
+        std::vector<double> xr(n, 0.0);
+        uint64_t outer = 0, inner = 0;
+        or_mg_mixed(1, nullptr, &a64, b.read_elements(), xr.data(), nullptr, a32.data(), d32.data(),
+                    prolmat ? pr.data() : nullptr, prolmat ? rs.data() : nullptr, &p, &outer, &inner, nullptr, nullptr);
+        std::printf("  MIXED ELL %s: outer %lu inner %lu (oracle %lu / %lu)\n", prolmat ? "PROLMAT" : "NONE",
+                    info.outer_iterations, info.inner_iterations, (unsigned long)outer, (unsigned long)inner);
+        TEST_CHECK(info.outer_iterations == outer && info.inner_iterations == inner);
+        double num = 0, den = 0;
+        for (unsigned long i = 0; i < n; ++i)
+        {
+            num += (x[i] - xr[i]) * (x[i] - xr[i]);
+            den += xr[i] * xr[i];
+        }
+        TEST_CHECK(std::sqrt(num / den) <= 1e-6);
+    }
+}
+
 static void test_lock_semantics()
 {
     DenseVector<double> v(1000, 1.0), w(1000);
@@ -630,6 +865,8 @@ int main(int argc, char ** argv)
         {"cg", test_cg},
         {"multigrid_mixed", test_multigrid_mixed},
         {"lock_semantics", test_lock_semantics},
+        {"multigrid_fixed", test_multigrid_fixed},
+        {"multigrid_ell", test_multigrid_ell},
         {"mgop_cycles", test_mgop_cycles},
         {"ir_mgop", test_ir_mgop},
     };

@@ -20,5 +20,5 @@ def test_template_layer_against_oracle(hip):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
     for name in ("ell_product_defect<double>", "q1_and_blas1<float>", "cg", "multigrid_mixed", "lock_semantics",
-                 "mgop_cycles", "ir_mgop"):
+                 "multigrid_fixed", "multigrid_ell", "mgop_cycles", "ir_mgop"):
         assert f"PASS {name}" in r.stdout
