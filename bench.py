@@ -193,6 +193,85 @@ def serve_case(hip, count, budget_gib=32, max_slots=64, load=0.7, seed=11):
                 graph_captures=st["graph_captures"], ticks=st["ticks"])
 
 
+def diffusion_case(hip, P, level=11, min_level=2, nrhs=64, tol=1e-8, pcg_max=200000, pcg=True):
+    """Variable-coefficient diffusion -div(k grad u) = f on the (2^level+1)^2
+    Q1 grid (k = exp(N(0,1)) on 64 x 64 tiles; Dirichlet left/right, Neumann
+    bottom/top) as SparseMatrixELL; nrhs seeded U[-1,1] right-hand sides
+    solved in one batch to rel. residual tol by
+      * Jacobi-PCG, CGSolver<VAR> in FP64 on a column-blocked multivector
+        (one pass over the matrix per iteration for all columns), and
+      * MG-IR, FP32 V-cycles (levels min_level..level, re-discretised with
+        cell-averaged k) inside FP64 refinement, one engine slot per RHS.
+    The true FP64 residual ||b - A x|| / ||b|| of every column is checked."""
+    side = P.side_of_level(level)
+    n = side * side
+    t0 = time.perf_counter()
+    bands = P.diffusion_q1_bands(level)
+    A64 = P.q1_to_ell(bands)
+    a32 = {L: P.diffusion_ell(L, dtype=np.float32) for L in range(min_level, level + 1)}
+    log(f"diffusion {side}^2 hierarchy built in {time.perf_counter() - t0:.1f}s")
+    out = dict(n=n, nrhs=nrhs, levels=f"{min_level}..{level}", tol=tol, nnz=P.ell_nnz(A64))
+    A = hip.EllMatrix.from_dict(A64)
+
+    def true_rel(X, B):
+        Rm = hip.DeviceVector.zeros(n * nrhs)
+        A.spmm(nrhs, X, Rm, rhs=B)
+        r = Rm.get().reshape(n, nrhs)
+        del Rm
+        b = B.get().reshape(n, nrhs)
+        return np.linalg.norm(r, axis=0) / np.linalg.norm(b, axis=0)
+
+    Xp = None
+    if pcg:
+        Pd = hip.DeviceVector.from_numpy(1.0 / bands[4])
+        B = hip.DeviceVector.zeros(n * nrhs)
+        hip.rhs_uniform_multi(B, side, nrhs, nrhs, 1, mask=P.DIFFUSION_MASK)
+        Xp = hip.DeviceVector.zeros(n * nrhs)
+        hip.synchronize()
+        t1 = time.perf_counter()
+        used, rel = hip.cg_ell_multi(A, Pd, B, Xp, nrhs, pcg_max, tol, raise_on_cap=False)
+        t2 = time.perf_counter()
+        tr = true_rel(Xp, B)
+        out["jacobi_pcg_fp64"] = dict(ms=(t2 - t1) * 1e3, solves_per_s=nrhs / (t2 - t1),
+                                      iters_min=int(used.min()), iters_max=int(used.max()),
+                                      max_rel_defect=float(rel.max()), max_true_rel_residual=float(tr.max()),
+                                      ms_per_iteration=(t2 - t1) * 1e3 / float(used.max()))
+        log("diffusion pcg", out["jacobi_pcg_fp64"])
+        Xp = Xp.get().reshape(n, nrhs)
+        del B, Pd
+    mg = hip.MGSolver.from_ell(hip.mg_params(min_level, level, n_max_iter=2, tolerance=tol, mask=P.DIFFUSION_MASK),
+                               A64, a32, None, slots=nrhs)
+    del a32
+    Bs = hip.DeviceVector.zeros(n * nrhs)
+    hip.rhs_uniform_multi(Bs, side, nrhs, 1, n, mask=P.DIFFUSION_MASK)
+    Xs = hip.DeviceVector.zeros(n * nrhs)
+    bs = [Bs.view(j * n, n) for j in range(nrhs)]
+    xs = [Xs.view(j * n, n) for j in range(nrhs)]
+    mg.solve_batch(bs[:1], xs[:1])                 # warm-up (one slot)
+    Xs.fill(0.0)
+    hip.synchronize()
+    t1 = time.perf_counter()
+    infos = mg.solve_batch(bs, xs, raise_on_cap=False)
+    t2 = time.perf_counter()
+    tmp = hip.DeviceVector.zeros(n)
+    tr = []
+    for j in range(nrhs):
+        A.spmv(xs[j], tmp, rhs=bs[j])
+        tr.append(np.linalg.norm(tmp.get()) / np.linalg.norm(bs[j].get()))
+    outer = [i["outer"] for i in infos]
+    inner = [i["inner"] for i in infos]
+    out["mg_ir_fp32_fp64"] = dict(ms=(t2 - t1) * 1e3, solves_per_s=nrhs / (t2 - t1), outer_min=min(outer),
+                                  outer_max=max(outer), inner_max=max(inner),
+                                  max_rel_defect=max(i["final_defect"] / i["initial_defect"] for i in infos),
+                                  max_true_rel_residual=float(max(tr)), device_bytes=mg.device_bytes)
+    if Xp is not None:
+        xm = Xs.get().reshape(nrhs, n).T
+        out["max_rel_diff_mg_vs_pcg"] = float(np.max(np.linalg.norm(xm - Xp, axis=0) / np.linalg.norm(Xp, axis=0)))
+        out["mg_ir_speedup_vs_pcg"] = out["jacobi_pcg_fp64"]["ms"] / out["mg_ir_fp32_fp64"]["ms"]
+    log("diffusion mg", out["mg_ir_fp32_fp64"])
+    return out
+
+
 def cg_case(hip, P, side, max_iters, tol=1e-8):
     n = side * side
     A = hip.Q1Matrix(P.poisson_q1_bands(n))
@@ -241,7 +320,16 @@ def main():
                     help="only time REPS MG-IR solves on the 4097^2 Poisson problem (for profiling)")
     ap.add_argument("--serve-only", type=int, default=0, metavar="REQUESTS",
                     help="only run the served mixed-size stream with REQUESTS requests")
+    ap.add_argument("--diffusion-only", type=int, default=0, metavar="LEVEL",
+                    help="only run the 64-RHS variable-coefficient diffusion case on the 2^LEVEL+1 grid")
+    ap.add_argument("--no-pcg", action="store_true", help="with --diffusion-only: MG-IR leg only")
     args = ap.parse_args()
+    if args.diffusion_only:
+        from honei_amd import device as hip
+        from honei_amd import problems as P
+        print(json.dumps({f"diffusion_{P.side_of_level(args.diffusion_only)}_64rhs":
+                          diffusion_case(hip, P, level=args.diffusion_only, pcg=not args.no_pcg)}))
+        return
     if args.serve_only:
         from honei_amd import device as hip
         print(json.dumps({"serve_mixed_poisson": serve_case(hip, args.serve_only)}))

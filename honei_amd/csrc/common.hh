@@ -44,6 +44,17 @@ namespace hsolve
     constexpr int kBlock = 256;
     constexpr int kReduceBlocks = 2048;   // partial sums per one-pass reduction
 
+    // SplitMix64 finaliser: output number i of the stream keyed by (seed, s)
+    // is mix64(key + G (i + 1)) with key = mix64(seed + G (s + 1)),
+    // G = 0x9E3779B97F4A7C15.
+    __host__ __device__ inline uint64_t mix64(uint64_t z)
+    {
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+
     inline unsigned grid_for(uint64_t work, unsigned per_block)
     {
         uint64_t g = (work + per_block - 1) / per_block;

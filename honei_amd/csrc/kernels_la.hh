@@ -493,28 +493,30 @@ namespace k
         __syncthreads();
     }
 
-    // Last-block finish of a two-value reduction over the blocks along x;
-    // `fin(a, b)` runs on one thread of the last block with the totals, and
-    // that block resets the ticket for the next use.
+    // Last-block finish of a two-value reduction over `nb` blocks, this one
+    // being block `bx` of them; `fin(a, b)` runs on one thread of the last
+    // block with the totals (partials summed in block order), and that block
+    // resets the ticket for the next use.
     template <typename F>
-    __device__ __forceinline__ void finish2(double a, double b, double * partials, unsigned * ticket, F && fin)
+    __device__ __forceinline__ void finish2_at(double a, double b, double * partials, unsigned * ticket, unsigned bx,
+                                               unsigned nb, F && fin)
     {
         __shared__ double lds[32];
         __shared__ bool last;
         block_sum2(a, b, lds);
         if (threadIdx.x == 0)
         {
-            partials[2 * blockIdx.x] = a;
-            partials[2 * blockIdx.x + 1] = b;
+            partials[2 * bx] = a;
+            partials[2 * bx + 1] = b;
             __threadfence();
-            last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+            last = (atomicAdd(ticket, 1u) == nb - 1);
         }
         __syncthreads();
         if (! last)
             return;
         __threadfence();
         double va = 0.0, vb = 0.0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+        for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
         {
             va += __builtin_nontemporal_load(partials + 2 * i);
             vb += __builtin_nontemporal_load(partials + 2 * i + 1);
@@ -525,6 +527,13 @@ namespace k
             fin(va, vb);
             *ticket = 0u;
         }
+    }
+
+    // the same over the blocks along x
+    template <typename F>
+    __device__ __forceinline__ void finish2(double a, double b, double * partials, unsigned * ticket, F && fin)
+    {
+        finish2_at(a, b, partials, ticket, blockIdx.x, gridDim.x, static_cast<F &&>(fin));
     }
 
     // sum_i a_i * b_i (b == nullptr: sum a_i^2), accumulated in double.

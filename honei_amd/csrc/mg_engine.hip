@@ -29,6 +29,23 @@ namespace
 {
     __device__ __forceinline__ bool live(const Slot & sl) { return sl.s != nullptr; }
 
+    // Virtual block coordinates of a slot launch (grid x [, y] per slot,
+    // blockIdx.z = slot).  The hardware hands out workgroups in linear order,
+    // x fastest; this bijection makes the slot the fastest-varying index, so
+    // consecutive workgroups take the same row block of every slot in turn
+    // and a level matrix that all slots share is fetched from HBM about once
+    // per row block instead of once per slot.
+    struct VB
+    {
+        unsigned x, y, z;
+    };
+    __device__ __forceinline__ VB vblock()
+    {
+        const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        const unsigned z = lin % gridDim.z, r = lin / gridDim.z;
+        return VB{r % gridDim.x, r / gridDim.x, z};
+    }
+
     // Row i of y = A x for an engine ELL matrix, in the reference's order
     // (slots ascending, summed from zero: honei/la/product.hh:148-206).
     template <typename DT>
@@ -65,12 +82,6 @@ namespace
         return ell_row<float>(a.E, x, i);
     }
 
-    // rows per lane of the row kernels: 4 (16-byte vectors) on Q1 bands, 1 on ELL
-    template <int MK> __device__ __forceinline__ int64_t lane_id()
-    {
-        return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    }
-
     // the level operator for the one-workgroup coarse CG
     template <int MK> struct LevelRowOp
     {
@@ -104,10 +115,11 @@ namespace
     // x32 of the finest level = (float)result
     __global__ __launch_bounds__(256) void start_init_kernel(const Slot * T, int L, int64_t n, int64_t sq)
     {
-        const Slot & sl = T[blockIdx.z];
+        const VB vb = vblock();
+        const Slot & sl = T[vb.z];
         if (! live(sl) || sl.s->do_start == 0)
             return;
-        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
         if (i >= n)
             return;
         double g = sl.s->zero_guess ? 0.0 : sl.result[i];
@@ -120,10 +132,11 @@ namespace
     // iterates of the coarser levels start at zero
     __global__ __launch_bounds__(256) void start_zero_kernel(const Slot * T, int minl, int L, LevelSizes sz)
     {
-        const Slot & sl = T[blockIdx.z];
+        const VB vb = vblock();
+        const Slot & sl = T[vb.z];
         if (! live(sl) || sl.s->do_start == 0)
             return;
-        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
         for (int l = minl; l < L; ++l)
             if (i < sz.n[l])
                 sl.v[l][F_X][i] = 0.0f;
@@ -133,10 +146,11 @@ namespace
     // result += scale * (double)x32
     __global__ __launch_bounds__(256) void outer_accumulate_kernel(const Slot * T, int L, int64_t n)
     {
-        const Slot & sl = T[blockIdx.z];
+        const VB vb = vblock();
+        const Slot & sl = T[vb.z];
         if (! live(sl) || sl.s->do_outer == 0)
             return;
-        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
         if (i >= n)
             return;
         const double scale = sl.s->scale;
@@ -150,7 +164,8 @@ namespace
     __global__ __launch_bounds__(256) void outer_defect_kernel(k::Bands B, EllMat64 E, const Slot * T, int64_t n,
             int64_t m, double outer_eps, uint64_t max_outer)
     {
-        const Slot & sl = T[blockIdx.z];
+        const VB vb = vblock();
+        const Slot & sl = T[vb.z];
         if (! live(sl))
             return;
         Scalars * s = sl.s;
@@ -162,19 +177,21 @@ namespace
         double * __restrict__ od = sl.od;
         const bool vec = MK == 0 && ((uintptr_t)b & 15u) == 0;
         double acc = 0.0;
-        const int64_t stride = (int64_t)gridDim.x * blockDim.x * 2;
-        for (int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i0 < n; i0 += stride)
+        if (MK == 1)
         {
-            if (MK == 1)
+            // one row per lane and pass, lanes on consecutive rows
+            const int64_t stride1 = (int64_t)gridDim.x * blockDim.x;
+            for (int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x; i < n; i += stride1)
             {
-                for (int64_t i = i0; i < i0 + 2 && i < n; ++i)
-                {
-                    const double di = b[i] - ell_row<double>(E, x, i);
-                    od[i] = di;
-                    acc += di * di;
-                }
+                const double di = b[i] - ell_row<double>(E, x, i);
+                od[i] = di;
+                acc += di * di;
             }
-            else if (vec && i0 - m - 1 >= 0 && i0 + 2 + m < n)
+        }
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x * 2;
+        for (int64_t i0 = ((int64_t)vb.x * blockDim.x + threadIdx.x) * 2; MK == 0 && i0 < n; i0 += stride)
+        {
+            if (vec && i0 - m - 1 >= 0 && i0 + 2 + m < n)
             {
                 double prod[2];
                 k::q1_rows_v<double>(B, x, i0, m, prod);
@@ -198,7 +215,7 @@ namespace
                 }
             }
         }
-        k::finish2(acc, 0.0, sl.partials, &s->ticket, [&](double dd, double) {
+        k::finish2_at(acc, 0.0, sl.partials, &s->ticket, vb.x, gridDim.x, [&](double dd, double) {
             const double nrm = sqrt(dd);
             if (start)
             {
@@ -243,10 +260,11 @@ namespace
     // the unscaled defect); a start also clears the result array
     __global__ __launch_bounds__(256) void to_inner_kernel(const Slot * T, int L, int64_t n)
     {
-        const Slot & sl = T[blockIdx.z];
+        const VB vb = vblock();
+        const Slot & sl = T[vb.z];
         if (! live(sl) || sl.s->open == 0)
             return;
-        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
         if (i >= n)
             return;
         const double inv = sl.s->inv;
@@ -260,11 +278,12 @@ namespace
     template <int INITIAL_ZERO, int MK>
     __global__ __launch_bounds__(256) void open_inner_kernel(LevelArgs a, const Slot * T, int L)
     {
-        const Slot & sl = T[blockIdx.z];
+        const VB vb = vblock();
+        const Slot & sl = T[vb.z];
         if (! live(sl) || sl.s->open == 0)
             return;
         const int64_t n = a.n;
-        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t lane = (int64_t)vb.x * blockDim.x + threadIdx.x;
         const float * rhs = sl.v[L][F_RHS];
         float * d = sl.v[L][F_D];
         if (INITIAL_ZERO)
@@ -282,10 +301,10 @@ namespace
         else
         {
             const float * x = sl.v[L][F_X];
-            for (int64_t i = lane * 4; i < lane * 4 + 4 && i < n; ++i)
+            for (int64_t i = lane; i < n; i += (int64_t)gridDim.x * blockDim.x)
                 d[i] = rhs[i] - ell_row<float>(a.E, x, i);
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0)
+        if (vb.x == 0 && threadIdx.x == 0)
         {
             sl.s->run = 1;
             sl.s->need_outer = 0;
@@ -299,7 +318,8 @@ namespace
     __global__ __launch_bounds__(256) void inner_norm_kernel(const Slot * T, int L, int64_t n, float tol,
             int convergence_check, uint64_t n_max_iter)
     {
-        const Slot & sl = T[blockIdx.z];
+        const VB vb = vblock();
+        const Slot & sl = T[vb.z];
         if (! live(sl))
             return;
         Scalars * s = sl.s;
@@ -309,7 +329,7 @@ namespace
         double acc = 0.0;
         typedef k::Vec16<float>::type F4;
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-        const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t tid = (int64_t)vb.x * blockDim.x + threadIdx.x;
         const int64_t n4 = n >> 2;
         const F4 * d4 = reinterpret_cast<const F4 *>(d);
         int64_t i = tid;
@@ -341,7 +361,7 @@ namespace
             const double v = (double)d[t];
             acc += v * v;
         }
-        k::finish2(acc, 0.0, sl.partials, &s->ticket, [&](double ss, double) {
+        k::finish2_at(acc, 0.0, sl.partials, &s->ticket, vb.x, gridDim.x, [&](double ss, double) {
             const float nrm = sqrtf((float)ss);
             if (MODE == 0)
             {
@@ -385,7 +405,7 @@ namespace
 
     // --------------------------------------------------------------- V-cycle
 #define HS_GATE_RUN                                     \
-    const Slot & sl = T[blockIdx.z];                    \
+    const Slot & sl = T[vb.z];                          \
     if (! live(sl) || sl.s->run == 0)                   \
         return;
 
@@ -393,8 +413,9 @@ namespace
     template <int ACC>
     __global__ __launch_bounds__(256) void v_zero1_kernel(LevelArgs a, const Slot * T, int L, int fsrc, int fout)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
-        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
         if (i >= a.n)
             return;
         const float * d = sl.v[L][fsrc];
@@ -403,16 +424,19 @@ namespace
         out[i] = ACC ? out[i] + c : c;
     }
 
-    // The row kernels below cover 4 rows per lane: on Q1 bands as 16-byte
-    // vectors (kernels_la.hh / kernels_mg.hh bodies), on ELL one row at a time.
+    // The row kernels below cover 4 rows per lane: on Q1 bands 4 consecutive
+    // rows as 16-byte vectors (kernels_la.hh / kernels_mg.hh bodies), on ELL
+    // one row per pass of a grid-stride loop (lanes on consecutive rows, so
+    // every slot load of a wave is one coalesced access).
 
     // the first two sweeps from zero (c1 = dinv*d, c2 = c1 + dinv*(d - A c1));
     // ACC: out += c2, else out = c2
     template <int ACC, int MK>
     __global__ __launch_bounds__(256) void v_zero2_kernel(LevelArgs a, const Slot * T, int L, int fsrc, int fout)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
-        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t lane = (int64_t)vb.x * blockDim.x + threadIdx.x;
         const float * d = sl.v[L][fsrc];
         float * out = sl.v[L][fout];
         if (MK == 0)
@@ -420,7 +444,7 @@ namespace
             k::jacobi_zero2_body<float, ACC>(a.B, d, a.dinv, out, a.n, a.m, lane);
             return;
         }
-        for (int64_t i = lane * 4; i < lane * 4 + 4 && i < a.n; ++i)
+        for (int64_t i = lane; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
         {
             const uint32_t l = a.E.len[i];
             float s = 0.0f;
@@ -442,8 +466,9 @@ namespace
     template <int MK>
     __global__ __launch_bounds__(256) void v_sweep_kernel(LevelArgs a, const Slot * T, int L, int fx, int fy, int frhs)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
-        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t lane = (int64_t)vb.x * blockDim.x + threadIdx.x;
         const float * x = sl.v[L][fx];
         float * y = sl.v[L][fy];
         const float * rhs = sl.v[L][frhs];
@@ -452,7 +477,7 @@ namespace
             k::q1_body_v<float, 2>(a.B, x, y, rhs, a.dinv, a.n, a.m, lane);
             return;
         }
-        for (int64_t i = lane * 4; i < lane * 4 + 4 && i < a.n; ++i)
+        for (int64_t i = lane; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
         {
             const float t = rhs[i] - ell_row<float>(a.E, x, i);
             y[i] = x[i] + t * a.dinv[i];
@@ -463,8 +488,9 @@ namespace
     template <int MK>
     __global__ __launch_bounds__(256) void v_defect_kernel(LevelArgs a, const Slot * T, int L)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
-        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t lane = (int64_t)vb.x * blockDim.x + threadIdx.x;
         const float * x = sl.v[L][F_X];
         float * d = sl.v[L][F_D];
         const float * rhs = sl.v[L][F_RHS];
@@ -473,7 +499,7 @@ namespace
             k::q1_body_v<float, 1>(a.B, x, d, rhs, nullptr, a.n, a.m, lane);
             return;
         }
-        for (int64_t i = lane * 4; i < lane * 4 + 4 && i < a.n; ++i)
+        for (int64_t i = lane; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
             d[i] = rhs[i] - ell_row<float>(a.E, x, i);
     }
 
@@ -481,19 +507,21 @@ namespace
     __global__ __launch_bounds__(256) void v_apply_ell_kernel(EllMat32 E, const Slot * T, int Lsrc, int fsrc, int Ldst,
                                                               int fdst)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
         const float * x = sl.v[Lsrc][fsrc];
         float * y = sl.v[Ldst][fdst];
-        const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        for (int64_t i = lane * 4; i < lane * 4 + 4 && i < (int64_t)E.rows; ++i)
+        const int64_t lane = (int64_t)vb.x * blockDim.x + threadIdx.x;
+        for (int64_t i = lane; i < (int64_t)E.rows; i += (int64_t)gridDim.x * blockDim.x)
             y[i] = ell_row<float>(E, x, i);
     }
 
     // x += alpha * c (ScaledSum with the constant correction factor)
     __global__ __launch_bounds__(256) void v_axpy_const_kernel(const Slot * T, int L, int64_t n, float alpha)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
-        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
         if (i >= n)
             return;
         float * x = sl.v[L][F_X];
@@ -504,11 +532,12 @@ namespace
     template <int ADD>
     __global__ __launch_bounds__(256) void v_move_kernel(const Slot * T, int L, int64_t n, int fsrc, int fdst)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
         typedef k::Vec16<float>::type F4;
         const float * src = sl.v[L][fsrc];
         float * dst = sl.v[L][fdst];
-        const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+        const int64_t i0 = ((int64_t)vb.x * blockDim.x + threadIdx.x) * 4;
         if (i0 + 4 <= n)
         {
             const F4 s = *(const F4 *)(src + i0);
@@ -532,9 +561,10 @@ namespace
     // rhs of level L-1 = restriction of d of level L
     __global__ __launch_bounds__(256) void v_restrict_kernel(const Slot * T, int L, int64_t Nc, k::Mask mk)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
-        k::restriction_body<float>(sl.v[L - 1][F_RHS], sl.v[L][F_D], Nc, mk, blockIdx.y,
-                                   (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+        k::restriction_body<float>(sl.v[L - 1][F_RHS], sl.v[L][F_D], Nc, mk, vb.y,
+                                   (int64_t)vb.x * blockDim.x + threadIdx.x);
     }
 
     // ConjugateGradients<NONE> (konv_rad form) on the coarsest level
@@ -542,6 +572,7 @@ namespace
     __global__ __launch_bounds__(256) void v_coarse_kernel(LevelArgs a, const Slot * T, int L, int frhs,
                                                            double konv_rad, uint64_t max_iters)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
         LevelRowOp<MK> A{a};
         k::cg_konv_body_op<float>(A, sl.v[L][frhs], sl.v[L][F_X], sl.cg_work, a.n, konv_rad, max_iters,
@@ -553,34 +584,37 @@ namespace
     __global__ __launch_bounds__(256) void v_prolong_kernel(const Slot * T, int L, int Nf, float alpha, k::Mask mk,
                                                             int fout)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
-        k::prolongation2_body<float, AXPY>(sl.v[L][fout], sl.v[L - 1][F_X], Nf, alpha, mk, blockIdx.y,
-                                           blockIdx.x * blockDim.x + threadIdx.x);
+        k::prolongation2_body<float, AXPY>(sl.v[L][fout], sl.v[L - 1][F_X], Nf, alpha, mk, vb.y,
+                                           vb.x * blockDim.x + threadIdx.x);
     }
 
     // adapt_correction_factor == 0: alpha = (d.c) / ((A c).c)
     template <int MK>
     __global__ __launch_bounds__(256) void v_adapt_alpha_kernel(LevelArgs a, const Slot * T, int L)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
         const float * d = sl.v[L][F_D];
         const float * c = sl.v[L][F_C];
         double a1 = 0.0, a2 = 0.0;
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride)
+        for (int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x; i < a.n; i += stride)
         {
             const float t = level_row<MK>(a, c, i);
             a1 += (double)d[i] * (double)c[i];
             a2 += (double)t * (double)c[i];
         }
         Scalars * s = sl.s;
-        k::finish2(a1, a2, sl.partials, &s->ticket, [&](double d1, double d2) { s->alpha = (float)d1 / (float)d2; });
+        k::finish2_at(a1, a2, sl.partials, &s->ticket, vb.x, gridDim.x, [&](double d1, double d2) { s->alpha = (float)d1 / (float)d2; });
     }
 
     __global__ __launch_bounds__(256) void v_axpy_alpha_kernel(const Slot * T, int L, int64_t n)
     {
+        const VB vb = vblock();
         HS_GATE_RUN
-        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
         if (i >= n)
             return;
         const float alpha = sl.s->alpha;
@@ -619,13 +653,6 @@ namespace
     }
 
     // ----------------------------------------------------------------- setup
-    __host__ __device__ inline uint64_t mix64(uint64_t z)
-    {
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        return z ^ (z >> 31);
-    }
-
     // b_i = -1 + 2 u_i, u_i = (bits_i >> 11) 2^-53 with SplitMix64 counters;
     // 0 on the boundary nodes of the square grid
     __global__ __launch_bounds__(256) void rhs_uniform_kernel(double * b, int64_t n, int64_t sq, uint64_t key)

@@ -37,6 +37,14 @@ class DeviceBuffer:
             pass
 
 
+class _Span:
+    """Part of another buffer (keeps the owner alive, frees nothing)."""
+
+    def __init__(self, owner, ptr):
+        self.owner = owner
+        self.ptr = ptr
+
+
 class DeviceVector:
     """DenseVector<DT> kept on the device."""
 
@@ -56,6 +64,15 @@ class DeviceVector:
     def zeros(cls, size, dtype=np.float64):
         v = cls(size, dtype)
         v.fill(0)
+        return v
+
+    def view(self, offset, size):
+        """Elements [offset, offset + size) as a vector sharing this one's memory."""
+        offset, size = int(offset), int(size)
+        assert 0 <= offset and offset + size <= self.size
+        v = DeviceVector.__new__(DeviceVector)
+        v.dtype, v.size = self.dtype, size
+        v.buf = _Span(self.buf, self.buf.ptr + offset * self.dtype.itemsize)
         return v
 
     @property
@@ -133,6 +150,11 @@ class EllMatrix:
     def spmv(self, x, y, rhs=None, stream=None):
         check(lib().hsolve_ell_spmv(vp(self.handle), vp(x.ptr), vp(y.ptr), vp(rhs.ptr) if rhs else None,
                                     vp(stream)), "ell_spmv")
+
+    def spmm(self, nrhs, X, Y, rhs=None, stream=None):
+        """Y = A X (or RHS - A X) on column-blocked multivectors of nrhs columns."""
+        check(lib().hsolve_ell_spmm(vp(self.handle), u64(nrhs), vp(X.ptr), vp(Y.ptr), vp(rhs.ptr) if rhs else None,
+                                    vp(stream)), "ell_spmm")
 
     def jacobi(self, rhs, dinv, x_old, x_new, stream=None):
         check(lib().hsolve_ell_jacobi(vp(self.handle), vp(rhs.ptr), vp(dinv.ptr), vp(x_old.ptr), vp(x_new.ptr),
@@ -277,6 +299,30 @@ def pcg_ell(A: EllMatrix, P: DeviceVector, b: DeviceVector, x: DeviceVector, max
     if rc != 5 or raise_on_cap:
         check(rc, "pcg_ell")
     return used.value, defect.value
+
+
+def cg_ell_multi(A: EllMatrix, P, B: DeviceVector, X: DeviceVector, nrhs, max_iters=10000, eps=1e-8, stream=None,
+                 raise_on_cap=True):
+    """CGSolver<VAR> (P a DeviceVector) or <NONE> (P None) on every column of
+    the column-blocked multivector B; returns (iterations, relative defects)."""
+    used = np.zeros(nrhs, np.uint64)
+    defect = np.zeros(nrhs, np.float64)
+    rc = lib().hsolve_cg_ell_multi(vp(A.handle), u64(nrhs), vp(P.ptr) if P is not None else None, vp(B.ptr),
+                                   vp(X.ptr), u64(max_iters), float(eps), used.ctypes.data_as(vp),
+                                   defect.ctypes.data_as(vp), vp(stream))
+    if rc != 5 or raise_on_cap:
+        check(rc, "cg_ell_multi")
+    return used, defect
+
+
+def rhs_uniform_multi(b: DeviceVector, side, count, row_stride, col_stride, mask=None, seed=None, stream_id=1,
+                      stream=None):
+    """Seeded U[-1, 1] right-hand sides, column j from seed + j, zero on the Dirichlet border nodes."""
+    from . import problems
+    m = None if mask is None else (u64 * 8)(*mask)
+    check(lib().hsolve_rhs_uniform_multi(vp(b.ptr), u64(side), u64(count), u64(row_stride), u64(col_stride), m,
+                                         u64(problems.SEED if seed is None else seed), u64(stream_id), vp(stream)),
+          "rhs_uniform_multi")
 
 
 def mg_params(min_level, max_level, n_max_iter=2, tolerance=1e-8, n_pre_smooth=2, adapt=1.0, mask=(2,) * 8,

@@ -184,6 +184,85 @@ def matrix_b_ell(n, seed=SEED, dtype=np.float64, halfwidth=2048, min_len=5, max_
                 Arl=lens.astype(np.uint64))
 
 
+# -- variable-coefficient diffusion -div(k grad u) = f on the unit square ----
+# Q1 elements, k constant per cell.  Dirichlet (u = 0) on the left and right
+# edges, natural Neumann on the bottom and top edges.
+
+DIFFUSION_TILES = 64
+# macro border mask (corners 0..3, then bottom, right, top, left edges): 2 =
+# Dirichlet, 1 = Neumann
+DIFFUSION_MASK = (2, 2, 2, 2, 1, 2, 1, 2)
+
+
+def ktile_field(seed=SEED, tiles=DIFFUSION_TILES):
+    """k = exp(N(0,1)) on a tiles x tiles piecewise-constant tiling of the
+    square, tile (ty, tx) from counter ty * tiles + tx of stream S_KTILE."""
+    idx = np.arange(tiles * tiles, dtype=np.uint64)
+    return np.exp(rng_normal(seed, S_KTILE, idx)).reshape(tiles, tiles)
+
+
+def cell_coefficients(level, seed=SEED, tiles=DIFFUSION_TILES):
+    """k per cell of the 2^level x 2^level cell grid: the tile value where a
+    cell lies inside one tile, else the arithmetic mean of the tiles it covers."""
+    cells = 1 << level
+    kt = ktile_field(seed, tiles)
+    if cells >= tiles:
+        rep = cells // tiles
+        return np.repeat(np.repeat(kt, rep, axis=0), rep, axis=1)
+    f = tiles // cells
+    return kt.reshape(cells, f, cells, f).mean(axis=(1, 3))
+
+
+def dirichlet_left_right(side):
+    m = np.zeros((side, side), bool)
+    m[:, 0] = m[:, -1] = True
+    return m.ravel()
+
+
+def diffusion_q1_bands(level, seed=SEED, tiles=DIFFUSION_TILES, dtype=np.float64, cells=None):
+    """The 9 Q1 bands (LL, LD, LU, DL, DD, DU, UL, UD, UU) of -div(k grad u)
+    on the side 2^level + 1 grid.  The bilinear element matrix of a square
+    cell is k * (2/3 on the diagonal, -1/6 to the two edge neighbours, -1/3
+    to the opposite corner).  Rows of Dirichlet nodes are identity rows and
+    their columns are eliminated from the other rows (u = 0 there), so the
+    matrix is symmetric positive definite.  `cells` overrides the per-cell
+    coefficients (2^level x 2^level, row = y)."""
+    side = side_of_level(level)
+    K = np.zeros((side + 1, side + 1))
+    K[1:-1, 1:-1] = cell_coefficients(level, seed, tiles) if cells is None else cells
+    SW, SE, NW, NE = K[:-1, :-1], K[:-1, 1:], K[1:, :-1], K[1:, 1:]
+    third, sixth = 1.0 / 3.0, 1.0 / 6.0
+    B = [-third * SW, -sixth * (SW + SE), -third * SE,
+         -sixth * (SW + NW), (2.0 / 3.0) * ((SW + SE) + (NW + NE)), -sixth * (SE + NE),
+         -third * NW, -sixth * (NW + NE), -third * NE]
+    B = [np.ascontiguousarray(b).ravel() for b in B]
+    n = side * side
+    dmask = dirichlet_left_right(side)
+    offs = q1_offsets(n)
+    i = np.arange(n, dtype=np.int64)
+    for b in range(9):
+        if b == 4:
+            continue
+        j = np.clip(i + offs[b], 0, n - 1)
+        B[b][dmask | dmask[j]] = 0.0
+    B[4][dmask] = 1.0
+    return [b.astype(dtype) for b in B]
+
+
+def diffusion_ell(level, seed=SEED, tiles=DIFFUSION_TILES, dtype=np.float64):
+    """SparseMatrixELL (threads = 1, ascending columns) of diffusion_q1_bands."""
+    return q1_to_ell(diffusion_q1_bands(level, seed, tiles, np.float64), dtype)
+
+
+def rhs_diffusion(level, j, seed=SEED, stream=S_RHS):
+    """Right-hand side j of a batch: U[-1, 1] from seed + j, 0 on the
+    Dirichlet (left and right edge) nodes."""
+    side = side_of_level(level)
+    b = rng_uniform(seed + j, stream, np.arange(side * side, dtype=np.uint64), -1.0, 1.0)
+    b[dirichlet_left_right(side)] = 0.0
+    return b
+
+
 def ell_nnz(E):
     return int(np.asarray(E["Arl"], np.uint64).sum()) * int(E.get("threads", 1))
 

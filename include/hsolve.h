@@ -272,6 +272,36 @@ int hsolve_cg_ell(hsolve_ell m, const void *b, void *x, uint64_t max_iters, doub
 int hsolve_pcg_ell(hsolve_ell m, const void *P, const void *b, void *x, uint64_t max_iters,
                    double eps, uint64_t *used_iters, double *defect_out, void *stream);
 
+/* ---------------- many right-hand sides on one ELL matrix -------------------
+ * A column-blocked multivector of nrhs <= 64 columns stores row i's entries
+ * at [i*nrhs, i*nrhs + nrhs) (device memory, the handle's value type).  The
+ * handle must have threads == 1.  The reference applies Product / Defect /
+ * CGSolver to one DenseVector at a time; these run every column in the same
+ * launches with each column's arithmetic unchanged. */
+/* Y = A X, or Y = RHS - A X when RHS != NULL: Product<Tag>::value(y, A, x)
+ * (honei/la/product.hh:148-206) and Defect<Tag> (honei/math/defect.hh:357-384)
+ * per column, bit-identical to hsolve_ell_spmv on that column. */
+int hsolve_ell_spmm(hsolve_ell m, uint64_t nrhs, const void *X, void *Y, const void *RHS,
+                    void *stream);
+/* CGSolver<Tag, methods::VAR> (P != NULL: diagonal preconditioner, z = P * r)
+ * or CGSolver<Tag, methods::NONE> (P == NULL), honei/math/cg.hh:63-219, on
+ * every column of B at once; X holds the initial guesses and the results.
+ * Each column keeps its own scalars, stopping test and iteration count;
+ * used_iters[nrhs] and defect_out[nrhs] (final relative defect) are host
+ * arrays or NULL.  HSOLVE_ERR_NOT_CONVERGED when a column hits max_iters. */
+int hsolve_cg_ell_multi(hsolve_ell m, uint64_t nrhs, const void *P, const void *B, void *X,
+                        uint64_t max_iters, double eps, uint64_t *used_iters, double *defect_out,
+                        void *stream);
+/* Seeded right-hand sides on a side x side grid (FP64): entry i of column j
+ * at b[i*row_stride + j*col_stride] is -1 + 2u, u = (SplitMix64 output i of
+ * stream (seed + j, stream_id) >> 11) * 2^-53, and 0 on the nodes whose
+ * macro_border_mask part (corners 0..3, bottom, right, top, left edges) is
+ * 2 (Dirichlet); mask NULL = every border node.  No reference counterpart
+ * (synthetic input generator of the benchmarks). */
+int hsolve_rhs_uniform_multi(void *b, uint64_t side, uint64_t count, uint64_t row_stride,
+                             uint64_t col_stride, const uint64_t *macro_border_mask, uint64_t seed,
+                             uint64_t stream_id, void *stream);
+
 /* Mixed-precision MG-IR: Multigrid<Tag, OuterTag, NONE, JAC, CYCLE::V, MIXED>
  * on Q1 bands (honei/math/multigrid.hh:1616-2393).  A hierarchy holds the
  * FP64 finest matrix, FP32 matrices and inverted diagonals on every level
