@@ -63,13 +63,12 @@ namespace hsolve
         return (unsigned)g;
     }
 
-    // Per-stream scratch used by the one-pass reductions: a partials array
-    // and a ticket counter that the last block resets, so a reduction is
-    // safe to replay inside a captured graph without any memset.
+    // Per-stream scratch of the dot / norm reductions: the block partials,
+    // summed by a following one-block launch (no memset, so a reduction can
+    // be replayed inside a captured graph).
     struct ReduceScratch
     {
         double * partials;      // kReduceBlocks doubles
-        unsigned * ticket;      // 1 counter, zero at rest
     };
     ReduceScratch & reduce_scratch(hipStream_t stream);
 }

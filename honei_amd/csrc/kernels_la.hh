@@ -436,36 +436,6 @@ namespace k
         return r;   // valid in thread 0
     }
 
-    // Finishes a one-pass reduction: the block that takes the last ticket
-    // sums all block partials in index order and writes the result.
-    template <typename RT, typename F>
-    __device__ __forceinline__ void finish_reduce(double block_total, double * partials,
-            unsigned * ticket, F && write_result)
-    {
-        __shared__ double lds[16];
-        __shared__ bool last;
-        if (threadIdx.x == 0)
-        {
-            partials[blockIdx.x] = block_total;
-            __threadfence();
-            const unsigned t = atomicAdd(ticket, 1u);
-            last = (t == gridDim.x - 1);
-        }
-        __syncthreads();
-        if (! last)
-            return;
-        __threadfence();
-        double v = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-            v += __builtin_nontemporal_load(partials + b);
-        const double total = block_sum(v, lds);
-        if (threadIdx.x == 0)
-        {
-            write_result(total);
-            *ticket = 0u;
-        }
-    }
-
     // Two sums at once: wave shuffles, then the waves' totals in order.
     // Valid in thread 0 on return.
     __device__ __forceinline__ void block_sum2(double & a, double & b, double * lds)
@@ -493,54 +463,46 @@ namespace k
         __syncthreads();
     }
 
-    // Last-block finish of a two-value reduction over `nb` blocks, this one
-    // being block `bx` of them; `fin(a, b)` runs on one thread of the last
-    // block with the totals (partials summed in block order), and that block
-    // resets the ticket for the next use.
-    template <typename F>
-    __device__ __forceinline__ void finish2_at(double a, double b, double * partials, unsigned * ticket, unsigned bx,
-                                               unsigned nb, F && fin)
+    // Two-value reduction over the blocks of a launch, in two launches: each
+    // block stores its pair of block sums (put_partials2: wave shuffles, then
+    // the waves in order; no fence and no atomic), and a following one-block
+    // launch of 256 threads forms the totals (sum_partials2: thread-strided
+    // sums of the partials in block order, then the block tree), so the
+    // result depends only on the grid.  The kernel boundary orders the
+    // partial stores before the final sum.  (A last-block finish instead
+    // needs a device-scope fence per block, which costs ~50 ns of serialised
+    // time on gfx950 and dominates a reduction over thousands of blocks.)
+    __device__ __forceinline__ void put_partials2(double a, double b, double * partials, unsigned bx)
     {
         __shared__ double lds[32];
-        __shared__ bool last;
         block_sum2(a, b, lds);
         if (threadIdx.x == 0)
         {
             partials[2 * bx] = a;
             partials[2 * bx + 1] = b;
-            __threadfence();
-            last = (atomicAdd(ticket, 1u) == nb - 1);
         }
-        __syncthreads();
-        if (! last)
-            return;
-        __threadfence();
+    }
+    // totals over nb partial pairs; valid on thread 0
+    __device__ __forceinline__ void sum_partials2(const double * partials, unsigned nb, double & a, double & b)
+    {
+        __shared__ double lds[32];
         double va = 0.0, vb = 0.0;
         for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
         {
-            va += __builtin_nontemporal_load(partials + 2 * i);
-            vb += __builtin_nontemporal_load(partials + 2 * i + 1);
+            va += partials[2 * i];
+            vb += partials[2 * i + 1];
         }
         block_sum2(va, vb, lds);
-        if (threadIdx.x == 0)
-        {
-            fin(va, vb);
-            *ticket = 0u;
-        }
+        a = va;
+        b = vb;
     }
 
-    // the same over the blocks along x
-    template <typename F>
-    __device__ __forceinline__ void finish2(double a, double b, double * partials, unsigned * ticket, F && fin)
-    {
-        finish2_at(a, b, partials, ticket, blockIdx.x, gridDim.x, static_cast<F &&>(fin));
-    }
-
-    // sum_i a_i * b_i (b == nullptr: sum a_i^2), accumulated in double.
+    // sum_i a_i * b_i (b == nullptr: sum a_i^2), accumulated in double:
+    // block sums here, the total in dot_finish_kernel (one block: thread-
+    // strided sums of the partials in index order, then the block tree).
     template <typename DT>
     __global__ __launch_bounds__(256) void dot_kernel(const DT * __restrict__ a,
-            const DT * __restrict__ b, uint64_t n, double * partials, unsigned * ticket,
-            DT * out, int root)
+            const DT * __restrict__ b, uint64_t n, double * partials)
     {
         __shared__ double lds[16];
         double v = 0.0;
@@ -551,9 +513,20 @@ namespace k
             v += ai * (b ? (double)b[i] : ai);
         }
         const double bt = block_sum(v, lds);
-        finish_reduce<DT>(bt, partials, ticket, [&](double total) {
+        if (threadIdx.x == 0)
+            partials[blockIdx.x] = bt;
+    }
+
+    template <typename DT>
+    __global__ __launch_bounds__(256) void dot_finish_kernel(const double * partials, unsigned nb, DT * out, int root)
+    {
+        __shared__ double lds[16];
+        double v = 0.0;
+        for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
+            v += partials[b];
+        const double total = block_sum(v, lds);
+        if (threadIdx.x == 0)
             *out = root ? (DT)sqrt(total) : (DT)total;
-        });
     }
 
     // ------------------------------------------------------------ BLAS-1

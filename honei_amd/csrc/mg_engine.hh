@@ -52,10 +52,8 @@ namespace mg
         int open;            // an inner solve opens in this tick
         int status;          // 1: outer step cap reached without convergence
         int zero_guess;      // start from x = 0 instead of the result array
-        unsigned ticket;     // last-block counter of this slot's reductions
         float alpha;         // correction factor when it is adapted
         float inner_initial, inner_defect;
-        int pad;
         uint64_t outer_iters, inner_total, inner_last, cg_iters;
         uint64_t ticks;      // ticks that found the solve started or running
         double initial_defect, def_norm, scale, inv;

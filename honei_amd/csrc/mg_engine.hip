@@ -46,17 +46,47 @@ namespace
         return VB{r % gridDim.x, r / gridDim.x, z};
     }
 
-    // Row i of y = A x for an engine ELL matrix, in the reference's order
-    // (slots ascending, summed from zero: honei/la/product.hh:148-206).
-    template <typename DT>
-    __device__ __forceinline__ DT ell_row(const EllMat<DT> & E, const DT * __restrict__ x, int64_t i)
+    // Row i of y = A x' for an engine ELL matrix, x'_c = gx(c), in the
+    // reference's order (slots ascending, summed from zero:
+    // honei/la/product.hh:148-206).  With 9 stored slots per row (the Q1
+    // stencils) every slot load and gather is issued before the first add;
+    // padding slots hold valid columns (checked at upload) and are loaded but
+    // not summed.
+    template <typename DT, typename GX>
+    __device__ __forceinline__ DT ell_row_g(const EllMat<DT> & E, int64_t i, GX && gx)
     {
         const uint32_t l = E.len[i];
         DT s = DT(0);
+        if (E.ncpr == 9)
+        {
+            uint32_t c[9];
+            DT v[9], xv[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+            {
+                const uint64_t j = (uint64_t)i + (uint64_t)k * E.stride;
+                c[k] = E.col[j];
+                v[k] = E.val[j];
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                xv[k] = gx(c[k]);
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                if ((uint32_t)k < l)
+                    s += v[k] * xv[k];
+            return s;
+        }
         uint64_t j = (uint64_t)i;
         for (uint32_t k = 0; k < l; ++k, j += E.stride)
-            s += E.val[j] * x[E.col[j]];
+            s += E.val[j] * gx(E.col[j]);
         return s;
+    }
+
+    template <typename DT>
+    __device__ __forceinline__ DT ell_row(const EllMat<DT> & E, const DT * __restrict__ x, int64_t i)
+    {
+        return ell_row_g<DT>(E, i, [&](uint32_t c) { return x[c]; });
     }
 
     struct LevelSizes
@@ -162,7 +192,7 @@ namespace
     // (two rows per lane, 16-byte loads inside); MK 1: the ELL matrix E.
     template <int MK>
     __global__ __launch_bounds__(256) void outer_defect_kernel(k::Bands B, EllMat64 E, const Slot * T, int64_t n,
-            int64_t m, double outer_eps, uint64_t max_outer)
+            int64_t m)
     {
         const VB vb = vblock();
         const Slot & sl = T[vb.z];
@@ -215,7 +245,26 @@ namespace
                 }
             }
         }
-        k::finish2_at(acc, 0.0, sl.partials, &s->ticket, vb.x, gridDim.x, [&](double dd, double) {
+        k::put_partials2(acc, 0.0, sl.partials, vb.x);
+    }
+
+    // |od| from the partials of outer_defect_kernel (nb blocks per slot), then
+    // the outer step's decisions; one block per slot (blockIdx.z)
+    __global__ __launch_bounds__(256) void outer_defect_finish_kernel(const Slot * T, unsigned nb, double outer_eps,
+                                                                      uint64_t max_outer)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl))
+            return;
+        Scalars * s = sl.s;
+        const int start = s->do_start;
+        if (start == 0 && s->do_outer == 0)
+            return;
+        double dd, unused;
+        k::sum_partials2(sl.partials, nb, dd, unused);
+        if (threadIdx.x != 0)
+            return;
+        {
             const double nrm = sqrt(dd);
             if (start)
             {
@@ -253,7 +302,7 @@ namespace
                         s->open = 1;
                 }
             }
-        });
+        }
     }
 
     // rhs32 = (float)(od * inv) (inv == 1 on the first step, so the cast of
@@ -315,8 +364,7 @@ namespace
     // `open`); MODE 1: end of a V-cycle (gate `run`): count it, stop on
     // |d| <= tol |d_0| or after n_max_iter cycles.
     template <int MODE>
-    __global__ __launch_bounds__(256) void inner_norm_kernel(const Slot * T, int L, int64_t n, float tol,
-            int convergence_check, uint64_t n_max_iter)
+    __global__ __launch_bounds__(256) void inner_norm_kernel(const Slot * T, int L, int64_t n)
     {
         const VB vb = vblock();
         const Slot & sl = T[vb.z];
@@ -361,7 +409,26 @@ namespace
             const double v = (double)d[t];
             acc += v * v;
         }
-        k::finish2_at(acc, 0.0, sl.partials, &s->ticket, vb.x, gridDim.x, [&](double ss, double) {
+        k::put_partials2(acc, 0.0, sl.partials, vb.x);
+    }
+
+    // the inner stopping test from inner_norm_kernel's partials (nb blocks
+    // per slot); one block per slot (blockIdx.z)
+    template <int MODE>
+    __global__ __launch_bounds__(256) void inner_norm_finish_kernel(const Slot * T, unsigned nb, float tol,
+            int convergence_check, uint64_t n_max_iter)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl))
+            return;
+        Scalars * s = sl.s;
+        if (MODE == 0 ? s->open == 0 : s->run == 0)
+            return;
+        double ss, unused;
+        k::sum_partials2(sl.partials, nb, ss, unused);
+        if (threadIdx.x != 0)
+            return;
+        {
             const float nrm = sqrtf((float)ss);
             if (MODE == 0)
             {
@@ -386,7 +453,7 @@ namespace
                     s->need_outer = 1;
                 }
             }
-        });
+        }
     }
 
     // one-level hierarchy: the cycle loop is left after the coarse solve
@@ -446,14 +513,7 @@ namespace
         }
         for (int64_t i = lane; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
         {
-            const uint32_t l = a.E.len[i];
-            float s = 0.0f;
-            uint64_t j = (uint64_t)i;
-            for (uint32_t kk = 0; kk < l; ++kk, j += a.E.stride)
-            {
-                const uint32_t c = a.E.col[j];
-                s += a.E.val[j] * (a.dinv[c] * d[c]);
-            }
+            const float s = ell_row_g<float>(a.E, i, [&](uint32_t c) { return a.dinv[c] * d[c]; });
             const float di = a.dinv[i];
             const float c1 = di * d[i];
             const float t = d[i] - s;
@@ -606,8 +666,19 @@ namespace
             a1 += (double)d[i] * (double)c[i];
             a2 += (double)t * (double)c[i];
         }
-        Scalars * s = sl.s;
-        k::finish2_at(a1, a2, sl.partials, &s->ticket, vb.x, gridDim.x, [&](double d1, double d2) { s->alpha = (float)d1 / (float)d2; });
+        k::put_partials2(a1, a2, sl.partials, vb.x);
+    }
+
+    // alpha from v_adapt_alpha_kernel's partials; one block per slot
+    __global__ __launch_bounds__(256) void v_adapt_alpha_finish_kernel(const Slot * T, unsigned nb)
+    {
+        const Slot & sl = T[blockIdx.z];
+        if (! live(sl) || sl.s->run == 0)
+            return;
+        double d1, d2;
+        k::sum_partials2(sl.partials, nb, d1, d2);
+        if (threadIdx.x == 0)
+            sl.s->alpha = (float)d1 / (float)d2;
     }
 
     __global__ __launch_bounds__(256) void v_axpy_alpha_kernel(const Slot * T, int L, int64_t n)
@@ -993,9 +1064,10 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                            (int)minl, (int)maxl, sz);
     }
     hipLaunchKernelGGL(outer_accumulate_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
-    hipLaunchKernelGGL((outer_defect_kernel<MK>),
-                       dim3(std::min<unsigned>(grid_for((top.n + 1) / 2, kBlock), kNormBlocks), 1, S), blk, 0, st,
-                       MK == 0 ? h.B64(maxl) : k::Bands{}, h.ell64[maxl], T, n, top.side, (double)p.tolerance,
+    const unsigned od_blocks = std::min<unsigned>(grid_for((top.n + 1) / 2, kBlock), kNormBlocks);
+    hipLaunchKernelGGL((outer_defect_kernel<MK>), dim3(od_blocks, 1, S), blk, 0, st,
+                       MK == 0 ? h.B64(maxl) : k::Bands{}, h.ell64[maxl], T, n, top.side);
+    hipLaunchKernelGGL(outer_defect_finish_kernel, dim3(1, 1, S), blk, 0, st, T, od_blocks, (double)p.tolerance,
                        p.max_outer);
     hipLaunchKernelGGL(to_inner_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
     if (p.initial_zero)
@@ -1004,7 +1076,8 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     else
         hipLaunchKernelGGL((open_inner_kernel<0, MK>), dim3(lanes4(top.n), 1, S), blk, 0, st, largs(maxl), T,
                            (int)maxl);
-    hipLaunchKernelGGL((inner_norm_kernel<0>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n, p.tolerance,
+    hipLaunchKernelGGL((inner_norm_kernel<0>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n);
+    hipLaunchKernelGGL((inner_norm_finish_kernel<0>), dim3(1, 1, S), blk, 0, st, T, norm_grid(top.n).x, p.tolerance,
                        p.convergence_check, p.n_max_iter);
 
     // ---- one V-cycle (multigrid.hh:1660-1905)
@@ -1119,8 +1192,9 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
             hipLaunchKernelGGL((v_prolong_kernel<false>), pg, blk, 0, st, T, (int)L, Nf, 0.0f, p.mask, (int)F_C);
         if (p.adapt == 0.0f)
         {
-            hipLaunchKernelGGL((v_adapt_alpha_kernel<MK>), dim3(std::min<unsigned>(grid_for(a.n, kBlock), kNormBlocks), 1, S),
-                               blk, 0, st, a, T, (int)L);
+            const unsigned ab = std::min<unsigned>(grid_for(a.n, kBlock), kNormBlocks);
+            hipLaunchKernelGGL((v_adapt_alpha_kernel<MK>), dim3(ab, 1, S), blk, 0, st, a, T, (int)L);
+            hipLaunchKernelGGL(v_adapt_alpha_finish_kernel, dim3(1, 1, S), blk, 0, st, T, ab);
             hipLaunchKernelGGL(v_axpy_alpha_kernel, dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, T, (int)L, a.n);
         }
         // post-smoothing: n_pre_smooth sweeps (multigrid.hh:1851)
@@ -1131,7 +1205,8 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         if (L == maxl)
             defect(L);
     }
-    hipLaunchKernelGGL((inner_norm_kernel<1>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n, p.tolerance,
+    hipLaunchKernelGGL((inner_norm_kernel<1>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n);
+    hipLaunchKernelGGL((inner_norm_finish_kernel<1>), dim3(1, 1, S), blk, 0, st, T, norm_grid(top.n).x, p.tolerance,
                        p.convergence_check, p.n_max_iter);
     HIP_CHECK(hipGetLastError());
 }

@@ -22,7 +22,8 @@
 // coalesced load per slot (lane l on row base + l) and hands them to the
 // row loop by readlane.  Per-column reductions accumulate per lane in
 // double in the wave's row order, then over the 4 waves, then over the
-// blocks in block order (last-block finish), so results are deterministic.
+// blocks in block order (a one-block finish launch), so results are
+// deterministic.
 #include "ell_handle.hh"
 #include "kernels_mg.hh"
 
@@ -36,8 +37,9 @@ namespace hsolve
 namespace
 {
     constexpr int kMaxCols = 64;           // columns of a multivector (one per lane)
-    constexpr int kTileK = 16;             // ELL slots a tile keeps in registers
+    constexpr int kTileK = 16;             // most ELL slots per row a tile keeps in registers
     constexpr unsigned kMvBlocks = 2048;   // grid cap of the multivector kernels
+    constexpr int kFinThreads = 1024;      // threads of the one-block finish launches
 
     template <typename DT, typename IT, typename LT> struct MvEll
     {
@@ -83,10 +85,14 @@ namespace
 
     // Calls f(row, sum_j) for every row of the tile at `base`, sum_j = row
     // of A times column j of X (lane j), the row's slots summed from zero in
-    // ascending slot order (product.hh:192-202).
-    template <typename DT, typename IT, typename LT, typename F>
-    __device__ __forceinline__ void tile_rows(const MvEll<DT, IT, LT> & A, const DT * __restrict__ X, unsigned R,
-                                              int64_t base, bool on, F && f)
+    // ascending slot order (product.hh:192-202), and own_j = OWN[row][j]
+    // (0 when OWN is null).  K >= num_cols_per_row: the tile keeps K slots
+    // per row in registers (unused slots point at the row itself), and each
+    // row issues all K gathers and the OWN load before the first add, so a
+    // row costs one memory round trip.
+    template <int K, typename DT, typename IT, typename LT, typename F>
+    __device__ __forceinline__ void tile_rows(const MvEll<DT, IT, LT> & A, const DT * __restrict__ X,
+                                              const DT * __restrict__ OWN, unsigned R, int64_t base, bool on, F && f)
     {
         const int lane = threadIdx.x & 63;
         const int64_t myrow = base + lane;
@@ -94,36 +100,51 @@ namespace
         uint32_t len = 0;
         if (myrow < A.n)
             len = (uint32_t)A.Arl[myrow];
-        uint32_t c[kTileK];
-        DT a[kTileK];
+        const uint32_t self = myrow < A.n ? (uint32_t)myrow : 0u;
+        // every stored slot (k < num_cols_per_row) of the tile's rows in one
+        // batch of coalesced loads, then the row lengths applied
+        IT rc[K];
+        DT ra[K];
 #pragma unroll
-        for (int kk = 0; kk < kTileK; ++kk)
+        for (int kk = 0; kk < K; ++kk)
         {
-            c[kk] = 0;
-            a[kk] = DT(0);
-            if ((uint32_t)kk < len)
+            rc[kk] = IT(0);
+            ra[kk] = DT(0);
+            if ((uint32_t)kk < A.ncpr)
             {
-                const uint64_t at = (uint64_t)kk * A.stride + (uint64_t)myrow;
-                c[kk] = (uint32_t)k::col_of<IT>(k::ld_stream(A.Aj + at), (uint64_t)myrow);
-                a[kk] = k::ld_stream(A.Ax + at);
+                const uint64_t at = (uint64_t)kk * A.stride + (uint64_t)self;
+                rc[kk] = k::ld_stream(A.Aj + at);
+                ra[kk] = k::ld_stream(A.Ax + at);
             }
+        }
+        uint32_t c[K];
+        DT a[K];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk)
+        {
+            const bool in = (uint32_t)kk < len;
+            c[kk] = in ? (uint32_t)k::col_of<IT>(rc[kk], (uint64_t)myrow) : self;
+            a[kk] = in ? ra[kk] : DT(0);
         }
         for (int r = 0; r < rows; ++r)
         {
+            DT xv[K];
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+            {
+                const uint32_t col = rlane(c[kk], r);
+                xv[kk] = on ? X[(uint64_t)col * R + lane] : DT(0);
+            }
+            const DT ov = (on && OWN) ? OWN[(uint64_t)(base + r) * R + lane] : DT(0);
             const uint32_t lr = rlane(len, r);
             DT sum = DT(0);
 #pragma unroll
-            for (int kk = 0; kk < kTileK; ++kk)
+            for (int kk = 0; kk < K; ++kk)
             {
                 if ((uint32_t)kk < lr)
-                {
-                    const uint32_t col = rlane(c[kk], r);
-                    const DT val = rlane(a[kk], r);
-                    if (on)
-                        sum += val * X[(uint64_t)col * R + lane];
-                }
+                    sum += rlane(a[kk], r) * xv[kk];
             }
-            f(base + r, sum);
+            f(base + r, sum, ov);
         }
     }
 
@@ -131,7 +152,8 @@ namespace
     // slots are read with wave-uniform addresses.
     template <typename DT, typename IT, typename LT, typename F>
     __device__ __forceinline__ void tile_rows_long(const MvEll<DT, IT, LT> & A, const DT * __restrict__ X,
-                                                   unsigned R, int64_t base, bool on, F && f)
+                                                   const DT * __restrict__ OWN, unsigned R, int64_t base, bool on,
+                                                   F && f)
     {
         const int lane = threadIdx.x & 63;
         const int rows = (int)(A.n - base < 64 ? A.n - base : 64);
@@ -148,49 +170,53 @@ namespace
                 if (on)
                     sum += val * X[col * R + lane];
             }
-            f(row, sum);
+            f(row, sum, (on && OWN) ? OWN[(uint64_t)row * R + lane] : DT(0));
         }
     }
 
-    template <typename DT, typename IT, typename LT, bool LONG, typename F>
+    // K: slots per row kept in registers (9 or 16), 0 for longer rows
+    template <typename DT, typename IT, typename LT, int K, typename F>
     __device__ __forceinline__ void for_rows_spmv(const MvEll<DT, IT, LT> & A, const DT * __restrict__ X,
-                                                  unsigned R, bool on, F && f)
+                                                  const DT * __restrict__ OWN, unsigned R, bool on, F && f)
     {
         const TileRange t = wave_tiles((A.n + 63) / 64);
         for (int64_t tile = t.first; tile < t.end; tile += t.step)
         {
-            if (LONG)
-                tile_rows_long(A, X, R, tile * 64, on, f);
+            if constexpr (K == 0)
+                tile_rows_long(A, X, OWN, R, tile * 64, on, f);
             else
-                tile_rows(A, X, R, tile * 64, on, f);
+                tile_rows<K>(A, X, OWN, R, tile * 64, on, f);
         }
     }
 
+    // f(row0, cnt) for the wave's rows in groups of 4 (cnt < 4 only at the
+    // end of the matrix), in ascending row order: the callers issue the loads
+    // of a group before its first store.
     template <typename F>
-    __device__ __forceinline__ void for_rows(int64_t n, F && f)
+    __device__ __forceinline__ void for_rows4(int64_t n, F && f)
     {
         const TileRange t = wave_tiles((n + 63) / 64);
         for (int64_t tile = t.first; tile < t.end; tile += t.step)
         {
             const int64_t base = tile * 64;
             const int rows = (int)(n - base < 64 ? n - base : 64);
-            for (int r = 0; r < rows; ++r)
-                f(base + r);
+            for (int r = 0; r < rows; r += 4)
+                f(base + r, rows - r < 4 ? rows - r : 4);
         }
     }
 
     // Y = A X (MODE 0) or Y = RHS - A X (MODE 1), every column
-    template <typename DT, typename IT, typename LT, bool LONG, int MODE>
+    template <typename DT, typename IT, typename LT, int K, int MODE>
     __global__ __launch_bounds__(256) void spmm_kernel(MvEll<DT, IT, LT> A, const DT * __restrict__ X,
                                                        DT * __restrict__ Y, const DT * __restrict__ RHS, unsigned R)
     {
         const unsigned lane = threadIdx.x & 63;
         const bool on = lane < R;
-        for_rows_spmv<DT, IT, LT, LONG>(A, X, R, on, [&](int64_t row, DT sum) {
+        for_rows_spmv<DT, IT, LT, K>(A, X, MODE == 0 ? nullptr : RHS, R, on, [&](int64_t row, DT sum, DT rhs) {
             if (! on)
                 return;
             const uint64_t e = (uint64_t)row * R + lane;
-            Y[e] = MODE == 0 ? sum : RHS[e] - sum;
+            Y[e] = MODE == 0 ? sum : rhs - sum;
         });
     }
 
@@ -204,7 +230,6 @@ namespace
         DT eps_relative;
         uint64_t max_iters;
         int active;          // columns still iterating
-        unsigned ticket;     // last-block counter, zero at rest
     };
 
     template <typename DT> struct Eps;
@@ -220,15 +245,16 @@ namespace
         return (fabs(v) > Eps<DT>::v) ? v : Eps<DT>::v * (v / (DT)fabs(v));
     }
 
-    // Per-column totals of V per-lane sums over the grid: waves of a block
-    // in order, then blocks in order (the last block to arrive).  fin(j, t)
-    // runs on lane j < R of the last block's first wave.
-    template <int V, typename F>
-    __device__ __forceinline__ void mfinish(const double (&acc)[V], double * partials, unsigned * ticket, unsigned R,
-                                            F && fin)
+    // Per-column reductions in two launches.  Each block stores, per column,
+    // the sum of its 4 waves' lane sums in wave order (mput, no fence and no
+    // atomic); a following one-block launch sums those stores per column
+    // (msum: wave w over blocks w, w + W, ..., then the waves in order) and
+    // applies the scalar step on lane j < R of its first wave.  The kernel
+    // boundary orders the stores.
+    template <int V>
+    __device__ __forceinline__ void mput(const double (&acc)[V], double * partials)
     {
         __shared__ double lds[4][V][64];
-        __shared__ bool last;
         const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
         for (int v = 0; v < V; ++v)
@@ -245,26 +271,27 @@ namespace
                 s += lds[3][v][lane];
                 partials[((uint64_t)blockIdx.x * V + v) * 64 + lane] = s;
             }
-            __threadfence();
         }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-        __syncthreads();
-        if (! last)
-            return;
-        __threadfence();
+    }
+
+    // in a one-block launch of kFinThreads threads (kFinThreads / 64 waves)
+    template <int V, typename F>
+    __device__ __forceinline__ void msum(const double * partials, unsigned nb, unsigned R, F && fin)
+    {
+        constexpr int W = kFinThreads / 64;
+        __shared__ double lds[W][V][64];
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         double t[V];
 #pragma unroll
         for (int v = 0; v < V; ++v)
             t[v] = 0.0;
-        for (unsigned b = (unsigned)wid; b < gridDim.x; b += 4)
+#pragma unroll 8
+        for (unsigned b = (unsigned)wid; b < nb; b += W)
         {
 #pragma unroll
             for (int v = 0; v < V; ++v)
-                t[v] += __builtin_nontemporal_load(partials + ((uint64_t)b * V + v) * 64 + lane);
+                t[v] += partials[((uint64_t)b * V + v) * 64 + lane];
         }
-        __syncthreads();
 #pragma unroll
         for (int v = 0; v < V; ++v)
             lds[wid][v][lane] = t[v];
@@ -276,39 +303,43 @@ namespace
             for (int v = 0; v < V; ++v)
             {
                 double s = lds[0][v][lane];
-                s += lds[1][v][lane];
-                s += lds[2][v][lane];
-                s += lds[3][v][lane];
+                for (int w = 1; w < W; ++w)
+                    s += lds[w][v][lane];
                 tot[v] = s;
             }
             if ((unsigned)lane < R)
                 fin(lane, tot);
-            if (lane == 0)
-                *ticket = 0u;
         }
     }
 
     // R = B - A X; P = Pc * R (VAR) or R (NONE); per column r.r, r.p
-    template <typename DT, typename IT, typename LT, bool LONG, bool VAR>
+    template <typename DT, typename IT, typename LT, int K, bool VAR>
     __global__ __launch_bounds__(256) void mcg_init_kernel(MvEll<DT, IT, LT> A, const DT * __restrict__ Bv,
             const DT * __restrict__ X, DT * __restrict__ Rv, DT * __restrict__ Pv, const DT * __restrict__ Pc,
-            unsigned R, double * partials, MState<DT> * s)
+            unsigned R, double * partials)
     {
         const unsigned lane = threadIdx.x & 63;
         const bool on = lane < R;
         double acc[2] = {0.0, 0.0};
-        for_rows_spmv<DT, IT, LT, LONG>(A, X, R, on, [&](int64_t row, DT sum) {
+        for_rows_spmv<DT, IT, LT, K>(A, X, Bv, R, on, [&](int64_t row, DT sum, DT bi) {
             if (! on)
                 return;
             const uint64_t e = (uint64_t)row * R + lane;
-            const DT ri = Bv[e] - sum;
+            const DT ri = bi - sum;
             Rv[e] = ri;
             const DT pi = VAR ? Pc[row] * ri : ri;
             Pv[e] = pi;
             acc[0] += (double)ri * (double)ri;
             acc[1] += (double)ri * (double)pi;
         });
-        mfinish<2>(acc, partials, &s->ticket, R, [&](int j, const double (&t)[2]) {
+        mput<2>(acc, partials);
+    }
+
+    template <typename DT, bool VAR>
+    __global__ __launch_bounds__(kFinThreads) void mcg_init_finish(const double * partials, unsigned nb, unsigned R,
+            MState<DT> * s)
+    {
+        msum<2>(partials, nb, R, [&](int j, const double (&t)[2]) {
             s->iters[j] = 0;
             s->converged[j] = 0;
             s->run[j] = (s->max_iters > 0) ? 1 : 0;
@@ -329,23 +360,31 @@ namespace
     }
 
     // V = A P; per column temp = v.p and lambda
-    template <typename DT, typename IT, typename LT, bool LONG, bool VAR>
+    template <typename DT, typename IT, typename LT, int K>
     __global__ __launch_bounds__(256) void mcg_spmv_kernel(MvEll<DT, IT, LT> A, const DT * __restrict__ Pv,
-            DT * __restrict__ Vv, unsigned R, double * partials, MState<DT> * s)
+            DT * __restrict__ Vv, unsigned R, double * partials, const MState<DT> * s)
     {
         if (s->active == 0)
             return;
         const unsigned lane = threadIdx.x & 63;
         const bool on = lane < R && s->run[lane] != 0;
         double acc[1] = {0.0};
-        for_rows_spmv<DT, IT, LT, LONG>(A, Pv, R, on, [&](int64_t row, DT sum) {
+        for_rows_spmv<DT, IT, LT, K>(A, Pv, Pv, R, on, [&](int64_t row, DT sum, DT pi) {
             if (! on)
                 return;
-            const uint64_t e = (uint64_t)row * R + lane;
-            Vv[e] = sum;
-            acc[0] += (double)sum * (double)Pv[e];
+            Vv[(uint64_t)row * R + lane] = sum;
+            acc[0] += (double)sum * (double)pi;
         });
-        mfinish<1>(acc, partials, &s->ticket, R, [&](int j, const double (&t)[1]) {
+        mput<1>(acc, partials);
+    }
+
+    template <typename DT, bool VAR>
+    __global__ __launch_bounds__(kFinThreads) void mcg_spmv_finish(const double * partials, unsigned nb, unsigned R,
+            MState<DT> * s)
+    {
+        if (s->active == 0)
+            return;
+        msum<1>(partials, nb, R, [&](int j, const double (&t)[1]) {
             if (s->run[j] == 0)
                 return;
             const DT temp = (DT)t[0];
@@ -360,7 +399,7 @@ namespace
     template <typename DT, bool VAR>
     __global__ __launch_bounds__(256) void mcg_update_kernel(DT * __restrict__ X, DT * __restrict__ Rv,
             const DT * __restrict__ Pv, const DT * __restrict__ Vv, const DT * __restrict__ Pc, int64_t n,
-            unsigned R, double * partials, MState<DT> * s)
+            unsigned R, double * partials, const MState<DT> * s)
     {
         if (s->active == 0)
             return;
@@ -371,17 +410,45 @@ namespace
         double acc[2] = {0.0, 0.0};
         if (on)
         {
-            for_rows(n, [&](int64_t row) {
-                const uint64_t e = (uint64_t)row * R + lane;
-                X[e] = X[e] + lambda * Pv[e];
-                const DT ri = Rv[e] + mlambda * Vv[e];
-                Rv[e] = ri;
-                acc[0] += (double)ri * (double)ri;
-                if (VAR)
-                    acc[1] += (double)ri * (double)(Pc[row] * ri);
+            for_rows4(n, [&](int64_t row0, int cnt) {
+                DT xv[4], pv[4], rv[4], vv[4], pc[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    const int64_t row = row0 + (u < cnt ? u : 0);
+                    const uint64_t e = (uint64_t)row * R + lane;
+                    xv[u] = X[e];
+                    pv[u] = Pv[e];
+                    rv[u] = Rv[e];
+                    vv[u] = Vv[e];
+                    pc[u] = VAR ? Pc[row] : DT(0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    if (u < cnt)
+                    {
+                        const uint64_t e = (uint64_t)(row0 + u) * R + lane;
+                        X[e] = xv[u] + lambda * pv[u];
+                        const DT ri = rv[u] + mlambda * vv[u];
+                        Rv[e] = ri;
+                        acc[0] += (double)ri * (double)ri;
+                        if (VAR)
+                            acc[1] += (double)ri * (double)(pc[u] * ri);
+                    }
+                }
             });
         }
-        mfinish<2>(acc, partials, &s->ticket, R, [&](int j, const double (&t)[2]) {
+        mput<2>(acc, partials);
+    }
+
+    template <typename DT, bool VAR>
+    __global__ __launch_bounds__(kFinThreads) void mcg_update_finish(const double * partials, unsigned nb, unsigned R,
+            MState<DT> * s)
+    {
+        if (s->active == 0)
+            return;
+        msum<2>(partials, nb, R, [&](int j, const double (&t)[2]) {
             if (s->run[j] == 0)
                 return;
             const DT eps = s->eps_relative;
@@ -424,7 +491,7 @@ namespace
             if (stop)
             {
                 s->run[j] = 0;
-                atomicSub(&s->active, 1);
+                atomicSub(&s->active, 1);   // lanes of one wave, block-local
             }
         });
     }
@@ -440,10 +507,26 @@ namespace
         if (! (lane < R && s->run[lane] != 0))
             return;
         const DT talpha = s->talpha[lane];
-        for_rows(n, [&](int64_t row) {
-            const uint64_t e = (uint64_t)row * R + lane;
-            const DT t = Pv[e] * talpha;
-            Pv[e] = t + (VAR ? Pc[row] * Rv[e] : Rv[e]);
+        for_rows4(n, [&](int64_t row0, int cnt) {
+            DT pv[4], rv[4], pc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                const int64_t row = row0 + (u < cnt ? u : 0);
+                const uint64_t e = (uint64_t)row * R + lane;
+                pv[u] = Pv[e];
+                rv[u] = Rv[e];
+                pc[u] = VAR ? Pc[row] : DT(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                if (u < cnt)
+                {
+                    const DT t = pv[u] * talpha;
+                    Pv[(uint64_t)(row0 + u) * R + lane] = t + (VAR ? pc[u] * rv[u] : rv[u]);
+                }
+            }
         });
     }
 
@@ -464,10 +547,12 @@ namespace
             auto run = [&](auto dt_tag) {
                 using DT = decltype(dt_tag);
                 MvEll<DT, IT, LT> A{Aj, (const DT *)m->Ax, Arl, m->stride, (int64_t)m->rows, (uint32_t)m->ncpr};
-                if (m->ncpr <= (uint64_t)kTileK)
-                    f(A, std::false_type());
+                if (m->ncpr <= 9)
+                    f(A, std::integral_constant<int, 9>());
+                else if (m->ncpr <= (uint64_t)kTileK)
+                    f(A, std::integral_constant<int, kTileK>());
                 else
-                    f(A, std::true_type());
+                    f(A, std::integral_constant<int, 0>());
             };
             if (m->dtype)
                 run(double(0));
@@ -476,7 +561,7 @@ namespace
         });
     }
 
-    template <typename DT, typename IT, typename LT, bool LONG, bool VAR>
+    template <typename DT, typename IT, typename LT, int K, bool VAR>
     void run_mcg(const MvEll<DT, IT, LT> & A, unsigned R, const DT * Bv, DT * X, const DT * Pc, uint64_t max_iters,
                  double eps, uint64_t * used, double * defect, int * converged, hipStream_t st)
     {
@@ -504,18 +589,25 @@ namespace
         h.eps_relative = (DT)eps;
         h.max_iters = max_iters;
         HIP_CHECK(hipMemcpyAsync(s, &h, sizeof(h), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL((mcg_init_kernel<DT, IT, LT, LONG, VAR>), dim3(grid), dim3(kBlock), 0, st, A, Bv,
-                           (const DT *)X, Rv, Pv, Pc, R, partials, s);
+        hipLaunchKernelGGL((mcg_init_kernel<DT, IT, LT, K, VAR>), dim3(grid), dim3(kBlock), 0, st, A, Bv,
+                           (const DT *)X, Rv, Pv, Pc, R, partials);
+        hipLaunchKernelGGL((mcg_init_finish<DT, VAR>), dim3(1), dim3(kFinThreads), 0, st, (const double *)partials, grid,
+                           R, s);
         HIP_CHECK(hipGetLastError());
         uint64_t enqueued = 0, chunk = 8;
         while (true)
         {
             for (uint64_t i = 0; i < chunk && enqueued < max_iters; ++i, ++enqueued)
             {
-                hipLaunchKernelGGL((mcg_spmv_kernel<DT, IT, LT, LONG, VAR>), dim3(grid), dim3(kBlock), 0, st, A,
-                                   (const DT *)Pv, Vv, R, partials, s);
+                hipLaunchKernelGGL((mcg_spmv_kernel<DT, IT, LT, K>), dim3(grid), dim3(kBlock), 0, st, A,
+                                   (const DT *)Pv, Vv, R, partials, (const MState<DT> *)s);
+                hipLaunchKernelGGL((mcg_spmv_finish<DT, VAR>), dim3(1), dim3(kFinThreads), 0, st,
+                                   (const double *)partials, grid, R, s);
                 hipLaunchKernelGGL((mcg_update_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, X, Rv,
-                                   (const DT *)Pv, (const DT *)Vv, Pc, (int64_t)n, R, partials, s);
+                                   (const DT *)Pv, (const DT *)Vv, Pc, (int64_t)n, R, partials,
+                                   (const MState<DT> *)s);
+                hipLaunchKernelGGL((mcg_update_finish<DT, VAR>), dim3(1), dim3(kFinThreads), 0, st,
+                                   (const double *)partials, grid, R, s);
                 hipLaunchKernelGGL((mcg_direction_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, Pv,
                                    (const DT *)Rv, Pc, (int64_t)n, R, (const MState<DT> *)s);
             }
@@ -573,18 +665,18 @@ int hsolve_ell_spmm(hsolve_ell m, uint64_t nrhs, const void * X, void * Y, const
     require(nrhs >= 1 && nrhs <= (uint64_t)kMaxCols, HSOLVE_ERR_INVALID, "ell_spmm: nrhs in [1, 64]");
     hipStream_t st = as_stream(stream);
     const unsigned R = (unsigned)nrhs;
-    with_mv(m, [&](auto A, auto long_tag) {
+    with_mv(m, [&](auto A, auto k_tag) {
         using MV = decltype(A);
         using DT = std::remove_const_t<std::remove_pointer_t<decltype(MV::Ax)>>;
-        constexpr bool LONG = decltype(long_tag)::value;
+        constexpr int K = decltype(k_tag)::value;
         using IT = std::remove_const_t<std::remove_pointer_t<decltype(MV::Aj)>>;
         using LT = std::remove_const_t<std::remove_pointer_t<decltype(MV::Arl)>>;
         const unsigned grid = mv_grid(m->rows);
         if (RHS)
-            hipLaunchKernelGGL((spmm_kernel<DT, IT, LT, LONG, 1>), dim3(grid), dim3(kBlock), 0, st, A,
+            hipLaunchKernelGGL((spmm_kernel<DT, IT, LT, K, 1>), dim3(grid), dim3(kBlock), 0, st, A,
                                (const DT *)X, (DT *)Y, (const DT *)RHS, R);
         else
-            hipLaunchKernelGGL((spmm_kernel<DT, IT, LT, LONG, 0>), dim3(grid), dim3(kBlock), 0, st, A,
+            hipLaunchKernelGGL((spmm_kernel<DT, IT, LT, K, 0>), dim3(grid), dim3(kBlock), 0, st, A,
                                (const DT *)X, (DT *)Y, (const DT *)nullptr, R);
         HIP_CHECK(hipGetLastError());
     });
@@ -601,17 +693,17 @@ int hsolve_cg_ell_multi(hsolve_ell m, uint64_t nrhs, const void * P, const void 
     hipStream_t st = as_stream(stream);
     const unsigned R = (unsigned)nrhs;
     std::vector<int> conv(R, 0);
-    with_mv(m, [&](auto A, auto long_tag) {
+    with_mv(m, [&](auto A, auto k_tag) {
         using MV = decltype(A);
         using DT = std::remove_const_t<std::remove_pointer_t<decltype(MV::Ax)>>;
-        constexpr bool LONG = decltype(long_tag)::value;
+        constexpr int K = decltype(k_tag)::value;
         using IT = std::remove_const_t<std::remove_pointer_t<decltype(MV::Aj)>>;
         using LT = std::remove_const_t<std::remove_pointer_t<decltype(MV::Arl)>>;
         if (P)
-            run_mcg<DT, IT, LT, LONG, true>(A, R, (const DT *)B, (DT *)X, (const DT *)P, max_iters, eps, used_iters,
+            run_mcg<DT, IT, LT, K, true>(A, R, (const DT *)B, (DT *)X, (const DT *)P, max_iters, eps, used_iters,
                                             defect_out, conv.data(), st);
         else
-            run_mcg<DT, IT, LT, LONG, false>(A, R, (const DT *)B, (DT *)X, nullptr, max_iters, eps, used_iters,
+            run_mcg<DT, IT, LT, K, false>(A, R, (const DT *)B, (DT *)X, nullptr, max_iters, eps, used_iters,
                                              defect_out, conv.data(), st);
     });
     if (max_iters > 0)

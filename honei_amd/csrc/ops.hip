@@ -32,12 +32,8 @@ namespace hsolve
             return it->second;
         ReduceScratch s;
         void * p = nullptr;
-        HIP_CHECK(hipMalloc(&p, sizeof(double) * kReduceBlocks + 256));
+        HIP_CHECK(hipMalloc(&p, sizeof(double) * kReduceBlocks));
         s.partials = (double *)p;
-        s.ticket = (unsigned *)((char *)p + sizeof(double) * kReduceBlocks);
-        // the ticket is zero at rest; it is set once here, outside any capture
-        HIP_CHECK(hipMemset(s.ticket, 0, 256));
-        HIP_CHECK(hipDeviceSynchronize());
         return g_reduce.emplace(key, s).first->second;
     }
 
@@ -49,8 +45,9 @@ namespace hsolve
         unsigned grid = grid_for(n, kBlock * 4);
         if (grid > (unsigned)kReduceBlocks)
             grid = kReduceBlocks;
-        hipLaunchKernelGGL((k::dot_kernel<DT>), dim3(grid), dim3(kBlock), 0, st, a, b, n, s.partials,
-                           s.ticket, out, root);
+        hipLaunchKernelGGL((k::dot_kernel<DT>), dim3(grid), dim3(kBlock), 0, st, a, b, n, s.partials);
+        hipLaunchKernelGGL((k::dot_finish_kernel<DT>), dim3(1), dim3(kBlock), 0, st, (const double *)s.partials, grid,
+                           out, root);
         HIP_CHECK(hipGetLastError());
     }
 

@@ -5,8 +5,9 @@
 //
 // Design: every scalar of the iteration (alpha, lambda, norms, iteration
 // counts, convergence flags) lives on the device.  Each reduction is fused
-// into the kernel that produces its operand and finished by the last block
-// to arrive, which also performs the scalar update and the stopping test.
+// into the kernel that produces its operand (block partials) and finished
+// by a one-block kernel, which also performs the scalar update and the
+// stopping test.
 // Every kernel of a step first reads a `run` flag and returns if the solve
 // has already stopped, so the host enqueues whole chunks of iterations and
 // synchronises once per chunk, while the iteration count stays exactly the
@@ -39,7 +40,6 @@ namespace
 
     template <typename DT, typename IT, typename LT> using EllOp = EllRowOp<DT, IT, LT>;
 
-    using k::finish2;
 
     // --------------------------------------------------------------- CG state
     template <typename DT> struct CgScalars
@@ -66,10 +66,10 @@ namespace
     }
 
     // r = b - A x; p = r (NONE) or p = P*r (VAR); reductions r.r and r.p
+    // (block partials; cg_init_finish forms the totals)
     template <typename DT, typename Op, bool VAR>
     __global__ __launch_bounds__(256) void cg_init_kernel(Op A, const DT * __restrict__ b, const DT * __restrict__ x,
-            DT * __restrict__ r, DT * __restrict__ p, const DT * __restrict__ P, int64_t n, double * partials,
-            unsigned * ticket, CgScalars<DT> * s)
+            DT * __restrict__ r, DT * __restrict__ p, const DT * __restrict__ P, int64_t n, double * partials)
     {
         double acc_rr = 0.0, acc_rp = 0.0;
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -83,7 +83,16 @@ namespace
             acc_rr += (double)ri * (double)ri;
             acc_rp += (double)ri * (double)pi;
         }
-        finish2(acc_rr, acc_rp, partials, ticket, [&](double rr, double rp) {
+        k::put_partials2(acc_rr, acc_rp, partials, blockIdx.x);
+    }
+
+    template <typename DT, bool VAR>
+    __global__ __launch_bounds__(256) void cg_init_finish(const double * partials, unsigned nb, CgScalars<DT> * s)
+    {
+        double rr, rp;
+        k::sum_partials2(partials, nb, rr, rp);
+        if (threadIdx.x == 0)
+        {
             s->iters = 0;
             s->converged = 0;
             s->run = (s->max_iters > 0) ? 1 : 0;
@@ -98,13 +107,13 @@ namespace
                 s->initial_defect = (DT)sqrt((DT)rr);
             }
             s->current_defect = s->initial_defect;
-        });
+        }
     }
 
-    // v = A p; temp = v.p; lambda
-    template <typename DT, typename Op, bool VAR>
+    // v = A p; partials of v.p
+    template <typename DT, typename Op>
     __global__ __launch_bounds__(256) void cg_spmv_kernel(Op A, const DT * __restrict__ p, DT * __restrict__ v,
-            int64_t n, double * partials, unsigned * ticket, CgScalars<DT> * s)
+            int64_t n, double * partials, const CgScalars<DT> * s)
     {
         if (s->run == 0)
             return;
@@ -116,21 +125,33 @@ namespace
             v[i] = vi;
             acc += (double)vi * (double)p[i];
         }
-        finish2(acc, 0.0, partials, ticket, [&](double vp, double) {
+        k::put_partials2(acc, 0.0, partials, blockIdx.x);
+    }
+
+    // temp = v.p; lambda
+    template <typename DT, bool VAR>
+    __global__ __launch_bounds__(256) void cg_spmv_finish(const double * partials, unsigned nb, CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        double vp, unused;
+        k::sum_partials2(partials, nb, vp, unused);
+        if (threadIdx.x == 0)
+        {
             const DT temp = (DT)vp;
             s->temp = temp;
             if (VAR)
                 s->lambda = s->alpha_new / guard_signed<DT>(temp);
             else
                 s->lambda = s->alpha / guard<DT>(temp);
-        });
+        }
     }
 
-    // x += lambda p; r += (-lambda) v; r.r (and r.(P r) for VAR); stopping test
+    // x += lambda p; r += (-lambda) v; partials of r.r (and r.(P r) for VAR)
     template <typename DT, bool VAR>
     __global__ __launch_bounds__(256) void cg_update_kernel(DT * __restrict__ x, DT * __restrict__ r,
             const DT * __restrict__ p, const DT * __restrict__ v, const DT * __restrict__ P, int64_t n,
-            double * partials, unsigned * ticket, CgScalars<DT> * s)
+            double * partials, const CgScalars<DT> * s)
     {
         if (s->run == 0)
             return;
@@ -147,7 +168,20 @@ namespace
             if (VAR)
                 acc_rz += (double)ri * (double)(P[i] * ri);
         }
-        finish2(acc_rr, acc_rz, partials, ticket, [&](double rr, double rz) {
+        k::put_partials2(acc_rr, acc_rz, partials, blockIdx.x);
+    }
+
+    // the stopping test and the next direction's factor
+    template <typename DT, bool VAR>
+    __global__ __launch_bounds__(256) void cg_update_finish(const double * partials, unsigned nb, CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        double rr, rz;
+        k::sum_partials2(partials, nb, rr, rz);
+        if (threadIdx.x != 0)
+            return;
+        {
             const DT eps = s->eps_relative;
             if (VAR)
             {
@@ -185,7 +219,7 @@ namespace
                 else if (s->iters == s->max_iters)
                     s->run = 0;
             }
-        });
+        }
     }
 
     // p = p * talpha + r (NONE) or p = p * talpha + P*r (VAR)
@@ -209,14 +243,13 @@ namespace
     {
         DT *r = nullptr, *p = nullptr, *v = nullptr;
         double * partials = nullptr;
-        unsigned * ticket = nullptr;
         CgScalars<DT> * s = nullptr;
         void * block = nullptr;
 
-        CgWork(uint64_t n, hipStream_t st)
+        explicit CgWork(uint64_t n)
         {
             const size_t vb = ((sizeof(DT) * n + 255) / 256) * 256;
-            const size_t bytes = 3 * vb + sizeof(double) * 2 * kSolverBlocks + 256 + sizeof(CgScalars<DT>) + 256;
+            const size_t bytes = 3 * vb + sizeof(double) * 2 * kSolverBlocks + sizeof(CgScalars<DT>) + 256;
             hipError_t e = hipMalloc(&block, bytes);
             if (e != hipSuccess)
                 throw Error(HSOLVE_ERR_ALLOC, "CG workspace allocation failed");
@@ -225,9 +258,7 @@ namespace
             p = (DT *)c; c += vb;
             v = (DT *)c; c += vb;
             partials = (double *)c; c += sizeof(double) * 2 * kSolverBlocks;
-            ticket = (unsigned *)c; c += 256;
             s = (CgScalars<DT> *)c;
-            HIP_CHECK(hipMemsetAsync(ticket, 0, 256, st));
         }
         ~CgWork()
         {
@@ -240,7 +271,7 @@ namespace
     void run_cg(const Op & A, uint64_t n, const DT * b, DT * x, const DT * P, uint64_t max_iters, double eps,
                 uint64_t * used_iters, double * defect_out, hipStream_t st, int * converged_out)
     {
-        CgWork<DT> w(n, st);
+        CgWork<DT> w(n);
         CgScalars<DT> h;
         std::memset(&h, 0, sizeof(h));
         h.eps_relative = (DT)eps;
@@ -248,7 +279,9 @@ namespace
         HIP_CHECK(hipMemcpyAsync(w.s, &h, sizeof(h), hipMemcpyHostToDevice, st));
         const unsigned grid = std::min<unsigned>(grid_for(n, kBlock), kSolverBlocks);
         hipLaunchKernelGGL((cg_init_kernel<DT, Op, VAR>), dim3(grid), dim3(kBlock), 0, st, A, b, x, w.r, w.p, P,
-                           (int64_t)n, w.partials, w.ticket, w.s);
+                           (int64_t)n, w.partials);
+        hipLaunchKernelGGL((cg_init_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials, grid,
+                           w.s);
         HIP_CHECK(hipGetLastError());
         // chunks of iterations between host checks; every kernel of an
         // iteration that starts after the stop is a no-op
@@ -258,10 +291,14 @@ namespace
         {
             for (uint64_t c = 0; c < chunk && enqueued < max_iters; ++c, ++enqueued)
             {
-                hipLaunchKernelGGL((cg_spmv_kernel<DT, Op, VAR>), dim3(grid), dim3(kBlock), 0, st, A, w.p, w.v,
-                                   (int64_t)n, w.partials, w.ticket, w.s);
+                hipLaunchKernelGGL((cg_spmv_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A, w.p, w.v,
+                                   (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
+                hipLaunchKernelGGL((cg_spmv_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
+                                   (const double *)w.partials, grid, w.s);
                 hipLaunchKernelGGL((cg_update_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, x, w.r, w.p, w.v, P,
-                                   (int64_t)n, w.partials, w.ticket, w.s);
+                                   (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
+                hipLaunchKernelGGL((cg_update_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
+                                   (const double *)w.partials, grid, w.s);
                 hipLaunchKernelGGL((cg_direction_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, w.p, w.r, P,
                                    (int64_t)n, (const CgScalars<DT> *)w.s);
             }
