@@ -323,12 +323,15 @@ def main():
     ap.add_argument("--diffusion-only", type=int, default=0, metavar="LEVEL",
                     help="only run the 64-RHS variable-coefficient diffusion case on the 2^LEVEL+1 grid")
     ap.add_argument("--no-pcg", action="store_true", help="with --diffusion-only: MG-IR leg only")
+    ap.add_argument("--pcg-max", type=int, default=200000,
+                    help="with --diffusion-only: iteration cap of the PCG leg (a short cap for profiling)")
     args = ap.parse_args()
     if args.diffusion_only:
         from honei_amd import device as hip
         from honei_amd import problems as P
         print(json.dumps({f"diffusion_{P.side_of_level(args.diffusion_only)}_64rhs":
-                          diffusion_case(hip, P, level=args.diffusion_only, pcg=not args.no_pcg)}))
+                          diffusion_case(hip, P, level=args.diffusion_only, pcg=not args.no_pcg,
+                                         pcg_max=args.pcg_max)}))
         return
     if args.serve_only:
         from honei_amd import device as hip

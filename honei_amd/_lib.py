@@ -109,6 +109,7 @@ _SIGS = {
     "hsolve_ell_destroy": [vp],
     "hsolve_ell_info": [vp] + [C.POINTER(u64)] * 7,
     "hsolve_ell_set_variant": [vp, C.c_int],
+    "hsolve_ell_q1_side": [vp, C.POINTER(u64)],
     "hsolve_ell_spmv": [vp, vp, vp, vp, vp],
     "hsolve_ell_jacobi": [vp, vp, vp, vp, vp, vp],
     "hsolve_cg_konv_ell": [vp, vp, vp, dbl, u64, vp, vp],

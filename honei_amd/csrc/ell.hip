@@ -180,6 +180,26 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
             }
     }
     const int idx_kind = (maxoff <= 32767) ? 1 : 0;
+    // 9-point structure on a side-M grid (Q1 stencils stored as ELL)
+    uint64_t q1_side = 0;
+    {
+        const uint64_t M = (uint64_t)std::llround(std::sqrt((double)rows));
+        bool q1 = threads == 1 && rows == columns && M >= 3 && M * M == rows && num_cols_per_row <= 9;
+        for (uint64_t r = 0; q1 && r < rows; ++r)
+        {
+            const int64_t y = (int64_t)(r / M), x = (int64_t)(r % M);
+            int64_t prev = -1;
+            for (uint64_t k = 0; q1 && k < rl[r]; ++k)
+            {
+                const int64_t c = (int64_t)Aj_host[r + k * stride];
+                const int64_t dy = c / (int64_t)M - y, dx = c % (int64_t)M - x;
+                q1 = c > prev && dy >= -1 && dy <= 1 && dx >= -1 && dx <= 1;
+                prev = c;
+            }
+        }
+        if (q1)
+            q1_side = M;
+    }
     std::vector<uint8_t> idx(size * (idx_kind ? sizeof(int16_t) : sizeof(uint32_t)));
     if (idx_kind)
     {
@@ -209,6 +229,7 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
     hsolve_ell_s * m = new hsolve_ell_s();
     m->dtype = dtype; m->rows = rows; m->cols = columns; m->stride = stride; m->ncpr = num_cols_per_row;
     m->threads = threads; m->nnz = nnz; m->idx_kind = idx_kind; m->len_bytes = lb; m->variant = 0;
+    m->q1_side = q1_side;
     m->Aj = nullptr; m->Ax = nullptr; m->Arl = nullptr;
     try
     {
@@ -269,6 +290,14 @@ int hsolve_ell_set_variant(hsolve_ell m, int variant)
     require(m != nullptr, HSOLVE_ERR_INVALID, "ell_set_variant: null handle");
     require(variant >= 0 && variant < 8, HSOLVE_ERR_INVALID, "ell_set_variant: variant bits out of range");
     m->variant = variant ? (variant | HSOLVE_ELL_SET) : 0;
+    HSOLVE_API_END
+}
+
+int hsolve_ell_q1_side(hsolve_ell m, uint64_t * side)
+{
+    HSOLVE_API_BEGIN
+    require(m && side, HSOLVE_ERR_INVALID, "ell_q1_side: null pointer");
+    *side = m->q1_side;
     HSOLVE_API_END
 }
 

@@ -15,6 +15,9 @@ struct hsolve_ell_s
     void * Arl;                // rows row lengths
     uint64_t device_bytes;
     int variant;               // kernel variant bits (HSOLVE_ELL_*), 0 = automatic
+    uint64_t q1_side;          // M > 0 when rows = cols = M^2, threads = 1 and every stored
+                               // column of row (y, x) is a neighbour (y + dy, x + dx),
+                               // |dy|, |dx| <= 1, in ascending slot order; else 0
 };
 
 // kernel variant bits of hsolve_ell_set_variant

@@ -220,11 +220,31 @@ namespace k
         const void * b[9];   // LL LD LU DL DD DU UL UD UU
     };
 
-    template <typename DT>
+    // Row i of a Q1 product.  ORD 0: Product<Q1>'s order (DD first, then
+    // LL, LD, LU, DL, DU, UL, UD, UU; product.hh:370-403).  ORD 1: the
+    // order of an ELL row with ascending columns (LL .. UU, summed from
+    // zero), for bands that hold a 9-point structured ELL matrix with zeros
+    // at its absent entries: a zero entry adds +-0 to a sum that is never
+    // -0, so with finite x the result is the ELL product's, bit for bit.
+    template <typename DT, int ORD = 0>
     __device__ __forceinline__ DT q1_row(const Bands & B, const DT * __restrict__ x, int64_t i,
                                          int64_t n, int64_t m)
     {
         const DT * const * b = reinterpret_cast<const DT * const *>(B.b);
+        if (ORD == 1)
+        {
+            DT s = DT(0);
+            if (i - m - 1 >= 0) s += ld_stream(b[0] + i) * x[i - m - 1];
+            if (i - m >= 0)     s += ld_stream(b[1] + i) * x[i - m];
+            if (i - m + 1 >= 0) s += ld_stream(b[2] + i) * x[i - m + 1];
+            if (i - 1 >= 0)     s += ld_stream(b[3] + i) * x[i - 1];
+            s += ld_stream(b[4] + i) * x[i];
+            if (i + 1 < n)      s += ld_stream(b[5] + i) * x[i + 1];
+            if (i + m - 1 < n)  s += ld_stream(b[6] + i) * x[i + m - 1];
+            if (i + m < n)      s += ld_stream(b[7] + i) * x[i + m];
+            if (i + m + 1 < n)  s += ld_stream(b[8] + i) * x[i + m + 1];
+            return s;
+        }
         DT s = ld_stream(b[4] + i) * x[i];
         if (i - m - 1 >= 0) s += ld_stream(b[0] + i) * x[i - m - 1];
         if (i - m >= 0)     s += ld_stream(b[1] + i) * x[i - m];
@@ -278,7 +298,7 @@ namespace k
     // ends of the vector (every index guard of q1_row holds): bands come in
     // as one 16-byte load each, x from the three row windows.  Each row's
     // sum runs in q1_row's order.
-    template <typename DT>
+    template <typename DT, int ORD = 0>
     __device__ __forceinline__ void q1_rows_v(const Bands & B, const DT * __restrict__ x, int64_t i0, int64_t m,
                                               DT (&out)[Vec16<DT>::V])
     {
@@ -296,6 +316,25 @@ namespace k
             xm[j] = x[i0 - m - 1 + j];
             xc[j] = x[i0 - 1 + j];
             xp[j] = x[i0 + m - 1 + j];
+        }
+        if (ORD == 1)
+        {
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+            {
+                DT s = DT(0);
+                s += bv[0][v] * xm[v];
+                s += bv[1][v] * xm[v + 1];
+                s += bv[2][v] * xm[v + 2];
+                s += bv[3][v] * xc[v];
+                s += bv[4][v] * xc[v + 1];
+                s += bv[5][v] * xc[v + 2];
+                s += bv[6][v] * xp[v];
+                s += bv[7][v] * xp[v + 1];
+                s += bv[8][v] * xp[v + 2];
+                out[v] = s;
+            }
+            return;
         }
 #pragma unroll
         for (int v = 0; v < V; ++v)
@@ -316,7 +355,7 @@ namespace k
     // Vectorised form of q1_kernel: V = 16 / sizeof(DT) rows per lane; lane
     // `lane` covers rows lane*V .. lane*V+V-1.  Needs 16-byte aligned bands,
     // y, rhs and dinv (checked by the launchers).
-    template <typename DT, int MODE>
+    template <typename DT, int MODE, int ORD = 0>
     __device__ __forceinline__ void q1_body_v(const Bands & B, const DT * __restrict__ x,
             DT * __restrict__ y, const DT * __restrict__ rhs, const DT * __restrict__ dinv,
             int64_t n, int64_t m, int64_t lane)
@@ -331,7 +370,7 @@ namespace k
             for (int v = 0; v < V && i0 + v < n; ++v)
             {
                 const int64_t i = i0 + v;
-                const DT s = q1_row<DT>(B, x, i, n, m);
+                const DT s = q1_row<DT, ORD>(B, x, i, n, m);
                 if (MODE == 0)
                     y[i] = s;
                 else if (MODE == 1)
@@ -345,7 +384,7 @@ namespace k
             return;
         }
         DT s[V];
-        q1_rows_v<DT>(B, x, i0, m, s);
+        q1_rows_v<DT, ORD>(B, x, i0, m, s);
         VT o;
         if (MODE == 0)
         {

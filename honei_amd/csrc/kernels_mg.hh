@@ -196,7 +196,8 @@ namespace k
     // V = 16 / sizeof(DT) rows per lane; bands and out must be 16-byte
     // aligned.  Lanes whose rows touch either end of the vector take the
     // guarded per-row form; every row sums in q1_row's order.
-    template <typename DT, int ACC>
+    // ORD as q1_row: 0 Product<Q1>'s order, 1 ascending-column (ELL) order.
+    template <typename DT, int ACC, int ORD = 0>
     __device__ __forceinline__ void jacobi_zero2_body(const Bands & B, const DT * __restrict__ d,
             const DT * __restrict__ dinv, DT * __restrict__ out, int64_t n, int64_t m, int64_t lane)
     {
@@ -212,11 +213,12 @@ namespace k
             {
                 const int64_t i = i0 + v;
 #define HS_C1(j) (dinv[(j)] * d[(j)])
-                DT s = ld_stream(b[4] + i) * HS_C1(i);
+                DT s = ORD == 1 ? DT(0) : ld_stream(b[4] + i) * HS_C1(i);
                 if (i - m - 1 >= 0) s += ld_stream(b[0] + i) * HS_C1(i - m - 1);
                 if (i - m >= 0)     s += ld_stream(b[1] + i) * HS_C1(i - m);
                 if (i - m + 1 >= 0) s += ld_stream(b[2] + i) * HS_C1(i - m + 1);
                 if (i - 1 >= 0)     s += ld_stream(b[3] + i) * HS_C1(i - 1);
+                if (ORD == 1)       s += ld_stream(b[4] + i) * HS_C1(i);
                 if (i + 1 < n)      s += ld_stream(b[5] + i) * HS_C1(i + 1);
                 if (i + m - 1 < n)  s += ld_stream(b[6] + i) * HS_C1(i + m - 1);
                 if (i + m < n)      s += ld_stream(b[7] + i) * HS_C1(i + m);
@@ -250,11 +252,13 @@ namespace k
 #pragma unroll
         for (int v = 0; v < V; ++v)
         {
-            DT s = bv[4][v] * cc[v + 1];
+            DT s = ORD == 1 ? DT(0) : bv[4][v] * cc[v + 1];
             s += bv[0][v] * cm[v];
             s += bv[1][v] * cm[v + 1];
             s += bv[2][v] * cm[v + 2];
             s += bv[3][v] * cc[v];
+            if (ORD == 1)
+                s += bv[4][v] * cc[v + 1];
             s += bv[5][v] * cc[v + 2];
             s += bv[6][v] * cp[v];
             s += bv[7][v] * cp[v + 1];
@@ -326,13 +330,13 @@ namespace k
     template <> struct Eps<double> { static constexpr double v = 2.2204460492503131e-16; };
 
     // A row of a Q1 banded matrix for the generic one-workgroup kernels.
-    template <typename DT> struct Q1RowOp
+    template <typename DT, int ORD = 0> struct Q1RowOp
     {
         Bands B;
         int64_t n, m;
         __device__ __forceinline__ DT row(const DT * __restrict__ x, int64_t i) const
         {
-            return q1_row<DT>(B, x, i, n, m);
+            return q1_row<DT, ORD>(B, x, i, n, m);
         }
     };
 

@@ -136,7 +136,8 @@ namespace mg
     struct Hierarchy
     {
         Params p;
-        int kind = 0;                        // 0: Q1 bands, 1: ELL matrices
+        int kind = 0;                        // 0: Q1 bands, 1: ELL matrices, 2: bands holding
+                                             // 9-point structured ELL matrices (ELL sum order)
         int prolmat = 0;                     // 1: transfers by the prol/res matrices
         uint64_t max_level = 0;
         HierLevel lv[kMaxLevels];

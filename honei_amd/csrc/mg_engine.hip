@@ -30,11 +30,13 @@ namespace
     __device__ __forceinline__ bool live(const Slot & sl) { return sl.s != nullptr; }
 
     // Virtual block coordinates of a slot launch (grid x [, y] per slot,
-    // blockIdx.z = slot).  The hardware hands out workgroups in linear order,
-    // x fastest; this bijection makes the slot the fastest-varying index, so
-    // consecutive workgroups take the same row block of every slot in turn
-    // and a level matrix that all slots share is fetched from HBM about once
-    // per row block instead of once per slot.
+    // blockIdx.z = slot).  The hardware hands out workgroups in linear order
+    // (x fastest), dealing consecutive ones to the 8 XCDs in turn.  Logical
+    // blocks are ordered row block major, slot fastest, and every XCD gets
+    // one contiguous range of them (k::xcd_block): all slots of a row block
+    // run on one XCD, one after another, so the level matrix that they
+    // share is fetched into that XCD's L2 once per row block, and the rows
+    // +-side away that a stencil row gathers stay in the same L2.
     struct VB
     {
         unsigned x, y, z;
@@ -42,7 +44,8 @@ namespace
     __device__ __forceinline__ VB vblock()
     {
         const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-        const unsigned z = lin % gridDim.z, r = lin / gridDim.z;
+        const unsigned L = k::xcd_block(lin, gridDim.x * gridDim.y * gridDim.z);
+        const unsigned z = L % gridDim.z, r = L / gridDim.z;
         return VB{r % gridDim.x, r / gridDim.x, z};
     }
 
@@ -104,11 +107,17 @@ namespace
         int64_t n, m;
     };
 
+    // Matrix kinds of the level kernels: MK 0 Q1 bands (Product<Q1>'s sum
+    // order), MK 1 ELL matrices, MK 2 bands holding 9-point structured ELL
+    // matrices (ascending-column order, the same bits as MK 1).
+    template <int MK> constexpr bool kBands = MK != 1;
+    template <int MK> constexpr int kOrd = MK == 2 ? 1 : 0;
+
     template <int MK>
     __device__ __forceinline__ float level_row(const LevelArgs & a, const float * __restrict__ x, int64_t i)
     {
-        if (MK == 0)
-            return k::q1_row<float>(a.B, x, i, a.n, a.m);
+        if (kBands<MK>)
+            return k::q1_row<float, kOrd<MK>>(a.B, x, i, a.n, a.m);
         return ell_row<float>(a.E, x, i);
     }
 
@@ -205,7 +214,6 @@ namespace
         const double * __restrict__ b = sl.b;
         const double * __restrict__ x = sl.result;
         double * __restrict__ od = sl.od;
-        const bool vec = MK == 0 && ((uintptr_t)b & 15u) == 0;
         double acc = 0.0;
         if (MK == 1)
         {
@@ -219,17 +227,17 @@ namespace
             }
         }
         const int64_t stride = (int64_t)gridDim.x * blockDim.x * 2;
-        for (int64_t i0 = ((int64_t)vb.x * blockDim.x + threadIdx.x) * 2; MK == 0 && i0 < n; i0 += stride)
+        for (int64_t i0 = ((int64_t)vb.x * blockDim.x + threadIdx.x) * 2; kBands<MK> && i0 < n; i0 += stride)
         {
-            if (vec && i0 - m - 1 >= 0 && i0 + 2 + m < n)
+            if (i0 - m - 1 >= 0 && i0 + 2 + m < n)
             {
+                // b and x are the caller's arrays (8-byte aligned), od is the slot's
                 double prod[2];
-                k::q1_rows_v<double>(B, x, i0, m, prod);
+                k::q1_rows_v<double, kOrd<MK>>(B, x, i0, m, prod);
                 typedef k::Vec16<double>::type V2;
-                const V2 bv = __builtin_nontemporal_load((const V2 *)(b + i0));
                 V2 o;
-                o[0] = bv[0] - prod[0];
-                o[1] = bv[1] - prod[1];
+                o[0] = __builtin_nontemporal_load(b + i0) - prod[0];
+                o[1] = __builtin_nontemporal_load(b + i0 + 1) - prod[1];
                 *(V2 *)(od + i0) = o;
                 acc += o[0] * o[0];
                 acc += o[1] * o[1];
@@ -238,7 +246,7 @@ namespace
             {
                 for (int64_t i = i0; i < i0 + 2 && i < n; ++i)
                 {
-                    const double prod = k::q1_row<double>(B, x, i, n, m);
+                    const double prod = k::q1_row<double, kOrd<MK>>(B, x, i, n, m);
                     const double di = b[i] - prod;
                     od[i] = di;
                     acc += di * di;
@@ -345,8 +353,8 @@ namespace
                 for (int64_t i = i0; i < n; ++i)
                     d[i] = rhs[i];
         }
-        else if (MK == 0)
-            k::q1_body_v<float, 1>(a.B, sl.v[L][F_X], d, rhs, nullptr, n, a.m, lane);
+        else if (kBands<MK>)
+            k::q1_body_v<float, 1, kOrd<MK>>(a.B, sl.v[L][F_X], d, rhs, nullptr, n, a.m, lane);
         else
         {
             const float * x = sl.v[L][F_X];
@@ -506,9 +514,9 @@ namespace
         const int64_t lane = (int64_t)vb.x * blockDim.x + threadIdx.x;
         const float * d = sl.v[L][fsrc];
         float * out = sl.v[L][fout];
-        if (MK == 0)
+        if (kBands<MK>)
         {
-            k::jacobi_zero2_body<float, ACC>(a.B, d, a.dinv, out, a.n, a.m, lane);
+            k::jacobi_zero2_body<float, ACC, kOrd<MK>>(a.B, d, a.dinv, out, a.n, a.m, lane);
             return;
         }
         for (int64_t i = lane; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
@@ -532,9 +540,9 @@ namespace
         const float * x = sl.v[L][fx];
         float * y = sl.v[L][fy];
         const float * rhs = sl.v[L][frhs];
-        if (MK == 0)
+        if (kBands<MK>)
         {
-            k::q1_body_v<float, 2>(a.B, x, y, rhs, a.dinv, a.n, a.m, lane);
+            k::q1_body_v<float, 2, kOrd<MK>>(a.B, x, y, rhs, a.dinv, a.n, a.m, lane);
             return;
         }
         for (int64_t i = lane; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
@@ -554,9 +562,9 @@ namespace
         const float * x = sl.v[L][F_X];
         float * d = sl.v[L][F_D];
         const float * rhs = sl.v[L][F_RHS];
-        if (MK == 0)
+        if (kBands<MK>)
         {
-            k::q1_body_v<float, 1>(a.B, x, d, rhs, nullptr, a.n, a.m, lane);
+            k::q1_body_v<float, 1, kOrd<MK>>(a.B, x, d, rhs, nullptr, a.n, a.m, lane);
             return;
         }
         for (int64_t i = lane; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
@@ -897,6 +905,56 @@ namespace
     }
 }
 
+namespace
+{
+    // neighbour position (dy + 1) * 3 + (dx + 1) of column c in row r of a
+    // side-M grid, or -1 when c is not one of the 9 neighbours
+    int q1_position(uint64_t r, uint64_t c, uint64_t M)
+    {
+        const int64_t dy = (int64_t)(c / M) - (int64_t)(r / M), dx = (int64_t)(c % M) - (int64_t)(r % M);
+        if (dy < -1 || dy > 1 || dx < -1 || dx > 1)
+            return -1;
+        return (int)((dy + 1) * 3 + (dx + 1));
+    }
+
+    // 9-point structure of a host ELL on the (2^L+1)^2 grid: every stored
+    // column of a row is one of its neighbours, in ascending order
+    bool ell_is_q1(const EllHost & e, uint64_t L)
+    {
+        const uint64_t M = (uint64_t)side_of(L);
+        if (e.rows != size_of(L) || e.cols != e.rows || e.ncpr > 9 || ! e.Aj || ! e.Ax || ! e.Arl)
+            return false;
+        for (uint64_t r = 0; r < e.rows; ++r)
+        {
+            if (e.Arl[r] > e.ncpr)
+                return false;
+            int prev = -1;
+            for (uint64_t k = 0; k < e.Arl[r]; ++k)
+            {
+                const int q = q1_position(r, e.Aj[r + k * e.stride], M);
+                if (q <= prev)
+                    return false;
+                prev = q;
+            }
+        }
+        return true;
+    }
+
+    // the matrix as 9 bands LL .. UU (band b of row r at b * bs + r), zero
+    // where the row has no entry
+    template <typename DT>
+    std::vector<DT> ell_bands(const EllHost & e, uint64_t L, uint64_t bs)
+    {
+        const uint64_t M = (uint64_t)side_of(L);
+        std::vector<DT> out(9 * bs, DT(0));
+        const DT * ax = (const DT *)e.Ax;
+        for (uint64_t r = 0; r < e.rows; ++r)
+            for (uint64_t k = 0; k < e.Arl[r]; ++k)
+                out[(uint64_t)q1_position(r, e.Aj[r + k * e.stride], M) * bs + r] = ax[r + k * e.stride];
+        return out;
+    }
+}
+
 void hierarchy_alloc_ell(Hierarchy & h, const Params & p, uint64_t max_level, const EllHost * a32,
                          const EllHost & A64, const float * const * dinv32, const EllHost * prol32,
                          const EllHost * res32)
@@ -907,6 +965,39 @@ void hierarchy_alloc_ell(Hierarchy & h, const Params & p, uint64_t max_level, co
     require(p.n_max_iter >= 1, HSOLVE_ERR_INVALID, "mg: n_max_iter must be >= 1");
     require(a32 != nullptr, HSOLVE_ERR_INVALID, "mg: null level matrices");
     require((prol32 == nullptr) == (res32 == nullptr), HSOLVE_ERR_INVALID, "mg: give both prol and res or neither");
+    // 9-point structured levels with structured transfers: kept as bands
+    // (kind 2, summed in the ELL rows' order), read with 16-byte vectors
+    bool banded = prol32 == nullptr && ell_is_q1(A64, max_level) && size_of(p.min_level) <= 16641;
+    for (uint64_t L = p.min_level; banded && L <= max_level; ++L)
+        banded = ell_is_q1(a32[L], L);
+    if (banded)
+    {
+        hierarchy_alloc(h, p, max_level);
+        hierarchy_alloc64(h, max_level);
+        h.kind = 2;
+        h.prolmat = 0;
+        for (uint64_t L = p.min_level; L <= max_level; ++L)
+        {
+            HierLevel & l = h.lv[L];
+            const std::vector<float> b = ell_bands<float>(a32[L], L, l.bs);
+            HIP_CHECK(hipMemcpy(l.bands, b.data(), sizeof(float) * b.size(), hipMemcpyHostToDevice));
+            std::vector<float> dv(l.n);
+            if (dinv32 && dinv32[L])
+                std::memcpy(dv.data(), dinv32[L], sizeof(float) * l.n);
+            else
+                for (uint64_t r = 0; r < l.n; ++r)
+                {
+                    // diags_inverted as the reference tests build it
+                    const float dd = b[4 * l.bs + r];
+                    const float inv = dd != 0.0f ? 1.0f / dd : 0.0f;
+                    dv[r] = (float)((double)inv * 0.7);
+                }
+            HIP_CHECK(hipMemcpy(l.dinv, dv.data(), sizeof(float) * l.n, hipMemcpyHostToDevice));
+        }
+        const std::vector<double> b64 = ell_bands<double>(A64, max_level, h.bs64[max_level]);
+        HIP_CHECK(hipMemcpy(h.bands64[max_level], b64.data(), sizeof(double) * b64.size(), hipMemcpyHostToDevice));
+        return;
+    }
     h.p = p;
     h.kind = 1;
     h.prolmat = prol32 ? 1 : 0;
@@ -1050,6 +1141,8 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         return a;
     };
     auto norm_grid = [&](uint64_t nn) { return dim3(std::min<unsigned>(lanes4(nn), kNormBlocks), 1, S); };
+    // row kernels: 4 rows per lane on Q1 bands, one row per lane on ELL
+    auto rgrid = [&](uint64_t nn) { return kBands<MK> ? lanes4(nn) : grid_for(nn, kBlock); };
 
     // ---- gates, start of new solves, FP64 outer step, opening of inner solves
     hipLaunchKernelGGL(plan_kernel, dim3(grid_for(S, 64)), dim3(64), 0, st, T, S);
@@ -1066,7 +1159,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     hipLaunchKernelGGL(outer_accumulate_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
     const unsigned od_blocks = std::min<unsigned>(grid_for((top.n + 1) / 2, kBlock), kNormBlocks);
     hipLaunchKernelGGL((outer_defect_kernel<MK>), dim3(od_blocks, 1, S), blk, 0, st,
-                       MK == 0 ? h.B64(maxl) : k::Bands{}, h.ell64[maxl], T, n, top.side);
+                       kBands<MK> ? h.B64(maxl) : k::Bands{}, h.ell64[maxl], T, n, top.side);
     hipLaunchKernelGGL(outer_defect_finish_kernel, dim3(1, 1, S), blk, 0, st, T, od_blocks, (double)p.tolerance,
                        p.max_outer);
     hipLaunchKernelGGL(to_inner_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
@@ -1074,7 +1167,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         hipLaunchKernelGGL((open_inner_kernel<1, MK>), dim3(lanes4(top.n), 1, S), blk, 0, st, largs(maxl), T,
                            (int)maxl);
     else
-        hipLaunchKernelGGL((open_inner_kernel<0, MK>), dim3(lanes4(top.n), 1, S), blk, 0, st, largs(maxl), T,
+        hipLaunchKernelGGL((open_inner_kernel<0, MK>), dim3(rgrid(top.n), 1, S), blk, 0, st, largs(maxl), T,
                            (int)maxl);
     hipLaunchKernelGGL((inner_norm_kernel<0>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n);
     hipLaunchKernelGGL((inner_norm_finish_kernel<0>), dim3(1, 1, S), blk, 0, st, T, norm_grid(top.n).x, p.tolerance,
@@ -1086,7 +1179,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         const LevelArgs a = largs(L);
         for (uint64_t i = 0; i < count; ++i)
         {
-            hipLaunchKernelGGL((v_sweep_kernel<MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, xc, xa, frhs);
+            hipLaunchKernelGGL((v_sweep_kernel<MK>), dim3(rgrid(a.n), 1, S), blk, 0, st, a, T, (int)L, xc, xa, frhs);
             std::swap(xc, xa);
         }
     };
@@ -1099,7 +1192,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     };
     auto defect = [&](uint64_t L) {
         const LevelArgs a = largs(L);
-        hipLaunchKernelGGL((v_defect_kernel<MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L);
+        hipLaunchKernelGGL((v_defect_kernel<MK>), dim3(rgrid(a.n), 1, S), blk, 0, st, a, T, (int)L);
     };
 
     if (minl != maxl)
@@ -1117,11 +1210,11 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                     hipLaunchKernelGGL((v_zero1_kernel<1>), dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, a, T,
                                        (int)L, dsrc, (int)F_X);
                 else if (npre == 2)
-                    hipLaunchKernelGGL((v_zero2_kernel<1, MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
+                    hipLaunchKernelGGL((v_zero2_kernel<1, MK>), dim3(rgrid(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
                                        (int)F_X);
                 else
                 {
-                    hipLaunchKernelGGL((v_zero2_kernel<0, MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
+                    hipLaunchKernelGGL((v_zero2_kernel<0, MK>), dim3(rgrid(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
                                        (int)F_C);
                     int cc = F_C, ca = F_TMP;
                     sweeps(L, cc, ca, dsrc, npre - 2);
@@ -1136,7 +1229,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                                        (int)L, dsrc, (int)F_X);
                 else
                 {
-                    hipLaunchKernelGGL((v_zero2_kernel<0, MK>), dim3(lanes4(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
+                    hipLaunchKernelGGL((v_zero2_kernel<0, MK>), dim3(rgrid(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
                                        (int)F_X);
                     int xc = F_X, xa = F_TMP;
                     sweeps(L, xc, xa, dsrc, npre - 2);
@@ -1146,7 +1239,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
             }
             defect(L);
             if (h.prolmat)
-                hipLaunchKernelGGL(v_apply_ell_kernel, dim3(lanes4(h.lv[L].res.rows), 1, S), blk, 0, st, h.lv[L].res,
+                hipLaunchKernelGGL(v_apply_ell_kernel, dim3(grid_for(h.lv[L].res.rows, kBlock), 1, S), blk, 0, st, h.lv[L].res,
                                    T, (int)L, (int)F_D, (int)(L - 1), (int)F_RHS);
             else
             {
@@ -1180,7 +1273,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         if (h.prolmat)
         {
             // c = P x_coarse, then x += alpha c
-            hipLaunchKernelGGL(v_apply_ell_kernel, dim3(lanes4(h.lv[L].prol.rows), 1, S), blk, 0, st, h.lv[L].prol, T,
+            hipLaunchKernelGGL(v_apply_ell_kernel, dim3(grid_for(h.lv[L].prol.rows, kBlock), 1, S), blk, 0, st, h.lv[L].prol, T,
                                (int)(L - 1), (int)F_X, (int)L, (int)F_C);
             if (p.adapt != 0.0f)
                 hipLaunchKernelGGL(v_axpy_const_kernel, dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, T, (int)L, a.n,
@@ -1216,6 +1309,8 @@ void enqueue_tick(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S
 {
     if (h.kind == 1)
         tick_impl<1>(h, maxl, T, S, st);
+    else if (h.kind == 2)
+        tick_impl<2>(h, maxl, T, S, st);
     else
         tick_impl<0>(h, maxl, T, S, st);
 }

@@ -49,6 +49,7 @@ namespace
         uint64_t stride;
         int64_t n;
         uint32_t ncpr;
+        int64_t q1_side;       // hsolve_ell_s::q1_side
     };
 
     __device__ __forceinline__ uint32_t rlane(uint32_t v, int l)
@@ -148,6 +149,153 @@ namespace
         }
     }
 
+    // The same for Q1-structured matrices (every stored column of row
+    // (y, x) is a neighbour (y + dy, x + dx), |dy|, |dx| <= 1, ascending, on
+    // a side-M grid).  The tile's slots are first sorted into their 9
+    // neighbour positions (value, presence bit) per lane.  The wave then
+    // walks the rows in order keeping the 3 x 3 window of X rows around the
+    // current row in registers: one step right along a grid line drops the
+    // window's left column and gathers the 3 entries of its new right column,
+    // so a row costs 3 gathers instead of 9; 4 rows of one grid line issue
+    // their 12 gathers together.  Present entries are summed in
+    // position order, which is the slot order, so the result is the same
+    // bits as tile_rows.  Window entries off the grid are read at a clamped
+    // (valid) address and never summed.
+    template <typename DT, typename IT, typename LT, typename F>
+    __device__ __forceinline__ void tile_rows_q1(const MvEll<DT, IT, LT> & A, const DT * __restrict__ X,
+                                                 const DT * __restrict__ OWN, unsigned R, int64_t base, bool on,
+                                                 F && f)
+    {
+        const int lane = threadIdx.x & 63;
+        const int64_t n = A.n, M = A.q1_side;
+        const int64_t myrow = base + lane;
+        const int rows = (int)(n - base < 64 ? n - base : 64);
+        const uint32_t self = myrow < n ? (uint32_t)myrow : 0u;
+        const uint32_t len = myrow < n ? (uint32_t)A.Arl[myrow] : 0u;
+        IT rc[9];
+        DT ra[9];
+#pragma unroll
+        for (int kk = 0; kk < 9; ++kk)
+        {
+            rc[kk] = IT(0);
+            ra[kk] = DT(0);
+            if ((uint32_t)kk < A.ncpr)
+            {
+                const uint64_t at = (uint64_t)kk * A.stride + (uint64_t)self;
+                rc[kk] = k::ld_stream(A.Aj + at);
+                ra[kk] = k::ld_stream(A.Ax + at);
+            }
+        }
+        DT ap[9];
+        uint32_t mask = 0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            ap[q] = DT(0);
+#pragma unroll
+        for (int kk = 0; kk < 9; ++kk)
+        {
+            if ((uint32_t)kk < len)
+            {
+                const int64_t d = (int64_t)k::col_of<IT>(rc[kk], (uint64_t)myrow) - myrow;
+                const int p = d < -1 ? (int)(d + M) + 1 : (d > 1 ? (int)(d - M) + 7 : (int)d + 4);
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    ap[q] = (p == q) ? ra[kk] : ap[q];
+                mask |= 1u << p;
+            }
+        }
+        auto at = [&](int64_t c) { return X[(uint64_t)(c < 0 ? 0 : (c >= n ? n - 1 : c)) * R + lane]; };
+        auto row_sum = [&](int r, const DT (&w)[9]) {
+            const uint32_t mr = rlane(mask, r);
+            DT sum = DT(0);
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+            {
+                if ((mr >> q) & 1u)
+                    sum += rlane(ap[q], r) * w[q];
+            }
+            return sum;
+        };
+        // c0[b], c1[b]: window columns x - 1 and x of the next row, band b
+        // (grid line y - 1, y, y + 1); valid after a row of the same line
+        DT c0[3], c1[3];
+        bool have = false;
+        int64_t x = base % M;
+        int r = 0;
+        while (r < rows)
+        {
+            const int64_t i = base + r;
+            if (x == 0 || ! have)
+            {
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+                {
+                    c0[b] = on ? at(i + (b - 1) * M - 1) : DT(0);
+                    c1[b] = on ? at(i + (b - 1) * M) : DT(0);
+                }
+            }
+            if (r + 4 <= rows && x + 3 < M)
+            {
+                // 4 rows of one grid line: their 12 new gathers in flight at once
+                DT v[3][4], ov[4];
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        v[b][u] = on ? at(i + (b - 1) * M + 1 + u) : DT(0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    ov[u] = (on && OWN && OWN != X) ? OWN[(uint64_t)(i + u) * R + lane] : DT(0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    DT w[9];
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d)
+                        {
+                            const int k = u + d;       // column x - 1 + k of band b
+                            w[b * 3 + d] = k == 0 ? c0[b] : (k == 1 ? c1[b] : v[b][k - 2]);
+                        }
+                    f(i + u, row_sum(r + u, w), OWN == X ? w[4] : ov[u]);
+                }
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+                {
+                    c0[b] = v[b][2];
+                    c1[b] = v[b][3];
+                }
+                r += 4;
+                x += 4;
+            }
+            else
+            {
+                DT w[9];
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+                {
+                    w[b * 3] = c0[b];
+                    w[b * 3 + 1] = c1[b];
+                    w[b * 3 + 2] = on ? at(i + (b - 1) * M + 1) : DT(0);
+                }
+                const DT ov = OWN == X ? w[4] : ((on && OWN) ? OWN[(uint64_t)i * R + lane] : DT(0));
+                f(i, row_sum(r, w), ov);
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+                {
+                    c0[b] = w[b * 3 + 1];
+                    c1[b] = w[b * 3 + 2];
+                }
+                r += 1;
+                x += 1;
+            }
+            if (x >= M)
+                x -= M;
+            have = true;
+        }
+    }
+
     // The same for matrices with more than kTileK slots per row: the row's
     // slots are read with wave-uniform addresses.
     template <typename DT, typename IT, typename LT, typename F>
@@ -174,7 +322,8 @@ namespace
         }
     }
 
-    // K: slots per row kept in registers (9 or 16), 0 for longer rows
+    // K: slots per row kept in registers (9 or 16), 0 for longer rows, -1
+    // for Q1-structured matrices
     template <typename DT, typename IT, typename LT, int K, typename F>
     __device__ __forceinline__ void for_rows_spmv(const MvEll<DT, IT, LT> & A, const DT * __restrict__ X,
                                                   const DT * __restrict__ OWN, unsigned R, bool on, F && f)
@@ -184,6 +333,8 @@ namespace
         {
             if constexpr (K == 0)
                 tile_rows_long(A, X, OWN, R, tile * 64, on, f);
+            else if constexpr (K < 0)
+                tile_rows_q1(A, X, OWN, R, tile * 64, on, f);
             else
                 tile_rows<K>(A, X, OWN, R, tile * 64, on, f);
         }
@@ -546,8 +697,11 @@ namespace
             using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
             auto run = [&](auto dt_tag) {
                 using DT = decltype(dt_tag);
-                MvEll<DT, IT, LT> A{Aj, (const DT *)m->Ax, Arl, m->stride, (int64_t)m->rows, (uint32_t)m->ncpr};
-                if (m->ncpr <= 9)
+                MvEll<DT, IT, LT> A{Aj, (const DT *)m->Ax, Arl, m->stride, (int64_t)m->rows, (uint32_t)m->ncpr,
+                                    (int64_t)m->q1_side};
+                if (m->q1_side != 0 && m->ncpr <= 9 && (m->variant & HSOLVE_ELL_SET) == 0)
+                    f(A, std::integral_constant<int, -1>());
+                else if (m->ncpr <= 9)
                     f(A, std::integral_constant<int, 9>());
                 else if (m->ncpr <= (uint64_t)kTileK)
                     f(A, std::integral_constant<int, kTileK>());

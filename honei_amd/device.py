@@ -147,6 +147,13 @@ class EllMatrix:
     def set_variant(self, variant: int):
         check(lib().hsolve_ell_set_variant(vp(self.handle), int(variant)), "ell_set_variant")
 
+    @property
+    def q1_side(self):
+        """Grid side when the matrix was recognised as a 9-point structured matrix, else 0."""
+        v = u64()
+        check(lib().hsolve_ell_q1_side(vp(self.handle), C.byref(v)), "ell_q1_side")
+        return v.value
+
     def spmv(self, x, y, rhs=None, stream=None):
         check(lib().hsolve_ell_spmv(vp(self.handle), vp(x.ptr), vp(y.ptr), vp(rhs.ptr) if rhs else None,
                                     vp(stream)), "ell_spmv")

@@ -117,7 +117,7 @@ int hsolve_defect_q1_double(const void *rhs, const void *ll, const void *ld, con
                             const void *ud, const void *uu, const void *x, void *y,
                             uint64_t size, uint64_t m, void *stream);
 /* Dot product and squared L2 norm (root = 0) or L2 norm (root = 1), one
- * pass, deterministic order; result_dev is a device scalar of the vector's
+ * pass over the data plus a one-block finish, deterministic order; result_dev is a device scalar of the vector's
  * type.  Replace cuda_dot_product_two_* (dot_product.cu:77-155) and
  * cuda_norm_l2_one_* (norm.cu:77-155). */
 int hsolve_dot_product_two_float(const void *a, const void *b, uint64_t size, void *result_dev,
@@ -243,6 +243,13 @@ int hsolve_ell_info(hsolve_ell m, uint64_t *rows, uint64_t *columns, uint64_t *s
  * num_cols_per_row slots without waiting for row lengths, 4 = XCD-contiguous
  * block order.  Results are identical for every variant. */
 int hsolve_ell_set_variant(hsolve_ell m, int variant);
+/* Grid side M when the handle holds a 9-point structured matrix (rows =
+ * columns = M^2, threads == 1, every stored column of row (y, x) one of
+ * (y + dy, x + dx), |dy|, |dx| <= 1, ascending), else 0.  The many-RHS
+ * kernels then keep a sliding 3 x 3 window of rows (same results); an
+ * explicit hsolve_ell_set_variant choice turns that off.  No reference
+ * counterpart. */
+int hsolve_ell_q1_side(hsolve_ell m, uint64_t *side);
 /* y = A x, or y = rhs - A x when rhs != NULL */
 int hsolve_ell_spmv(hsolve_ell m, const void *x, void *y, const void *rhs, void *stream);
 /* ConjugateGradients<Tag, NONE>::value(ELL, rhs, x, konv_rad) in one

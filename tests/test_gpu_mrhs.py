@@ -192,3 +192,29 @@ def test_mg_ir_diffusion_batch_matches_oracle(hip, oracle):
         single = mg.solve(bd[j], x1)
         assert np.array_equal(x1.get(), x)
         assert (single["outer"], single["inner"]) == (infos[j]["outer"], infos[j]["inner"])
+
+
+@pytest.mark.parametrize("R", [3, 64])
+@pytest.mark.parametrize("level", [4, 6])
+def test_spmm_stencil_window_equals_generic(hip, oracle, level, R):
+    """The 3 x 3 window path for 9-point structured matrices (chosen at
+    upload) gives the same bits as the generic tile path and the oracle."""
+    E = P.diffusion_ell(level)
+    n = E["rows"]
+    A = hip.EllMatrix.from_dict(E)
+    G = hip.EllMatrix.from_dict(E)
+    G.set_variant(1)                        # an explicit variant selects the generic path
+    assert A.q1_side == P.side_of_level(level)
+    assert hip.EllMatrix.from_dict(P.matrix_b_ell(4096, halfwidth=100)).q1_side == 0
+    xs = [P.rng_uniform(P.SEED + 7 * j, P.S_X, np.arange(n, dtype=np.uint64), -1, 1) for j in range(R)]
+    X = hip.DeviceVector.from_numpy(_mv(xs))
+    RHS = hip.DeviceVector.from_numpy(_mv(xs[::-1]))
+    Y1, Y2 = hip.DeviceVector.zeros(n * R), hip.DeviceVector.zeros(n * R)
+    for rhs in (None, RHS):
+        A.spmm(R, X, Y1, rhs=rhs)
+        G.spmm(R, X, Y2, rhs=rhs)
+        y1 = Y1.get()
+        assert np.array_equal(y1, Y2.get())
+        j = R - 1
+        ref = oracle.ell_product(E, xs[j]) if rhs is None else oracle.ell_defect(E, xs[::-1][j], xs[j])
+        assert np.array_equal(_col(y1, R, j), ref)
