@@ -10,8 +10,11 @@ Headline (one "step" = one ELL SpMV y = A x over the whole matrix):
   lengths).
 
 Also reported under "extra": defect and FP32 forms, Matrix B (random ELL,
-2^24 rows, 5..13 per row), Q1 banded SpMV, and Poisson solves/s of
-mixed-precision MG-IR (FP32 V-cycles, FP64 refinement) and plain FP64 CG.
+2^24 rows, 5..13 per row), Q1 banded SpMV, Poisson solves/s of
+mixed-precision MG-IR (FP32 V-cycles, FP64 refinement) and plain FP64 CG,
+the served stream of 4096 mixed-size MG-IR requests (solves/s, p50/p99),
+and 64 right-hand sides of the variable-coefficient diffusion problem on
+2049^2 (batched Jacobi-PCG and MG-IR).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -395,6 +398,9 @@ def main():
         extra["cg_fp64_poisson_4097_tol1e-8"] = cg
         extra["mg_ir_speedup_vs_fp64_cg_4097"] = cg["ms"] / mg["ms_p50"]
         log("cg", cg)
+        extra["serve_mixed_poisson"] = serve_case(hip, 4096)
+        log("serve", {k: v for k, v in extra["serve_mixed_poisson"].items() if k != "burst"})
+        extra["diffusion_2049_64rhs"] = diffusion_case(hip, P, level=11)
 
     cpu = None
     if not args.quick and not args.no_cpu:

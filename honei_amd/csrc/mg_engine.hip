@@ -43,6 +43,8 @@ namespace
     };
     __device__ __forceinline__ VB vblock()
     {
+        if (gridDim.z == 1)     // one slot: the hardware order
+            return VB{blockIdx.x, blockIdx.y, 0u};
         const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
         const unsigned L = k::xcd_block(lin, gridDim.x * gridDim.y * gridDim.z);
         const unsigned z = L % gridDim.z, r = L / gridDim.z;
