@@ -47,7 +47,18 @@ namespace
             return m->variant;
         if (m->nnz * 10 >= m->rows * m->ncpr * 9)
             return HSOLVE_ELL_FULL;
-        return HSOLVE_ELL_XCD;
+        return HSOLVE_ELL_XCD | HSOLVE_ELL_LDS;
+    }
+
+    // LDS bytes of ell_spmv_lds's x window, or 0 when the handle does not
+    // qualify (threads == 1, columns within +-max_offset of the row, a
+    // window of at most 48 KiB so that 3 blocks fit on a CU)
+    size_t lds_window_bytes(const hsolve_ell_s * m, size_t vs)
+    {
+        const uint64_t w = (uint64_t)k::kLdsRows + 2 * m->max_offset;
+        if (m->threads != 1 || w * vs > 48 * 1024)
+            return 0;
+        return (size_t)(w * vs);
     }
 
     template <typename DT, typename IT, typename LT, int MODE, int K>
@@ -81,6 +92,20 @@ namespace
         with_ell_types(m, [&](auto Aj, auto Arl) {
             using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
             using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
+            const size_t lds = lds_window_bytes(m, sizeof(DT));
+            if (r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->ncpr <= 16)
+            {
+                const unsigned g = grid_for(m->rows, k::kLdsRows);
+#define HS_L(K)                                                                                                               hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE>), dim3(g), dim3(kBlock), lds, st, x, y,                                    Aj, Ax, Arl, rhs, dinv, m->rows, m->cols, m->stride, (uint32_t)m->ncpr,                                        m->max_offset)
+                if (m->ncpr <= 9)
+                    HS_L(9);
+                else if (m->ncpr <= 13)
+                    HS_L(13);
+                else
+                    HS_L(16);
+#undef HS_L
+                return;
+            }
             if (r2_ok && (ell_launch_r2<DT, IT, LT, MODE, 4>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
                           ell_launch_r2<DT, IT, LT, MODE, 9>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
                           ell_launch_r2<DT, IT, LT, MODE, 13>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
@@ -230,6 +255,7 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
     m->dtype = dtype; m->rows = rows; m->cols = columns; m->stride = stride; m->ncpr = num_cols_per_row;
     m->threads = threads; m->nnz = nnz; m->idx_kind = idx_kind; m->len_bytes = lb; m->variant = 0;
     m->q1_side = q1_side;
+    m->max_offset = (uint64_t)maxoff;
     m->Aj = nullptr; m->Ax = nullptr; m->Arl = nullptr;
     try
     {
@@ -288,7 +314,8 @@ int hsolve_ell_set_variant(hsolve_ell m, int variant)
 {
     HSOLVE_API_BEGIN
     require(m != nullptr, HSOLVE_ERR_INVALID, "ell_set_variant: null handle");
-    require(variant >= 0 && variant < 8, HSOLVE_ERR_INVALID, "ell_set_variant: variant bits out of range");
+    require(variant >= 0 && variant < 32 && (variant & HSOLVE_ELL_SET) == 0, HSOLVE_ERR_INVALID,
+            "ell_set_variant: variant bits out of range");
     m->variant = variant ? (variant | HSOLVE_ELL_SET) : 0;
     HSOLVE_API_END
 }

@@ -15,6 +15,7 @@ struct hsolve_ell_s
     void * Arl;                // rows row lengths
     uint64_t device_bytes;
     int variant;               // kernel variant bits (HSOLVE_ELL_*), 0 = automatic
+    uint64_t max_offset;       // largest |column - row| over the stored entries
     uint64_t q1_side;          // M > 0 when rows = cols = M^2, threads = 1 and every stored
                                // column of row (y, x) is a neighbour (y + dy, x + dx),
                                // |dy|, |dx| <= 1, in ascending slot order; else 0
@@ -26,7 +27,8 @@ enum
     HSOLVE_ELL_ONE_ROW = 1,    // one row per lane (else two rows per lane)
     HSOLVE_ELL_FULL = 2,       // load all num_cols_per_row slots without waiting for row lengths
     HSOLVE_ELL_XCD = 4,        // XCD-contiguous block order
-    HSOLVE_ELL_SET = 8         // marks an explicit choice
+    HSOLVE_ELL_SET = 8,        // marks an explicit choice
+    HSOLVE_ELL_LDS = 16        // x window in LDS (banded matrices; else ignored)
 };
 
 namespace hsolve

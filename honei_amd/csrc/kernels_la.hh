@@ -185,6 +185,90 @@ namespace k
         }
     }
 
+    // Rows per block of ell_spmv_lds (two per lane).
+    constexpr int kLdsRows = 512;
+
+    // ell_spmv_r2 for matrices whose stored columns lie within +-maxoff of
+    // their row (banded but unstructured, e.g. random columns in a window):
+    // the block first copies x[w0, w1), the columns its 512 rows can touch,
+    // into LDS with coalesced loads, and the rows gather from there instead
+    // of sending one scattered L2 request per nonzero.  Blocks in
+    // XCD-contiguous order, so neighbouring windows overlap in one L2.
+    // Dynamic LDS: (kLdsRows + 2 maxoff) * sizeof(DT) bytes.  Same sums, in
+    // the same order, as ell_spmv_r2.
+    template <typename DT, typename IT, typename LT, int MAXK, int MODE>
+    __global__ __launch_bounds__(256) void ell_spmv_lds(
+            const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
+            const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
+            const DT * __restrict__ dinv, uint64_t rows, uint64_t cols, uint64_t stride, uint32_t ncpr,
+            uint64_t maxoff)
+    {
+        extern __shared__ __align__(16) unsigned char lds_raw[];
+        DT * xw = reinterpret_cast<DT *>(lds_raw);
+        typedef typename Vec2<DT>::type V;
+        typedef typename Vec2<IT>::type VI;
+        const uint64_t rb0 = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kLdsRows;
+        const uint64_t w0 = rb0 > maxoff ? rb0 - maxoff : 0;
+        const uint64_t wend = rb0 + kLdsRows + maxoff;
+        const uint64_t w1 = wend < cols ? wend : cols;
+        for (uint64_t j = threadIdx.x; w0 + j < w1; j += blockDim.x)
+            xw[j] = x[w0 + j];
+        __syncthreads();
+        const uint64_t r0 = rb0 + (uint64_t)threadIdx.x * 2;
+        if (r0 >= rows)
+            return;
+        const bool two = r0 + 1 < rows;
+        const uint32_t len0 = (uint32_t)Arl[r0];
+        const uint32_t len1 = two ? (uint32_t)Arl[r0 + 1] : 0u;
+        const uint32_t lmax = len0 > len1 ? len0 : len1;
+        DT a0[MAXK], a1[MAXK];
+        IT c0[MAXK], c1[MAXK];
+#pragma unroll
+        for (int k = 0; k < MAXK; ++k)
+        {
+            if ((uint32_t)k < lmax)
+            {
+                const V v = __builtin_nontemporal_load((const V *)(Ax + r0 + (uint64_t)k * stride));
+                const VI c = __builtin_nontemporal_load((const VI *)(Aj + r0 + (uint64_t)k * stride));
+                a0[k] = v.x;
+                a1[k] = v.y;
+                c0[k] = (IT)c.x;
+                c1[k] = (IT)c.y;
+            }
+        }
+        DT s0 = DT(0), s1 = DT(0);
+#pragma unroll
+        for (int k = 0; k < MAXK; ++k)
+        {
+            if ((uint32_t)k < len0)
+                s0 += a0[k] * xw[col_of<IT>(c0[k], r0) - w0];
+            if ((uint32_t)k < len1)
+                s1 += a1[k] * xw[col_of<IT>(c1[k], r0 + 1) - w0];
+        }
+        if (MODE == 0)
+        {
+            y[r0] = s0;
+            if (two)
+                y[r0 + 1] = s1;
+        }
+        else if (MODE == 1)
+        {
+            y[r0] = rhs[r0] - s0;
+            if (two)
+                y[r0 + 1] = rhs[r0 + 1] - s1;
+        }
+        else
+        {
+            const DT t0 = rhs[r0] - s0;
+            y[r0] = x[r0] + t0 * dinv[r0];
+            if (two)
+            {
+                const DT t1 = rhs[r0 + 1] - s1;
+                y[r0 + 1] = x[r0 + 1] + t1 * dinv[r0 + 1];
+            }
+        }
+    }
+
     // ELL with threads > 1: still one lane per row, walking k then t so the
     // summation order is the reference's (product.hh:192-202).
     template <typename DT, typename IT, typename LT, int MODE>

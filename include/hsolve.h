@@ -241,7 +241,10 @@ int hsolve_ell_info(hsolve_ell m, uint64_t *rows, uint64_t *columns, uint64_t *s
 /* Kernel variant of the handle (tuning knob, no reference counterpart):
  * 0 = automatic; else bits 1 = one row per lane (default two), 2 = load all
  * num_cols_per_row slots without waiting for row lengths, 4 = XCD-contiguous
- * block order.  Results are identical for every variant. */
+ * block order, 16 = stage the x window of each 512-row block in LDS (used
+ * when every stored column lies within a window of at most 48 KiB around
+ * its row; automatic for such matrices unless they fill 90% of their
+ * slots).  Results are identical for every variant. */
 int hsolve_ell_set_variant(hsolve_ell m, int variant);
 /* Grid side M when the handle holds a 9-point structured matrix (rows =
  * columns = M^2, threads == 1, every stored column of row (y, x) one of

@@ -111,12 +111,20 @@ def test_ell_matrix_b_and_a_small(hip, oracle, dtype):
         M = hip.EllMatrix.from_dict(E)
         yd = hip.DeviceVector.zeros(n, dtype)
         xd, rd = hip.DeviceVector.from_numpy(x), hip.DeviceVector.from_numpy(rhs)
-        for variant in range(8):          # every kernel variant gives the same bits
+        for variant in list(range(8)) + [16, 20, 22]:   # every kernel variant gives the same bits
             M.set_variant(variant)
             M.spmv(xd, yd)
             assert np.array_equal(yd.get(), ref), variant
             M.spmv(xd, yd, rhs=rd)
             assert np.array_equal(yd.get(), ref_d), variant
+        # the Jacobi sweep through the default and the LDS-window kernels
+        dinv = (1.0 / (np.abs(x) + 1.0)).astype(dtype)
+        ref_j = oracle.jacobi_ell(x, E, rhs, dinv, 1)
+        dd = hip.DeviceVector.from_numpy(dinv)
+        for variant in (0, 16):
+            M.set_variant(variant)
+            M.jacobi(rd, dd, xd, yd)
+            assert np.array_equal(yd.get(), ref_j), variant
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
