@@ -69,6 +69,17 @@ namespace
             return false;
         const unsigned grid = grid_for((m->rows + 1) / 2, kBlock);
         const bool full = v & HSOLVE_ELL_FULL, xcd = v & HSOLVE_ELL_XCD;
+        if constexpr (K == 9)
+        {
+            // 9-point structured (automatic choice only): columns implied
+            if (m->q1_side != 0 && full && ! xcd && (m->variant & HSOLVE_ELL_SET) == 0)
+            {
+                hipLaunchKernelGGL((k::ell_spmv_r2<DT, IT, LT, 9, MODE, true, false, true>), dim3(grid), dim3(kBlock),
+                                   0, st, x, y, Aj, Ax, Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride,
+                                   (uint32_t)m->ncpr, (int64_t)m->q1_side);
+                return true;
+            }
+        }
 #define HS_R2(F, X)                                                                                           \
         hipLaunchKernelGGL((k::ell_spmv_r2<DT, IT, LT, K, MODE, F, X>), dim3(grid), dim3(kBlock), 0, st, x, y, Aj, \
                            Ax, Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride, (uint32_t)m->ncpr)

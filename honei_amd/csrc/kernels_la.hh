@@ -120,12 +120,16 @@ namespace k
     // nearly all full, e.g. stencils); the sums still stop at each row's
     // length, in ascending slot order.  Requires row_start even and the
     // slot arrays readable at row_end (stride > rows or rows even).
-    template <typename DT, typename IT, typename LT, int MAXK, int MODE, bool FULL, bool XCD>
+    // STENCIL (9-point structured matrices on a side-m grid, MAXK == 9): a
+    // row with all 9 slots stored holds exactly the neighbours (y + dy,
+    // x + dx) in ascending order, so its columns are row + offset and its
+    // index slots are not read; only shorter (boundary) rows read them.
+    template <typename DT, typename IT, typename LT, int MAXK, int MODE, bool FULL, bool XCD, bool STENCIL = false>
     __global__ __launch_bounds__(256) void ell_spmv_r2(
             const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
             const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
             const DT * __restrict__ dinv, uint64_t row_start, uint64_t row_end, uint64_t stride,
-            uint32_t ncpr)
+            uint32_t ncpr, int64_t m = 0)
     {
         typedef typename Vec2<DT>::type V;
         typedef typename Vec2<IT>::type VI;
@@ -137,6 +141,8 @@ namespace k
         const uint32_t len0 = (uint32_t)Arl[r0];
         const uint32_t len1 = two ? (uint32_t)Arl[r0 + 1] : 0u;
         const uint32_t lmax = len0 > len1 ? len0 : len1;
+        // STENCIL: both rows full, columns from the offsets
+        const bool implied = STENCIL && len0 == 9 && (len1 == 9 || ! two);
         DT a0[MAXK], a1[MAXK];
         IT c0[MAXK], c1[MAXK];
 #pragma unroll
@@ -145,21 +151,38 @@ namespace k
             if (FULL ? (uint32_t)k < ncpr : (uint32_t)k < lmax)
             {
                 const V v = __builtin_nontemporal_load((const V *)(Ax + r0 + (uint64_t)k * stride));
-                const VI c = __builtin_nontemporal_load((const VI *)(Aj + r0 + (uint64_t)k * stride));
                 a0[k] = v.x;
                 a1[k] = v.y;
-                c0[k] = (IT)c.x;
-                c1[k] = (IT)c.y;
+                if (! implied)
+                {
+                    const VI c = __builtin_nontemporal_load((const VI *)(Aj + r0 + (uint64_t)k * stride));
+                    c0[k] = (IT)c.x;
+                    c1[k] = (IT)c.y;
+                }
             }
         }
         DT s0 = DT(0), s1 = DT(0);
-#pragma unroll
-        for (int k = 0; k < MAXK; ++k)
+        if (implied)
         {
-            if ((uint32_t)k < len0)
-                s0 += a0[k] * x[col_of<IT>(c0[k], r0)];
-            if ((uint32_t)k < len1)
-                s1 += a1[k] * x[col_of<IT>(c1[k], r0 + 1)];
+#pragma unroll
+            for (int k = 0; k < MAXK; ++k)
+            {
+                const int64_t off = (int64_t)(k / 3 - 1) * m + (k % 3 - 1);
+                s0 += a0[k] * x[(int64_t)r0 + off];
+                if (two)
+                    s1 += a1[k] * x[(int64_t)r0 + 1 + off];
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < MAXK; ++k)
+            {
+                if ((uint32_t)k < len0)
+                    s0 += a0[k] * x[col_of<IT>(c0[k], r0)];
+                if ((uint32_t)k < len1)
+                    s1 += a1[k] * x[col_of<IT>(c1[k], r0 + 1)];
+            }
         }
         if (MODE == 0)
         {
