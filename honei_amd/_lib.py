@@ -116,6 +116,7 @@ _SIGS = {
     "hsolve_cg_q1": [C.c_int, vp, u64, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_cg_ell": [vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_pcg_ell": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
+    "hsolve_pcg_jac_ell": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_ell_spmm": [vp, u64, vp, vp, vp, vp],
     "hsolve_cg_ell_multi": [vp, u64, vp, vp, vp, u64, dbl, vp, vp, vp],
     "hsolve_rhs_uniform_multi": [vp, u64, u64, u64, u64, vp, u64, u64, vp],

@@ -238,6 +238,142 @@ namespace
         }
     }
 
+    // ----------------------------------------- ConjugateGradients<Tag, JAC>
+    // conjugate_gradients.hh:1110-1155 with the ELL cg_kernel :885-913.
+    // Scalars: temp = beta (r.z), alpha = step, talpha = the direction factor.
+    template <typename DT> __device__ __forceinline__ DT div_guard(DT num, DT den)
+    {
+        return fabs(den) >= Eps<DT>::v ? num / den : num / Eps<DT>::v;
+    }
+
+    // r = b - A x; z = dd * r; d = z; partials of r.r and r.z
+    template <typename DT, typename Op>
+    __global__ __launch_bounds__(256) void jac_init_kernel(Op A, const DT * __restrict__ b,
+            const DT * __restrict__ x, const DT * __restrict__ dd, DT * __restrict__ r, DT * __restrict__ z,
+            DT * __restrict__ d, int64_t n, double * partials)
+    {
+        double acc_rr = 0.0, acc_rz = 0.0;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        {
+            const DT ri = b[i] - A.row(x, i);
+            r[i] = ri;
+            const DT zi = dd[i] * ri;
+            z[i] = zi;
+            d[i] = zi;
+            acc_rr += (double)ri * (double)ri;
+            acc_rz += (double)ri * (double)zi;
+        }
+        k::put_partials2(acc_rr, acc_rz, partials, blockIdx.x);
+    }
+
+    template <typename DT>
+    __global__ __launch_bounds__(256) void jac_init_finish(const double * partials, unsigned nb, CgScalars<DT> * s)
+    {
+        double rr, rz;
+        k::sum_partials2(partials, nb, rr, rz);
+        if (threadIdx.x == 0)
+        {
+            s->iters = 0;
+            s->converged = 0;
+            s->run = (s->max_iters > 0) ? 1 : 0;
+            s->initial_defect = (DT)sqrt((DT)rr);
+            s->current_defect = s->initial_defect;
+            s->temp = (DT)rz;
+        }
+    }
+
+    // alpha = (r.z) / (d.t), t = A d computed by cg_spmv_kernel
+    template <typename DT>
+    __global__ __launch_bounds__(256) void jac_spmv_finish(const double * partials, unsigned nb, CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        double dt, unused;
+        k::sum_partials2(partials, nb, dt, unused);
+        if (threadIdx.x == 0)
+            s->alpha = div_guard<DT>(s->temp, (DT)dt);
+    }
+
+    // x += alpha d; r -= alpha t; z = dd * r; partials of z.r and r.r
+    template <typename DT>
+    __global__ __launch_bounds__(256) void jac_update_kernel(DT * __restrict__ x, DT * __restrict__ r,
+            DT * __restrict__ z, const DT * __restrict__ d, const DT * __restrict__ t, const DT * __restrict__ dd,
+            int64_t n, double * partials, const CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        const DT alpha = s->alpha;
+        const DT malpha = -alpha;
+        double acc_zr = 0.0, acc_rr = 0.0;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        {
+            x[i] = x[i] + alpha * d[i];
+            const DT ri = r[i] + malpha * t[i];
+            r[i] = ri;
+            const DT zi = dd[i] * ri;
+            z[i] = zi;
+            acc_zr += (double)zi * (double)ri;
+            acc_rr += (double)ri * (double)ri;
+        }
+        k::put_partials2(acc_zr, acc_rr, partials, blockIdx.x);
+    }
+
+    // beta = (z.r) / (r.z)_old; the stopping test ||r|| < eps ||r0|| or the cap
+    template <typename DT>
+    __global__ __launch_bounds__(256) void jac_update_finish(const double * partials, unsigned nb,
+                                                             CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        double zr, rr;
+        k::sum_partials2(partials, nb, zr, rr);
+        if (threadIdx.x != 0)
+            return;
+        s->talpha = div_guard<DT>((DT)zr, s->temp);
+        s->temp = (DT)zr;
+        ++s->iters;
+        const DT cur = (DT)sqrt((DT)rr);
+        s->current_defect = cur;
+        if (cur < s->initial_defect * s->eps_relative)
+        {
+            s->converged = 1;
+            s->run = 0;
+        }
+        else if (s->iters == s->max_iters)
+            s->run = 0;
+    }
+
+    // d = z + beta d (ScaledSum(d, z, temp_0, beta) with temp_0 a copy of d)
+    template <typename DT>
+    __global__ __launch_bounds__(256) void jac_direction_kernel(DT * __restrict__ d, const DT * __restrict__ z,
+            int64_t n, const CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        const DT beta = s->talpha;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+            d[i] = beta * d[i] + z[i];
+    }
+
+    // One device array, freed with the object.
+    template <typename DT> struct DeviceArray
+    {
+        DT * p = nullptr;
+        explicit DeviceArray(uint64_t n)
+        {
+            if (hipMalloc((void **)&p, sizeof(DT) * (n ? n : 1)) != hipSuccess)
+                throw Error(HSOLVE_ERR_ALLOC, "solver workspace allocation failed");
+        }
+        ~DeviceArray()
+        {
+            if (p)
+                (void)hipFree(p);
+        }
+    };
+
     // Device workspace of one CG solve.
     template <typename DT> struct CgWork
     {
@@ -331,6 +467,60 @@ namespace
         return B;
     }
 
+    // ConjugateGradients<JAC> on an ELL handle: CgWork's r, p, v hold r, d, t;
+    // z has its own vector.
+    template <typename DT, typename Op>
+    void run_pcg_jac(const Op & A, uint64_t n, const DT * b, DT * x, const DT * dd, uint64_t max_iters, double eps,
+                     uint64_t * used_iters, double * defect_out, hipStream_t st, int * converged_out)
+    {
+        CgWork<DT> w(n);
+        DeviceArray<DT> zbuf(n);
+        DT * z = zbuf.p;
+        CgScalars<DT> h;
+        std::memset(&h, 0, sizeof(h));
+        h.eps_relative = (DT)eps;
+        h.max_iters = max_iters;
+        HIP_CHECK(hipMemcpyAsync(w.s, &h, sizeof(h), hipMemcpyHostToDevice, st));
+        const unsigned grid = std::min<unsigned>(grid_for(n, kBlock), kSolverBlocks);
+        hipLaunchKernelGGL((jac_init_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A, b, (const DT *)x, dd, w.r,
+                           z, w.p, (int64_t)n, w.partials);
+        hipLaunchKernelGGL((jac_init_finish<DT>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials, grid,
+                           w.s);
+        HIP_CHECK(hipGetLastError());
+        uint64_t enqueued = 0, chunk = 8;
+        while (true)
+        {
+            for (uint64_t c = 0; c < chunk && enqueued < max_iters; ++c, ++enqueued)
+            {
+                hipLaunchKernelGGL((cg_spmv_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A, (const DT *)w.p,
+                                   w.v, (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
+                hipLaunchKernelGGL((jac_spmv_finish<DT>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials,
+                                   grid, w.s);
+                hipLaunchKernelGGL((jac_update_kernel<DT>), dim3(grid), dim3(kBlock), 0, st, x, w.r, z,
+                                   (const DT *)w.p, (const DT *)w.v, dd, (int64_t)n, w.partials,
+                                   (const CgScalars<DT> *)w.s);
+                hipLaunchKernelGGL((jac_update_finish<DT>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials,
+                                   grid, w.s);
+                hipLaunchKernelGGL((jac_direction_kernel<DT>), dim3(grid), dim3(kBlock), 0, st, w.p, (const DT *)z,
+                                   (int64_t)n, (const CgScalars<DT> *)w.s);
+            }
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(&h, w.s, sizeof(h), hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            if (h.run == 0 || enqueued >= max_iters)
+                break;
+            if (chunk < 64)
+                chunk *= 2;
+        }
+        if (used_iters)
+            *used_iters = h.iters;
+        if (defect_out)
+            *defect_out = (h.initial_defect > 0) ? (double)h.current_defect / (double)h.initial_defect
+                                                 : (double)h.current_defect;
+        if (converged_out)
+            *converged_out = h.converged;
+    }
+
     template <typename DT, bool VAR>
     void run_cg_ell(const hsolve_ell_s * m, const DT * b, DT * x, const DT * P, uint64_t max_iters, double eps,
                     uint64_t * used, double * defect, hipStream_t st, int * conv)
@@ -395,6 +585,38 @@ int hsolve_cg_ell(hsolve_ell m, const void * b, void * x, uint64_t max_iters, do
     if (! conv && max_iters > 0)
     {
         set_last_error("cg_ell: max_iters reached");
+        return HSOLVE_ERR_NOT_CONVERGED;
+    }
+    HSOLVE_API_END
+}
+
+int hsolve_pcg_jac_ell(hsolve_ell m, const void * dd_inverted, const void * b, void * x, uint64_t max_iters,
+                       double eps, uint64_t * used_iters, double * defect_out, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(m && dd_inverted && b && x, HSOLVE_ERR_INVALID, "pcg_jac_ell: null pointer");
+    require(m->rows == m->cols, HSOLVE_ERR_SIZE, "pcg_jac_ell: matrix is not square");
+    int conv = 0;
+    hipStream_t st = as_stream(stream);
+    with_ell_types(m, [&](auto Aj, auto Arl) {
+        using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
+        using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
+        if (m->dtype)
+        {
+            EllOp<double, IT, LT> A{Aj, (const double *)m->Ax, Arl, m->stride, (uint32_t)m->threads};
+            run_pcg_jac<double>(A, m->rows, (const double *)b, (double *)x, (const double *)dd_inverted, max_iters,
+                                eps, used_iters, defect_out, st, &conv);
+        }
+        else
+        {
+            EllOp<float, IT, LT> A{Aj, (const float *)m->Ax, Arl, m->stride, (uint32_t)m->threads};
+            run_pcg_jac<float>(A, m->rows, (const float *)b, (float *)x, (const float *)dd_inverted, max_iters,
+                               eps, used_iters, defect_out, st, &conv);
+        }
+    });
+    if (! conv && max_iters > 0)
+    {
+        set_last_error("pcg_jac_ell: max_iters reached");
         return HSOLVE_ERR_NOT_CONVERGED;
     }
     HSOLVE_API_END

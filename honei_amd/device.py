@@ -308,6 +308,17 @@ def pcg_ell(A: EllMatrix, P: DeviceVector, b: DeviceVector, x: DeviceVector, max
     return used.value, defect.value
 
 
+def pcg_jac_ell(A: EllMatrix, dd_inverted: DeviceVector, b: DeviceVector, x: DeviceVector, max_iters=10000,
+                eps=1e-8, stream=None, raise_on_cap=True):
+    """ConjugateGradients<JAC> on ELL; returns (iterations, final relative defect)."""
+    used, defect = u64(), C.c_double()
+    rc = lib().hsolve_pcg_jac_ell(vp(A.handle), vp(dd_inverted.ptr), vp(b.ptr), vp(x.ptr), u64(max_iters),
+                                  float(eps), C.byref(used), C.byref(defect), vp(stream))
+    if rc != 5 or raise_on_cap:
+        check(rc, "pcg_jac_ell")
+    return used.value, defect.value
+
+
 def cg_ell_multi(A: EllMatrix, P, B: DeviceVector, X: DeviceVector, nrhs, max_iters=10000, eps=1e-8, stream=None,
                  raise_on_cap=True):
     """CGSolver<VAR> (P a DeviceVector) or <NONE> (P None) on every column of

@@ -304,6 +304,30 @@ namespace honei
         }
     };
 
+    template <> struct ConjugateGradients<tags::GPU::MI355X, methods::JAC>
+    {
+        /// PCG with the inverted diagonal dd_inverted on an ELL matrix
+        /// (conjugate_gradients.hh:1110-1155, kernel :885-913): stops at
+        /// ||r|| < eps*||r0|| or after max_iters; used_iters is set by either.
+        template <typename DT>
+        static void value(const SparseMatrixELL<DT> & A, const DenseVector<DT> & rhs, DenseVector<DT> & x,
+                          const DenseVector<DT> & dd_inverted, unsigned long max_iters, unsigned long & used_iters,
+                          DT eps_relative = 1e-8)
+        {
+            using namespace mi355x;
+            same_size(rhs.size(), A.rows());
+            same_size(x.size(), A.rows());
+            same_size(dd_inverted.size(), A.rows());
+            uint64_t u = 0;
+            double def = 0;
+            int st = hsolve_pcg_jac_ell(A.device_handle(), rd(dd_inverted), rd(rhs), rw(x), max_iters,
+                                        (double)eps_relative, &u, &def, stream());
+            if (st != HSOLVE_ERR_NOT_CONVERGED)
+                check(st, "ConjugateGradients JAC");
+            used_iters = u;
+        }
+    };
+
     /// Configuration and level data of the multigrid solvers (multigrid.hh:65-149).
     template <typename Prec_, typename DataLayout_> struct MGInfo
     {

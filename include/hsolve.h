@@ -282,6 +282,13 @@ int hsolve_cg_ell(hsolve_ell m, const void *b, void *x, uint64_t max_iters, doub
 int hsolve_pcg_ell(hsolve_ell m, const void *P, const void *b, void *x, uint64_t max_iters,
                    double eps, uint64_t *used_iters, double *defect_out, void *stream);
 
+/* ConjugateGradients<Tag, methods::JAC>::value(ELL A, rhs, x, dd_inverted,
+ * max_iters, used_iters, eps) (honei/math/conjugate_gradients.hh:1110-1155,
+ * cg_kernel :885-913): PCG with z = dd_inverted * r; stops when ||r|| <
+ * eps*||r0|| (no absolute test) or after max_iters. */
+int hsolve_pcg_jac_ell(hsolve_ell m, const void *dd_inverted, const void *b, void *x, uint64_t max_iters,
+                       double eps, uint64_t *used_iters, double *defect_out, void *stream);
+
 /* ---------------- many right-hand sides on one ELL matrix -------------------
  * A column-blocked multivector of nrhs <= 64 columns stores row i's entries
  * at [i*nrhs, i*nrhs + nrhs) (device memory, the handle's value type).  The

@@ -118,6 +118,9 @@ int or_cg_none_ell_##S(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,    
 int or_cg_var_ell_##S(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,              \
                       uint64_t stride, uint64_t threads, const T *P, const T *b, T *x,   \
                       uint64_t n, uint64_t max_iters, T eps, uint64_t *used_iters);      \
+int or_pcg_jac_ell_##S(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,             \
+                       uint64_t stride, uint64_t threads, const T *dd_inv, const T *rhs, \
+                       T *x, uint64_t n, uint64_t max_iters, T eps, uint64_t *used_iters);\
 int or_cg_konv_q1_##S(const T *const *bands, const T *rhs, T *x, uint64_t n,            \
                       double konv_rad, uint64_t max_iters, uint64_t *iters);             \
 int or_cg_konv_ell_##S(const or_ell_##S *M, const T *rhs, T *x, double konv_rad,          \

@@ -555,6 +555,51 @@ int FN(or_cg_var_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
     return converged ? 0 : 1;
 }
 
+/* ConjugateGradients<Tag, JAC>::value(ELL A, rhs, x, dd_inverted,
+ * max_iters, used_iters, eps): honei/math/conjugate_gradients.hh:1110-1155
+ * with the ELL cg_kernel :885-913.  Per step: beta = r.z; t = A d;
+ * alpha = beta / (d.t) (eps-guarded); x += alpha d; r -= alpha t;
+ * z = dd_inv * r; beta = (z.r) / beta (eps-guarded); d = z + beta d.
+ * Stops when ||r|| < eps ||r0|| (no absolute test) or after max_iters;
+ * used_iters is set by either.  Returns 0 when converged. */
+int FN(or_pcg_jac_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl, uint64_t stride,
+                       uint64_t threads, const T *dd_inv, const T *rhs, T *x, uint64_t n,
+                       uint64_t max_iters, T eps_relative, uint64_t *used_iters)
+{
+    FN(or_matrix) A = {1, 0, Aj, Ax, Arl, stride, threads};
+    T *r = (T *)malloc(sizeof(T) * n), *z = (T *)malloc(sizeof(T) * n);
+    T *d = (T *)malloc(sizeof(T) * n), *t0 = (T *)malloc(sizeof(T) * n);
+    int converged = 0;
+    FN(or_defect_any)(&A, r, rhs, x, n);
+    const T initial = FN(or_norm_l2)(r, n, 1);
+    FN(or_element_product3)(z, dd_inv, r, n);
+    memcpy(d, z, sizeof(T) * n);
+    *used_iters = 0;
+    for (uint64_t i = 0; i < max_iters; ++i) {
+        T beta = FN(or_dot)(r, z, n);
+        FN(or_apply)(&A, t0, d, n);
+        T alpha = FN(or_dot)(d, t0, n);
+        alpha = fabs(alpha) >= OR_EPS ? beta / alpha : beta / (T)OR_EPS;
+        FN(or_scaled_sum)(x, d, alpha, n);
+        FN(or_scaled_sum)(r, t0, -alpha, n);
+        FN(or_element_product3)(z, dd_inv, r, n);
+        const T zr = FN(or_dot)(z, r, n);
+        beta = fabs(beta) >= OR_EPS ? zr / beta : zr / (T)OR_EPS;
+        memcpy(t0, d, sizeof(T) * n);                    /* copy<Tag_>(d, temp_0) */
+        FN(or_scaled_sum3)(d, z, t0, beta, n);           /* d = z + beta temp_0 */
+        const T current = FN(or_norm_l2)(r, n, 1);
+        if (current < initial * eps_relative) {
+            *used_iters = i + 1;
+            converged = 1;
+            break;
+        }
+        if (i == max_iters - 1)
+            *used_iters = i + 1;
+    }
+    free(r); free(z); free(d); free(t0);
+    return converged ? 0 : 1;
+}
+
 /* ConjugateGradients<Tag, NONE>::value(Q1, rhs, x, konv_rad):
  * honei/math/conjugate_gradients.hh:490-526 with cg_kernel :108-151.
  * max_iters is a safety cap the reference does not have (0 = none). */

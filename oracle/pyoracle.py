@@ -92,6 +92,7 @@ class Oracle:
             getattr(L, f"or_cg_none_q1_{s}").argtypes = [vp, vp, vp, u64, u64, T, vp]
             getattr(L, f"or_cg_none_ell_{s}").argtypes = [vp, vp, vp, u64, u64, vp, vp, u64, u64, T, vp]
             getattr(L, f"or_cg_var_ell_{s}").argtypes = [vp, vp, vp, u64, u64, vp, vp, vp, u64, u64, T, vp]
+            getattr(L, f"or_pcg_jac_ell_{s}").argtypes = [vp, vp, vp, u64, u64, vp, vp, vp, u64, u64, T, vp]
             getattr(L, f"or_cg_konv_q1_{s}").argtypes = [vp, vp, vp, u64, C.c_double, u64, vp]
         L.or_rng_bits.restype = u64
         L.or_rng_bits.argtypes = [u64, u64, u64]
@@ -286,6 +287,15 @@ class Oracle:
             _p(P), _p(b), _p(x), u64(b.size), u64(max_iters), eps, C.byref(used))
         return x, used.value, st
 
+    def pcg_jac_ell(self, E, dd_inv, b, x0, max_iters=10000, eps=1e-8):
+        """ConjugateGradients<JAC> on ELL (conjugate_gradients.hh:1110-1155)."""
+        x = x0.copy()
+        used = u64()
+        st = getattr(self.lib, f"or_pcg_jac_ell_{_sfx(b.dtype)}")(
+            _p(E["Aj"]), _p(E["Ax"]), _p(E["Arl"]), u64(E["stride"]), u64(E["threads"]),
+            _p(dd_inv), _p(b), _p(x), u64(b.size), u64(max_iters), eps, C.byref(used))
+        return x, used.value, st
+
     def cg_konv_q1(self, bands, rhs, konv_rad, max_iters=0):
         x = np.zeros_like(rhs)
         it = u64()
@@ -368,6 +378,8 @@ class Reference:
             getattr(L, f"ref_cg_none_q1_{s}").argtypes = [vp, vp, vp, u64, u64, T]
             getattr(L, f"ref_cg_none_ell_{s}").argtypes = [u64, u64, u64, u64, vp, vp, vp, vp, u64, T]
             getattr(L, f"ref_cg_var_ell_{s}").argtypes = [u64, u64, u64, u64, vp, vp, vp, vp, vp, u64, T]
+            getattr(L, f"ref_pcg_jac_ell_{s}").restype = u64
+            getattr(L, f"ref_pcg_jac_ell_{s}").argtypes = [u64, u64, u64, u64, vp, vp, vp, vp, vp, u64, T]
             getattr(L, f"ref_cg_konv_q1_{s}").argtypes = [vp, vp, vp, u64, C.c_double]
         L.ref_mg_mixed_q1.argtypes = [vp, vp, vp, vp, u64, u64, u64, C.c_double, u64, C.c_double,
                                       vp, vp, vp]
@@ -466,6 +478,13 @@ class Reference:
         used = getattr(self.lib, f"ref_cg_var_ell_{_sfx(b.dtype)}")(
             u64(E["rows"]), u64(E["stride"]), u64(E["ncpr"]), u64(E["threads"]), _p(E["Aj"]),
             _p(E["Ax"]), _p(P), _p(b), _p(x), u64(max_iters), eps)
+        return x, used
+
+    def pcg_jac_ell(self, E, dd_inv, b, x0, max_iters=10000, eps=1e-8):
+        x = x0.copy()
+        used = getattr(self.lib, f"ref_pcg_jac_ell_{_sfx(b.dtype)}")(
+            u64(E["rows"]), u64(E["stride"]), u64(E["ncpr"]), u64(E["threads"]), _p(E["Aj"]),
+            _p(E["Ax"]), _p(dd_inv), _p(b), _p(x), u64(max_iters), eps)
         return x, used
 
     def cg_konv_q1(self, bands, rhs, konv_rad):

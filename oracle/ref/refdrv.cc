@@ -231,6 +231,19 @@ uint64_t ref_cg_var_ell_##S(uint64_t rows, uint64_t stride, uint64_t ncpr, uint6
     return used;                                                                                           \
 }                                                                                                          \
                                                                                                            \
+uint64_t ref_pcg_jac_ell_##S(uint64_t rows, uint64_t stride, uint64_t ncpr, uint64_t threads,            \
+        const uint64_t * Aj, const DT * Ax, const DT * dd, const DT * b, DT * x, uint64_t max_iters,       \
+        DT eps)                                                                                            \
+{                                                                                                          \
+    SparseMatrixELL<DT> A(ell<DT>(rows, rows, stride, ncpr, threads, Aj, Ax));                            \
+    DenseVector<DT> bv(dv(b, rows)), xv(dv(x, rows)), ddv(dv(dd, rows));                                    \
+    unsigned long used(0);                                                                                 \
+    ConjugateGradients<tags::CPU, methods::JAC>::value(A, bv, xv, ddv, (unsigned long)max_iters, used,    \
+                                                       eps);                                               \
+    out(x, xv);                                                                                            \
+    return used;                                                                                           \
+}                                                                                                          \
+                                                                                                           \
 void ref_cg_konv_q1_##S(const DT * const * bands, const DT * rhs, DT * x, uint64_t n, double konv_rad)    \
 {                                                                                                          \
     BandedMatrixQx<Q1Type, DT> A(q1(bands, n));                                                            \

@@ -371,6 +371,27 @@ static void test_cg()
     TEST_CHECK(bit_equal(cx, ccx.data()));
 }
 
+template <typename DT> static void test_pcg_jac()
+{
+    // ConjugateGradients<JAC> on the FEATFLOW2 fixture of conjugate_gradients_TEST.cc:620-700
+    SparseMatrixELL<DT> E(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/l2/area51_full_0.ell", DT(0)));
+    DenseVector<DT> rhs(VectorIO<io_formats::EXP>::read_vector(g_dir + "/l2/area51_rhs_0", DT(0)));
+    DenseVector<DT> x(VectorIO<io_formats::EXP>::read_vector(g_dir + "/l2/area51_init_0", DT(0)));
+    DenseVector<DT> sol(VectorIO<io_formats::EXP>::read_vector(g_dir + "/l2/area51_sol_0", DT(0)));
+    DenseVector<DT> dd(E.rows(), DT(0));
+    for (unsigned long i = 0; i < E.rows(); ++i)
+        dd[i] = DT(1) / E(i, i);
+    unsigned long used = 0;
+    ConjugateGradients<Tag, methods::JAC>::value(E, rhs, x, dd, 10000ul, used, DT(1e-8));
+    const double tol = sizeof(DT) == 8 ? 3e-5 : 3e-3;
+    // eps 1e-8 lies below FP32's reach: the float solve may run to max_iters,
+    // as in the reference test, which checks only the solution
+    bool ok = used > 0 && (sizeof(DT) == 4 || used < 10000);
+    for (unsigned long i = 0; i < E.rows(); ++i)
+        ok = ok && std::fabs(double(x[i]) - double(sol[i])) <= tol;
+    TEST_CHECK(ok);
+}
+
 static void test_multigrid_mixed()
 {
     // poisson_mg_mixed_banded_TEST.cc:243-427, with the .bin hierarchy
@@ -863,6 +884,8 @@ int main(int argc, char ** argv)
         {"transfers_and_jacobi<double>", test_transfers_and_jacobi<double>},
         {"transfers_and_jacobi<float>", test_transfers_and_jacobi<float>},
         {"cg", test_cg},
+        {"pcg_jac<double>", test_pcg_jac<double>},
+        {"pcg_jac<float>", test_pcg_jac<float>},
         {"multigrid_mixed", test_multigrid_mixed},
         {"lock_semantics", test_lock_semantics},
         {"multigrid_fixed", test_multigrid_fixed},

@@ -124,3 +124,50 @@ def test_mg_mixed_poisson_levels(hip, oracle, maxl, minl):
     assert _rel(xd.get(), ref["x"]) <= 1e-6
     assert info["final_defect"] < (1e-10 * 1.0000001) * info["initial_defect"] or \
         info["final_defect"] < float(np.float32(1e-10)) * info["initial_defect"]
+
+
+def _jac_inverse(E, dtype):
+    n = E["rows"]
+    Aj = E["Aj"].reshape(E["ncpr"], E["stride"])[:, :n]
+    Ax = E["Ax"].reshape(E["ncpr"], E["stride"])[:, :n]
+    diag = np.where((Aj == np.arange(n)) & (Ax != 0), Ax, 0).sum(axis=0).astype(dtype)
+    return (dtype(1) / diag).astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_pcg_jac_area51(hip, oracle, dtype):
+    """conjugate_gradients_TEST.cc:620-700: ConjugateGradients<JAC> on l2/area51_full_0.ell
+    with eps 1e-8 against FEATFLOW2's solution (3e-5 FP64, 3e-3 FP32)."""
+    F = load_ell("l2/area51_full_0.ell")
+    E = dict(F)
+    E["Arl"] = oracle.ell_row_length(F["rows"], F["stride"], 1, F["Ax"])
+    E["Ax"] = F["Ax"].astype(dtype)
+    dd = _jac_inverse(E, dtype)
+    rhs = load_exp("l2/area51_rhs_0").astype(dtype)
+    init = load_exp("l2/area51_init_0").astype(dtype)
+    sol = load_exp("l2/area51_sol_0")
+    x_ref, it_ref, _ = oracle.pcg_jac_ell(E, dd, rhs, init, 10000, dtype(1e-8))
+    M = hip.EllMatrix.from_dict(E)
+    xd = hip.DeviceVector.from_numpy(init)
+    used, _ = hip.pcg_jac_ell(M, hip.DeviceVector.from_numpy(dd), hip.DeviceVector.from_numpy(rhs), xd, 10000,
+                              1e-8, raise_on_cap=False)
+    x = xd.get().astype(np.float64)
+    assert abs(used - it_ref) <= 1, (used, it_ref)
+    assert _rel(x, x_ref.astype(np.float64)) <= (1e-6 if dtype == np.float64 else 1e-4)
+    assert np.max(np.abs(x - sol)) <= (3e-5 if dtype == np.float64 else 3e-3)
+
+
+def test_pcg_jac_diffusion_level7(hip, oracle):
+    """ConjugateGradients<JAC> on the variable-coefficient diffusion matrix (129^2 nodes):
+    counts within +-1 of the oracle and the solution within 1e-6 relative."""
+    D = P.diffusion_ell(7)
+    dd = _jac_inverse(D, np.float64)
+    b = P.rhs_diffusion(7, 3)
+    x_ref, it_ref, _ = oracle.pcg_jac_ell(D, dd, b, np.zeros(D["rows"]), 20000, 1e-8)
+    M = hip.EllMatrix.from_dict(D)
+    xd = hip.DeviceVector.from_numpy(np.zeros(D["rows"]))
+    used, defect = hip.pcg_jac_ell(M, hip.DeviceVector.from_numpy(dd), hip.DeviceVector.from_numpy(b), xd, 20000,
+                                   1e-8)
+    assert abs(used - it_ref) <= 1, (used, it_ref)
+    assert defect <= 1e-8
+    assert _rel(xd.get(), x_ref) <= 1e-6

@@ -123,3 +123,36 @@ def test_mg_mixed_bin_fixture(oracle, reference):
     r2 = reference.mg_mixed_q1(F["bands"], F["b"], np.zeros(F["n"]), a32, min_level=1, max_level=6)
     assert (r1["outer"], r1["inner"]) == (r2["outer"], r2["inner"])
     assert np.array_equal(r1["x"], r2["x"])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_pcg_jac_matches_reference_and_featflow(oracle, reference, dtype):
+    """ConjugateGradients<JAC> on ELL: the oracle is the compiled reference bit
+    for bit (iterations included) on the area51 fixture of
+    conjugate_gradients_TEST.cc:620-700 and on the diffusion matrix, and meets
+    FEATFLOW2's solution within that test's eps (3e-5 FP64, 3e-3 FP32)."""
+    F = load_ell("l2/area51_full_0.ell")
+    E = dict(F)
+    E["Arl"] = oracle.ell_row_length(F["rows"], F["stride"], 1, F["Ax"])
+    E["Ax"] = F["Ax"].astype(dtype)
+    n = F["rows"]
+    Aj = F["Aj"].reshape(F["ncpr"], F["stride"])[:, :n]
+    Ax = F["Ax"].reshape(F["ncpr"], F["stride"])[:, :n]
+    diag = np.where((Aj == np.arange(n)) & (Ax != 0), Ax, 0).sum(axis=0).astype(dtype)
+    dd = (dtype(1) / diag).astype(dtype)
+    rhs = load_exp("l2/area51_rhs_0").astype(dtype)
+    init = load_exp("l2/area51_init_0").astype(dtype)
+    sol = load_exp("l2/area51_sol_0")
+    eps = dtype(1e-8)
+    x1, it1, st = oracle.pcg_jac_ell(E, dd, rhs, init, 10000, eps)
+    x2, it2 = reference.pcg_jac_ell(E, dd, rhs, init, 10000, eps)
+    assert it1 == it2 and np.array_equal(x1, x2)
+    assert np.abs(x1.astype(np.float64) - sol).max() <= (3e-5 if dtype == np.float64 else 3e-3)
+    D = P.diffusion_ell(5)
+    D["Ax"] = D["Ax"].astype(dtype)
+    ddd = (dtype(1) / np.where(D["Aj"].reshape(D["ncpr"], D["stride"]) == np.arange(D["stride"]),
+                               D["Ax"].reshape(D["ncpr"], D["stride"]), 0).sum(axis=0)[:D["rows"]]).astype(dtype)
+    b = P.rhs_diffusion(5, 0).astype(dtype)
+    y1, j1, _ = oracle.pcg_jac_ell(D, ddd, b, np.zeros(D["rows"], dtype), 10000, dtype(1e-6))
+    y2, j2 = reference.pcg_jac_ell(D, ddd, b, np.zeros(D["rows"], dtype), 10000, dtype(1e-6))
+    assert j1 == j2 and np.array_equal(y1, y2)
