@@ -103,6 +103,32 @@ def spmv_case(hip, P, E, steps, warmup, defect=False):
     return out
 
 
+def csr_case(hip, P, E, steps, warmup, defect=False):
+    """Product / Defect of the same matrix stored as SparseMatrixCSR
+    (blocksize 1, uint32 offsets and columns in the handle)."""
+    n = E["rows"]
+    M = P.ell_to_csr(E)
+    dtype = M["Ax"].dtype
+    s = dtype.itemsize
+    z = M["nblocks"]
+    H = hip.CsrMatrix.from_dict(M)
+    del M
+    x = hip.DeviceVector.from_numpy(
+        P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype))
+    y = hip.DeviceVector.zeros(n, dtype)
+    rhs = None
+    if defect:
+        rhs = hip.DeviceVector.from_numpy(
+            P.rng_uniform(P.SEED, P.S_RHS, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype))
+    ms = timed(lambda: H.spmv(x, y, rhs=rhs), steps, warmup)
+    nbytes = P.csr_spmv_bytes(n, z, s) + (n * s if defect else 0)
+    out = dict(rows=n, nnz=z, ms=ms, bytes=nbytes, GBps=nbytes / (ms * 1e-3) / 1e9,
+               device_bytes=H.device_bytes)
+    out["frac_hbm"] = out["GBps"] / HBM_PEAK_GBS
+    del H, x, y, rhs
+    return out
+
+
 def q1_case(hip, P, side, dtype, steps, warmup):
     n = side * side
     A = hip.Q1Matrix(P.matrix_a_bands(side, dtype=dtype))
@@ -326,9 +352,24 @@ def main():
     ap.add_argument("--diffusion-only", type=int, default=0, metavar="LEVEL",
                     help="only run the 64-RHS variable-coefficient diffusion case on the 2^LEVEL+1 grid")
     ap.add_argument("--no-pcg", action="store_true", help="with --diffusion-only: MG-IR leg only")
+    ap.add_argument("--csr-only", action="store_true",
+                    help="SparseMatrixCSR product / defect on Matrix A (FP64, FP32) and Matrix B (FP64) only")
     ap.add_argument("--pcg-max", type=int, default=200000,
                     help="with --diffusion-only: iteration cap of the PCG leg (a short cap for profiling)")
     args = ap.parse_args()
+    if args.csr_only:
+        from honei_amd import device as hip
+        from honei_amd import problems as P
+        E = P.q1_to_ell(P.matrix_a_bands(args.side, dtype=np.float64))
+        out = {"csr_spmv_matrix_a_f64": csr_case(hip, P, E, args.steps, args.warmup),
+               "csr_defect_matrix_a_f64": csr_case(hip, P, E, args.steps, args.warmup, defect=True)}
+        E["Ax"] = E["Ax"].astype(np.float32)
+        out["csr_spmv_matrix_a_f32"] = csr_case(hip, P, E, args.steps, args.warmup)
+        del E
+        EB = P.matrix_b_ell(1 << 24)
+        out["csr_spmv_matrix_b_f64"] = csr_case(hip, P, EB, args.steps, args.warmup)
+        print(json.dumps(out))
+        return
     if args.diffusion_only:
         from honei_amd import device as hip
         from honei_amd import problems as P
@@ -379,11 +420,13 @@ def main():
         extra["ell_spmv_matrix_a_f32"] = spmv_case(hip, P, E32, args.steps, args.warmup)
         extra["ell_defect_matrix_a_f32"] = spmv_case(hip, P, E32, args.steps, args.warmup, defect=True)
         del E32
+        extra["csr_spmv_matrix_a_f64"] = csr_case(hip, P, E64, args.steps, args.warmup)
         log("matrix A extras done")
         t1 = time.perf_counter()
         EB = P.matrix_b_ell(1 << 24)
         log(f"Matrix B built in {time.perf_counter() - t1:.1f}s")
         extra["ell_spmv_matrix_b_f64"] = spmv_case(hip, P, EB, args.steps, args.warmup)
+        extra["csr_spmv_matrix_b_f64"] = csr_case(hip, P, EB, args.steps, args.warmup)
         EB["Ax"] = EB["Ax"].astype(np.float32)
         extra["ell_spmv_matrix_b_f32"] = spmv_case(hip, P, EB, args.steps, args.warmup)
         del EB

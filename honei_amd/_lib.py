@@ -118,6 +118,15 @@ _SIGS = {
     "hsolve_pcg_ell": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_pcg_jac_ell": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_ell_spmm": [vp, u64, vp, vp, vp, vp],
+    "hsolve_product_csr_dv_float": [vp, vp, vp, vp, vp, u64, u64, u64, u64, vp],
+    "hsolve_product_csr_dv_double": [vp, vp, vp, vp, vp, u64, u64, u64, u64, vp],
+    "hsolve_defect_csr_dv_float": [vp, vp, vp, vp, vp, vp, u64, u64, u64, vp],
+    "hsolve_defect_csr_dv_double": [vp, vp, vp, vp, vp, vp, u64, u64, u64, vp],
+    "hsolve_csr_create": [C.c_int, u64, u64, u64, vp, vp, vp, u64, C.POINTER(vp)],
+    "hsolve_csr_destroy": [vp],
+    "hsolve_csr_info": [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64),
+                        C.POINTER(u64)],
+    "hsolve_csr_spmv": [vp, vp, vp, vp, u64, u64, vp],
     "hsolve_cg_ell_multi": [vp, u64, vp, vp, vp, u64, dbl, vp, vp, vp],
     "hsolve_rhs_uniform_multi": [vp, u64, u64, u64, u64, vp, u64, u64, vp],
     "hsolve_mg_create": [C.POINTER(MGParams), vp, vp, vp, C.POINTER(vp)],

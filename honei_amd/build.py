@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libhsolve.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["ops.hip", "ell.hip", "solvers.hip", "mg_engine.hip", "mg_solve.hip", "serve.hip", "mrhs.hip"]
+SOURCES = ["ops.hip", "ell.hip", "solvers.hip", "mg_engine.hip", "mg_solve.hip", "serve.hip", "mrhs.hip", "csr.hip"]
 
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

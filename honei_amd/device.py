@@ -176,6 +176,44 @@ class EllMatrix:
             pass
 
 
+class CsrMatrix:
+    """SparseMatrixCSR<DT> (blocked CSR: Ar block offsets, Aj block start
+    columns, Ax `bs` values per block) uploaded into a device handle."""
+
+    def __init__(self, rows, columns, blocksize, Ar, Aj, Ax, used_elements=0):
+        Ar = np.ascontiguousarray(Ar, np.uint64)
+        Aj = np.ascontiguousarray(Aj, np.uint64)
+        Ax = np.ascontiguousarray(Ax)
+        assert Ar.size == rows + 1 and Aj.size >= int(Ar[rows]) and Ax.size >= int(Ar[rows]) * blocksize
+        self.dtype = Ax.dtype
+        h = vp()
+        check(lib().hsolve_csr_create(_DT[self.dtype], u64(rows), u64(columns), u64(blocksize),
+                                      Ar.ctypes.data_as(vp), Aj.ctypes.data_as(vp), Ax.ctypes.data_as(vp),
+                                      u64(used_elements), C.byref(h)), "csr_create")
+        self.handle = h.value
+        vals = [u64() for _ in range(6)]
+        check(lib().hsolve_csr_info(vp(self.handle), *[C.byref(v) for v in vals]), "csr_info")
+        (self.rows, self.columns, self.blocksize, self.nblocks, self.used,
+         self.device_bytes) = (v.value for v in vals)
+
+    @classmethod
+    def from_dict(cls, M):
+        return cls(M["rows"], M["cols"], M["bs"], M["Ar"], M["Aj"], M["Ax"], M.get("used", 0))
+
+    def spmv(self, x, y, rhs=None, row_start=0, row_end=0, stream=None):
+        """y = A x (or rhs - A x) on rows [row_start, row_end), row_end 0 = all rows."""
+        check(lib().hsolve_csr_spmv(vp(self.handle), vp(x.ptr), vp(y.ptr), vp(rhs.ptr) if rhs else None,
+                                    u64(row_start), u64(row_end), vp(stream)), "csr_spmv")
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().hsolve_csr_destroy(vp(self.handle))
+                self.handle = None
+        except Exception:
+            pass
+
+
 class Event:
     """hipEvent_t for timing work enqueued on a stream."""
 

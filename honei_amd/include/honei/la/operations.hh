@@ -14,6 +14,7 @@
 #include <honei/backends/mi355x/abi.hh>
 #include <honei/la/banded_matrix_qx.hh>
 #include <honei/la/dense_vector.hh>
+#include <honei/la/sparse_matrix_csr.hh>
 #include <honei/la/sparse_matrix_ell.hh>
 
 namespace honei
@@ -78,6 +79,36 @@ namespace honei
 
         template <typename DT>
         static DenseVector<DT> value(const SparseMatrixELL<DT> & a, const DenseVector<DT> & b)
+        {
+            DenseVector<DT> result(a.rows());
+            return value(result, a, b);
+        }
+
+        /// y = A x (blocked CSR) on rows [row_start, row_end) (row_end 0: all
+        /// rows; the others keep their values); product.hh:208-245,
+        /// product-cuda.cc:643-700
+        template <typename DT>
+        static DenseVector<DT> & value(DenseVector<DT> & result, const SparseMatrixCSR<DT> & a, const DenseVector<DT> & b,
+                                       unsigned long row_start = 0, unsigned long row_end = 0)
+        {
+            using namespace mi355x;
+            if (b.size() != a.columns())
+                throw VectorSizeDoesNotMatch(b.size(), a.columns());
+            if (a.rows() != result.size())
+                throw VectorSizeDoesNotMatch(a.rows(), result.size());
+            if (b.memid() == result.memid())
+                throw InternalError("Product: result must not share storage with the operand");
+            if (row_end == 0)
+                row_end = a.rows();
+            const bool all = row_start == 0 && row_end == a.rows();
+            check(hsolve_csr_spmv(a.device_handle(), rd(b), all ? wr(result) : rw(result), nullptr, row_start, row_end,
+                                  stream()),
+                  "Product CSR");
+            return result;
+        }
+
+        template <typename DT>
+        static DenseVector<DT> value(const SparseMatrixCSR<DT> & a, const DenseVector<DT> & b)
         {
             DenseVector<DT> result(a.rows());
             return value(result, a, b);

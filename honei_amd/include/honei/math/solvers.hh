@@ -1,6 +1,6 @@
 // honei/math/solvers.hh -- Defect, transfers, smoother, CG variants and
 // mixed-precision multigrid for tags::GPU::MI355X, with HONEI's interfaces:
-//   Defect<Tag>                       honei/math/defect.hh:42-768
+//   Defect<Tag>                       honei/math/defect.hh:42-768 (ELL, CSR, Q1)
 //   ApplyDirichletBoundaries<Tag>     honei/math/apply_dirichlet_boundaries.hh:28-85
 //   Restriction/Prolongation<Tag, M>  honei/math/restriction.hh:40-200, prolongation.hh:41-170
 //   Jacobi<Tag> (MG overloads)        honei/math/jacobi.hh:398-523
@@ -46,6 +46,31 @@ namespace honei
             const DT * pr = rd(rhs);
             check(hsolve_ell_spmv(a.device_handle(), rd(x), r.memid() == rhs.memid() ? rw(r) : wr(r), pr, stream()),
                   "Defect ELL");
+            return r;
+        }
+
+        /// r = rhs - A x (blocked CSR) on rows [row_start, row_end) (row_end
+        /// 0: all rows); defect.hh:386-418, defect-cuda.cc:309-390, :660-725
+        template <typename DT>
+        static DenseVector<DT> & value(DenseVector<DT> & r, const DenseVector<DT> & rhs, const SparseMatrixCSR<DT> & a,
+                                       const DenseVector<DT> & x, unsigned long row_start = 0,
+                                       unsigned long row_end = 0)
+        {
+            using namespace mi355x;
+            if (x.size() != a.columns())
+                throw VectorSizeDoesNotMatch(x.size(), a.columns());
+            if (rhs.size() != a.rows())
+                throw VectorSizeDoesNotMatch(rhs.size(), a.rows());
+            same_size(r.size(), a.rows());
+            if (x.memid() == r.memid())
+                throw InternalError("Defect: result must not share storage with x");
+            if (row_end == 0)
+                row_end = a.rows();
+            const bool all = row_start == 0 && row_end == a.rows();
+            const DT * pr = rd(rhs);
+            check(hsolve_csr_spmv(a.device_handle(), rd(x), (r.memid() == rhs.memid() || ! all) ? rw(r) : wr(r), pr,
+                                  row_start, row_end, stream()),
+                  "Defect CSR");
             return r;
         }
 

@@ -263,6 +263,29 @@ def rhs_diffusion(level, j, seed=SEED, stream=S_RHS):
     return b
 
 
+def ell_to_csr(E):
+    """Blocked CSR with blocksize 1 of an ELL matrix with threads = 1 whose
+    rows keep their columns ascending (what q1_to_ell and matrix_b_ell
+    build): the slots holding a non-zero value, row by row, as
+    SparseMatrixCSR(SparseMatrixELL) stores them; a dict rows/cols/bs/Ar/Aj/Ax."""
+    assert E.get("threads", 1) == 1
+    n, stride, ncpr = E["rows"], E["stride"], E["ncpr"]
+    Aj = np.asarray(E["Aj"]).reshape(ncpr, stride)[:, :n].T
+    Ax = np.asarray(E["Ax"]).reshape(ncpr, stride)[:, :n].T
+    keep = Ax != 0
+    counts = keep.sum(axis=1)
+    Ar = np.zeros(n + 1, np.uint64)
+    np.cumsum(counts, out=Ar[1:])
+    return dict(rows=n, cols=E.get("cols", n), bs=1, Ar=Ar, Aj=Aj[keep].astype(np.uint64), Ax=Ax[keep],
+                nblocks=int(Ar[n]), used=int(Ar[n]))
+
+
+def csr_spmv_bytes(n, z, s):
+    """Algorithmic bytes of y = A x from a CSR matrix with 32-bit offsets and
+    columns: z(s + 4) + n(2s + 4)."""
+    return z * (s + 4) + n * (2 * s + 4)
+
+
 def ell_nnz(E):
     return int(np.asarray(E["Arl"], np.uint64).sum()) * int(E.get("threads", 1))
 

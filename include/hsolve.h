@@ -97,6 +97,28 @@ int hsolve_defect_smell_dv_double(const void *rhs, void *result, const void *Aj,
                                   const void *Arl, const void *b, uint64_t row_start,
                                   uint64_t row_end, uint64_t num_cols_per_row, uint64_t stride,
                                   uint64_t threads, void *stream);
+/* Blocked-CSR SpMV y = A x on rows [row_start, row_end) (row_end 0 = all
+ * rows) in HONEI's SparseMatrixCSR layout: Ar uint64 block offsets (rows + 1),
+ * Aj uint64 block start columns, Ax atomicsize values per block; positions at
+ * or past `columns` (the length of x) add nothing.  Row sums in the CPU
+ * order, bit-identical to Product<tags::CPU>.  Replaces
+ * cuda_product_csr_dv_{float,double} (operations.hh:95-99, product.cu:257-296,
+ * :404-447); the CUDA launcher's blocksize is dropped and `columns` is added
+ * (the CUDA kernel leans on texture bounds for that check). */
+int hsolve_product_csr_dv_float(const void *x, void *y, const void *Aj, const void *Ax,
+                                const void *Ar, uint64_t row_start, uint64_t row_end,
+                                uint64_t atomicsize, uint64_t columns, void *stream);
+int hsolve_product_csr_dv_double(const void *x, void *y, const void *Aj, const void *Ax,
+                                 const void *Ar, uint64_t row_start, uint64_t row_end,
+                                 uint64_t atomicsize, uint64_t columns, void *stream);
+/* Blocked-CSR defect result = rhs - A b on rows [0, rows).  Replaces
+ * cuda_defect_csr_dv_{float,double} (operations.hh:49-53, defect.cu:260-301, :419-466). */
+int hsolve_defect_csr_dv_float(const void *rhs, void *result, const void *Aj, const void *Ax,
+                               const void *Ar, const void *b, uint64_t rows, uint64_t atomicsize,
+                               uint64_t columns, void *stream);
+int hsolve_defect_csr_dv_double(const void *rhs, void *result, const void *Aj, const void *Ax,
+                                const void *Ar, const void *b, uint64_t rows, uint64_t atomicsize,
+                                uint64_t columns, void *stream);
 /* Q1 banded SpMV y = A x, m = sqrt(size).  Replaces cuda_product_bmdv_q1_*
  * (product.cu:300-354); sums in tags::CPU order DD,LL,LD,LU,DL,DU,UL,UD,UU. */
 int hsolve_product_bmdv_q1_float(const void *ll, const void *ld, const void *lu, const void *dl,
@@ -223,6 +245,24 @@ int hsolve_cg_konv_q1_float(const void *const *bands9, const void *rhs, void *x,
 int hsolve_cg_konv_q1_double(const void *const *bands9, const void *rhs, void *x,
                              uint64_t size, double konv_rad, uint64_t max_iters, void *iters_dev,
                              void *stream);
+
+/* ---------------- device-resident CSR handle -------------------------------- */
+typedef struct hsolve_csr_s *hsolve_csr;
+/* Uploads a SparseMatrixCSR given in HONEI's host layout (uint64 Ar/Aj,
+ * blocksize values per block in Ax); offsets and columns are kept as uint32
+ * when the element count and columns + blocksize fit.  dtype: 0 = float,
+ * 1 = double.  used_elements is stored for hsolve_csr_info only.  Backs
+ * SparseMatrixCSR of the template layer (sparse_matrix_csr.hh:55-160). */
+int hsolve_csr_create(int dtype, uint64_t rows, uint64_t columns, uint64_t blocksize,
+                      const uint64_t *Ar_host, const uint64_t *Aj_host, const void *Ax_host,
+                      uint64_t used_elements, hsolve_csr *out);
+int hsolve_csr_destroy(hsolve_csr m);
+int hsolve_csr_info(hsolve_csr m, uint64_t *rows, uint64_t *columns, uint64_t *blocksize,
+                    uint64_t *nblocks, uint64_t *used_elements, uint64_t *device_bytes);
+/* y = A x, or y = rhs - A x when rhs != NULL, on rows [row_start, row_end)
+ * (row_end 0 = all rows); other rows of y are left as they are. */
+int hsolve_csr_spmv(hsolve_csr m, const void *x, void *y, const void *rhs, uint64_t row_start,
+                    uint64_t row_end, void *stream);
 
 /* ---------------- device-resident ELL handle -------------------------------- */
 typedef struct hsolve_ell_s *hsolve_ell;

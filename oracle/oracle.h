@@ -84,6 +84,15 @@ void or_ell_product_##S(T *y, const uint64_t *Aj, const T *Ax, const uint64_t *A
 void or_ell_defect_##S(T *r, const T *rhs, const uint64_t *Aj, const T *Ax,              \
                        const uint64_t *Arl, const T *x, uint64_t rows,                   \
                        uint64_t stride, uint64_t threads);                               \
+void or_csr_from_rows_##S(uint64_t rows, const uint64_t *rowptr, const uint64_t *col,    \
+                          const T *val, uint64_t bs, uint64_t *Ar, uint64_t *Aj, T *Ax, \
+                          uint64_t *nblocks, uint64_t *used);                           \
+void or_csr_product_##S(T *y, const uint64_t *Ar, const uint64_t *Aj, const T *Ax,      \
+                        const T *x, uint64_t columns, uint64_t bs,                      \
+                        uint64_t row_start, uint64_t row_end);                          \
+void or_csr_defect_##S(T *r, const T *rhs, const uint64_t *Ar, const uint64_t *Aj,      \
+                       const T *Ax, const T *x, uint64_t columns, uint64_t bs,          \
+                       uint64_t row_start, uint64_t row_end);                           \
 void or_q1_product_##S(T *y, const T *const *bands, const T *x, uint64_t n);             \
 void or_q1_defect_##S(T *r, const T *rhs, const T *const *bands, const T *x, uint64_t n);\
 T or_dot_##S(const T *x, const T *y, uint64_t n);                                        \

@@ -89,6 +89,82 @@ void FN(or_ell_defect)(T *r, const T *rhs, const uint64_t *Aj, const T *Ax,
     free(temp);
 }
 
+/* SparseMatrixCSR<T>(SparseMatrix) with csr::blocksize = bs:
+ * honei/la/sparse_matrix_csr-impl.hh:98-136.  The input is a row-wise
+ * sparse matrix (rowptr/col/val, columns ascending per row; stored zeros
+ * stay stored).  Each stored block starts at the column rounded down to a
+ * multiple of bs and takes the row's next entries whose columns fall on its
+ * bs consecutive positions; the other positions hold 0.  Outputs: Ar
+ * (rows + 1 block offsets), Aj (block start columns), Ax (bs values per
+ * block); *nblocks = Ar[rows], *used = stored entries.  Aj and Ax must hold
+ * rowptr[rows] and rowptr[rows] * bs elements. */
+void FN(or_csr_from_rows)(uint64_t rows, const uint64_t *rowptr, const uint64_t *col,
+                          const T *val, uint64_t bs, uint64_t *Ar, uint64_t *Aj, T *Ax,
+                          uint64_t *nblocks, uint64_t *used)
+{
+    uint64_t gi = 0, ue = 0, ax = 0;
+    for (uint64_t r = 0; r < rows; ++r) {
+        const uint64_t n = rowptr[r + 1] - rowptr[r];
+        const uint64_t *c = col + rowptr[r];
+        const T *v = val + rowptr[r];
+        Ar[r] = gi;
+        uint64_t i = 0;
+        while (i < n) {
+            const uint64_t start = c[i] - c[i] % bs;
+            uint64_t taken = 0;
+            Aj[gi] = start;
+            for (uint64_t k = 0; k < bs; ++k) {
+                if (i + taken < n && start + k == c[i + taken]) {
+                    Ax[ax++] = v[i + taken];
+                    ++ue;
+                    ++taken;
+                } else {
+                    Ax[ax++] = (T)0;
+                }
+            }
+            i += taken;
+            ++gi;
+        }
+    }
+    Ar[rows] = gi;
+    *nblocks = gi;
+    *used = ue;
+}
+
+/* Product<tags::CPU>::value(y, SparseMatrixCSR, x, row_start, row_end):
+ * honei/la/product.hh:208-245.  Positions at or past the last column
+ * (x has `columns` entries) are skipped; rows outside the range keep y. */
+void FN(or_csr_product)(T *y, const uint64_t *Ar, const uint64_t *Aj, const T *Ax,
+                        const T *x, uint64_t columns, uint64_t bs,
+                        uint64_t row_start, uint64_t row_end)
+{
+    for (uint64_t row = row_start; row < row_end; ++row) {
+        T sum = 0;
+        for (uint64_t i = Ar[row]; i < Ar[row + 1]; ++i)
+            for (uint64_t k = 0; k < bs; ++k)
+                if (Aj[i] + k < columns)
+                    sum += Ax[i * bs + k] * x[Aj[i] + k];
+        y[row] = sum;
+    }
+}
+
+/* Defect<tags::CPU>::value(r, rhs, SparseMatrixCSR, x, row_start, row_end):
+ * honei/math/defect.hh:386-418 (the row sum of the product, then
+ * r = rhs - sum). */
+void FN(or_csr_defect)(T *r, const T *rhs, const uint64_t *Ar, const uint64_t *Aj,
+                       const T *Ax, const T *x, uint64_t columns, uint64_t bs,
+                       uint64_t row_start, uint64_t row_end)
+{
+    for (uint64_t row = row_start; row < row_end; ++row) {
+        T sum = 0;
+        for (uint64_t i = Ar[row]; i < Ar[row + 1]; ++i)
+            for (uint64_t k = 0; k < bs; ++k)
+                if (Aj[i] + k < columns)
+                    sum += Ax[i * bs + k] * x[Aj[i] + k];
+        r[row] = rhs[row] - sum;
+    }
+}
+
 /* Product<tags::CPU>::value(y, BandedMatrixQx<Q1Type>, x):
  * honei/la/product.hh:370-403, summation order DD,LL,LD,LU,DL,DU,UL,UD,UU. */
 void FN(or_q1_product)(T *y, const T *const *bands, const T *x, uint64_t n)

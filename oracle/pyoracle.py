@@ -142,6 +142,68 @@ class Oracle:
             u64(E["stride"]), u64(E["threads"]))
         return r
 
+    # -- CSR ---------------------------------------------------------------
+    def csr_from_rows(self, rows, cols, rowptr, col, val, bs=1):
+        """SparseMatrixCSR(SparseMatrix) with csr::blocksize bs
+        (sparse_matrix_csr-impl.hh:98-136); a dict rows/cols/bs/Ar/Aj/Ax/nblocks/used."""
+        rowptr = np.ascontiguousarray(rowptr, np.uint64)
+        col = np.ascontiguousarray(col, np.uint64)
+        val = np.ascontiguousarray(val)
+        cap = max(int(rowptr[rows]), 1)
+        Ar = np.zeros(rows + 1, np.uint64)
+        Aj = np.zeros(cap, np.uint64)
+        Ax = np.zeros(cap * bs, val.dtype)
+        nb, used = u64(), u64()
+        getattr(self.lib, f"or_csr_from_rows_{_sfx(val.dtype)}")(
+            u64(rows), _p(rowptr), _p(col), _p(val), u64(bs), _p(Ar), _p(Aj), _p(Ax), C.byref(nb), C.byref(used))
+        n = nb.value
+        return dict(rows=rows, cols=cols, bs=bs, Ar=Ar, Aj=Aj[:n].copy(), Ax=Ax[:n * bs].copy(), nblocks=n,
+                    used=used.value)
+
+    @staticmethod
+    def ell_rows(E, cols=None):
+        """SparseMatrix(SparseMatrixELL) (sparse_matrix.hh:157-186): every slot
+        with a non-zero value, per row, columns ascending (a later slot with
+        the same column overwrites)."""
+        rows, stride, thr = E["rows"], E["stride"], E.get("threads", 1)
+        Aj, Ax = E["Aj"], E["Ax"]
+        rowptr, col, val = [0], [], []
+        for r in range(rows):
+            d = {}
+            for i in range(r * thr, Aj.size, stride):
+                for t in range(thr):
+                    if Ax[i + t] != 0:
+                        d[int(Aj[i + t])] = Ax[i + t]
+            for c in sorted(d):
+                col.append(c)
+                val.append(d[c])
+            rowptr.append(len(col))
+        return (np.asarray(rowptr, np.uint64), np.asarray(col, np.uint64), np.asarray(val, Ax.dtype))
+
+    def csr_from_ell(self, E, bs=1, cols=None):
+        """SparseMatrixCSR(SparseMatrixELL) (sparse_matrix_csr-impl.hh:85-96)."""
+        rowptr, col, val = self.ell_rows(E)
+        return self.csr_from_rows(E["rows"], cols if cols is not None else E.get("cols", E["rows"]),
+                                  rowptr, col, val, bs)
+
+    def csr_product(self, M, x, row_start=0, row_end=None, y=None):
+        """Product<tags::CPU>(y, SparseMatrixCSR, x, row_start, row_end) (product.hh:208-245)."""
+        y = np.zeros(M["rows"], x.dtype) if y is None else y.copy()
+        row_end = M["rows"] if row_end is None else row_end
+        getattr(self.lib, f"or_csr_product_{_sfx(x.dtype)}")(
+            _p(y), _p(M["Ar"]), _p(M["Aj"]), _p(M["Ax"]), _p(x), u64(M["cols"]), u64(M["bs"]),
+            u64(row_start), u64(row_end))
+        return y
+
+    def csr_defect(self, M, rhs, x, row_start=0, row_end=None, r=None):
+        """Defect<tags::CPU>(r, rhs, SparseMatrixCSR, x) (defect.hh:386-418)."""
+        r = np.zeros(M["rows"], x.dtype) if r is None else r.copy()
+        row_end = M["rows"] if row_end is None else row_end
+        getattr(self.lib, f"or_csr_defect_{_sfx(x.dtype)}")(
+            _p(r), _p(rhs), _p(M["Ar"]), _p(M["Aj"]), _p(M["Ax"]), _p(x), u64(M["cols"]), u64(M["bs"]),
+            u64(row_start), u64(row_end))
+        return r
+
     # -- Q1 ----------------------------------------------------------------
     def q1_product(self, bands, x):
         y = np.zeros_like(x)
@@ -486,6 +548,48 @@ class Reference:
             u64(E["rows"]), u64(E["stride"]), u64(E["ncpr"]), u64(E["threads"]), _p(E["Aj"]),
             _p(E["Ax"]), _p(dd_inv), _p(b), _p(x), u64(max_iters), eps)
         return x, used
+
+    def _csr_result(self, f, args, rows, cols, bs, dtype):
+        nb, used, axs = u64(), u64(), u64()
+        f(*args, None, None, None, C.byref(nb), C.byref(used), C.byref(axs))
+        Ar = np.zeros(rows + 1, np.uint64)
+        Aj = np.zeros(max(nb.value, 1), np.uint64)
+        Ax = np.zeros(max(axs.value, 1), dtype)
+        f(*args, _p(Ar), _p(Aj), _p(Ax), C.byref(nb), C.byref(used), C.byref(axs))
+        n = nb.value
+        return dict(rows=rows, cols=cols, bs=bs, Ar=Ar, Aj=Aj[:n].copy(), Ax=Ax[:axs.value].copy(), nblocks=n,
+                    used=used.value)
+
+    def csr_from_rows(self, rows, cols, rowptr, col, val, bs=1):
+        rowptr = np.ascontiguousarray(rowptr, np.uint64)
+        col = np.ascontiguousarray(col, np.uint64)
+        val = np.ascontiguousarray(val)
+        f = getattr(self.lib, f"ref_csr_from_rows_{_sfx(val.dtype)}")
+        return self._csr_result(f, (u64(rows), u64(cols), _p(rowptr), _p(col), _p(val), u64(bs)),
+                                rows, cols, bs, val.dtype)
+
+    def csr_from_ell(self, E, bs=1, cols=None):
+        cols = cols if cols is not None else E.get("cols", E["rows"])
+        f = getattr(self.lib, f"ref_csr_from_ell_{_sfx(E['Ax'].dtype)}")
+        return self._csr_result(f, (u64(E["rows"]), u64(cols), u64(E["stride"]), u64(E["ncpr"]),
+                                    u64(E.get("threads", 1)), _p(E["Aj"]), _p(E["Ax"]), u64(bs)),
+                                E["rows"], cols, bs, E["Ax"].dtype)
+
+    def csr_product(self, M, x, row_start=0, row_end=None, y=None):
+        y = np.zeros(M["rows"], x.dtype) if y is None else y.copy()
+        row_end = M["rows"] if row_end is None else row_end
+        getattr(self.lib, f"ref_csr_product_{_sfx(x.dtype)}")(
+            _p(y), u64(M["rows"]), u64(M["cols"]), u64(M["bs"]), u64(M["nblocks"]), _p(M["Ar"]), _p(M["Aj"]),
+            _p(M["Ax"]), _p(x), u64(row_start), u64(row_end))
+        return y
+
+    def csr_defect(self, M, rhs, x, row_start=0, row_end=None, r=None):
+        r = np.zeros(M["rows"], x.dtype) if r is None else r.copy()
+        row_end = M["rows"] if row_end is None else row_end
+        getattr(self.lib, f"ref_csr_defect_{_sfx(x.dtype)}")(
+            _p(r), _p(rhs), u64(M["rows"]), u64(M["cols"]), u64(M["bs"]), u64(M["nblocks"]), _p(M["Ar"]),
+            _p(M["Aj"]), _p(M["Ax"]), _p(x), u64(row_start), u64(row_end))
+        return r
 
     def cg_konv_q1(self, bands, rhs, konv_rad):
         x = np.zeros_like(rhs)

@@ -11,6 +11,7 @@
 #include <honei/la/banded_matrix_qx.hh>
 #include <honei/la/sparse_matrix.hh>
 #include <honei/la/sparse_matrix_ell.hh>
+#include <honei/la/sparse_matrix_csr.hh>
 #include <honei/la/product.hh>
 #include <honei/la/dot_product.hh>
 #include <honei/la/norm.hh>
@@ -334,6 +335,120 @@ int ref_mg_mixed_q1(const double * const * A64, const double * b, double * x,
     return (int)*outer;
 }
 
+}  // extern "C"
+
+/* ---- SparseMatrixCSR: construction (from a row-wise SparseMatrix or from an
+ * ELL matrix, csr::blocksize = bs), Product and Defect on row ranges. */
+namespace
+{
+    template <typename DT_> void csr_out(const SparseMatrixCSR<DT_> & c, uint64_t * Ar, uint64_t * Aj, DT_ * Ax,
+            uint64_t * nblocks, uint64_t * used, uint64_t * axsize)
+    {
+        const unsigned long nb(c.Ar()[c.rows()]);
+        *nblocks = nb;
+        *used = c.used_elements();
+        *axsize = c.Ax().size();
+        if (Ar != 0)
+        {
+            for (unsigned long i(0) ; i <= c.rows() ; ++i)
+                Ar[i] = c.Ar()[i];
+            for (unsigned long i(0) ; i < nb ; ++i)
+                Aj[i] = c.Aj()[i];
+            for (unsigned long i(0) ; i < c.Ax().size() ; ++i)
+                Ax[i] = c.Ax()[i];
+        }
+    }
+
+    template <typename DT_> void csr_from_rows(uint64_t rows, uint64_t cols, const uint64_t * rowptr,
+            const uint64_t * col, const DT_ * val, uint64_t bs, uint64_t * Ar, uint64_t * Aj, DT_ * Ax,
+            uint64_t * nblocks, uint64_t * used, uint64_t * axsize)
+    {
+        Configuration::instance()->set_value("csr::blocksize", (int)bs);
+        SparseMatrix<DT_> sm(rows, cols);
+        for (unsigned long r(0) ; r < rows ; ++r)
+            for (uint64_t i(rowptr[r]) ; i < rowptr[r + 1] ; ++i)
+                sm(r, col[i], val[i]);
+        SparseMatrixCSR<DT_> c(sm);
+        csr_out(c, Ar, Aj, Ax, nblocks, used, axsize);
+        Configuration::instance()->set_value("csr::blocksize", 1);
+    }
+
+    template <typename DT_> void csr_from_ell(uint64_t rows, uint64_t cols, uint64_t stride, uint64_t ncpr,
+            uint64_t threads, const uint64_t * eAj, const DT_ * eAx, uint64_t bs, uint64_t * Ar, uint64_t * Aj,
+            DT_ * Ax, uint64_t * nblocks, uint64_t * used, uint64_t * axsize)
+    {
+        Configuration::instance()->set_value("csr::blocksize", (int)bs);
+        SparseMatrixELL<DT_> e(ell<DT_>(rows, cols, stride, ncpr, threads, eAj, eAx));
+        SparseMatrixCSR<DT_> c(e);
+        csr_out(c, Ar, Aj, Ax, nblocks, used, axsize);
+        Configuration::instance()->set_value("csr::blocksize", 1);
+    }
+
+    template <typename DT_> SparseMatrixCSR<DT_> csr(uint64_t rows, uint64_t cols, uint64_t bs,
+            uint64_t nblocks, uint64_t used, const uint64_t * Ar, const uint64_t * Aj, const DT_ * Ax)
+    {
+        DenseVector<unsigned long> ar(rows + 1), aj(nblocks ? nblocks : 1, 0ul);
+        DenseVector<DT_> ax(nblocks ? nblocks * bs : 1, DT_(0));
+        for (unsigned long i(0) ; i <= rows ; ++i)
+            ar[i] = Ar[i];
+        for (unsigned long i(0) ; i < nblocks ; ++i)
+            aj[i] = Aj[i];
+        for (unsigned long i(0) ; i < nblocks * bs ; ++i)
+            ax[i] = Ax[i];
+        return SparseMatrixCSR<DT_>(bs, rows, cols, aj, ax, ar, used);
+    }
+
+    template <typename DT_> void csr_product(DT_ * y, uint64_t rows, uint64_t cols, uint64_t bs,
+            uint64_t nblocks, const uint64_t * Ar, const uint64_t * Aj, const DT_ * Ax, const DT_ * x,
+            uint64_t row_start, uint64_t row_end)
+    {
+        SparseMatrixCSR<DT_> A(csr<DT_>(rows, cols, bs, nblocks, 0, Ar, Aj, Ax));
+        DenseVector<DT_> xv(dv(x, cols)), yv(dv(y, rows));
+        Product<tags::CPU>::value(yv, A, xv, row_start, row_end);
+        out(y, yv);
+    }
+
+    template <typename DT_> void csr_defect(DT_ * r, const DT_ * rhs, uint64_t rows, uint64_t cols,
+            uint64_t bs, uint64_t nblocks, const uint64_t * Ar, const uint64_t * Aj, const DT_ * Ax,
+            const DT_ * x, uint64_t row_start, uint64_t row_end)
+    {
+        SparseMatrixCSR<DT_> A(csr<DT_>(rows, cols, bs, nblocks, 0, Ar, Aj, Ax));
+        DenseVector<DT_> xv(dv(x, cols)), rv(dv(r, rows)), bv(dv(rhs, rows));
+        Defect<tags::CPU>::value(rv, bv, A, xv, row_start, row_end);
+        out(r, rv);
+    }
+}
+
+extern "C"
+{
+#define REF_CSR(DT, S)                                                                                     \
+void ref_csr_from_rows_##S(uint64_t rows, uint64_t cols, const uint64_t * rowptr, const uint64_t * col,    \
+        const DT * val, uint64_t bs, uint64_t * Ar, uint64_t * Aj, DT * Ax, uint64_t * nblocks,             \
+        uint64_t * used, uint64_t * axsize)                                                                \
+{                                                                                                          \
+    csr_from_rows<DT>(rows, cols, rowptr, col, val, bs, Ar, Aj, Ax, nblocks, used, axsize);                \
+}                                                                                                          \
+void ref_csr_from_ell_##S(uint64_t rows, uint64_t cols, uint64_t stride, uint64_t ncpr, uint64_t threads,  \
+        const uint64_t * eAj, const DT * eAx, uint64_t bs, uint64_t * Ar, uint64_t * Aj, DT * Ax,          \
+        uint64_t * nblocks, uint64_t * used, uint64_t * axsize)                                            \
+{                                                                                                          \
+    csr_from_ell<DT>(rows, cols, stride, ncpr, threads, eAj, eAx, bs, Ar, Aj, Ax, nblocks, used, axsize);  \
+}                                                                                                          \
+void ref_csr_product_##S(DT * y, uint64_t rows, uint64_t cols, uint64_t bs, uint64_t nblocks,              \
+        const uint64_t * Ar, const uint64_t * Aj, const DT * Ax, const DT * x, uint64_t row_start,          \
+        uint64_t row_end)                                                                                  \
+{                                                                                                          \
+    csr_product<DT>(y, rows, cols, bs, nblocks, Ar, Aj, Ax, x, row_start, row_end);                        \
+}                                                                                                          \
+void ref_csr_defect_##S(DT * r, const DT * rhs, uint64_t rows, uint64_t cols, uint64_t bs,                 \
+        uint64_t nblocks, const uint64_t * Ar, const uint64_t * Aj, const DT * Ax, const DT * x,            \
+        uint64_t row_start, uint64_t row_end)                                                              \
+{                                                                                                          \
+    csr_defect<DT>(r, rhs, rows, cols, bs, nblocks, Ar, Aj, Ax, x, row_start, row_end);                    \
+}
+REF_CSR(double, f64)
+REF_CSR(float, f32)
+#undef REF_CSR
 }  // extern "C"
 
 /* Operator-list MG from the reference's own loader and templates:

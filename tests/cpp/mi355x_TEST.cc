@@ -6,6 +6,7 @@
 //
 // Usage: mi355x_TEST <fixture dir>   (exit status 0 when every check passes)
 #include <honei/la/product.hh>
+#include <honei/la/sparse_matrix_csr.hh>
 #include <honei/la/dot_product.hh>
 #include <honei/la/norm.hh>
 #include <honei/la/scaled_sum.hh>
@@ -369,6 +370,73 @@ static void test_cg()
     bands_host(C, cb);
     or_cg_konv_q1_f32(cb, cr.read_elements(), ccx.data(), 25, std::numeric_limits<float>::epsilon(), 0, &it);
     TEST_CHECK(bit_equal(cx, ccx.data()));
+}
+
+static void or_csr_product(double * y, const uint64_t * Ar, const uint64_t * Aj, const double * Ax, const double * x,
+                           uint64_t cols, uint64_t bs, uint64_t rs, uint64_t re)
+{ or_csr_product_f64(y, Ar, Aj, Ax, x, cols, bs, rs, re); }
+static void or_csr_product(float * y, const uint64_t * Ar, const uint64_t * Aj, const float * Ax, const float * x,
+                           uint64_t cols, uint64_t bs, uint64_t rs, uint64_t re)
+{ or_csr_product_f32(y, Ar, Aj, Ax, x, cols, bs, rs, re); }
+
+template <typename DT> static void test_csr()
+{
+    // product_TEST.cc:917-957: size x (size + 3), every fifth element of the
+    // row-major index stored, csr::blocksize 1..8; compared bit for bit with
+    // the oracle on all rows and on a row range
+    for (unsigned long blocks(1) ; blocks <= 8 ; blocks *= 2)
+    {
+        Configuration::instance()->set_value("csr::blocksize", (int)blocks);
+        for (unsigned long size(11) ; size < (1ul << 9) ; size <<= 1)
+        {
+            SparseMatrix<DT> sms(size, size + 3);
+            for (unsigned long i(0) ; i < size * (size + 3) ; ++i)
+                if (i % 5 == 0)
+                    sms(i / (size + 3), i % (size + 3), DT(i) / DT(1.234));
+            SparseMatrixCSR<DT> sm0(sms);
+            TEST_CHECK(sm0.blocksize() == blocks);
+            DenseVector<DT> dv1(size + 3, DT(4));
+            dv1[0] = 1;
+            dv1[1] = 2;
+            DenseVector<DT> prod(size, DT(4711));
+            Product<Tag>::value(prod, sm0, dv1);
+            std::vector<DT> ref(size, DT(4711));
+            or_csr_product(ref.data(), (const uint64_t *)sm0.Ar().read_elements(),
+                           (const uint64_t *)sm0.Aj().read_elements(), sm0.Ax().read_elements(),
+                           dv1.read_elements(), size + 3, blocks, 0, size);
+            TEST_CHECK(bit_equal(prod, ref.data()));
+            bool near = true;
+            for (unsigned long r(0) ; r < size ; ++r)
+            {
+                double s = 0;
+                for (unsigned long c(0) ; c < size + 3 ; ++c)
+                    s += double(sms(r, c)) * double(dv1[c]);
+                near = near && std::fabs(s - double(prod[r])) <= 1e-5 * (1 + std::fabs(s));
+            }
+            TEST_CHECK(near);
+            DenseVector<DT> part(size, DT(-3));
+            Product<Tag>::value(part, sm0, dv1, 2, size - 1);
+            std::vector<DT> refp(size, DT(-3));
+            or_csr_product(refp.data(), (const uint64_t *)sm0.Ar().read_elements(),
+                           (const uint64_t *)sm0.Aj().read_elements(), sm0.Ax().read_elements(),
+                           dv1.read_elements(), size + 3, blocks, 2, size - 1);
+            TEST_CHECK(bit_equal(part, refp.data()));
+        }
+    }
+    Configuration::instance()->set_value("csr::blocksize", 1);
+    // defect_TEST.cc:376-420: area51 defect through CSR equals the ELL defect
+    SparseMatrixELL<DT> E(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/l2/area51_full_0.ell", DT(0)));
+    SparseMatrixCSR<DT> C(E);
+    DenseVector<DT> rhs(VectorIO<io_formats::EXP>::read_vector(g_dir + "/l2/area51_rhs_0", DT(0)));
+    DenseVector<DT> x(E.rows(), DT(0));
+    for (unsigned long i(0) ; i < x.size() ; ++i)
+        if (i % 2 == 0)
+            x[i] = 1;
+    DenseVector<DT> r1(E.rows()), r2(E.rows());
+    Defect<Tag>::value(r1, rhs, C, x);
+    Defect<Tag>::value(r2, rhs, E, x);
+    TEST_CHECK(bit_equal(r1, r2.read_elements()));
+    TEST_CHECK(C(3, 3) == E(3, 3) && C(0, E.columns() - 1) == E(0, E.columns() - 1));
 }
 
 template <typename DT> static void test_pcg_jac()
@@ -883,6 +951,8 @@ int main(int argc, char ** argv)
         {"q1_and_blas1<float>", test_q1_and_blas1<float>},
         {"transfers_and_jacobi<double>", test_transfers_and_jacobi<double>},
         {"transfers_and_jacobi<float>", test_transfers_and_jacobi<float>},
+        {"csr<double>", test_csr<double>},
+        {"csr<float>", test_csr<float>},
         {"cg", test_cg},
         {"pcg_jac<double>", test_pcg_jac<double>},
         {"pcg_jac<float>", test_pcg_jac<float>},

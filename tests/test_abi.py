@@ -24,7 +24,7 @@ def test_header_declares_the_reference_launcher_set():
     for base in ("product_smell_dv", "defect_smell_dv", "product_bmdv_q1", "defect_q1", "dot_product_two",
                  "norm_l2_one", "scaled_sum_three", "sum_two", "scale_one", "difference_two",
                  "difference_three", "element_product_three", "element_inverse_one", "prolongation",
-                 "restriction"):
+                 "restriction", "product_csr_dv", "defect_csr_dv"):
         for t in ("float", "double"):
             assert f"hsolve_{base}_{t}" in names, base
     for n in ("hsolve_convert_float_double", "hsolve_convert_double_float", "hsolve_malloc", "hsolve_free",

@@ -73,3 +73,31 @@ def test_matrix_b_lds_window_sampled_rows(hip):
     A.set_variant(4)                      # scattered gathers, no LDS window
     A.spmv(hip.DeviceVector.from_numpy(x), y)
     assert np.array_equal(y.get(), got)
+
+
+@pytest.mark.parametrize("which", ["a_f64", "a_f32", "b_f64"])
+def test_csr_equals_ell_full_size(hip, which):
+    """The same matrix as SparseMatrixCSR (blocksize 1): its rows hold the
+    ELL slots in the same order, so Product and Defect must equal the ELL
+    results bit for bit on every row (Matrix A 4097^2, Matrix B 2^22 rows);
+    rows sampled across the matrix are also recomputed on the host."""
+    if which == "b_f64":
+        E = P.matrix_b_ell(1 << 22)
+    else:
+        E = P.q1_to_ell(P.matrix_a_bands(4097, dtype=np.float64 if which == "a_f64" else np.float32))
+    n, dtype = E["rows"], E["Ax"].dtype
+    A = hip.EllMatrix.from_dict(E)
+    C = hip.CsrMatrix.from_dict(P.ell_to_csr(E))
+    x = P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype)
+    rhs = P.rng_uniform(P.SEED, P.S_RHS, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype)
+    dx, dr = hip.DeviceVector.from_numpy(x), hip.DeviceVector.from_numpy(rhs)
+    ye, yc = hip.DeviceVector.zeros(n, dtype), hip.DeviceVector.zeros(n, dtype)
+    A.spmv(dx, ye)
+    C.spmv(dx, yc)
+    got = yc.get()
+    assert np.array_equal(got, ye.get())
+    rows = _sample(n, np.random.default_rng(3))
+    assert np.array_equal(got[rows], _host_rows(E, x, rows))
+    A.spmv(dx, ye, rhs=dr)
+    C.spmv(dx, yc, rhs=dr)
+    assert np.array_equal(yc.get(), ye.get())

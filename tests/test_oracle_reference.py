@@ -156,3 +156,61 @@ def test_pcg_jac_matches_reference_and_featflow(oracle, reference, dtype):
     y1, j1, _ = oracle.pcg_jac_ell(D, ddd, b, np.zeros(D["rows"], dtype), 10000, dtype(1e-6))
     y2, j2 = reference.pcg_jac_ell(D, ddd, b, np.zeros(D["rows"], dtype), 10000, dtype(1e-6))
     assert j1 == j2 and np.array_equal(y1, y2)
+
+
+def _every_fifth(size, dtype):
+    """product_TEST.cc:917-957: size x (size + 3), element index i of the
+    row-major matrix stored when i % 5 == 0, with value i / 1.234."""
+    cols = size + 3
+    idx = np.arange(size * cols)
+    sel = idx[idx % 5 == 0]
+    r, c = sel // cols, sel % cols
+    rowptr = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=size))]).astype(np.uint64)
+    return cols, rowptr, c.astype(np.uint64), (sel / 1.234).astype(dtype), r
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("bs", [1, 2, 3, 4, 8])
+def test_csr_matches_reference(oracle, reference, rng, dtype, bs):
+    """SparseMatrixCSR construction (from rows and from ELL), Product and
+    Defect on row ranges: the oracle is the compiled reference bit for bit,
+    for csr::blocksize 1..8 (the blocks of product_TEST.cc:929) and the
+    matrices of product_TEST.cc:917-957 and defect_TEST.cc:376-420."""
+    for size in (11, 22, 44, 88, 176, 352):
+        cols, rowptr, col, val, r = _every_fifth(size, dtype)
+        A = oracle.csr_from_rows(size, cols, rowptr, col, val, bs)
+        B = reference.csr_from_rows(size, cols, rowptr, col, val, bs)
+        for k in ("Ar", "Aj", "Ax", "nblocks", "used"):
+            assert np.array_equal(A[k], B[k]), k
+        assert A["used"] == col.size
+        x = np.full(cols, 4, dtype)
+        x[0], x[1] = 1, 2
+        y = oracle.csr_product(A, x, y=np.full(size, 4711, dtype))
+        assert np.array_equal(y, reference.csr_product(B, x, y=np.full(size, 4711, dtype)))
+        dense = np.zeros((size, cols))
+        dense[r, col.astype(np.int64)] = val
+        assert np.allclose(y, dense @ x.astype(np.float64), rtol=1e-5 if dtype == np.float32 else 1e-13)
+        lo, hi = 2, size - 3
+        rhs = rng.standard_normal(size).astype(dtype)
+        xr = rng.standard_normal(cols).astype(dtype)
+        y0 = np.full(size, 7, dtype)
+        assert np.array_equal(oracle.csr_product(A, xr, lo, hi, y=y0), reference.csr_product(B, xr, lo, hi, y=y0))
+        d = oracle.csr_defect(A, rhs, xr, lo, hi, r=y0)
+        assert np.array_equal(d, reference.csr_defect(B, rhs, xr, lo, hi, r=y0))
+        assert np.array_equal(d[:lo], y0[:lo]) and np.array_equal(d[hi:], y0[hi:])
+    for name in ("poisson/A_6.ell", "l2/area51_full_0.ell", "poisson/prol_6.ell"):
+        F = load_ell(name)
+        E = dict(F)
+        E["Ax"] = F["Ax"].astype(dtype)
+        A = oracle.csr_from_ell(E, bs, cols=F["cols"])
+        B = reference.csr_from_ell(E, bs, cols=F["cols"])
+        for k in ("Ar", "Aj", "Ax", "nblocks", "used"):
+            assert np.array_equal(A[k], B[k]), (name, k)
+        x = rng.standard_normal(F["cols"]).astype(dtype)
+        rhs = rng.standard_normal(F["rows"]).astype(dtype)
+        y = oracle.csr_product(A, x)
+        assert np.array_equal(y, reference.csr_product(B, x))
+        assert np.array_equal(oracle.csr_defect(A, rhs, x), reference.csr_defect(B, rhs, x))
+        if bs == 1:
+            E["Arl"] = oracle.ell_row_length(F["rows"], F["stride"], 1, E["Ax"])
+            assert np.array_equal(y, oracle.ell_product(E, x))
