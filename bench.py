@@ -361,8 +361,15 @@ def main():
         from honei_amd import device as hip
         from honei_amd import problems as P
         E = P.q1_to_ell(P.matrix_a_bands(args.side, dtype=np.float64))
-        out = {"csr_spmv_matrix_a_f64": csr_case(hip, P, E, args.steps, args.warmup),
-               "csr_defect_matrix_a_f64": csr_case(hip, P, E, args.steps, args.warmup, defect=True)}
+        out = {}
+        if args.copy_calibration:
+            nb = 2 << 30
+            src, dst = hip.DeviceBuffer(nb), hip.DeviceBuffer(nb)
+            ms = timed(lambda: hip.copy_kernel(src, dst, nb), max(args.steps // 5, 3), 1)
+            out["copy_kernel_2GiB"] = dict(bytes=2 * nb, ms=ms, GBps=2 * nb / (ms * 1e-3) / 1e9)
+            del src, dst
+        out.update({"csr_spmv_matrix_a_f64": csr_case(hip, P, E, args.steps, args.warmup),
+               "csr_defect_matrix_a_f64": csr_case(hip, P, E, args.steps, args.warmup, defect=True)})
         E["Ax"] = E["Ax"].astype(np.float32)
         out["csr_spmv_matrix_a_f32"] = csr_case(hip, P, E, args.steps, args.warmup)
         del E
