@@ -82,7 +82,8 @@ def read_traffic():
         return None
 
 
-def spmv_case(hip, P, E, steps, warmup, defect=False):
+def spmv_case(hip, P, E, steps, warmup, defect=False, keep_y=None):
+    """keep_y: a list that receives the host copy of the last y."""
     n = E["rows"]
     dtype = E["Ax"].dtype
     s = dtype.itemsize
@@ -99,6 +100,8 @@ def spmv_case(hip, P, E, steps, warmup, defect=False):
     nbytes = P.ell_defect_bytes(n, z, s) if defect else P.ell_spmv_bytes(n, z, s)
     out = dict(rows=n, nnz=z, ms=ms, bytes=nbytes, GBps=nbytes / (ms * 1e-3) / 1e9)
     out["frac_hbm"] = out["GBps"] / HBM_PEAK_GBS
+    if keep_y is not None:
+        keep_y.append(y.get())
     del M, x, y, rhs
     return out
 
@@ -313,26 +316,82 @@ def cg_case(hip, P, side, max_iters, tol=1e-8):
     return dict(n=n, iters=used, rel_defect=rel, ms=(t1 - t0) * 1e3, solves_per_s=1.0 / (t1 - t0))
 
 
+def cpu_threads():
+    """Host threads for the multi-core baseline: this process's CPU set,
+    at most 16 (one GPU's share of a shared host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline_spmv(E, budget_s=10.0):
-    """The oracle (single-thread C restatement of Product<tags::CPU> on ELL)
-    over the same Matrix A, repeated until about budget_s seconds."""
+    """The oracle (C restatement of Product<tags::CPU> on ELL) over the same
+    Matrix A and x, repeated until about budget_s seconds: single-threaded,
+    and with the rows split into one equal range per thread as
+    Product<tags::CPU::MultiCore> does (product.hh:2087-2106).  Returns
+    (multi-thread result, single-thread result, the oracle's y)."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
     from honei_amd import problems as P
     from oracle.pyoracle import Oracle
     O = Oracle()
+    fn = O.lib.or_ell_product_f64      # ctypes releases the GIL during the call
     n = E["rows"]
     x = P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1)
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        O.ell_product(E, x)
-        reps += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = (time.perf_counter() - t0) / reps
+    y = np.zeros(n)
+    Aj, Ax, Arl = E["Aj"], E["Ax"], np.ascontiguousarray(E["Arl"], np.uint64)
+    ptr = [a.ctypes.data_as(C.c_void_p) for a in (y, Aj, Ax, Arl, x)]
+
+    def rows(r0, r1):
+        fn(*ptr, C.c_uint64(r0), C.c_uint64(r1), C.c_uint64(E["stride"]), C.c_uint64(E.get("threads", 1)))
+
     nbytes = P.ell_spmv_bytes(n, P.ell_nnz(E), 8)
-    return dict(value=nbytes / dt / 1e9, unit="GB/s", cores=1, kind="port",
-                sample=f"{reps} full Product<tags::CPU> ELL SpMVs of Matrix A FP64 ({n} rows), "
-                       f"{dt * 1e3:.0f} ms each, on {cpu_model()}")
+    out = []
+    for T in (1, cpu_threads()):
+        bounds = [(n * t // T, n * (t + 1) // T) for t in range(T)]
+        with ThreadPoolExecutor(max_workers=T) as ex:
+            t0 = time.perf_counter()
+            reps = 0
+            while True:
+                list(ex.map(lambda b: rows(*b), bounds))
+                reps += 1
+                if time.perf_counter() - t0 > budget_s:
+                    break
+            dt = (time.perf_counter() - t0) / reps
+        out.append(dict(value=nbytes / dt / 1e9, unit="GB/s", cores=T, kind="port",
+                        sample=f"{reps} full Product<tags::CPU> ELL SpMVs of Matrix A FP64 ({n} rows), "
+                               f"{dt * 1e3:.1f} ms each, {T} thread(s) over equal row ranges, on {cpu_model()}"))
+    return out[1], out[0], y
+
+
+def mg_vs_oracle(P, level, min_level, tol, x_gpu, info_gpu):
+    """The oracle's MG-IR (C restatement of Multigrid<tags::CPU, ..., MIXED>,
+    one thread) on the same problem: iteration counts, relative difference of
+    the solutions, and the CPU solve time."""
+    from oracle.pyoracle import Oracle
+    O = Oracle()
+    n = P.side_of_level(level) ** 2
+    a32, d32 = {}, {}
+    for L in range(min_level, level + 1):
+        bands = [b.astype(np.float32) for b in P.poisson_q1_bands(P.side_of_level(L) ** 2)]
+        a32[L] = bands
+        # ElementInverse, then Scale by 0.7 in double (poisson_mg_mixed_banded_TEST.cc:411-419)
+        inv = bands[4].copy()
+        nz = inv != 0
+        inv[nz] = np.float32(1.0) / inv[nz]
+        d32[L] = (inv.astype(np.float64) * 0.7).astype(np.float32)
+    A64 = P.poisson_q1_bands(n)
+    b = P.rhs_uniform(n)
+    t0 = time.perf_counter()
+    r = O.mg_mixed_q1(A64, b, np.zeros(n), a32, d32, min_level=min_level, max_level=level, tolerance=tol)
+    cpu_s = time.perf_counter() - t0
+    rel = float(np.linalg.norm(x_gpu - r["x"]) / np.linalg.norm(r["x"]))
+    return dict(oracle_outer=r["outer"], oracle_inner=r["inner"], gpu_outer=info_gpu["outer"],
+                gpu_inner=info_gpu["inner"], rel_diff_vs_oracle=rel,
+                oracle_rel_defect=r["final_defect"] / r["initial_defect"],
+                cpu_solve_s=cpu_s, cpu_threads=1)
 
 
 def main():
@@ -417,7 +476,8 @@ def main():
         log("copy", extra["copy_kernel_2GiB"])
         del src, dst
 
-    head = spmv_case(hip, P, E64, args.steps, args.warmup)
+    y_head = []
+    head = spmv_case(hip, P, E64, args.steps, args.warmup, keep_y=y_head)
     log("headline", head)
     traffic = read_traffic()
     if not args.quick:
@@ -439,8 +499,12 @@ def main():
         del EB
         extra["q1_spmv_matrix_a_f64"] = q1_case(hip, P, side, np.float64, args.steps, args.warmup)
         log("spmv extras done")
-        mg, _ = mg_case(hip, P, 12, 2, reps=5, tol=1e-8)
+        mg, x_mg = mg_case(hip, P, 12, 2, reps=5, tol=1e-8)
         extra["mg_ir_poisson_4097_tol1e-8"] = mg
+        if not args.no_cpu:
+            mg["vs_oracle"] = mg_vs_oracle(P, 12, 2, 1e-8, x_mg.get(), mg)
+            log("mg vs oracle", mg["vs_oracle"])
+        del x_mg
         mg10, _ = mg_case(hip, P, 12, 2, reps=3, tol=1e-10)
         extra["mg_ir_poisson_4097_tol1e-10"] = mg10
         log("mg", mg, mg10)
@@ -454,8 +518,12 @@ def main():
 
     cpu = None
     if not args.quick and not args.no_cpu:
-        cpu = cpu_baseline_spmv(E64)
-        log("cpu", cpu)
+        cpu, cpu1, y_oracle = cpu_baseline_spmv(E64)
+        extra["cpu_baseline_1thread"] = cpu1
+        diff = np.abs(y_head[0] - y_oracle)
+        extra["headline_vs_oracle"] = dict(rows=int(y_oracle.size), rows_differing=int(np.count_nonzero(diff)),
+                                           max_abs_diff=float(diff.max()))
+        log("cpu", cpu, cpu1, extra["headline_vs_oracle"])
 
     res = {
         "metric": METRIC,
