@@ -411,11 +411,23 @@ def main():
     ap.add_argument("--diffusion-only", type=int, default=0, metavar="LEVEL",
                     help="only run the 64-RHS variable-coefficient diffusion case on the 2^LEVEL+1 grid")
     ap.add_argument("--no-pcg", action="store_true", help="with --diffusion-only: MG-IR leg only")
+    ap.add_argument("--matb-only", action="store_true",
+                    help="ELL product / defect on Matrix B (FP64, FP32) only")
     ap.add_argument("--csr-only", action="store_true",
                     help="SparseMatrixCSR product / defect on Matrix A (FP64, FP32) and Matrix B (FP64) only")
     ap.add_argument("--pcg-max", type=int, default=200000,
                     help="with --diffusion-only: iteration cap of the PCG leg (a short cap for profiling)")
     args = ap.parse_args()
+    if args.matb_only:
+        from honei_amd import device as hip
+        from honei_amd import problems as P
+        EB = P.matrix_b_ell(1 << 24)
+        out = {"ell_spmv_matrix_b_f64": spmv_case(hip, P, EB, args.steps, args.warmup),
+               "ell_defect_matrix_b_f64": spmv_case(hip, P, EB, args.steps, args.warmup, defect=True)}
+        EB["Ax"] = EB["Ax"].astype(np.float32)
+        out["ell_spmv_matrix_b_f32"] = spmv_case(hip, P, EB, args.steps, args.warmup)
+        print(json.dumps(out))
+        return
     if args.csr_only:
         from honei_amd import device as hip
         from honei_amd import problems as P

@@ -55,6 +55,14 @@ namespace hsolve
     }
     constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
 
+    // Workgroup barrier for LDS traffic only: waits for this wave's LDS
+    // operations, not for its outstanding global loads (which
+    // __syncthreads() would drain), so loads issued before it stay in flight.
+    __device__ inline void lds_barrier()
+    {
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    }
+
     inline unsigned grid_for(uint64_t work, unsigned per_block)
     {
         uint64_t g = (work + per_block - 1) / per_block;

@@ -27,14 +27,6 @@ namespace hsolve
     constexpr int kCsrJ = 8;                       // elements per lane per chunk
     constexpr int kCsrChunk = kCsrRows * kCsrJ;    // elements staged per pass (16 KiB FP64)
 
-    // Workgroup barrier for LDS traffic only: waits for this wave's LDS
-    // operations, not for its outstanding global loads (which
-    // __syncthreads() would drain), so loads issued before it stay in flight.
-    __device__ inline void lds_barrier()
-    {
-        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    }
-
     // Products of the chunk [c0, c0 + cn) into LDS.  Loads use indices
     // clamped into the chunk, so every lane issues all kCsrJ loads at once
     // without branches; only the LDS stores are guarded.

@@ -9,7 +9,9 @@
 //  * row lengths in the narrowest unsigned type that holds the longest row.
 // For an FP64 9-point stencil that is 10 instead of 16 bytes per nonzero and
 // 1 instead of 8 bytes per row length.  Which products are summed, and in
-// which order, does not change.
+// which order, does not change.  A matrix with rows of varying length whose
+// columns stay near the diagonal also gets a copy with the rows of every
+// 512-row block sorted by length, read by the LDS window kernel.
 #include "ell_handle.hh"
 #include "kernels_mg.hh"
 
@@ -47,7 +49,7 @@ namespace
             return m->variant;
         if (m->nnz * 10 >= m->rows * m->ncpr * 9)
             return HSOLVE_ELL_FULL;
-        return HSOLVE_ELL_XCD | HSOLVE_ELL_LDS;
+        return HSOLVE_ELL_XCD | HSOLVE_ELL_LDS | (m->sAx ? HSOLVE_ELL_SORT : 0);
     }
 
     // LDS bytes of ell_spmv_lds's x window, or 0 when the handle does not
@@ -107,14 +109,34 @@ namespace
             if (r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->ncpr <= 16)
             {
                 const unsigned g = grid_for(m->rows, k::kLdsRows);
-#define HS_L(K)                                                                                                               hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE>), dim3(g), dim3(kBlock), lds, st, x, y,                                    Aj, Ax, Arl, rhs, dinv, m->rows, m->cols, m->stride, (uint32_t)m->ncpr,                                        m->max_offset)
+                const bool sorted = (v & HSOLVE_ELL_SORT) && m->sAx;
+                const IT * sAj = (const IT *)m->sAj;
+                const LT * sArl = (const LT *)m->sArl;
+                const DT * sAx = (const DT *)m->sAx;
+                const uint16_t * perm = (const uint16_t *)m->sperm;
+#define HS_L(K)                                                                                               \
+                if (sorted)                                                                                   \
+                    hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE, true>), dim3(g), dim3(kBlock), lds, \
+                                       st, x, y, sAj, sAx, sArl, rhs, dinv, m->rows, m->cols, m->stride,        \
+                                       (uint32_t)m->ncpr, m->max_offset, perm);                               \
+                else                                                                                          \
+                    hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE, false>), dim3(g), dim3(kBlock), lds, \
+                                       st, x, y, Aj, Ax, Arl, rhs, dinv, m->rows, m->cols, m->stride,          \
+                                       (uint32_t)m->ncpr, m->max_offset, (const uint16_t *)nullptr)
                 if (m->ncpr <= 9)
+                {
                     HS_L(9);
+                }
                 else if (m->ncpr <= 13)
+                {
                     HS_L(13);
+                }
                 else
+                {
                     HS_L(16);
+                }
 #undef HS_L
+                HIP_CHECK(hipGetLastError());
                 return;
             }
             if (r2_ok && (ell_launch_r2<DT, IT, LT, MODE, 4>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
@@ -261,8 +283,50 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
     if (maxlen <= 0xFF) { lens = pack_lengths<uint8_t>(rl); lb = 1; }
     else if (maxlen <= 0xFFFF) { lens = pack_lengths<uint16_t>(rl); lb = 2; }
     else { lens = pack_lengths<uint32_t>(rl); lb = 4; }
+    // one zero length past the last row: the two-rows-per-lane kernels read
+    // the pair (r, r + 1) without a branch
+    lens.resize(lens.size() + 4, 0);
+
+    // the LDS window kernel's copy with rows sorted by length inside each
+    // 512-row block, when that kernel is the automatic choice
+    const size_t isz = idx_kind ? sizeof(int16_t) : sizeof(uint32_t);
+    const bool sort_copy = threads == 1 && num_cols_per_row <= 16 && nnz * 10 < rows * num_cols_per_row * 9 &&
+                           ((uint64_t)k::kLdsRows + 2 * (uint64_t)maxoff) * vs <= 48 * 1024 &&
+                           (stride > rows || rows % 2 == 0) && rows > 0;
+    std::vector<uint16_t> perm;
+    std::vector<uint8_t> s_idx, s_ax, s_lens;
+    if (sort_copy)
+    {
+        perm.assign(rows + 2, 0);     // padding entries past the last row, as for the lengths
+        std::vector<uint64_t> order;
+        for (uint64_t b0 = 0; b0 < rows; b0 += (uint64_t)k::kLdsRows)
+        {
+            const uint64_t b1 = std::min<uint64_t>(rows, b0 + (uint64_t)k::kLdsRows);
+            order.resize(b1 - b0);
+            for (uint64_t i = 0; i < b1 - b0; ++i)
+                order[i] = b0 + i;
+            std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return rl[a] > rl[b]; });
+            for (uint64_t i = 0; i < b1 - b0; ++i)
+                perm[b0 + i] = (uint16_t)(order[i] - b0);
+        }
+        s_idx.assign(idx.size(), 0);
+        s_ax.assign(vs * size, 0);
+        s_lens.assign(lens.size(), 0);
+        const uint8_t * ax = (const uint8_t *)Ax_host;
+        for (uint64_t p = 0; p < rows; ++p)
+        {
+            const uint64_t o = (p / (uint64_t)k::kLdsRows) * (uint64_t)k::kLdsRows + perm[p];
+            std::memcpy(&s_lens[p * lb], &lens[o * lb], lb);
+            for (uint64_t kk = 0; kk < num_cols_per_row; ++kk)
+            {
+                std::memcpy(&s_idx[(kk * stride + p) * isz], &idx[(kk * stride + o) * isz], isz);
+                std::memcpy(&s_ax[(kk * stride + p) * vs], &ax[(kk * stride + o) * vs], vs);
+            }
+        }
+    }
 
     hsolve_ell_s * m = new hsolve_ell_s();
+    m->sAj = nullptr; m->sAx = nullptr; m->sArl = nullptr; m->sperm = nullptr;
     m->dtype = dtype; m->rows = rows; m->cols = columns; m->stride = stride; m->ncpr = num_cols_per_row;
     m->threads = threads; m->nnz = nnz; m->idx_kind = idx_kind; m->len_bytes = lb; m->variant = 0;
     m->q1_side = q1_side;
@@ -279,16 +343,34 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
         HIP_CHECK(hipMemcpy(m->Ax, Ax_host, vs * size, hipMemcpyHostToDevice));
         if (! lens.empty())
             HIP_CHECK(hipMemcpy(m->Arl, lens.data(), lens.size(), hipMemcpyHostToDevice));
+        if (sort_copy)
+        {
+            hipError_t f1 = hipMalloc(&m->sAj, s_idx.size());
+            hipError_t f2 = hipMalloc(&m->sAx, s_ax.size());
+            hipError_t f3 = hipMalloc(&m->sArl, s_lens.size());
+            hipError_t f4 = hipMalloc(&m->sperm, perm.size() * sizeof(uint16_t));
+            if (f1 != hipSuccess || f2 != hipSuccess || f3 != hipSuccess || f4 != hipSuccess)
+                throw Error(HSOLVE_ERR_ALLOC, "ell_create: device allocation of the sorted copy failed");
+            HIP_CHECK(hipMemcpy(m->sAj, s_idx.data(), s_idx.size(), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(m->sAx, s_ax.data(), s_ax.size(), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(m->sArl, s_lens.data(), s_lens.size(), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(m->sperm, perm.data(), perm.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+        }
     }
     catch (...)
     {
         if (m->Aj) (void)hipFree(m->Aj);
         if (m->Ax) (void)hipFree(m->Ax);
         if (m->Arl) (void)hipFree(m->Arl);
+        if (m->sAj) (void)hipFree(m->sAj);
+        if (m->sAx) (void)hipFree(m->sAx);
+        if (m->sArl) (void)hipFree(m->sArl);
+        if (m->sperm) (void)hipFree(m->sperm);
         delete m;
         throw;
     }
-    m->device_bytes = idx.size() + vs * size + lens.size();
+    m->device_bytes = idx.size() + vs * size + lens.size() +
+                      (sort_copy ? s_idx.size() + s_ax.size() + s_lens.size() + perm.size() * sizeof(uint16_t) : 0);
     *out = m;
     HSOLVE_API_END
 }
@@ -301,6 +383,13 @@ int hsolve_ell_destroy(hsolve_ell m)
         HIP_CHECK(hipFree(m->Aj));
         HIP_CHECK(hipFree(m->Ax));
         HIP_CHECK(hipFree(m->Arl));
+        if (m->sAx)
+        {
+            HIP_CHECK(hipFree(m->sAj));
+            HIP_CHECK(hipFree(m->sAx));
+            HIP_CHECK(hipFree(m->sArl));
+            HIP_CHECK(hipFree(m->sperm));
+        }
         delete m;
     }
     HSOLVE_API_END
@@ -325,7 +414,7 @@ int hsolve_ell_set_variant(hsolve_ell m, int variant)
 {
     HSOLVE_API_BEGIN
     require(m != nullptr, HSOLVE_ERR_INVALID, "ell_set_variant: null handle");
-    require(variant >= 0 && variant < 32 && (variant & HSOLVE_ELL_SET) == 0, HSOLVE_ERR_INVALID,
+    require(variant >= 0 && variant < 64 && (variant & HSOLVE_ELL_SET) == 0, HSOLVE_ERR_INVALID,
             "ell_set_variant: variant bits out of range");
     m->variant = variant ? (variant | HSOLVE_ELL_SET) : 0;
     HSOLVE_API_END

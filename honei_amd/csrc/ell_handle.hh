@@ -19,6 +19,14 @@ struct hsolve_ell_s
     uint64_t q1_side;          // M > 0 when rows = cols = M^2, threads = 1 and every stored
                                // column of row (y, x) is a neighbour (y + dy, x + dx),
                                // |dy|, |dx| <= 1, in ascending slot order; else 0
+    // Copy for the LDS window kernel, built at upload when that kernel is
+    // the automatic choice: Aj, Ax and Arl with each 512-row block's rows in
+    // descending order of length, and the block-local original row of every
+    // storage position (uint16).  nullptr otherwise.
+    void * sAj;
+    void * sAx;
+    void * sArl;
+    void * sperm;
 };
 
 // kernel variant bits of hsolve_ell_set_variant
@@ -28,7 +36,8 @@ enum
     HSOLVE_ELL_FULL = 2,       // load all num_cols_per_row slots without waiting for row lengths
     HSOLVE_ELL_XCD = 4,        // XCD-contiguous block order
     HSOLVE_ELL_SET = 8,        // marks an explicit choice
-    HSOLVE_ELL_LDS = 16        // x window in LDS (banded matrices; else ignored)
+    HSOLVE_ELL_LDS = 16,       // x window in LDS (banded matrices; else ignored)
+    HSOLVE_ELL_SORT = 32       // with LDS: rows sorted by length inside each block (when the copy exists)
 };
 
 namespace hsolve

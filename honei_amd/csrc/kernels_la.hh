@@ -219,12 +219,19 @@ namespace k
     // XCD-contiguous order, so neighbouring windows overlap in one L2.
     // Dynamic LDS: (kLdsRows + 2 maxoff) * sizeof(DT) bytes.  Same sums, in
     // the same order, as ell_spmv_r2.
-    template <typename DT, typename IT, typename LT, int MAXK, int MODE>
+    //
+    // SORTED: the arrays hold each 512-row block's rows in descending order
+    // of length (a copy built at upload) and perm[p] is the block-local
+    // original row of storage position p.  Slot k of a wave's rows is then
+    // used by a prefix of its lanes, so the cache lines of slots past a
+    // row's length are not fetched; each row keeps its own slot order and
+    // its result goes to its original index.
+    template <typename DT, typename IT, typename LT, int MAXK, int MODE, bool SORTED = false>
     __global__ __launch_bounds__(256) void ell_spmv_lds(
             const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
             const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
             const DT * __restrict__ dinv, uint64_t rows, uint64_t cols, uint64_t stride, uint32_t ncpr,
-            uint64_t maxoff)
+            uint64_t maxoff, const uint16_t * __restrict__ perm)
     {
         extern __shared__ __align__(16) unsigned char lds_raw[];
         DT * xw = reinterpret_cast<DT *>(lds_raw);
@@ -234,60 +241,108 @@ namespace k
         const uint64_t w0 = rb0 > maxoff ? rb0 - maxoff : 0;
         const uint64_t wend = rb0 + kLdsRows + maxoff;
         const uint64_t w1 = wend < cols ? wend : cols;
-        for (uint64_t j = threadIdx.x; w0 + j < w1; j += blockDim.x)
-            xw[j] = x[w0 + j];
-        __syncthreads();
+        const uint32_t wn = (uint32_t)(w1 - w0);
         const uint64_t r0 = rb0 + (uint64_t)threadIdx.x * 2;
-        if (r0 >= rows)
-            return;
+        const bool live = r0 < rows;
         const bool two = r0 + 1 < rows;
-        const uint32_t len0 = (uint32_t)Arl[r0];
-        const uint32_t len1 = two ? (uint32_t)Arl[r0 + 1] : 0u;
-        const uint32_t lmax = len0 > len1 ? len0 : len1;
+        // Row lengths first, then every slot load of the lane's two rows at
+        // once (a slot past both lengths re-reads slot 0, a cached line), then
+        // the x window: all of it is in flight together, and the window's
+        // LDS stores wait only for the loads issued before them.  No branch
+        // separates the loads (a lane past the last row reads the block's
+        // first pair; Arl and perm hold one padding entry), so the compiler
+        // need not drain them at a join.
+        const uint64_t rr = live ? r0 : rb0;
+        const uint32_t l0 = (uint32_t)Arl[rr], l1 = (uint32_t)Arl[rr + 1];
+        uint64_t o0 = rr, o1 = rr + 1;
+        if (SORTED)
+        {
+            // original rows of the two storage positions
+            o0 = rb0 + perm[rr];
+            o1 = rb0 + perm[rr + 1];
+        }
+        const uint32_t len0 = live ? l0 : 0u;
+        const uint32_t len1 = two ? l1 : 0u;
+        const uint32_t lmax = l0 > l1 ? l0 : l1;
         DT a0[MAXK], a1[MAXK];
         IT c0[MAXK], c1[MAXK];
 #pragma unroll
         for (int k = 0; k < MAXK; ++k)
         {
-            if ((uint32_t)k < lmax)
+            const uint64_t kk = (uint32_t)k < lmax ? (uint64_t)k : 0;
+            const V v = __builtin_nontemporal_load((const V *)(Ax + rr + kk * stride));
+            const VI c = __builtin_nontemporal_load((const VI *)(Aj + rr + kk * stride));
+            a0[k] = v.x;
+            a1[k] = v.y;
+            c0[k] = (IT)c.x;
+            c1[k] = (IT)c.y;
+        }
+        constexpr uint32_t kFill = 16;
+        {
+            DT wv[kFill];
+#pragma unroll
+            for (uint32_t i = 0; i < kFill; ++i)
             {
-                const V v = __builtin_nontemporal_load((const V *)(Ax + r0 + (uint64_t)k * stride));
-                const VI c = __builtin_nontemporal_load((const VI *)(Aj + r0 + (uint64_t)k * stride));
-                a0[k] = v.x;
-                a1[k] = v.y;
-                c0[k] = (IT)c.x;
-                c1[k] = (IT)c.y;
+                const uint32_t j = i * 256 + threadIdx.x;
+                wv[i] = x[w0 + (j < wn ? j : wn - 1)];
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < kFill; ++i)
+            {
+                const uint32_t j = i * 256 + threadIdx.x;
+                if (j < wn)
+                    xw[j] = wv[i];
             }
         }
+        for (uint32_t base = kFill * 256; base < wn; base += kFill * 256)
+        {
+            DT wv[kFill];
+#pragma unroll
+            for (uint32_t i = 0; i < kFill; ++i)
+            {
+                const uint32_t j = base + i * 256 + threadIdx.x;
+                wv[i] = x[w0 + (j < wn ? j : wn - 1)];
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < kFill; ++i)
+            {
+                const uint32_t j = base + i * 256 + threadIdx.x;
+                if (j < wn)
+                    xw[j] = wv[i];
+            }
+        }
+        lds_barrier();
+        if (! live)
+            return;
         DT s0 = DT(0), s1 = DT(0);
 #pragma unroll
         for (int k = 0; k < MAXK; ++k)
         {
             if ((uint32_t)k < len0)
-                s0 += a0[k] * xw[col_of<IT>(c0[k], r0) - w0];
+                s0 += a0[k] * xw[col_of<IT>(c0[k], o0) - w0];
             if ((uint32_t)k < len1)
-                s1 += a1[k] * xw[col_of<IT>(c1[k], r0 + 1) - w0];
+                s1 += a1[k] * xw[col_of<IT>(c1[k], o1) - w0];
         }
         if (MODE == 0)
         {
-            y[r0] = s0;
+            y[o0] = s0;
             if (two)
-                y[r0 + 1] = s1;
+                y[o1] = s1;
         }
         else if (MODE == 1)
         {
-            y[r0] = rhs[r0] - s0;
+            y[o0] = rhs[o0] - s0;
             if (two)
-                y[r0 + 1] = rhs[r0 + 1] - s1;
+                y[o1] = rhs[o1] - s1;
         }
         else
         {
-            const DT t0 = rhs[r0] - s0;
-            y[r0] = x[r0] + t0 * dinv[r0];
+            const DT t0 = rhs[o0] - s0;
+            y[o0] = x[o0] + t0 * dinv[o0];
             if (two)
             {
-                const DT t1 = rhs[r0 + 1] - s1;
-                y[r0 + 1] = x[r0 + 1] + t1 * dinv[r0 + 1];
+                const DT t1 = rhs[o1] - s1;
+                y[o1] = x[o1] + t1 * dinv[o1];
             }
         }
     }

@@ -284,7 +284,11 @@ int hsolve_ell_info(hsolve_ell m, uint64_t *rows, uint64_t *columns, uint64_t *s
  * block order, 16 = stage the x window of each 512-row block in LDS (used
  * when every stored column lies within a window of at most 48 KiB around
  * its row; automatic for such matrices unless they fill 90% of their
- * slots).  Results are identical for every variant. */
+ * slots), 32 = with 16: read a copy whose rows are sorted by length inside
+ * each 512-row block, so slots past a row's length are not fetched (the
+ * copy is built at upload when the LDS form is the automatic choice, and
+ * the automatic choice then uses it).  Results are identical for every
+ * variant. */
 int hsolve_ell_set_variant(hsolve_ell m, int variant);
 /* Grid side M when the handle holds a 9-point structured matrix (rows =
  * columns = M^2, threads == 1, every stored column of row (y, x) one of

@@ -101,7 +101,7 @@ def test_ell_known_answer_threads(hip, oracle, dtype, threads):
 
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_ell_matrix_b_and_a_small(hip, oracle, dtype):
-    for E in (P.matrix_b_ell(1 << 15, dtype=dtype),
+    for E in (P.matrix_b_ell(1 << 15, dtype=dtype), P.matrix_b_ell((1 << 15) - 7, dtype=dtype),
               P.q1_to_ell(P.matrix_a_bands(129, dtype=dtype))):
         n = E["rows"]
         x = P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype)
@@ -111,7 +111,8 @@ def test_ell_matrix_b_and_a_small(hip, oracle, dtype):
         M = hip.EllMatrix.from_dict(E)
         yd = hip.DeviceVector.zeros(n, dtype)
         xd, rd = hip.DeviceVector.from_numpy(x), hip.DeviceVector.from_numpy(rhs)
-        for variant in list(range(8)) + [16, 20, 22]:   # every kernel variant gives the same bits
+        # every kernel variant gives the same bits (48, 52: rows sorted by length per LDS block)
+        for variant in list(range(8)) + [16, 20, 22, 48, 52]:
             M.set_variant(variant)
             M.spmv(xd, yd)
             assert np.array_equal(yd.get(), ref), variant
@@ -121,7 +122,7 @@ def test_ell_matrix_b_and_a_small(hip, oracle, dtype):
         dinv = (1.0 / (np.abs(x) + 1.0)).astype(dtype)
         ref_j = oracle.jacobi_ell(x, E, rhs, dinv, 1)
         dd = hip.DeviceVector.from_numpy(dinv)
-        for variant in (0, 16):
+        for variant in (0, 16, 48):
             M.set_variant(variant)
             M.jacobi(rd, dd, xd, yd)
             assert np.array_equal(yd.get(), ref_j), variant
