@@ -82,13 +82,16 @@ def read_traffic():
         return None
 
 
-def spmv_case(hip, P, E, steps, warmup, defect=False, keep_y=None):
-    """keep_y: a list that receives the host copy of the last y."""
+def spmv_case(hip, P, E, steps, warmup, defect=False, keep_y=None, variant=None):
+    """keep_y: a list that receives the host copy of the last y; variant: an
+    explicit ELL kernel variant (hsolve_ell_set_variant), else automatic."""
     n = E["rows"]
     dtype = E["Ax"].dtype
     s = dtype.itemsize
     z = P.ell_nnz(E)
     M = hip.EllMatrix.from_dict(E)
+    if variant is not None:
+        M.set_variant(variant)
     x = hip.DeviceVector.from_numpy(
         P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype))
     y = hip.DeviceVector.zeros(n, dtype)
@@ -423,7 +426,8 @@ def main():
         from honei_amd import problems as P
         EB = P.matrix_b_ell(1 << 24)
         out = {"ell_spmv_matrix_b_f64": spmv_case(hip, P, EB, args.steps, args.warmup),
-               "ell_defect_matrix_b_f64": spmv_case(hip, P, EB, args.steps, args.warmup, defect=True)}
+               "ell_defect_matrix_b_f64": spmv_case(hip, P, EB, args.steps, args.warmup, defect=True),
+               "ell_spmv_matrix_b_f64_no_lds": spmv_case(hip, P, EB, args.steps, args.warmup, variant=4)}
         EB["Ax"] = EB["Ax"].astype(np.float32)
         out["ell_spmv_matrix_b_f32"] = spmv_case(hip, P, EB, args.steps, args.warmup)
         print(json.dumps(out))

@@ -67,7 +67,7 @@ namespace
     bool ell_launch_r2(const hsolve_ell_s * m, int v, const DT * x, DT * y, const IT * Aj, const DT * Ax,
                        const LT * Arl, const DT * rhs, const DT * dinv, hipStream_t st)
     {
-        if (m->ncpr > (uint64_t)K)
+        if (m->ncpr > (uint64_t)K || m->ncpr == 0 || m->cols == 0)
             return false;
         const unsigned grid = grid_for((m->rows + 1) / 2, kBlock);
         const bool full = v & HSOLVE_ELL_FULL, xcd = v & HSOLVE_ELL_XCD;
@@ -106,7 +106,7 @@ namespace
             using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
             using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
             const size_t lds = lds_window_bytes(m, sizeof(DT));
-            if (r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->ncpr <= 16)
+            if (r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->ncpr >= 1 && m->ncpr <= 16)
             {
                 const unsigned g = grid_for(m->rows, k::kLdsRows);
                 const bool sorted = (v & HSOLVE_ELL_SORT) && m->sAx;

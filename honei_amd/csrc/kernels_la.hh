@@ -145,20 +145,22 @@ namespace k
         const bool implied = STENCIL && len0 == 9 && (len1 == 9 || ! two);
         DT a0[MAXK], a1[MAXK];
         IT c0[MAXK], c1[MAXK];
+        // A slot past the rows' lengths (past num_cols_per_row with FULL)
+        // re-reads slot 0, a cached line, instead of branching, so every
+        // slot load issues at once (num_cols_per_row >= 1 at launch).
+        const uint32_t lim = FULL ? ncpr : lmax;
 #pragma unroll
         for (int k = 0; k < MAXK; ++k)
         {
-            if (FULL ? (uint32_t)k < ncpr : (uint32_t)k < lmax)
+            const uint64_t kk = (uint32_t)k < lim ? (uint64_t)k : 0;
+            const V v = __builtin_nontemporal_load((const V *)(Ax + r0 + kk * stride));
+            a0[k] = v.x;
+            a1[k] = v.y;
+            if (! implied)
             {
-                const V v = __builtin_nontemporal_load((const V *)(Ax + r0 + (uint64_t)k * stride));
-                a0[k] = v.x;
-                a1[k] = v.y;
-                if (! implied)
-                {
-                    const VI c = __builtin_nontemporal_load((const VI *)(Aj + r0 + (uint64_t)k * stride));
-                    c0[k] = (IT)c.x;
-                    c1[k] = (IT)c.y;
-                }
+                const VI c = __builtin_nontemporal_load((const VI *)(Aj + r0 + kk * stride));
+                c0[k] = (IT)c.x;
+                c1[k] = (IT)c.y;
             }
         }
         DT s0 = DT(0), s1 = DT(0);
