@@ -319,6 +319,52 @@ def cg_case(hip, P, side, max_iters, tol=1e-8):
     return dict(n=n, iters=used, rel_defect=rel, ms=(t1 - t0) * 1e3, solves_per_s=1.0 / (t1 - t0))
 
 
+def cg257_case(hip, P, reps=5, tol=1e-8, max_iters=10000, cpu=True):
+    """Plain FP64 CG on the Q1 Poisson problem with 257 x 257 nodes, as
+    BandedMatrixQx<Q1> and as SparseMatrixELL (x0 = 0, seeded U[-1, 1] rhs
+    with zero boundary entries): GPU time per solve (median of reps), and the
+    oracle's CG (the tags::CPU restatement, one thread) on the same inputs:
+    its time, its iteration count and the relative difference of the
+    solutions.  The working set (8 MB) stays in cache, so this is a latency
+    figure, not a roofline one."""
+    from oracle.pyoracle import Oracle
+    n = 257 * 257
+    bands = P.poisson_q1_bands(n)
+    E = P.q1_to_ell(bands)
+    b = P.rhs_uniform(n)
+    out = {}
+    for form in ("q1", "ell"):
+        A = hip.Q1Matrix(bands) if form == "q1" else hip.EllMatrix.from_dict(E)
+        bd, xd = hip.DeviceVector.from_numpy(b), hip.DeviceVector.zeros(n)
+        lat = []
+        for r in range(reps + 1):
+            xd.fill(0.0)
+            hip.synchronize()
+            t0 = time.perf_counter()
+            if form == "q1":
+                used, rel = hip.cg_q1(A, bd, xd, max_iters, tol, raise_on_cap=False)
+            else:
+                used, rel = hip.cg_ell(A, bd, xd, max_iters, tol, raise_on_cap=False)
+            t1 = time.perf_counter()
+            if r > 0:
+                lat.append((t1 - t0) * 1e3)
+        res = dict(iters=used, rel_defect=rel, ms_p50=float(np.median(lat)), solves_per_s=float(1e3 / np.median(lat)))
+        if cpu:
+            O = Oracle()
+            t0 = time.perf_counter()
+            if form == "q1":
+                xr, it, _ = O.cg_none_q1(bands, b, np.zeros(n), max_iters, tol)
+            else:
+                xr, it, _ = O.cg_none_ell(E, b, np.zeros(n), max_iters, tol)
+            cpu_ms = (time.perf_counter() - t0) * 1e3
+            x = xd.get()
+            res.update(oracle_iters=it, oracle_ms=cpu_ms, speedup_vs_oracle=cpu_ms / res["ms_p50"],
+                       rel_diff_vs_oracle=float(np.linalg.norm(x - xr) / np.linalg.norm(xr)))
+        out[form] = res
+        del A, bd, xd
+    return out
+
+
 def cpu_threads():
     """Host threads for the multi-core baseline: this process's CPU set,
     at most 16 (one GPU's share of a shared host)."""
@@ -414,6 +460,8 @@ def main():
     ap.add_argument("--diffusion-only", type=int, default=0, metavar="LEVEL",
                     help="only run the 64-RHS variable-coefficient diffusion case on the 2^LEVEL+1 grid")
     ap.add_argument("--no-pcg", action="store_true", help="with --diffusion-only: MG-IR leg only")
+    ap.add_argument("--cg257-only", action="store_true",
+                    help="plain FP64 CG on the 257x257 Q1 Poisson problem (banded and ELL) with the oracle beside it")
     ap.add_argument("--matb-only", action="store_true",
                     help="ELL product / defect on Matrix B (FP64, FP32) only")
     ap.add_argument("--csr-only", action="store_true",
@@ -421,6 +469,11 @@ def main():
     ap.add_argument("--pcg-max", type=int, default=200000,
                     help="with --diffusion-only: iteration cap of the PCG leg (a short cap for profiling)")
     args = ap.parse_args()
+    if args.cg257_only:
+        from honei_amd import device as hip
+        from honei_amd import problems as P
+        print(json.dumps({"cg_fp64_poisson_257": cg257_case(hip, P)}))
+        return
     if args.matb_only:
         from honei_amd import device as hip
         from honei_amd import problems as P
@@ -524,6 +577,8 @@ def main():
         mg10, _ = mg_case(hip, P, 12, 2, reps=3, tol=1e-10)
         extra["mg_ir_poisson_4097_tol1e-10"] = mg10
         log("mg", mg, mg10)
+        extra["cg_fp64_poisson_257"] = cg257_case(hip, P, cpu=not args.no_cpu)
+        log("cg 257", extra["cg_fp64_poisson_257"])
         cg = cg_case(hip, P, 4097, 100000)
         extra["cg_fp64_poisson_4097_tol1e-8"] = cg
         extra["mg_ir_speedup_vs_fp64_cg_4097"] = cg["ms"] / mg["ms_p50"]
