@@ -478,9 +478,16 @@ def main():
         from honei_amd import device as hip
         from honei_amd import problems as P
         EB = P.matrix_b_ell(1 << 24)
-        out = {"ell_spmv_matrix_b_f64": spmv_case(hip, P, EB, args.steps, args.warmup),
+        out = {}
+        if args.copy_calibration:
+            nb = 2 << 30
+            src, dst = hip.DeviceBuffer(nb), hip.DeviceBuffer(nb)
+            ms = timed(lambda: hip.copy_kernel(src, dst, nb), max(args.steps // 5, 3), 1)
+            out["copy_kernel_2GiB"] = dict(bytes=2 * nb, ms=ms, GBps=2 * nb / (ms * 1e-3) / 1e9)
+            del src, dst
+        out.update({"ell_spmv_matrix_b_f64": spmv_case(hip, P, EB, args.steps, args.warmup),
                "ell_defect_matrix_b_f64": spmv_case(hip, P, EB, args.steps, args.warmup, defect=True),
-               "ell_spmv_matrix_b_f64_no_lds": spmv_case(hip, P, EB, args.steps, args.warmup, variant=4)}
+               "ell_spmv_matrix_b_f64_no_lds": spmv_case(hip, P, EB, args.steps, args.warmup, variant=4)})
         EB["Ax"] = EB["Ax"].astype(np.float32)
         out["ell_spmv_matrix_b_f32"] = spmv_case(hip, P, EB, args.steps, args.warmup)
         print(json.dumps(out))
