@@ -532,7 +532,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by grid row blocks, see DESIGN.md)", file=sys.stderr)
+              "(the path shards by grid row blocks: one rank per GPU, halo exchange and RCCL reductions)",
+              file=sys.stderr)
         sys.exit(2)
 
     from honei_amd import device as hip
