@@ -11,7 +11,8 @@
 // 1 instead of 8 bytes per row length.  Which products are summed, and in
 // which order, does not change.  A matrix with rows of varying length whose
 // columns stay near the diagonal also gets a copy with the rows of every
-// 512-row block sorted by length, read by the LDS window kernel.
+// LDS block (1024 rows FP64, 512 FP32) sorted by length, read by the LDS
+// window kernel.
 #include "ell_handle.hh"
 #include "kernels_mg.hh"
 
@@ -57,7 +58,7 @@ namespace
     // window of at most 48 KiB so that 3 blocks fit on a CU)
     size_t lds_window_bytes(const hsolve_ell_s * m, size_t vs)
     {
-        const uint64_t w = (uint64_t)k::kLdsRows + 2 * m->max_offset;
+        const uint64_t w = (uint64_t)(vs == 8 ? k::lds_rows<double>() : k::lds_rows<float>()) + 2 * m->max_offset;
         if (m->threads != 1 || w * vs > 48 * 1024)
             return 0;
         return (size_t)(w * vs);
@@ -108,7 +109,7 @@ namespace
             const size_t lds = lds_window_bytes(m, sizeof(DT));
             if (r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->ncpr >= 1 && m->ncpr <= 16)
             {
-                const unsigned g = grid_for(m->rows, k::kLdsRows);
+                const unsigned g = grid_for(m->rows, k::lds_rows<DT>());
                 const bool sorted = (v & HSOLVE_ELL_SORT) && m->sAx;
                 const IT * sAj = (const IT *)m->sAj;
                 const LT * sArl = (const LT *)m->sArl;
@@ -116,11 +117,11 @@ namespace
                 const uint16_t * perm = (const uint16_t *)m->sperm;
 #define HS_L(K)                                                                                               \
                 if (sorted)                                                                                   \
-                    hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE, true>), dim3(g), dim3(kBlock), lds, \
+                    hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE, true>), dim3(g), dim3(k::lds_rows<DT>() / 2), lds, \
                                        st, x, y, sAj, sAx, sArl, rhs, dinv, m->rows, m->cols, m->stride,        \
                                        (uint32_t)m->ncpr, m->max_offset, perm);                               \
                 else                                                                                          \
-                    hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE, false>), dim3(g), dim3(kBlock), lds, \
+                    hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE, false>), dim3(g), dim3(k::lds_rows<DT>() / 2), lds, \
                                        st, x, y, Aj, Ax, Arl, rhs, dinv, m->rows, m->cols, m->stride,          \
                                        (uint32_t)m->ncpr, m->max_offset, (const uint16_t *)nullptr)
                 if (m->ncpr <= 9)
@@ -288,10 +289,11 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
     lens.resize(lens.size() + 4, 0);
 
     // the LDS window kernel's copy with rows sorted by length inside each
-    // 512-row block, when that kernel is the automatic choice
+    // LDS block, when that kernel is the automatic choice
     const size_t isz = idx_kind ? sizeof(int16_t) : sizeof(uint32_t);
+    const uint64_t lds_rows = dtype ? (uint64_t)k::lds_rows<double>() : (uint64_t)k::lds_rows<float>();
     const bool sort_copy = threads == 1 && num_cols_per_row <= 16 && nnz * 10 < rows * num_cols_per_row * 9 &&
-                           ((uint64_t)k::kLdsRows + 2 * (uint64_t)maxoff) * vs <= 48 * 1024 &&
+                           (lds_rows + 2 * (uint64_t)maxoff) * vs <= 48 * 1024 &&
                            (stride > rows || rows % 2 == 0) && rows > 0;
     std::vector<uint16_t> perm;
     std::vector<uint8_t> s_idx, s_ax, s_lens;
@@ -299,9 +301,9 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
     {
         perm.assign(rows + 2, 0);     // padding entries past the last row, as for the lengths
         std::vector<uint64_t> order;
-        for (uint64_t b0 = 0; b0 < rows; b0 += (uint64_t)k::kLdsRows)
+        for (uint64_t b0 = 0; b0 < rows; b0 += lds_rows)
         {
-            const uint64_t b1 = std::min<uint64_t>(rows, b0 + (uint64_t)k::kLdsRows);
+            const uint64_t b1 = std::min<uint64_t>(rows, b0 + lds_rows);
             order.resize(b1 - b0);
             for (uint64_t i = 0; i < b1 - b0; ++i)
                 order[i] = b0 + i;
@@ -315,7 +317,7 @@ int hsolve_ell_create(int dtype, uint64_t rows, uint64_t columns, uint64_t strid
         const uint8_t * ax = (const uint8_t *)Ax_host;
         for (uint64_t p = 0; p < rows; ++p)
         {
-            const uint64_t o = (p / (uint64_t)k::kLdsRows) * (uint64_t)k::kLdsRows + perm[p];
+            const uint64_t o = (p / lds_rows) * lds_rows + perm[p];
             std::memcpy(&s_lens[p * lb], &lens[o * lb], lb);
             for (uint64_t kk = 0; kk < num_cols_per_row; ++kk)
             {

@@ -20,7 +20,7 @@ struct hsolve_ell_s
                                // column of row (y, x) is a neighbour (y + dy, x + dx),
                                // |dy|, |dx| <= 1, in ascending slot order; else 0
     // Copy for the LDS window kernel, built at upload when that kernel is
-    // the automatic choice: Aj, Ax and Arl with each 512-row block's rows in
+    // the automatic choice: Aj, Ax and Arl with each LDS block's rows in
     // descending order of length, and the block-local original row of every
     // storage position (uint16).  nullptr otherwise.
     void * sAj;

@@ -210,31 +210,35 @@ namespace k
         }
     }
 
-    // Rows per block of ell_spmv_lds (two per lane).
-    constexpr int kLdsRows = 512;
+    // Rows per block of ell_spmv_lds (two per lane): 1024 for FP64, 512
+    // for FP32 (measured on MI355X, Matrix B: FP64 437 -> 417 us with 1024
+    // rows, whose x window per row is half as large; FP32 245 -> 269 us).
+    template <typename DT> constexpr int lds_rows() { return sizeof(DT) == 8 ? 1024 : 512; }
 
     // ell_spmv_r2 for matrices whose stored columns lie within +-maxoff of
     // their row (banded but unstructured, e.g. random columns in a window):
-    // the block first copies x[w0, w1), the columns its 512 rows can touch,
+    // the block first copies x[w0, w1), the columns its rows can touch,
     // into LDS with coalesced loads, and the rows gather from there instead
     // of sending one scattered L2 request per nonzero.  Blocks in
     // XCD-contiguous order, so neighbouring windows overlap in one L2.
     // Dynamic LDS: (kLdsRows + 2 maxoff) * sizeof(DT) bytes.  Same sums, in
     // the same order, as ell_spmv_r2.
     //
-    // SORTED: the arrays hold each 512-row block's rows in descending order
+    // SORTED: the arrays hold each block's rows in descending order
     // of length (a copy built at upload) and perm[p] is the block-local
     // original row of storage position p.  Slot k of a wave's rows is then
     // used by a prefix of its lanes, so the cache lines of slots past a
     // row's length are not fetched; each row keeps its own slot order and
     // its result goes to its original index.
     template <typename DT, typename IT, typename LT, int MAXK, int MODE, bool SORTED = false>
-    __global__ __launch_bounds__(256) void ell_spmv_lds(
+    __global__ __launch_bounds__(lds_rows<DT>() / 2) void ell_spmv_lds(
             const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
             const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
             const DT * __restrict__ dinv, uint64_t rows, uint64_t cols, uint64_t stride, uint32_t ncpr,
             uint64_t maxoff, const uint16_t * __restrict__ perm)
     {
+        constexpr int kLdsRows = lds_rows<DT>();
+        constexpr int kLdsThreads = kLdsRows / 2;
         extern __shared__ __align__(16) unsigned char lds_raw[];
         DT * xw = reinterpret_cast<DT *>(lds_raw);
         typedef typename Vec2<DT>::type V;
@@ -285,30 +289,30 @@ namespace k
 #pragma unroll
             for (uint32_t i = 0; i < kFill; ++i)
             {
-                const uint32_t j = i * 256 + threadIdx.x;
+                const uint32_t j = i * kLdsThreads + threadIdx.x;
                 wv[i] = x[w0 + (j < wn ? j : wn - 1)];
             }
 #pragma unroll
             for (uint32_t i = 0; i < kFill; ++i)
             {
-                const uint32_t j = i * 256 + threadIdx.x;
+                const uint32_t j = i * kLdsThreads + threadIdx.x;
                 if (j < wn)
                     xw[j] = wv[i];
             }
         }
-        for (uint32_t base = kFill * 256; base < wn; base += kFill * 256)
+        for (uint32_t base = kFill * kLdsThreads; base < wn; base += kFill * kLdsThreads)
         {
             DT wv[kFill];
 #pragma unroll
             for (uint32_t i = 0; i < kFill; ++i)
             {
-                const uint32_t j = base + i * 256 + threadIdx.x;
+                const uint32_t j = base + i * kLdsThreads + threadIdx.x;
                 wv[i] = x[w0 + (j < wn ? j : wn - 1)];
             }
 #pragma unroll
             for (uint32_t i = 0; i < kFill; ++i)
             {
-                const uint32_t j = base + i * 256 + threadIdx.x;
+                const uint32_t j = base + i * kLdsThreads + threadIdx.x;
                 if (j < wn)
                     xw[j] = wv[i];
             }

@@ -281,11 +281,11 @@ int hsolve_ell_info(hsolve_ell m, uint64_t *rows, uint64_t *columns, uint64_t *s
 /* Kernel variant of the handle (tuning knob, no reference counterpart):
  * 0 = automatic; else bits 1 = one row per lane (default two), 2 = load all
  * num_cols_per_row slots without waiting for row lengths, 4 = XCD-contiguous
- * block order, 16 = stage the x window of each 512-row block in LDS (used
- * when every stored column lies within a window of at most 48 KiB around
- * its row; automatic for such matrices unless they fill 90% of their
- * slots), 32 = with 16: read a copy whose rows are sorted by length inside
- * each 512-row block, so slots past a row's length are not fetched (the
+ * block order, 16 = stage the x window of each block of 1024 (FP64) or
+ * 512 (FP32) rows in LDS (used when every stored column lies within a
+ * window of at most 48 KiB around its row; automatic for such matrices
+ * unless they fill 90% of their slots), 32 = with 16: read a copy whose rows are sorted by length inside
+ * each such block, so slots past a row's length are not fetched (the
  * copy is built at upload when the LDS form is the automatic choice, and
  * the automatic choice then uses it).  Results are identical for every
  * variant. */
