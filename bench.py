@@ -109,15 +109,17 @@ def spmv_case(hip, P, E, steps, warmup, defect=False, keep_y=None, variant=None)
     return out
 
 
-def csr_case(hip, P, E, steps, warmup, defect=False):
+def csr_case(hip, P, E, steps, warmup, defect=False, variant=0):
     """Product / Defect of the same matrix stored as SparseMatrixCSR
-    (blocksize 1, uint32 offsets and columns in the handle)."""
+    (blocksize 1) through a CSR handle: variant 0 lets the handle use its
+    ELL form where it holds one, variant 1 times the CSR kernel itself."""
     n = E["rows"]
     M = P.ell_to_csr(E)
     dtype = M["Ax"].dtype
     s = dtype.itemsize
     z = M["nblocks"]
     H = hip.CsrMatrix.from_dict(M)
+    H.set_variant(variant)
     del M
     x = hip.DeviceVector.from_numpy(
         P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype))
@@ -129,7 +131,7 @@ def csr_case(hip, P, E, steps, warmup, defect=False):
     ms = timed(lambda: H.spmv(x, y, rhs=rhs), steps, warmup)
     nbytes = P.csr_spmv_bytes(n, z, s) + (n * s if defect else 0)
     out = dict(rows=n, nnz=z, ms=ms, bytes=nbytes, GBps=nbytes / (ms * 1e-3) / 1e9,
-               device_bytes=H.device_bytes)
+               device_bytes=H.device_bytes, kernel="ell form" if (H.ell_form and variant == 0) else "csr")
     out["frac_hbm"] = out["GBps"] / HBM_PEAK_GBS
     del H, x, y, rhs
     return out
@@ -504,12 +506,15 @@ def main():
             out["copy_kernel_2GiB"] = dict(bytes=2 * nb, ms=ms, GBps=2 * nb / (ms * 1e-3) / 1e9)
             del src, dst
         out.update({"csr_spmv_matrix_a_f64": csr_case(hip, P, E, args.steps, args.warmup),
-               "csr_defect_matrix_a_f64": csr_case(hip, P, E, args.steps, args.warmup, defect=True)})
+                    "csr_spmv_matrix_a_f64_csr_kernel": csr_case(hip, P, E, args.steps, args.warmup, variant=1),
+                    "csr_defect_matrix_a_f64_csr_kernel": csr_case(hip, P, E, args.steps, args.warmup, defect=True,
+                                                                   variant=1)})
         E["Ax"] = E["Ax"].astype(np.float32)
-        out["csr_spmv_matrix_a_f32"] = csr_case(hip, P, E, args.steps, args.warmup)
+        out["csr_spmv_matrix_a_f32_csr_kernel"] = csr_case(hip, P, E, args.steps, args.warmup, variant=1)
         del E
         EB = P.matrix_b_ell(1 << 24)
         out["csr_spmv_matrix_b_f64"] = csr_case(hip, P, EB, args.steps, args.warmup)
+        out["csr_spmv_matrix_b_f64_csr_kernel"] = csr_case(hip, P, EB, args.steps, args.warmup, variant=1)
         print(json.dumps(out))
         return
     if args.diffusion_only:
@@ -565,12 +570,14 @@ def main():
         extra["ell_defect_matrix_a_f32"] = spmv_case(hip, P, E32, args.steps, args.warmup, defect=True)
         del E32
         extra["csr_spmv_matrix_a_f64"] = csr_case(hip, P, E64, args.steps, args.warmup)
+        extra["csr_spmv_matrix_a_f64_csr_kernel"] = csr_case(hip, P, E64, args.steps, args.warmup, variant=1)
         log("matrix A extras done")
         t1 = time.perf_counter()
         EB = P.matrix_b_ell(1 << 24)
         log(f"Matrix B built in {time.perf_counter() - t1:.1f}s")
         extra["ell_spmv_matrix_b_f64"] = spmv_case(hip, P, EB, args.steps, args.warmup)
         extra["csr_spmv_matrix_b_f64"] = csr_case(hip, P, EB, args.steps, args.warmup)
+        extra["csr_spmv_matrix_b_f64_csr_kernel"] = csr_case(hip, P, EB, args.steps, args.warmup, variant=1)
         EB["Ax"] = EB["Ax"].astype(np.float32)
         extra["ell_spmv_matrix_b_f32"] = spmv_case(hip, P, EB, args.steps, args.warmup)
         del EB

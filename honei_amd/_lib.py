@@ -127,6 +127,8 @@ _SIGS = {
     "hsolve_csr_info": [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64),
                         C.POINTER(u64)],
     "hsolve_csr_spmv": [vp, vp, vp, vp, u64, u64, vp],
+    "hsolve_csr_set_variant": [vp, C.c_int],
+    "hsolve_csr_form": [vp, C.POINTER(C.c_int)],
     "hsolve_cg_ell_multi": [vp, u64, vp, vp, vp, u64, dbl, vp, vp, vp],
     "hsolve_rhs_uniform_multi": [vp, u64, u64, u64, u64, vp, u64, u64, vp],
     "hsolve_mg_create": [C.POINTER(MGParams), vp, vp, vp, C.POINTER(vp)],

@@ -20,6 +20,7 @@
 #include "hsolve.h"
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 namespace hsolve
@@ -219,6 +220,11 @@ struct hsolve_csr_s
     void * blocks;      // row-block table: (first row, first element) pairs, nblk + 1 of them
     uint64_t nblk;
     std::vector<uint64_t> first_rows;   // host copy of the table's first rows (nblk + 1)
+    // ELL form of the same matrix (blocksize 1, short rows of similar
+    // length): a row's entries become its slots in the same order, so the
+    // ELL kernels give the same sums; used for full-range products
+    hsolve_ell ell = nullptr;
+    int variant = 0;                    // 0 automatic, 1 CSR kernel only
     uint64_t device_bytes;
 };
 
@@ -429,6 +435,42 @@ int hsolve_csr_create(int dtype, uint64_t rows, uint64_t columns, uint64_t block
         throw;
     }
     m->device_bytes = is * (rows + 1) + js * nb + vs * nb * blocksize + is * rb.size();
+    // the ELL form: blocksize 1, every column below `columns`, rows of at
+    // most 16 entries and at most 50% padding
+    if (blocksize == 1 && rows > 0 && nb > 0 && columns <= 0xFFFFFFFFull && rows <= 0xFFFFFFFFull)
+    {
+        uint64_t maxlen = 0;
+        bool ok = true;
+        for (uint64_t r = 0; r < rows; ++r)
+            maxlen = std::max<uint64_t>(maxlen, Ar_host[r + 1] - Ar_host[r]);
+        for (uint64_t i = 0; ok && i < nb; ++i)
+            ok = Aj_host[i] < columns;
+        const uint64_t stride = 32 * ((rows + 31) / 32);
+        if (ok && maxlen >= 1 && maxlen <= 16 && rows * maxlen * 2 <= nb * 3)
+        {
+            std::vector<uint64_t> eaj(maxlen * stride, 0), arl(rows);
+            std::vector<uint8_t> eax(vs * maxlen * stride, 0);
+            const uint8_t * ax = (const uint8_t *)Ax_host;
+            for (uint64_t r = 0; r < rows; ++r)
+            {
+                arl[r] = Ar_host[r + 1] - Ar_host[r];
+                for (uint64_t k = 0; k < arl[r]; ++k)
+                {
+                    eaj[k * stride + r] = Aj_host[Ar_host[r] + k];
+                    std::memcpy(&eax[(k * stride + r) * vs], &ax[(Ar_host[r] + k) * vs], vs);
+                }
+            }
+            hsolve_ell e = nullptr;
+            if (hsolve_ell_create(dtype, rows, columns, stride, maxlen, 1, eaj.data(), eax.data(), arl.data(), &e) ==
+                HSOLVE_OK)
+            {
+                uint64_t eb = 0;
+                (void)hsolve_ell_info(e, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &eb);
+                m->ell = e;
+                m->device_bytes += eb;
+            }
+        }
+    }
     *out = m;
     HSOLVE_API_END
 }
@@ -442,6 +484,8 @@ int hsolve_csr_destroy(hsolve_csr m)
         HIP_CHECK(hipFree(m->Aj));
         HIP_CHECK(hipFree(m->Ax));
         HIP_CHECK(hipFree(m->blocks));
+        if (m->ell)
+            (void)hsolve_ell_destroy(m->ell);
         delete m;
     }
     HSOLVE_API_END
@@ -461,6 +505,23 @@ int hsolve_csr_info(hsolve_csr m, uint64_t * rows, uint64_t * columns, uint64_t 
     HSOLVE_API_END
 }
 
+int hsolve_csr_set_variant(hsolve_csr m, int variant)
+{
+    HSOLVE_API_BEGIN
+    require(m != nullptr, HSOLVE_ERR_INVALID, "csr_set_variant: null handle");
+    require(variant == 0 || variant == 1, HSOLVE_ERR_INVALID, "csr_set_variant: variant must be 0 or 1");
+    m->variant = variant;
+    HSOLVE_API_END
+}
+
+int hsolve_csr_form(hsolve_csr m, int * form)
+{
+    HSOLVE_API_BEGIN
+    require(m && form, HSOLVE_ERR_INVALID, "csr_form: null pointer");
+    *form = m->ell ? 1 : 0;
+    HSOLVE_API_END
+}
+
 int hsolve_csr_spmv(hsolve_csr m, const void * x, void * y, const void * rhs, uint64_t row_start, uint64_t row_end,
                     void * stream)
 {
@@ -470,6 +531,8 @@ int hsolve_csr_spmv(hsolve_csr m, const void * x, void * y, const void * rhs, ui
     if (row_end == 0)
         row_end = m->rows;
     require(row_start <= row_end && row_end <= m->rows, HSOLVE_ERR_INVALID, "csr_spmv: row range out of bounds");
+    if (m->ell && m->variant == 0 && row_start == 0 && row_end == m->rows)
+        return hsolve_ell_spmv(m->ell, x, y, rhs, stream);
     hipStream_t st = as_stream(stream);
     if (m->dtype)
         csr_handle_launch<double>(m, x, y, rhs, row_start, row_end, st);

@@ -200,6 +200,17 @@ class CsrMatrix:
     def from_dict(cls, M):
         return cls(M["rows"], M["cols"], M["bs"], M["Ar"], M["Aj"], M["Ax"], M.get("used", 0))
 
+    def set_variant(self, variant: int):
+        """0 automatic, 1 always the CSR kernel."""
+        check(lib().hsolve_csr_set_variant(vp(self.handle), int(variant)), "csr_set_variant")
+
+    @property
+    def ell_form(self):
+        """True when full-range products run on the handle's ELL form."""
+        f = C.c_int()
+        check(lib().hsolve_csr_form(vp(self.handle), C.byref(f)), "csr_form")
+        return bool(f.value)
+
     def spmv(self, x, y, rhs=None, row_start=0, row_end=0, stream=None):
         """y = A x (or rhs - A x) on rows [row_start, row_end), row_end 0 = all rows."""
         check(lib().hsolve_csr_spmv(vp(self.handle), vp(x.ptr), vp(y.ptr), vp(rhs.ptr) if rhs else None,

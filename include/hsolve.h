@@ -260,9 +260,17 @@ int hsolve_csr_destroy(hsolve_csr m);
 int hsolve_csr_info(hsolve_csr m, uint64_t *rows, uint64_t *columns, uint64_t *blocksize,
                     uint64_t *nblocks, uint64_t *used_elements, uint64_t *device_bytes);
 /* y = A x, or y = rhs - A x when rhs != NULL, on rows [row_start, row_end)
- * (row_end 0 = all rows); other rows of y are left as they are. */
+ * (row_end 0 = all rows); other rows of y are left as they are.  With
+ * blocksize 1, rows of at most 16 entries and at most 50% padding, the
+ * upload also builds an ELL handle of the same matrix (each row's entries
+ * as its slots, in order) and full-range products run on it; results are
+ * identical either way. */
 int hsolve_csr_spmv(hsolve_csr m, const void *x, void *y, const void *rhs, uint64_t row_start,
                     uint64_t row_end, void *stream);
+/* 0 = automatic, 1 = always the CSR kernel (tuning knob, no reference
+ * counterpart); hsolve_csr_form reports 1 when the handle holds the ELL form. */
+int hsolve_csr_set_variant(hsolve_csr m, int variant);
+int hsolve_csr_form(hsolve_csr m, int *form);
 
 /* ---------------- device-resident ELL handle -------------------------------- */
 typedef struct hsolve_ell_s *hsolve_ell;

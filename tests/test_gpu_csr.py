@@ -54,8 +54,9 @@ def _raw(hip, M, x, rhs=None, row_start=0, row_end=None, y0=None):
     return yd.get()[:rows]
 
 
-def _handle(hip, M, x, rhs=None, row_start=0, row_end=0, y0=None):
+def _handle(hip, M, x, rhs=None, row_start=0, row_end=0, y0=None, variant=0):
     H = hip.CsrMatrix.from_dict(M)
+    H.set_variant(variant)
     rows = M["rows"]
     xd = hip.DeviceVector.from_numpy(x)
     yd = hip.DeviceVector.from_numpy(np.full(max(rows, 1), 4711, M["Ax"].dtype) if y0 is None else y0)
@@ -108,6 +109,11 @@ def test_csr_from_ell_fixtures(hip, oracle, rng, dtype, name):
         assert np.array_equal(_handle(hip, M, x), ref)
         assert np.array_equal(_raw(hip, M, x, rhs), refd)
         assert np.array_equal(_handle(hip, M, x, rhs), refd)
+        # the CSR kernel itself when the handle also holds the ELL form
+        assert np.array_equal(_handle(hip, M, x, variant=1), ref)
+        assert np.array_equal(_handle(hip, M, x, rhs, variant=1), refd)
+        if bs == 1 and name == "poisson/A_6.ell":   # at most 16 entries per row, at most 50% padding
+            assert hip.CsrMatrix.from_dict(M).ell_form
 
 
 @pytest.mark.parametrize("dtype", DTYPES)

@@ -92,12 +92,19 @@ def test_csr_equals_ell_full_size(hip, which):
     rhs = P.rng_uniform(P.SEED, P.S_RHS, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype)
     dx, dr = hip.DeviceVector.from_numpy(x), hip.DeviceVector.from_numpy(rhs)
     ye, yc = hip.DeviceVector.zeros(n, dtype), hip.DeviceVector.zeros(n, dtype)
+    assert C.ell_form
     A.spmv(dx, ye)
-    C.spmv(dx, yc)
-    got = yc.get()
-    assert np.array_equal(got, ye.get())
-    rows = _sample(n, np.random.default_rng(3))
-    assert np.array_equal(got[rows], _host_rows(E, x, rows))
+    ref = ye.get()
+    for variant in (0, 1):                 # the handle's ELL form, then the CSR kernel itself
+        C.set_variant(variant)
+        yc.fill(0.0)
+        C.spmv(dx, yc)
+        got = yc.get()
+        assert np.array_equal(got, ref), variant
+        rows = _sample(n, np.random.default_rng(3))
+        assert np.array_equal(got[rows], _host_rows(E, x, rows))
     A.spmv(dx, ye, rhs=dr)
-    C.spmv(dx, yc, rhs=dr)
-    assert np.array_equal(yc.get(), ye.get())
+    for variant in (0, 1):
+        C.set_variant(variant)
+        C.spmv(dx, yc, rhs=dr)
+        assert np.array_equal(yc.get(), ye.get()), variant
