@@ -459,6 +459,10 @@ def main():
                     help="only time REPS MG-IR solves on the 4097^2 Poisson problem (for profiling)")
     ap.add_argument("--serve-only", type=int, default=0, metavar="REQUESTS",
                     help="only run the served mixed-size stream with REQUESTS requests")
+    ap.add_argument("--serve-budget-gib", type=int, default=32,
+                    help="with --serve-only: device-memory budget of the request server (GiB)")
+    ap.add_argument("--serve-max-slots", type=int, default=64,
+                    help="with --serve-only: slots per grid level of the request server")
     ap.add_argument("--diffusion-only", type=int, default=0, metavar="LEVEL",
                     help="only run the 64-RHS variable-coefficient diffusion case on the 2^LEVEL+1 grid")
     ap.add_argument("--no-pcg", action="store_true", help="with --diffusion-only: MG-IR leg only")
@@ -526,7 +530,8 @@ def main():
         return
     if args.serve_only:
         from honei_amd import device as hip
-        print(json.dumps({"serve_mixed_poisson": serve_case(hip, args.serve_only)}))
+        print(json.dumps({"serve_mixed_poisson": serve_case(hip, args.serve_only, budget_gib=args.serve_budget_gib,
+                                                            max_slots=args.serve_max_slots)}))
         return
     if args.mg_only:
         from honei_amd import device as hip
