@@ -372,6 +372,12 @@ static void test_cg()
     TEST_CHECK(bit_equal(cx, ccx.data()));
 }
 
+static void or_csr_from_rows(uint64_t rows, const uint64_t * rp, const uint64_t * c, const double * v, uint64_t bs,
+                             uint64_t * Ar, uint64_t * Aj, double * Ax, uint64_t * nb, uint64_t * used)
+{ or_csr_from_rows_f64(rows, rp, c, v, bs, Ar, Aj, Ax, nb, used); }
+static void or_csr_from_rows(uint64_t rows, const uint64_t * rp, const uint64_t * c, const float * v, uint64_t bs,
+                             uint64_t * Ar, uint64_t * Aj, float * Ax, uint64_t * nb, uint64_t * used)
+{ or_csr_from_rows_f32(rows, rp, c, v, bs, Ar, Aj, Ax, nb, used); }
 static void or_csr_product(double * y, const uint64_t * Ar, const uint64_t * Aj, const double * Ax, const double * x,
                            uint64_t cols, uint64_t bs, uint64_t rs, uint64_t re)
 { or_csr_product_f64(y, Ar, Aj, Ax, x, cols, bs, rs, re); }
@@ -395,6 +401,36 @@ template <typename DT> static void test_csr()
                     sms(i / (size + 3), i % (size + 3), DT(i) / DT(1.234));
             SparseMatrixCSR<DT> sm0(sms);
             TEST_CHECK(sm0.blocksize() == blocks);
+            {
+                // the container's block grouping equals the oracle's restatement
+                // of sparse_matrix_csr-impl.hh:98-136, array for array
+                std::vector<uint64_t> rp(1, 0), cl;
+                std::vector<DT> vl;
+                for (unsigned long i(0) ; i < size * (size + 3) ; ++i)
+                {
+                    if (i % (size + 3) == 0 && i > 0)
+                        rp.push_back(cl.size());
+                    if (i % 5 == 0)
+                    {
+                        cl.push_back(i % (size + 3));
+                        vl.push_back(DT(i) / DT(1.234));
+                    }
+                }
+                rp.push_back(cl.size());
+                std::vector<uint64_t> ar(size + 1), aj(cl.size() + 1);
+                std::vector<DT> ax((cl.size() + 1) * blocks);
+                uint64_t nblk = 0, used = 0;
+                or_csr_from_rows(size, rp.data(), cl.data(), vl.data(), blocks, ar.data(), aj.data(), ax.data(), &nblk,
+                                 &used);
+                bool same = used == sm0.used_elements() && nblk == sm0.Ar()[size];
+                for (unsigned long r(0) ; same && r <= size ; ++r)
+                    same = ar[r] == sm0.Ar()[r];
+                for (uint64_t i(0) ; same && i < nblk ; ++i)
+                    same = aj[i] == sm0.Aj()[i];
+                for (uint64_t i(0) ; same && i < nblk * blocks ; ++i)
+                    same = std::memcmp(&ax[i], &sm0.Ax().read_elements()[i], sizeof(DT)) == 0;
+                TEST_CHECK(same);
+            }
             DenseVector<DT> dv1(size + 3, DT(4));
             dv1[0] = 1;
             dv1[1] = 2;
