@@ -166,13 +166,24 @@ namespace k
         DT s0 = DT(0), s1 = DT(0);
         if (implied)
         {
+            // the two rows' 3 x 3 neighbourhoods share three windows of four
+            // x values (12 loads instead of 18); slot k = 3 (dy + 1) + dx + 1
+            DT w[3][4];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                const int64_t c = (int64_t)r0 + (int64_t)(d - 1) * m - 1;
+                w[d][0] = x[c];
+                w[d][1] = x[c + 1];
+                w[d][2] = x[c + 2];
+                w[d][3] = x[two ? c + 3 : c + 2];
+            }
 #pragma unroll
             for (int k = 0; k < MAXK; ++k)
             {
-                const int64_t off = (int64_t)(k / 3 - 1) * m + (k % 3 - 1);
-                s0 += a0[k] * x[(int64_t)r0 + off];
+                s0 += a0[k] * w[k / 3][k % 3];
                 if (two)
-                    s1 += a1[k] * x[(int64_t)r0 + 1 + off];
+                    s1 += a1[k] * w[k / 3][k % 3 + 1];
             }
         }
         else
