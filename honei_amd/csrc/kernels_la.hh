@@ -138,6 +138,23 @@ namespace k
         if (r0 >= row_end)
             return;
         const bool two = r0 + 1 < row_end;
+        // the defect / sweep operands of both rows load up front (the second
+        // row's index clamped to the first when there is none), not after
+        // the sums
+        const uint64_t r1 = two ? r0 + 1 : r0;
+        DT b0 = DT(0), b1 = DT(0), x0 = DT(0), x1 = DT(0), d0 = DT(0), d1 = DT(0);
+        if (MODE != 0)
+        {
+            b0 = rhs[r0];
+            b1 = rhs[r1];
+        }
+        if (MODE == 2)
+        {
+            x0 = x[r0];
+            x1 = x[r1];
+            d0 = dinv[r0];
+            d1 = dinv[r1];
+        }
         const uint32_t len0 = (uint32_t)Arl[r0];
         const uint32_t len1 = two ? (uint32_t)Arl[r0 + 1] : 0u;
         const uint32_t lmax = len0 > len1 ? len0 : len1;
@@ -205,18 +222,18 @@ namespace k
         }
         else if (MODE == 1)
         {
-            y[r0] = rhs[r0] - s0;
+            y[r0] = b0 - s0;
             if (two)
-                y[r0 + 1] = rhs[r0 + 1] - s1;
+                y[r0 + 1] = b1 - s1;
         }
         else
         {
-            const DT t0 = rhs[r0] - s0;
-            y[r0] = x[r0] + t0 * dinv[r0];
+            const DT t0 = b0 - s0;
+            y[r0] = x0 + t0 * d0;
             if (two)
             {
-                const DT t1 = rhs[r0 + 1] - s1;
-                y[r0 + 1] = x[r0 + 1] + t1 * dinv[r0 + 1];
+                const DT t1 = b1 - s1;
+                y[r0 + 1] = x1 + t1 * d1;
             }
         }
     }
