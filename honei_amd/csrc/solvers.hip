@@ -148,14 +148,40 @@ namespace
     }
 
     // x += lambda p; r += (-lambda) v; partials of r.r (and r.(P r) for VAR)
-    template <typename DT, bool VAR>
+    // FUSED (small grids): every block first forms temp = v.p from the SpMV's
+    // partials itself (the same sum in the same order in every block) and
+    // lambda from it, so cg_spmv_finish is not launched; block 0 stores
+    // temp and lambda.  The SpMV's partials are read from sp_partials, this
+    // kernel's own go to `partials`, a different buffer.
+    template <typename DT, bool VAR, bool FUSED = false>
     __global__ __launch_bounds__(256) void cg_update_kernel(DT * __restrict__ x, DT * __restrict__ r,
             const DT * __restrict__ p, const DT * __restrict__ v, const DT * __restrict__ P, int64_t n,
-            double * partials, const CgScalars<DT> * s)
+            double * partials, CgScalars<DT> * s, const double * sp_partials = nullptr, unsigned sp_nb = 0)
     {
         if (s->run == 0)
             return;
-        const DT lambda = s->lambda;
+        DT lambda;
+        if (FUSED)
+        {
+            __shared__ DT sh_lambda;
+            double vp, unused;
+            k::sum_partials2(sp_partials, sp_nb, vp, unused);
+            if (threadIdx.x == 0)
+            {
+                const DT temp = (DT)vp;
+                const DT lam = VAR ? s->alpha_new / guard_signed<DT>(temp) : s->alpha / guard<DT>(temp);
+                sh_lambda = lam;
+                if (blockIdx.x == 0)
+                {
+                    s->temp = temp;
+                    s->lambda = lam;
+                }
+            }
+            __syncthreads();
+            lambda = sh_lambda;
+        }
+        else
+            lambda = s->lambda;
         const DT mlambda = -lambda;
         double acc_rr = 0.0, acc_rz = 0.0;
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -379,13 +405,14 @@ namespace
     {
         DT *r = nullptr, *p = nullptr, *v = nullptr;
         double * partials = nullptr;
+        double * partials2 = nullptr;     // a second set, for kernels that read one set and write the other
         CgScalars<DT> * s = nullptr;
         void * block = nullptr;
 
         explicit CgWork(uint64_t n)
         {
             const size_t vb = ((sizeof(DT) * n + 255) / 256) * 256;
-            const size_t bytes = 3 * vb + sizeof(double) * 2 * kSolverBlocks + sizeof(CgScalars<DT>) + 256;
+            const size_t bytes = 3 * vb + sizeof(double) * 4 * kSolverBlocks + sizeof(CgScalars<DT>) + 256;
             hipError_t e = hipMalloc(&block, bytes);
             if (e != hipSuccess)
                 throw Error(HSOLVE_ERR_ALLOC, "CG workspace allocation failed");
@@ -394,6 +421,7 @@ namespace
             p = (DT *)c; c += vb;
             v = (DT *)c; c += vb;
             partials = (double *)c; c += sizeof(double) * 2 * kSolverBlocks;
+            partials2 = (double *)c; c += sizeof(double) * 2 * kSolverBlocks;
             s = (CgScalars<DT> *)c;
         }
         ~CgWork()
@@ -414,6 +442,7 @@ namespace
         h.max_iters = max_iters;
         HIP_CHECK(hipMemcpyAsync(w.s, &h, sizeof(h), hipMemcpyHostToDevice, st));
         const unsigned grid = std::min<unsigned>(grid_for(n, kBlock), kSolverBlocks);
+        const bool fused = grid <= 512;
         hipLaunchKernelGGL((cg_init_kernel<DT, Op, VAR>), dim3(grid), dim3(kBlock), 0, st, A, b, x, w.r, w.p, P,
                            (int64_t)n, w.partials);
         hipLaunchKernelGGL((cg_init_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials, grid,
@@ -429,12 +458,23 @@ namespace
             {
                 hipLaunchKernelGGL((cg_spmv_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A, w.p, w.v,
                                    (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
-                hipLaunchKernelGGL((cg_spmv_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
-                                   (const double *)w.partials, grid, w.s);
-                hipLaunchKernelGGL((cg_update_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, x, w.r, w.p, w.v, P,
-                                   (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
-                hipLaunchKernelGGL((cg_update_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
-                                   (const double *)w.partials, grid, w.s);
+                if (fused)
+                {
+                    // small grids (launch-bound solves): lambda formed inside the update
+                    hipLaunchKernelGGL((cg_update_kernel<DT, VAR, true>), dim3(grid), dim3(kBlock), 0, st, x, w.r,
+                                       w.p, w.v, P, (int64_t)n, w.partials2, w.s, (const double *)w.partials, grid);
+                    hipLaunchKernelGGL((cg_update_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
+                                       (const double *)w.partials2, grid, w.s);
+                }
+                else
+                {
+                    hipLaunchKernelGGL((cg_spmv_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
+                                       (const double *)w.partials, grid, w.s);
+                    hipLaunchKernelGGL((cg_update_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, x, w.r, w.p,
+                                       w.v, P, (int64_t)n, w.partials, w.s, (const double *)nullptr, 0u);
+                    hipLaunchKernelGGL((cg_update_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
+                                       (const double *)w.partials, grid, w.s);
+                }
                 hipLaunchKernelGGL((cg_direction_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, w.p, w.r, P,
                                    (int64_t)n, (const CgScalars<DT> *)w.s);
             }
