@@ -51,7 +51,8 @@ namespace hsolve
         const LT * Arl;
         uint64_t stride;
         uint32_t threads;
-        __device__ __forceinline__ DT row(const DT * __restrict__ x, int64_t i) const
+        // x: a pointer, or any value whose x[j] yields the j-th entry
+        template <typename XV> __device__ __forceinline__ DT row(const XV & x, int64_t i) const
         {
             const uint32_t len = (uint32_t)Arl[i];
             if (threads == 1)

@@ -39,8 +39,8 @@ namespace k
     // ascending slot order as product.hh:192-202.  MAXK (compile-time bound
     // on num_cols_per_row) lets every slot's loads be issued before the
     // dependent sum, for memory-level parallelism.
-    template <typename DT, typename IT, int MAXK>
-    __device__ __forceinline__ DT ell_row_sum(const DT * __restrict__ x, const IT * __restrict__ aj,
+    template <typename DT, typename IT, int MAXK, typename XV>
+    __device__ __forceinline__ DT ell_row_sum(const XV & x, const IT * __restrict__ aj,
             const DT * __restrict__ ax, uint32_t len, uint64_t row, uint64_t stride)
     {
         DT sum = DT(0);
@@ -422,9 +422,9 @@ namespace k
     // zero), for bands that hold a 9-point structured ELL matrix with zeros
     // at its absent entries: a zero entry adds +-0 to a sum that is never
     // -0, so with finite x the result is the ELL product's, bit for bit.
-    template <typename DT, int ORD = 0>
-    __device__ __forceinline__ DT q1_row(const Bands & B, const DT * __restrict__ x, int64_t i,
-                                         int64_t n, int64_t m)
+    // x: a pointer, or any value whose x[j] yields the j-th entry
+    template <typename DT, int ORD = 0, typename XV>
+    __device__ __forceinline__ DT q1_row(const Bands & B, const XV & x, int64_t i, int64_t n, int64_t m)
     {
         const DT * const * b = reinterpret_cast<const DT * const *>(B.b);
         if (ORD == 1)

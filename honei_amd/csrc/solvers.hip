@@ -32,7 +32,7 @@ namespace
     {
         k::Bands B;
         int64_t n, m;
-        __device__ __forceinline__ DT row(const DT * __restrict__ x, int64_t i) const
+        template <typename XV> __device__ __forceinline__ DT row(const XV & x, int64_t i) const
         {
             return k::q1_row<DT>(B, x, i, n, m);
         }
@@ -197,7 +197,50 @@ namespace
         k::put_partials2(acc_rr, acc_rz, partials, blockIdx.x);
     }
 
-    // the stopping test and the next direction's factor
+    // the stopping test after an update and the next direction's factor,
+    // on a copy of the scalars (rr = r.r, rz = r.(P r) for VAR)
+    template <typename DT, bool VAR>
+    __device__ __forceinline__ void cg_test(CgScalars<DT> & s, double rr, double rz)
+    {
+        const DT eps = s.eps_relative;
+        if (VAR)
+        {
+            ++s.iters;
+            const DT cur = (DT)sqrt((DT)rr);
+            s.current_defect = cur;
+            if (cur < eps * s.initial_defect || cur < eps)
+            {
+                s.converged = 1;
+                s.run = 0;
+                return;
+            }
+            if (s.iters == s.max_iters)
+            {
+                s.run = 0;
+                return;
+            }
+            s.alpha = s.alpha_new;
+            s.alpha_new = (DT)rz;
+            s.talpha = s.alpha_new / guard_signed<DT>(s.alpha);
+        }
+        else
+        {
+            s.alpha_old = s.alpha;
+            s.alpha = (DT)rr;
+            s.talpha = s.alpha / guard<DT>(s.alpha_old);
+            ++s.iters;
+            const DT cur = (DT)sqrt(s.alpha);
+            s.current_defect = cur;
+            if (cur < eps * s.initial_defect || cur < eps)
+            {
+                s.converged = 1;
+                s.run = 0;
+            }
+            else if (s.iters == s.max_iters)
+                s.run = 0;
+        }
+    }
+
     template <typename DT, bool VAR>
     __global__ __launch_bounds__(256) void cg_update_finish(const double * partials, unsigned nb, CgScalars<DT> * s)
     {
@@ -207,45 +250,51 @@ namespace
         k::sum_partials2(partials, nb, rr, rz);
         if (threadIdx.x != 0)
             return;
+        CgScalars<DT> t = *s;
+        cg_test<DT, VAR>(t, rr, rz);
+        *s = t;
+    }
+
+    // The next direction p * talpha + r (NONE) or p * talpha + P*r (VAR),
+    // formed where it is read: x[j] of an operator row.
+    template <typename DT, bool VAR> struct NextDir
+    {
+        const DT * p;
+        const DT * r;
+        const DT * P;
+        DT talpha;
+        __device__ __forceinline__ DT operator[](int64_t j) const
         {
-            const DT eps = s->eps_relative;
-            if (VAR)
-            {
-                ++s->iters;
-                const DT cur = (DT)sqrt((DT)rr);
-                s->current_defect = cur;
-                if (cur < eps * s->initial_defect || cur < eps)
-                {
-                    s->converged = 1;
-                    s->run = 0;
-                    return;
-                }
-                if (s->iters == s->max_iters)
-                {
-                    s->run = 0;
-                    return;
-                }
-                s->alpha = s->alpha_new;
-                s->alpha_new = (DT)rz;
-                s->talpha = s->alpha_new / guard_signed<DT>(s->alpha);
-            }
-            else
-            {
-                s->alpha_old = s->alpha;
-                s->alpha = (DT)rr;
-                s->talpha = s->alpha / guard<DT>(s->alpha_old);
-                ++s->iters;
-                const DT cur = (DT)sqrt(s->alpha);
-                s->current_defect = cur;
-                if (cur < eps * s->initial_defect || cur < eps)
-                {
-                    s->converged = 1;
-                    s->run = 0;
-                }
-                else if (s->iters == s->max_iters)
-                    s->run = 0;
-            }
+            const DT t = p[j] * talpha;
+            return t + (VAR ? P[j] * r[j] : r[j]);
         }
+    };
+
+    // Small grids (launch-bound solves): the direction update folded into
+    // the next SpMV.  p_new = p_old * talpha + (P)r is formed at every read,
+    // with the operations of cg_direction_kernel, so every block sees the
+    // same values; each row stores its own p_new into the other direction
+    // buffer (p_old stays intact while other blocks still read it).  talpha
+    // and the stop flag come from cg_update_finish.
+    template <typename DT, typename Op, bool VAR>
+    __global__ __launch_bounds__(256) void cg_spmv_dir_kernel(Op A, const DT * __restrict__ p_old,
+            const DT * __restrict__ r, const DT * __restrict__ P, DT * __restrict__ p_new, DT * __restrict__ v,
+            int64_t n, double * partials, const CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        const NextDir<DT, VAR> d{p_old, r, P, s->talpha};
+        double acc = 0.0;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        {
+            const DT vi = A.row(d, i);
+            const DT pi = d[i];
+            p_new[i] = pi;
+            v[i] = vi;
+            acc += (double)vi * (double)pi;
+        }
+        k::put_partials2(acc, 0.0, partials, blockIdx.x);
     }
 
     // p = p * talpha + r (NONE) or p = p * talpha + P*r (VAR)
@@ -404,6 +453,7 @@ namespace
     template <typename DT> struct CgWork
     {
         DT *r = nullptr, *p = nullptr, *v = nullptr;
+        DT * p2 = nullptr;                // the other direction buffer of cg_spmv_dir_kernel
         double * partials = nullptr;
         double * partials2 = nullptr;     // a second set, for kernels that read one set and write the other
         CgScalars<DT> * s = nullptr;
@@ -412,7 +462,7 @@ namespace
         explicit CgWork(uint64_t n)
         {
             const size_t vb = ((sizeof(DT) * n + 255) / 256) * 256;
-            const size_t bytes = 3 * vb + sizeof(double) * 4 * kSolverBlocks + sizeof(CgScalars<DT>) + 256;
+            const size_t bytes = 4 * vb + sizeof(double) * 4 * kSolverBlocks + sizeof(CgScalars<DT>) + 256;
             hipError_t e = hipMalloc(&block, bytes);
             if (e != hipSuccess)
                 throw Error(HSOLVE_ERR_ALLOC, "CG workspace allocation failed");
@@ -420,6 +470,7 @@ namespace
             r = (DT *)c; c += vb;
             p = (DT *)c; c += vb;
             v = (DT *)c; c += vb;
+            p2 = (DT *)c; c += vb;
             partials = (double *)c; c += sizeof(double) * 2 * kSolverBlocks;
             partials2 = (double *)c; c += sizeof(double) * 2 * kSolverBlocks;
             s = (CgScalars<DT> *)c;
@@ -452,39 +503,72 @@ namespace
         // iteration that starts after the stop is a no-op
         uint64_t enqueued = 0;
         uint64_t chunk = 8;
-        while (true)
+        if (fused)
         {
-            for (uint64_t c = 0; c < chunk && enqueued < max_iters; ++c, ++enqueued)
+            // small grids (launch-bound solves), three launches per
+            // iteration: the SpMV forms the direction itself, the update forms
+            // lambda from the SpMV's partials.  pv[cp] holds the current
+            // direction.
+            DT * pv[2] = {w.p, w.p2};
+            int cp = 0;
+            bool first = true;      // pv[0] still holds the direction cg_init_kernel formed
+            while (true)
             {
-                hipLaunchKernelGGL((cg_spmv_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A, w.p, w.v,
-                                   (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
-                if (fused)
+                for (uint64_t c = 0; c < chunk && enqueued < max_iters; ++c, ++enqueued)
                 {
-                    // small grids (launch-bound solves): lambda formed inside the update
+                    if (first)
+                    {
+                        hipLaunchKernelGGL((cg_spmv_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A, pv[cp],
+                                           w.v, (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
+                        first = false;
+                    }
+                    else
+                    {
+                        hipLaunchKernelGGL((cg_spmv_dir_kernel<DT, Op, VAR>), dim3(grid), dim3(kBlock), 0, st, A,
+                                           (const DT *)pv[cp], (const DT *)w.r, P, pv[cp ^ 1], w.v, (int64_t)n,
+                                           w.partials, (const CgScalars<DT> *)w.s);
+                        cp ^= 1;
+                    }
                     hipLaunchKernelGGL((cg_update_kernel<DT, VAR, true>), dim3(grid), dim3(kBlock), 0, st, x, w.r,
-                                       w.p, w.v, P, (int64_t)n, w.partials2, w.s, (const double *)w.partials, grid);
+                                       pv[cp], w.v, P, (int64_t)n, w.partials2, w.s, (const double *)w.partials,
+                                       grid);
                     hipLaunchKernelGGL((cg_update_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
                                        (const double *)w.partials2, grid, w.s);
                 }
-                else
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipMemcpyAsync(&h, w.s, sizeof(h), hipMemcpyDeviceToHost, st));
+                HIP_CHECK(hipStreamSynchronize(st));
+                if (h.run == 0 || enqueued >= max_iters)
+                    break;
+                if (chunk < 64)
+                    chunk *= 2;
+            }
+        }
+        else
+        {
+            while (true)
+            {
+                for (uint64_t c = 0; c < chunk && enqueued < max_iters; ++c, ++enqueued)
                 {
+                    hipLaunchKernelGGL((cg_spmv_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A, w.p, w.v,
+                                       (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
                     hipLaunchKernelGGL((cg_spmv_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
                                        (const double *)w.partials, grid, w.s);
                     hipLaunchKernelGGL((cg_update_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, x, w.r, w.p,
                                        w.v, P, (int64_t)n, w.partials, w.s, (const double *)nullptr, 0u);
                     hipLaunchKernelGGL((cg_update_finish<DT, VAR>), dim3(1), dim3(kBlock), 0, st,
                                        (const double *)w.partials, grid, w.s);
+                    hipLaunchKernelGGL((cg_direction_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, w.p, w.r,
+                                       P, (int64_t)n, (const CgScalars<DT> *)w.s);
                 }
-                hipLaunchKernelGGL((cg_direction_kernel<DT, VAR>), dim3(grid), dim3(kBlock), 0, st, w.p, w.r, P,
-                                   (int64_t)n, (const CgScalars<DT> *)w.s);
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipMemcpyAsync(&h, w.s, sizeof(h), hipMemcpyDeviceToHost, st));
+                HIP_CHECK(hipStreamSynchronize(st));
+                if (h.run == 0 || enqueued >= max_iters)
+                    break;
+                if (chunk < 64)
+                    chunk *= 2;
             }
-            HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipMemcpyAsync(&h, w.s, sizeof(h), hipMemcpyDeviceToHost, st));
-            HIP_CHECK(hipStreamSynchronize(st));
-            if (h.run == 0 || enqueued >= max_iters)
-                break;
-            if (chunk < 64)
-                chunk *= 2;
         }
         if (used_iters)
             *used_iters = h.iters;
