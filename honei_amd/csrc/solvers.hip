@@ -371,14 +371,36 @@ namespace
     }
 
     // x += alpha d; r -= alpha t; z = dd * r; partials of z.r and r.r
-    template <typename DT>
+    // FUSED (small grids): every block forms d.t from the SpMV's partials
+    // (sp_partials, the same sum in the same order in every block) and alpha
+    // from it, so jac_spmv_finish is not launched; block 0 stores alpha.
+    // Other blocks read only s->temp and s->run here.
+    template <typename DT, bool FUSED = false>
     __global__ __launch_bounds__(256) void jac_update_kernel(DT * __restrict__ x, DT * __restrict__ r,
             DT * __restrict__ z, const DT * __restrict__ d, const DT * __restrict__ t, const DT * __restrict__ dd,
-            int64_t n, double * partials, const CgScalars<DT> * s)
+            int64_t n, double * partials, CgScalars<DT> * s, const double * sp_partials = nullptr,
+            unsigned sp_nb = 0)
     {
         if (s->run == 0)
             return;
-        const DT alpha = s->alpha;
+        DT alpha;
+        if (FUSED)
+        {
+            __shared__ DT sh_alpha;
+            double dt, unused;
+            k::sum_partials2(sp_partials, sp_nb, dt, unused);
+            if (threadIdx.x == 0)
+            {
+                const DT a = div_guard<DT>(s->temp, (DT)dt);
+                sh_alpha = a;
+                if (blockIdx.x == 0)
+                    s->alpha = a;
+            }
+            __syncthreads();
+            alpha = sh_alpha;
+        }
+        else
+            alpha = s->alpha;
         const DT malpha = -alpha;
         double acc_zr = 0.0, acc_rr = 0.0;
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -431,6 +453,44 @@ namespace
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
             d[i] = beta * d[i] + z[i];
+    }
+
+    // The next direction beta * d + z of jac_direction_kernel, formed where
+    // it is read.
+    template <typename DT> struct JacNextDir
+    {
+        const DT * d;
+        const DT * z;
+        DT beta;
+        __device__ __forceinline__ DT operator[](int64_t j) const
+        {
+            const DT t = beta * d[j];
+            return t + z[j];
+        }
+    };
+
+    // Small grids: the direction update folded into the next SpMV, as
+    // cg_spmv_dir_kernel (t = A d_new, partials of t.d_new; each row stores
+    // its d_new into the other direction buffer).
+    template <typename DT, typename Op>
+    __global__ __launch_bounds__(256) void jac_spmv_dir_kernel(Op A, const DT * __restrict__ d_old,
+            const DT * __restrict__ z, DT * __restrict__ d_new, DT * __restrict__ t, int64_t n, double * partials,
+            const CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        const JacNextDir<DT> nd{d_old, z, s->talpha};
+        double acc = 0.0;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        {
+            const DT ti = A.row(nd, i);
+            const DT di = nd[i];
+            d_new[i] = di;
+            t[i] = ti;
+            acc += (double)ti * (double)di;
+        }
+        k::put_partials2(acc, 0.0, partials, blockIdx.x);
     }
 
     // One device array, freed with the object.
@@ -612,7 +672,43 @@ namespace
                            w.s);
         HIP_CHECK(hipGetLastError());
         uint64_t enqueued = 0, chunk = 8;
-        while (true)
+        // small grids (launch-bound solves): three launches per iteration, as
+        // run_cg; pv[cp] holds the current direction
+        DT * pv[2] = {w.p, w.p2};
+        int cp = 0;
+        bool first = true;
+        while (grid <= 512)
+        {
+            for (uint64_t c = 0; c < chunk && enqueued < max_iters; ++c, ++enqueued)
+            {
+                if (first)
+                {
+                    hipLaunchKernelGGL((cg_spmv_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A,
+                                       (const DT *)pv[cp], w.v, (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
+                    first = false;
+                }
+                else
+                {
+                    hipLaunchKernelGGL((jac_spmv_dir_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A,
+                                       (const DT *)pv[cp], (const DT *)z, pv[cp ^ 1], w.v, (int64_t)n, w.partials,
+                                       (const CgScalars<DT> *)w.s);
+                    cp ^= 1;
+                }
+                hipLaunchKernelGGL((jac_update_kernel<DT, true>), dim3(grid), dim3(kBlock), 0, st, x, w.r, z,
+                                   (const DT *)pv[cp], (const DT *)w.v, dd, (int64_t)n, w.partials2, w.s,
+                                   (const double *)w.partials, grid);
+                hipLaunchKernelGGL((jac_update_finish<DT>), dim3(1), dim3(kBlock), 0, st,
+                                   (const double *)w.partials2, grid, w.s);
+            }
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(&h, w.s, sizeof(h), hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            if (h.run == 0 || enqueued >= max_iters)
+                break;
+            if (chunk < 64)
+                chunk *= 2;
+        }
+        while (grid > 512)
         {
             for (uint64_t c = 0; c < chunk && enqueued < max_iters; ++c, ++enqueued)
             {
@@ -621,8 +717,8 @@ namespace
                 hipLaunchKernelGGL((jac_spmv_finish<DT>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials,
                                    grid, w.s);
                 hipLaunchKernelGGL((jac_update_kernel<DT>), dim3(grid), dim3(kBlock), 0, st, x, w.r, z,
-                                   (const DT *)w.p, (const DT *)w.v, dd, (int64_t)n, w.partials,
-                                   (const CgScalars<DT> *)w.s);
+                                   (const DT *)w.p, (const DT *)w.v, dd, (int64_t)n, w.partials, w.s,
+                                   (const double *)nullptr, 0u);
                 hipLaunchKernelGGL((jac_update_finish<DT>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials,
                                    grid, w.s);
                 hipLaunchKernelGGL((jac_direction_kernel<DT>), dim3(grid), dim3(kBlock), 0, st, w.p, (const DT *)z,
