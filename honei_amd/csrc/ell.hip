@@ -107,7 +107,7 @@ namespace
             using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
             using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
             const size_t lds = lds_window_bytes(m, sizeof(DT));
-            if (r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->ncpr >= 1 && m->ncpr <= 16)
+            if (r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->cols != 0 && m->ncpr >= 1 && m->ncpr <= 16)
             {
                 const unsigned g = grid_for(m->rows, k::lds_rows<DT>());
                 const bool sorted = (v & HSOLVE_ELL_SORT) && m->sAx;

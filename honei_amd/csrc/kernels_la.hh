@@ -275,7 +275,9 @@ namespace k
         const uint64_t w0 = rb0 > maxoff ? rb0 - maxoff : 0;
         const uint64_t wend = rb0 + kLdsRows + maxoff;
         const uint64_t w1 = wend < cols ? wend : cols;
-        const uint32_t wn = (uint32_t)(w1 - w0);
+        // A block whose rows all lie past cols + maxoff (empty tail rows of a
+        // matrix with rows > cols) has an empty window: w1 <= w0.
+        const uint32_t wn = w1 > w0 ? (uint32_t)(w1 - w0) : 0u;
         const uint64_t r0 = rb0 + (uint64_t)threadIdx.x * 2;
         const bool live = r0 < rows;
         const bool two = r0 + 1 < rows;
@@ -312,6 +314,7 @@ namespace k
             c1[k] = (IT)c.y;
         }
         constexpr uint32_t kFill = 16;
+        if (wn != 0)
         {
             DT wv[kFill];
 #pragma unroll

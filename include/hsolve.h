@@ -76,9 +76,11 @@ int hsolve_event_elapsed_ms(void *start, void *stop, float *ms);   /* synchronis
 int hsolve_copy_kernel(const void *src, void *dst, uint64_t bytes, void *stream);
 
 /* ---------------- operations.hh mirror ------------------------------------- */
-/* ELL SpMV y = A x on rows [row_start, row_end) (row_end 0 = all rows), in
- * HONEI's SMELL layout: element k of lane t of row r at r*threads + t +
- * k*stride; Aj uint64 column indices, Arl uint64 row step counts.
+/* ELL SpMV y = A x on rows [row_start, row_end); pass row_end = rows for
+ * the whole matrix.  An empty range (row_end <= row_start) is a no-op: y is
+ * left unchanged and HSOLVE_OK is returned.  HONEI's SMELL layout: element
+ * k of lane t of row r at r*threads + t + k*stride; Aj uint64 column
+ * indices, Arl uint64 row step counts.
  * Replaces cuda_product_smell_dv_{float,double} (product.cu:356-402). */
 int hsolve_product_smell_dv_float(const void *x, void *y, const void *Aj, const void *Ax,
                                   const void *Arl, uint64_t row_start, uint64_t row_end,
@@ -97,8 +99,9 @@ int hsolve_defect_smell_dv_double(const void *rhs, void *result, const void *Aj,
                                   const void *Arl, const void *b, uint64_t row_start,
                                   uint64_t row_end, uint64_t num_cols_per_row, uint64_t stride,
                                   uint64_t threads, void *stream);
-/* Blocked-CSR SpMV y = A x on rows [row_start, row_end) (row_end 0 = all
- * rows) in HONEI's SparseMatrixCSR layout: Ar uint64 block offsets (rows + 1),
+/* Blocked-CSR SpMV y = A x on rows [row_start, row_end) (an empty range,
+ * row_end <= row_start, is a no-op returning HSOLVE_OK) in HONEI's
+ * SparseMatrixCSR layout: Ar uint64 block offsets (rows + 1),
  * Aj uint64 block start columns, Ax atomicsize values per block; positions at
  * or past `columns` (the length of x) add nothing.  Row sums in the CPU
  * order, bit-identical to Product<tags::CPU>.  Replaces

@@ -140,7 +140,16 @@ int or_mg_fixed_##S(int kind, const T *const *a_q1, const or_ell_##S *a_ell,    
                     T *final_defect, T *initial_defect);                                 \
 int or_mgop_solve_##S(const or_mgop_params *p, const or_ell_##S *A, const T *const *P,  \
                       const or_ell_##S *prol, const or_ell_##S *res, const T *b, T *x,   \
-                      uint64_t *used_iters, uint64_t *used_iters_coarse, T *final_norm);
+                      uint64_t *used_iters, uint64_t *used_iters_coarse, T *final_norm); \
+int or_mgop_solve_pm_##S(const or_mgop_params *p, const or_ell_##S *A, const or_ell_##S *Pm,\
+                         const or_ell_##S *prol, const or_ell_##S *res, const T *b, T *x,\
+                         uint64_t *used_iters, uint64_t *used_iters_coarse, T *final_norm);\
+int or_cg_var_ell_pm_##S(const or_ell_##S *A, const or_ell_##S *P, const T *b, T *x,     \
+                         uint64_t max_iters, T eps, uint64_t *used_iters);               \
+int or_cg_iters_q1_##S(const T *const *bands, const T *rhs, T *x, uint64_t n,           \
+                       uint64_t iters, uint64_t *used);                                  \
+int or_cg_iters_ell_##S(const or_ell_##S *M, const T *rhs, T *x, uint64_t max_iters,     \
+                        uint64_t *used);
 
 OR_DECLARE(double, f64)
 OR_DECLARE(float, f32)

@@ -579,26 +579,36 @@ int FN(or_cg_none_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
 
 static T FN(or_sign)(T v) { return v / (T)fabs(v); }
 
-/* CGSolver<Tag, methods::VAR>::value with a DenseVector preconditioner P
- * (Product(z, P, r) is the elementwise product): honei/math/cg.hh:141-219. */
-int FN(or_cg_var_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
-                      uint64_t stride, uint64_t threads, const T *P, const T *b, T *x,
-                      uint64_t n, uint64_t max_iters, T eps_relative, uint64_t *used_iters)
+/* The preconditioner of CGSolver<Tag, VAR>: z = Product(P, r), an
+ * elementwise product for a DenseVector P, an ELL product for a
+ * SparseMatrixELL P. */
+static void FN(or_precon)(const T *Pv, const OR_CAT(or_ell, OR_S) *Pm, T *z, const T *r, uint64_t n)
 {
-    FN(or_matrix) A = {1, 0, Aj, Ax, Arl, stride, threads};
+    if (Pm)
+        FN(or_ell_product)(z, Pm->Aj, Pm->Ax, Pm->Arl, r, 0, Pm->rows, Pm->stride, Pm->threads);
+    else
+        FN(or_element_product3)(z, Pv, r, n);
+}
+
+/* CGSolver<Tag, methods::VAR>::value: honei/math/cg.hh:141-219, with the
+ * preconditioner applied by or_precon. */
+static int FN(or_cg_var_any)(const FN(or_matrix) *A, const T *Pv, const OR_CAT(or_ell, OR_S) *Pm,
+                             const T *b, T *x, uint64_t n, uint64_t max_iters, T eps_relative,
+                             uint64_t *used_iters)
+{
     T *p = (T *)malloc(sizeof(T) * n), *r = (T *)malloc(sizeof(T) * n);
     T *v = (T *)malloc(sizeof(T) * n), *z = (T *)malloc(sizeof(T) * n);
     T alpha, alpha_new, lambda, initial_defect, temp;
     uint64_t iterations = 0;
     int converged = 0;
 
-    FN(or_defect_any)(&A, r, b, x, n);
-    FN(or_element_product3)(p, P, r, n);
+    FN(or_defect_any)(A, r, b, x, n);
+    FN(or_precon)(Pv, Pm, p, r, n);
     initial_defect = FN(or_norm_l2)(r, n, 1);
     alpha_new = FN(or_dot)(r, p, n);
     *used_iters = 0;
     while (iterations < max_iters) {
-        FN(or_apply)(&A, v, p, n);
+        FN(or_apply)(A, v, p, n);
         temp = FN(or_dot)(v, p, n);
         lambda = alpha_new / (fabs(temp) > OR_EPS ? temp : (T)OR_EPS * FN(or_sign)(temp));
         ++iterations;
@@ -620,7 +630,7 @@ int FN(or_cg_var_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
             *used_iters = iterations;
             break;
         }
-        FN(or_element_product3)(z, P, r, n);
+        FN(or_precon)(Pv, Pm, z, r, n);
         alpha = alpha_new;
         alpha_new = FN(or_dot)(r, z, n);
         T talpha_new = alpha_new / (fabs(alpha) > OR_EPS ? alpha : (T)OR_EPS * FN(or_sign)(alpha));
@@ -629,6 +639,24 @@ int FN(or_cg_var_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
     }
     free(p); free(r); free(v); free(z);
     return converged ? 0 : 1;
+}
+
+/* CGSolver<Tag, methods::VAR> with a DenseVector preconditioner P. */
+int FN(or_cg_var_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl,
+                      uint64_t stride, uint64_t threads, const T *P, const T *b, T *x,
+                      uint64_t n, uint64_t max_iters, T eps_relative, uint64_t *used_iters)
+{
+    FN(or_matrix) A = {1, 0, Aj, Ax, Arl, stride, threads};
+    return FN(or_cg_var_any)(&A, P, NULL, b, x, n, max_iters, eps_relative, used_iters);
+}
+
+/* CGSolver<Tag, methods::VAR> with a SparseMatrixELL preconditioner P
+ * (rows(P) = columns(P) = rows(A)). */
+int FN(or_cg_var_ell_pm)(const OR_CAT(or_ell, OR_S) *M, const OR_CAT(or_ell, OR_S) *P, const T *b, T *x,
+                         uint64_t max_iters, T eps_relative, uint64_t *used_iters)
+{
+    FN(or_matrix) A = {1, 0, M->Aj, M->Ax, M->Arl, M->stride, M->threads};
+    return FN(or_cg_var_any)(&A, NULL, P, b, x, M->rows, max_iters, eps_relative, used_iters);
 }
 
 /* ConjugateGradients<Tag, JAC>::value(ELL A, rhs, x, dd_inverted,
@@ -676,6 +704,35 @@ int FN(or_pcg_jac_ell)(const uint64_t *Aj, const T *Ax, const uint64_t *Arl, uin
     return converged ? 0 : 1;
 }
 
+/* One step of the "version 1" cg_kernel of ConjugateGradients<Tag, NONE>
+ * (Q1: honei/math/conjugate_gradients.hh:108-151; ELL: :153-200, the same
+ * steps on ELL products) on gradient g, iterate x and direction u;
+ * energy and ng are work vectors. */
+static void FN(or_cg_kernel_v1)(const FN(or_matrix) *A, const T *rhs, T *g, T *x, T *u, T *energy, T *ng,
+                                uint64_t n)
+{
+    T upper = FN(or_norm_l2)(g, n, 0);
+    FN(or_apply)(A, energy, u, n);
+    T lower = FN(or_dot)(energy, u, n);
+    memcpy(energy, u, sizeof(T) * n); /* u_c = copy(u); energy = u_c */
+    if (fabs(lower) >= OR_EPS)
+        FN(or_scale)(energy, (T)(upper / lower), n);
+    else
+        FN(or_scale)(energy, (T)(upper / (T)OR_EPS), n);
+    FN(or_sum)(energy, x, n);
+    lower = upper;
+    FN(or_apply)(A, ng, energy, n);
+    FN(or_difference)(ng, rhs, n);
+    upper = FN(or_norm_l2)(ng, n, 0);
+    if (fabs(lower) >= OR_EPS)
+        FN(or_scale)(u, (T)(upper / lower), n);
+    else
+        FN(or_scale)(u, (T)(upper / (T)OR_EPS), n);
+    FN(or_difference)(u, ng, n);
+    memcpy(g, ng, sizeof(T) * n);
+    memcpy(x, energy, sizeof(T) * n);
+}
+
 /* ConjugateGradients<Tag, NONE>::value(Q1, rhs, x, konv_rad):
  * honei/math/conjugate_gradients.hh:490-526 with cg_kernel :108-151.
  * max_iters is a safety cap the reference does not have (0 = none). */
@@ -700,27 +757,7 @@ static int FN(or_cg_konv_any)(const FN(or_matrix) *A, const T *rhs, T *x, uint64
             capped = 1;
             break;
         }
-        /* cg_kernel */
-        T upper = FN(or_norm_l2)(g, n, 0);
-        FN(or_apply)(A, energy, u, n);
-        T lower = FN(or_dot)(energy, u, n);
-        memcpy(energy, u, sizeof(T) * n); /* u_c = copy(u); energy = u_c */
-        if (fabs(lower) >= OR_EPS)
-            FN(or_scale)(energy, (T)(upper / lower), n);
-        else
-            FN(or_scale)(energy, (T)(upper / (T)OR_EPS), n);
-        FN(or_sum)(energy, x, n);
-        lower = upper;
-        FN(or_apply)(A, ng, energy, n);
-        FN(or_difference)(ng, rhs, n);
-        upper = FN(or_norm_l2)(ng, n, 0);
-        if (fabs(lower) >= OR_EPS)
-            FN(or_scale)(u, (T)(upper / lower), n);
-        else
-            FN(or_scale)(u, (T)(upper / (T)OR_EPS), n);
-        FN(or_difference)(u, ng, n);
-        memcpy(g, ng, sizeof(T) * n);
-        memcpy(x, energy, sizeof(T) * n);
+        FN(or_cg_kernel_v1)(A, rhs, g, x, u, energy, ng, n);
         ++it;
         norm_x_last = norm_x;
         norm_x = FN(or_norm_l2)(x, n, 0);
@@ -747,6 +784,83 @@ int FN(or_cg_konv_ell)(const OR_CAT(or_ell, OR_S) *M, const T *rhs, T *x, double
     return FN(or_cg_konv_any)(&A, rhs, x, M->rows, konv_rad, max_iters, iters);
 }
 
+/* ConjugateGradients<Tag, NONE>::value(Q1, rhs, x, unsigned long iters):
+ * honei/math/conjugate_gradients.hh:526-569 with cg_kernel :108-151.
+ * x = 0; g = A x - rhs; u = (copy of g) * -1; initial = ||g||^2 (the
+ * squared Norm<vnt_l_two, false>).  Up to `iters` times: one cg_kernel
+ * step, then the squared norm of the true defect rhs - A x, stopping when
+ * it is below initial * 1e-8 (the product formed in double).  *used = the
+ * steps taken; returns 0 when that test fired. */
+int FN(or_cg_iters_q1)(const T *const *bands, const T *rhs, T *x, uint64_t n, uint64_t iters,
+                       uint64_t *used)
+{
+    FN(or_matrix) A = {0, bands, NULL, NULL, NULL, 0, 0};
+    T *g = (T *)malloc(sizeof(T) * n), *u = (T *)malloc(sizeof(T) * n);
+    T *energy = (T *)malloc(sizeof(T) * n), *ng = (T *)malloc(sizeof(T) * n);
+    T *r = (T *)malloc(sizeof(T) * n);
+    int converged = 0;
+    for (uint64_t i = 0; i < n; ++i)
+        x[i] = 0;
+    FN(or_apply)(&A, g, x, n);
+    FN(or_difference)(g, rhs, n);
+    memcpy(u, g, sizeof(T) * n);        /* g_c = copy(g) */
+    FN(or_scale)(u, (T)-1., n);         /* Scale(g_c, -1); u = copy(g_c) */
+    const T initial_norm = FN(or_norm_l2)(g, n, 0);
+    uint64_t i = 0;
+    while (i < iters) {
+        ++i;
+        FN(or_cg_kernel_v1)(&A, rhs, g, x, u, energy, ng, n);
+        FN(or_q1_defect)(r, rhs, bands, x, n);
+        const T current_norm = FN(or_norm_l2)(r, n, 0);
+        if ((double)current_norm < (double)initial_norm * 1e-08) {
+            converged = 1;
+            break;
+        }
+    }
+    *used = i;
+    free(g); free(u); free(energy); free(ng); free(r);
+    return converged ? 0 : 1;
+}
+
+/* ConjugateGradients<Tag, NONE>::value(ELL, rhs, x, unsigned long
+ * max_iters): honei/math/conjugate_gradients.hh:571-586 with the
+ * "version 2" cg_kernel :203-248.  CGSolver<NONE>'s steps from the given
+ * x, without its epsilon guards (lambda = alpha / (v.p), p = p *
+ * (alpha / alpha_old) + r), stopping when ||r|| < 1e-8 ||r0|| or
+ * ||r|| < 1e-8 (both formed in double) or after max_iters.  *used = the
+ * iterations run; returns 0 when a stopping test fired. */
+int FN(or_cg_iters_ell)(const OR_CAT(or_ell, OR_S) *M, const T *rhs, T *x, uint64_t max_iters, uint64_t *used)
+{
+    FN(or_matrix) A = {1, NULL, M->Aj, M->Ax, M->Arl, M->stride, M->threads};
+    const uint64_t n = M->rows;
+    T *r = (T *)malloc(sizeof(T) * n), *p = (T *)malloc(sizeof(T) * n), *v = (T *)malloc(sizeof(T) * n);
+    int converged = 0;
+    FN(or_defect_any)(&A, r, rhs, x, n);
+    memcpy(p, r, sizeof(T) * n);
+    T alpha = FN(or_norm_l2)(r, n, 0);
+    const T initial_defect = (T)sqrt(alpha);
+    uint64_t iterations = 0;
+    while (iterations < max_iters) {
+        FN(or_apply)(&A, v, p, n);
+        const T lambda = alpha / FN(or_dot)(v, p, n);
+        FN(or_scaled_sum)(x, p, lambda, n);
+        FN(or_scaled_sum)(r, v, -lambda, n);
+        const T alpha_old = alpha;
+        alpha = FN(or_norm_l2)(r, n, 0);
+        FN(or_scale)(p, alpha / alpha_old, n);
+        FN(or_sum)(p, r, n);
+        ++iterations;
+        const T current_defect = (T)sqrt(alpha);
+        if ((double)current_defect < 1e-08 * (double)initial_defect || (double)current_defect < 1e-08) {
+            converged = 1;
+            break;
+        }
+    }
+    *used = iterations;
+    free(r); free(p); free(v);
+    return converged ? 0 : 1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Operator-list multigrid: MGCycleCreation (honei/math/mg.hh:560-1060),     */
 /* MGSolver (mg.hh:1064-1100), RISmoother (ri.hh:96-134), PROLMAT transfers */
@@ -755,7 +869,8 @@ int FN(or_cg_konv_ell)(const OR_CAT(or_ell, OR_S) *M, const T *rhs, T *x, double
 typedef struct FN(or_mgop) {
     const or_mgop_params *p;
     const OR_CAT(or_ell, OR_S) *A, *prol, *res;
-    const T *const *P;
+    const T *const *P;                     /* DenseVector preconditioners, or NULL */
+    const OR_CAT(or_ell, OR_S) *Pm;        /* SparseMatrixELL preconditioners, or NULL */
     T **c, **d, **store, **t0, **t1;    /* per internal level */
     uint64_t used_coarse;
 } FN(or_mgop);
@@ -776,7 +891,7 @@ static void FN(mgop_smooth)(FN(or_mgop) *g, uint64_t i, T *x, const T *b, uint64
     const uint64_t n = g->A[i].rows;
     for (uint64_t k = 0; k < iters; ++k) {
         FN(mgop_defect)(&g->A[i], g->t0[i], b, x);
-        FN(or_element_product3)(g->t1[i], g->P[i], g->t0[i], n);
+        FN(or_precon)(g->P ? g->P[i] : NULL, g->Pm ? &g->Pm[i] : NULL, g->t1[i], g->t0[i], n);
         FN(or_sum)(x, g->t1[i], n);
     }
 }
@@ -786,9 +901,11 @@ static void FN(mgop_coarse)(FN(or_mgop) *g, uint64_t i, T *x, const T *b)
 {
     const OR_CAT(or_ell, OR_S) *M = &g->A[i];
     uint64_t used = 0;
-    if (g->p->coarse_var)
-        FN(or_cg_var_ell)(M->Aj, M->Ax, M->Arl, M->stride, M->threads, g->P[i], b, x, M->rows,
+    if (g->p->coarse_var) {
+        FN(or_matrix) Am = {1, 0, M->Aj, M->Ax, M->Arl, M->stride, M->threads};
+        FN(or_cg_var_any)(&Am, g->P ? g->P[i] : NULL, g->Pm ? &g->Pm[i] : NULL, b, x, M->rows,
                           g->p->max_iters_coarse, (T)g->p->eps_relative, &used);
+    }
     else
         FN(or_cg_none_ell)(M->Aj, M->Ax, M->Arl, M->stride, M->threads, b, x, M->rows,
                            g->p->max_iters_coarse, (T)g->p->eps_relative, &used);
@@ -823,9 +940,10 @@ static void FN(mgop_cycle)(FN(or_mgop) *g, int shape, uint64_t level, T *x, cons
     FN(mgop_smooth)(g, i, x, b, g->p->n_post_smooth);
 }
 
-int FN(or_mgop_solve)(const or_mgop_params *p, const OR_CAT(or_ell, OR_S) *A, const T *const *P,
-                      const OR_CAT(or_ell, OR_S) *prol, const OR_CAT(or_ell, OR_S) *res, const T *b, T *x,
-                      uint64_t *used_iters, uint64_t *used_iters_coarse, T *final_norm)
+static int FN(or_mgop_solve_any)(const or_mgop_params *p, const OR_CAT(or_ell, OR_S) *A, const T *const *P,
+                                 const OR_CAT(or_ell, OR_S) *Pm, const OR_CAT(or_ell, OR_S) *prol,
+                                 const OR_CAT(or_ell, OR_S) *res, const T *b, T *x, uint64_t *used_iters,
+                                 uint64_t *used_iters_coarse, T *final_norm)
 {
     if (p->levels < 1 || p->min_level < 1 || p->min_level > p->levels || p->cycle < 0 || p->cycle > 2)
         return -1;
@@ -834,6 +952,7 @@ int FN(or_mgop_solve)(const or_mgop_params *p, const OR_CAT(or_ell, OR_S) *A, co
     g.p = p;
     g.A = A;
     g.P = P;
+    g.Pm = Pm;
     g.prol = prol;
     g.res = res;
     g.used_coarse = 0;
@@ -869,6 +988,25 @@ int FN(or_mgop_solve)(const or_mgop_params *p, const OR_CAT(or_ell, OR_S) *A, co
         free(vec[v]);
     }
     return 0;
+}
+
+/* MGSolver over the cycle with DenseVector preconditioners P[level]
+ * (PreconFill<DenseVector>: "jac"). */
+int FN(or_mgop_solve)(const or_mgop_params *p, const OR_CAT(or_ell, OR_S) *A, const T *const *P,
+                      const OR_CAT(or_ell, OR_S) *prol, const OR_CAT(or_ell, OR_S) *res, const T *b, T *x,
+                      uint64_t *used_iters, uint64_t *used_iters_coarse, T *final_norm)
+{
+    return FN(or_mgop_solve_any)(p, A, P, NULL, prol, res, b, x, used_iters, used_iters_coarse, final_norm);
+}
+
+/* The same with SparseMatrixELL preconditioners Pm[level] (PreconFill for a
+ * matrix PreconContType_, mg.hh:315-335: the level's <A_i>_<suffix>.ell
+ * file): RISmoother applies t1 = Pm t0 and CGSolver<VAR> z = Pm r. */
+int FN(or_mgop_solve_pm)(const or_mgop_params *p, const OR_CAT(or_ell, OR_S) *A, const OR_CAT(or_ell, OR_S) *Pm,
+                         const OR_CAT(or_ell, OR_S) *prol, const OR_CAT(or_ell, OR_S) *res, const T *b, T *x,
+                         uint64_t *used_iters, uint64_t *used_iters_coarse, T *final_norm)
+{
+    return FN(or_mgop_solve_any)(p, A, NULL, Pm, prol, res, b, x, used_iters, used_iters_coarse, final_norm);
 }
 
 /* ------------------------------------------------------------------------ */

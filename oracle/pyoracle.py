@@ -94,6 +94,9 @@ class Oracle:
             getattr(L, f"or_cg_var_ell_{s}").argtypes = [vp, vp, vp, u64, u64, vp, vp, vp, u64, u64, T, vp]
             getattr(L, f"or_pcg_jac_ell_{s}").argtypes = [vp, vp, vp, u64, u64, vp, vp, vp, u64, u64, T, vp]
             getattr(L, f"or_cg_konv_q1_{s}").argtypes = [vp, vp, vp, u64, C.c_double, u64, vp]
+            getattr(L, f"or_cg_iters_q1_{s}").argtypes = [vp, vp, vp, u64, u64, vp]
+            getattr(L, f"or_cg_iters_ell_{s}").argtypes = [vp, vp, vp, u64, vp]
+            getattr(L, f"or_cg_var_ell_pm_{s}").argtypes = [vp, vp, vp, vp, u64, T, vp]
         L.or_rng_bits.restype = u64
         L.or_rng_bits.argtypes = [u64, u64, u64]
         L.or_rng_uniform.restype = C.c_double
@@ -101,6 +104,8 @@ class Oracle:
         L.or_mg_mixed_q1.argtypes = [vp, vp, vp, vp, vp, C.POINTER(MGParams), vp, vp, vp, vp]
         for s in ("f64", "f32"):
             getattr(L, f"or_mgop_solve_{s}").argtypes = [C.POINTER(MGOpParams), vp, vp, vp, vp, vp, vp, vp, vp, vp]
+            getattr(L, f"or_mgop_solve_pm_{s}").argtypes = [C.POINTER(MGOpParams), vp, vp, vp, vp, vp, vp, vp, vp,
+                                                            vp]
         L.or_ir_mgop.argtypes = [C.POINTER(MGOpParams), vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, u64,
                                  vp, vp, vp]
 
@@ -365,6 +370,32 @@ class Oracle:
             _pp(bands), _p(rhs), _p(x), u64(rhs.size), C.c_double(konv_rad), u64(max_iters), C.byref(it))
         return x, it.value, st
 
+    def cg_iters_q1(self, bands, rhs, iters):
+        """ConjugateGradients<NONE>(Q1, rhs, x, unsigned long iters)
+        (conjugate_gradients.hh:526-569); x starts at zero."""
+        x = np.zeros_like(rhs)
+        used = u64()
+        st = getattr(self.lib, f"or_cg_iters_q1_{_sfx(rhs.dtype)}")(
+            _pp(bands), _p(rhs), _p(x), u64(rhs.size), u64(iters), C.byref(used))
+        return x, used.value, st
+
+    def cg_iters_ell(self, E, rhs, x0, max_iters):
+        """ConjugateGradients<NONE>(ELL, rhs, x, unsigned long max_iters)
+        (conjugate_gradients.hh:571-586, kernel :203-248) from x0."""
+        x = np.ascontiguousarray(x0, rhs.dtype).copy()
+        used = u64()
+        st = getattr(self.lib, f"or_cg_iters_ell_{_sfx(rhs.dtype)}")(
+            _ells([E]), _p(rhs), _p(x), u64(max_iters), C.byref(used))
+        return x, used.value, st
+
+    def cg_var_ell_pm(self, E, Pm, b, x0, max_iters=10000, eps=1e-8):
+        """CGSolver<VAR> with a SparseMatrixELL preconditioner Pm (cg.hh:141-219)."""
+        x = np.ascontiguousarray(x0, b.dtype).copy()
+        used = u64()
+        st = getattr(self.lib, f"or_cg_var_ell_pm_{_sfx(b.dtype)}")(
+            _ells([E]), _ells([Pm]), _p(b), _p(x), u64(max_iters), eps, C.byref(used))
+        return x, used.value, st
+
     def mg_mixed_q1(self, A64, b, x0, a32_levels, dinv32_levels, *, min_level, max_level,
                     n_max_iter=2, tolerance=1e-8, n_pre_smooth=2, adapt=1.0, mask=(2,) * 8,
                     max_outer=0):
@@ -399,6 +430,22 @@ class Oracle:
         fn = C.c_double() if dt == np.float64 else C.c_float()
         st = getattr(self.lib, f"or_mgop_solve_{_sfx(dt)}")(
             C.byref(p), _ells(data["A"]), _pp(data["P"]), _ells(data["prol"]), _ells(data["res"]),
+            _p(b), _p(x), C.byref(used), C.byref(used_c), C.byref(fn))
+        assert st == 0, st
+        return dict(x=x, used=used.value, used_coarse=used_c.value, final_norm=fn.value)
+
+    def mgop_solve_pm(self, data, b, x0, *, cycle="V", coarse_var=False, min_level=1, max_iters=100,
+                      max_iters_coarse=100, n_pre=4, n_post=4, eps=1e-8):
+        """mgop_solve with SparseMatrixELL preconditioners: data["Pm"] (one ELL
+        dict per level) in RISmoother and in CGSolver<VAR>."""
+        dt = np.dtype(b.dtype)
+        p = MGOpParams(CYCLES[cycle], int(coarse_var), len(data["A"]), min_level, max_iters, max_iters_coarse,
+                       n_pre, n_post, eps)
+        x = np.ascontiguousarray(x0, dt).copy()
+        used, used_c = u64(), u64()
+        fn = C.c_double() if dt == np.float64 else C.c_float()
+        st = getattr(self.lib, f"or_mgop_solve_pm_{_sfx(dt)}")(
+            C.byref(p), _ells(data["A"]), _ells(data["Pm"]), _ells(data["prol"]), _ells(data["res"]),
             _p(b), _p(x), C.byref(used), C.byref(used_c), C.byref(fn))
         assert st == 0, st
         return dict(x=x, used=used.value, used_coarse=used_c.value, final_norm=fn.value)
@@ -443,10 +490,17 @@ class Reference:
             getattr(L, f"ref_pcg_jac_ell_{s}").restype = u64
             getattr(L, f"ref_pcg_jac_ell_{s}").argtypes = [u64, u64, u64, u64, vp, vp, vp, vp, vp, u64, T]
             getattr(L, f"ref_cg_konv_q1_{s}").argtypes = [vp, vp, vp, u64, C.c_double]
+            getattr(L, f"ref_cg_iters_q1_{s}").argtypes = [vp, vp, vp, u64, u64]
+            getattr(L, f"ref_cg_iters_ell_{s}").argtypes = [u64, u64, u64, u64, vp, vp, vp, vp, u64]
+            getattr(L, f"ref_cg_var_ell_pm_{s}").restype = u64
+            getattr(L, f"ref_cg_var_ell_pm_{s}").argtypes = [u64, u64, u64, u64, vp, vp, u64, u64, u64, vp, vp, vp,
+                                                             vp, u64, T]
         L.ref_mg_mixed_q1.argtypes = [vp, vp, vp, vp, u64, u64, u64, C.c_double, u64, C.c_double,
                                       vp, vp, vp]
         L.ref_mgop_solve_f64.argtypes = [C.c_char_p, C.c_int, C.c_int, u64, u64, u64, u64, u64, u64,
                                          C.c_double, C.c_double, vp, vp, vp]
+        L.ref_mgop_solve_pm_f64.argtypes = [C.c_char_p, C.c_int, C.c_int, u64, u64, u64, u64, u64, u64,
+                                            C.c_double, C.c_double, C.c_char_p, vp, vp, vp]
         L.ref_ir_mgop.argtypes = [C.c_char_p, u64, u64, u64, u64, u64, u64, C.c_double, C.c_double,
                                   C.c_double, u64, vp, vp, vp, vp]
 
@@ -597,6 +651,27 @@ class Reference:
             _pp(bands), _p(rhs), _p(x), u64(rhs.size), C.c_double(konv_rad))
         return x
 
+    def cg_iters_q1(self, bands, rhs, iters):
+        x = np.zeros_like(rhs)
+        getattr(self.lib, f"ref_cg_iters_q1_{_sfx(rhs.dtype)}")(_pp(bands), _p(rhs), _p(x), u64(rhs.size),
+                                                                u64(iters))
+        return x
+
+    def cg_iters_ell(self, E, rhs, x0, max_iters):
+        x = np.ascontiguousarray(x0, rhs.dtype).copy()
+        getattr(self.lib, f"ref_cg_iters_ell_{_sfx(rhs.dtype)}")(
+            u64(E["rows"]), u64(E["stride"]), u64(E["ncpr"]), u64(E["threads"]), _p(E["Aj"]), _p(E["Ax"]),
+            _p(rhs), _p(x), u64(max_iters))
+        return x
+
+    def cg_var_ell_pm(self, E, Pm, b, x0, max_iters=10000, eps=1e-8):
+        x = np.ascontiguousarray(x0, b.dtype).copy()
+        used = getattr(self.lib, f"ref_cg_var_ell_pm_{_sfx(b.dtype)}")(
+            u64(E["rows"]), u64(E["stride"]), u64(E["ncpr"]), u64(E["threads"]), _p(E["Aj"]), _p(E["Ax"]),
+            u64(Pm["stride"]), u64(Pm["ncpr"]), u64(Pm["threads"]), _p(Pm["Aj"]), _p(Pm["Ax"]), _p(b), _p(x),
+            u64(max_iters), eps)
+        return x, used
+
     def mg_mixed_q1(self, A64, b, x0, a32_levels, *, min_level, max_level, n_max_iter=2,
                     tolerance=1e-8, n_pre_smooth=2, adapt=1.0, mask=(2,) * 8):
         x = x0.copy()
@@ -637,7 +712,21 @@ def _ref_ir_mgop(self, base, n, *, levels=4, min_level=1, max_iters_inner=8, max
     return dict(x=x, used=used.value, inner_used=iu.value, coarse_used=cu.value)
 
 
+def _ref_mgop_solve_pm(self, base, n, suffix, *, cycle="V", coarse_var=False, levels=6, min_level=1,
+                       max_iters=100, max_iters_coarse=100, n_pre=4, n_post=4, eps=1e-8, damping=1.0):
+    """MGUtil<..., PreconContType_ = SparseMatrixELL>::load_data(base, levels, damping, suffix) + MGSolver."""
+    base = base if base.endswith("/") else base + "/"
+    x = np.zeros(n, np.float64)
+    used, used_c = u64(), u64()
+    st = self.lib.ref_mgop_solve_pm_f64(base.encode(), CYCLES[cycle], int(coarse_var), levels, min_level, max_iters,
+                                        max_iters_coarse, n_pre, n_post, eps, damping, suffix.encode(), _p(x),
+                                        C.byref(used), C.byref(used_c))
+    assert st == 0
+    return dict(x=x, used=used.value, used_coarse=used_c.value)
+
+
 Reference.mgop_solve = _ref_mgop_solve
+Reference.mgop_solve_pm = _ref_mgop_solve_pm
 Reference.ir_mgop = _ref_ir_mgop
 
 

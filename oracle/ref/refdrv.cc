@@ -77,6 +77,15 @@ namespace
         return SparseMatrixELL<DT_>(rows, cols, stride, ncpr, aj, ax, threads);
     }
 
+    // swallows what a solver prints to std::cout while it lives
+    struct quiet
+    {
+        std::stringstream sink;
+        std::streambuf * old;
+        quiet() : old(std::cout.rdbuf(sink.rdbuf())) {}
+        ~quiet() { std::cout.rdbuf(old); }
+    };
+
     template <typename DT_> void ell_from_csr(unsigned long rows, unsigned long cols, const uint64_t * rowptr,
             const uint64_t * col, const DT_ * val, unsigned long threads,
             uint64_t * stride, uint64_t * ncpr, uint64_t * Aj, DT_ * Ax, uint64_t * Arl)
@@ -251,6 +260,38 @@ void ref_cg_konv_q1_##S(const DT * const * bands, const DT * rhs, DT * x, uint64
     DenseVector<DT> bv(dv(rhs, n)), xv(dv(x, n));                                                          \
     ConjugateGradients<tags::CPU, methods::NONE>::value(A, bv, xv, konv_rad);                             \
     out(x, xv);                                                                                            \
+}                                                                                                          \
+                                                                                                           \
+void ref_cg_iters_q1_##S(const DT * const * bands, const DT * rhs, DT * x, uint64_t n, uint64_t iters)    \
+{                                                                                                          \
+    BandedMatrixQx<Q1Type, DT> A(q1(bands, n));                                                            \
+    DenseVector<DT> bv(dv(rhs, n)), xv(dv(x, n));                                                          \
+    quiet q;                                                                                               \
+    ConjugateGradients<tags::CPU, methods::NONE>::value(A, bv, xv, (unsigned long)iters);                 \
+    out(x, xv);                                                                                            \
+}                                                                                                          \
+                                                                                                           \
+void ref_cg_iters_ell_##S(uint64_t rows, uint64_t stride, uint64_t ncpr, uint64_t threads,                 \
+        const uint64_t * Aj, const DT * Ax, const DT * rhs, DT * x, uint64_t max_iters)                    \
+{                                                                                                          \
+    SparseMatrixELL<DT> A(ell<DT>(rows, rows, stride, ncpr, threads, Aj, Ax));                            \
+    DenseVector<DT> bv(dv(rhs, rows)), xv(dv(x, rows));                                                    \
+    quiet q;                                                                                               \
+    ConjugateGradients<tags::CPU, methods::NONE>::value(A, bv, xv, (unsigned long)max_iters);             \
+    out(x, xv);                                                                                            \
+}                                                                                                          \
+                                                                                                           \
+uint64_t ref_cg_var_ell_pm_##S(uint64_t rows, uint64_t stride, uint64_t ncpr, uint64_t threads,           \
+        const uint64_t * Aj, const DT * Ax, uint64_t p_stride, uint64_t p_ncpr, uint64_t p_threads,        \
+        const uint64_t * PAj, const DT * PAx, const DT * b, DT * x, uint64_t max_iters, DT eps)            \
+{                                                                                                          \
+    SparseMatrixELL<DT> A(ell<DT>(rows, rows, stride, ncpr, threads, Aj, Ax));                            \
+    SparseMatrixELL<DT> P(ell<DT>(rows, rows, p_stride, p_ncpr, p_threads, PAj, PAx));                    \
+    DenseVector<DT> bv(dv(b, rows)), xv(dv(x, rows));                                                      \
+    unsigned long used(0);                                                                                 \
+    CGSolver<tags::CPU, methods::VAR>::value(A, P, bv, xv, (unsigned long)max_iters, used, eps);          \
+    out(x, xv);                                                                                            \
+    return used;                                                                                           \
 }
 
 REF_TYPED(double, f64)
@@ -456,15 +497,15 @@ REF_CSR(float, f32)
  * `cycle` (0 V, 1 W, 2 F::V) with CGSolver<NONE> (coarse_var 0) or <VAR>
  * (1), RISmoother and PROLMAT transfers, then MGSolver with the l2 norm.
  * x receives the finest iterate. */
-template <typename DT_, typename Cycle_, typename Coarse_>
+template <typename DT_, typename Cycle_, typename Coarse_, typename Precon_>
 static void mgop_run(const char * base, unsigned long levels, unsigned long min_level, unsigned long max_iters,
         unsigned long max_iters_coarse, unsigned long npre, unsigned long npost, double eps, double damping,
-        DT_ * x, unsigned long * used, unsigned long * used_coarse)
+        DT_ * x, unsigned long * used, unsigned long * used_coarse, const char * suffix)
 {
-    typedef MGUtil<tags::CPU, SparseMatrixELL<DT_>, DenseVector<DT_>, SparseMatrixELL<DT_>, DenseVector<DT_>,
+    typedef MGUtil<tags::CPU, SparseMatrixELL<DT_>, DenseVector<DT_>, SparseMatrixELL<DT_>, Precon_,
             MatrixIO<io_formats::ELL>, VectorIO<io_formats::EXP>, DT_> Util;
-    MGData<SparseMatrixELL<DT_>, DenseVector<DT_>, SparseMatrixELL<DT_>, DenseVector<DT_>, DT_> data(
-            Util::load_data(base, levels, DT_(damping), "jac"));
+    MGData<SparseMatrixELL<DT_>, DenseVector<DT_>, SparseMatrixELL<DT_>, Precon_, DT_> data(
+            Util::load_data(base, levels, DT_(damping), suffix));
     Util::configure(data, max_iters, max_iters_coarse, npre, npost, min_level, DT_(eps));
     OperatorList ol(MGCycleCreation<tags::CPU, Cycle_, Coarse_, RISmoother<tags::CPU>,
             Restriction<tags::CPU, methods::PROLMAT>, Prolongation<tags::CPU, methods::PROLMAT>, DT_>::value(data));
@@ -479,20 +520,20 @@ static void mgop_run(const char * base, unsigned long levels, unsigned long min_
     *used_coarse = data.used_iters_coarse;
 }
 
-template <typename DT_, typename Coarse_>
+template <typename DT_, typename Coarse_, typename Precon_>
 static int mgop_by_cycle(int cycle, const char * base, unsigned long levels, unsigned long min_level,
         unsigned long max_iters, unsigned long max_iters_coarse, unsigned long npre, unsigned long npost, double eps,
-        double damping, DT_ * x, unsigned long * used, unsigned long * used_coarse)
+        double damping, DT_ * x, unsigned long * used, unsigned long * used_coarse, const char * suffix)
 {
     if (cycle == 0)
-        mgop_run<DT_, methods::CYCLE::V::STATIC, Coarse_>(base, levels, min_level, max_iters, max_iters_coarse,
-                npre, npost, eps, damping, x, used, used_coarse);
+        mgop_run<DT_, methods::CYCLE::V::STATIC, Coarse_, Precon_>(base, levels, min_level, max_iters,
+                max_iters_coarse, npre, npost, eps, damping, x, used, used_coarse, suffix);
     else if (cycle == 1)
-        mgop_run<DT_, methods::CYCLE::W::STATIC, Coarse_>(base, levels, min_level, max_iters, max_iters_coarse,
-                npre, npost, eps, damping, x, used, used_coarse);
+        mgop_run<DT_, methods::CYCLE::W::STATIC, Coarse_, Precon_>(base, levels, min_level, max_iters,
+                max_iters_coarse, npre, npost, eps, damping, x, used, used_coarse, suffix);
     else if (cycle == 2)
-        mgop_run<DT_, methods::CYCLE::F::V::STATIC, Coarse_>(base, levels, min_level, max_iters, max_iters_coarse,
-                npre, npost, eps, damping, x, used, used_coarse);
+        mgop_run<DT_, methods::CYCLE::F::V::STATIC, Coarse_, Precon_>(base, levels, min_level, max_iters,
+                max_iters_coarse, npre, npost, eps, damping, x, used, used_coarse, suffix);
     else
         return -1;
     return 0;
@@ -506,10 +547,28 @@ int ref_mgop_solve_f64(const char * base, int cycle, int coarse_var, uint64_t le
 {
     unsigned long u(0), uc(0);
     int st = coarse_var ?
-        mgop_by_cycle<double, CGSolver<tags::CPU, methods::VAR> >(cycle, base, levels, min_level, max_iters,
-                max_iters_coarse, npre, npost, eps, damping, x, &u, &uc) :
-        mgop_by_cycle<double, CGSolver<tags::CPU, methods::NONE> >(cycle, base, levels, min_level, max_iters,
-                max_iters_coarse, npre, npost, eps, damping, x, &u, &uc);
+        mgop_by_cycle<double, CGSolver<tags::CPU, methods::VAR>, DenseVector<double> >(cycle, base, levels,
+                min_level, max_iters, max_iters_coarse, npre, npost, eps, damping, x, &u, &uc, "jac") :
+        mgop_by_cycle<double, CGSolver<tags::CPU, methods::NONE>, DenseVector<double> >(cycle, base, levels,
+                min_level, max_iters, max_iters_coarse, npre, npost, eps, damping, x, &u, &uc, "jac");
+    *used = u;
+    *used_coarse = uc;
+    return st;
+}
+
+/* The same with SparseMatrixELL preconditioners read by PreconFill from
+ * <base>A_<l>_<suffix>.ell (e.g. "spai", "sainv"), scaled by damping when
+ * that is not 1. */
+int ref_mgop_solve_pm_f64(const char * base, int cycle, int coarse_var, uint64_t levels, uint64_t min_level,
+        uint64_t max_iters, uint64_t max_iters_coarse, uint64_t npre, uint64_t npost, double eps, double damping,
+        const char * suffix, double * x, uint64_t * used, uint64_t * used_coarse)
+{
+    unsigned long u(0), uc(0);
+    int st = coarse_var ?
+        mgop_by_cycle<double, CGSolver<tags::CPU, methods::VAR>, SparseMatrixELL<double> >(cycle, base, levels,
+                min_level, max_iters, max_iters_coarse, npre, npost, eps, damping, x, &u, &uc, suffix) :
+        mgop_by_cycle<double, CGSolver<tags::CPU, methods::NONE>, SparseMatrixELL<double> >(cycle, base, levels,
+                min_level, max_iters, max_iters_coarse, npre, npost, eps, damping, x, &u, &uc, suffix);
     *used = u;
     *used_coarse = uc;
     return st;

@@ -264,3 +264,68 @@ def test_coarse_cg_bit_exact(hip, oracle, rng, n, dtype):
     it = hip.cg_konv_q1(A, hip.DeviceVector.from_numpy(rhs), xd, konv)
     assert it == it_ref
     assert np.array_equal(xd.get(), x_ref)
+
+
+def _banded_ell_tail(rows, cols, maxoff, ncpr, dtype, seed=5):
+    """An ELL matrix with rows > cols: rows below `cols` hold 1..ncpr-2
+    entries at distinct columns within +-maxoff of the row, every row from
+    `cols` on is empty (the LDS-window kernel sees whole blocks of them)."""
+    g = np.random.default_rng(seed)
+    stride = rows
+    Aj = np.zeros(ncpr * stride, np.uint64)
+    Ax = np.zeros(ncpr * stride, dtype)
+    Arl = np.zeros(rows, np.uint64)
+    for r in range(min(rows, cols)):
+        lo, hi = max(0, r - maxoff), min(cols, r + maxoff + 1)
+        k = int(g.integers(1, ncpr - 1))
+        c = np.sort(g.choice(np.arange(lo, hi), size=min(k, hi - lo), replace=False))
+        Arl[r] = c.size
+        for j, cc in enumerate(c):
+            Aj[j * stride + r] = cc
+            Ax[j * stride + r] = dtype(g.uniform(0.5, 1.5))
+    return dict(rows=rows, cols=cols, stride=stride, ncpr=ncpr, threads=1, Aj=Aj, Ax=Ax, Arl=Arl)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_ell_lds_empty_tail_and_zero_columns(hip, oracle, rng, dtype):
+    """Handles the LDS-window kernel is chosen for, with blocks whose x
+    window is empty: rows > cols with an empty tail of more than one LDS
+    block (1024 rows FP64, 512 FP32), and a matrix with no columns."""
+    E = _banded_ell_tail(4096, 1000, 30, 8, dtype)
+    x = rng.uniform(-1, 1, E["cols"]).astype(dtype)
+    ref = oracle.ell_product(E, x)
+    assert np.all(ref[1000:] == 0)
+    M = hip.EllMatrix.from_dict(E)
+    xd = hip.DeviceVector.from_numpy(x)
+    yd = hip.DeviceVector.from_numpy(np.full(E["rows"], 7, dtype))
+    for variant in (0, 16, 20, 48, 52):
+        M.set_variant(variant)
+        M.spmv(xd, yd)
+        assert np.array_equal(yd.get(), ref), variant
+    # no columns: every row empty, y = 0
+    Z = dict(rows=2048, cols=0, stride=2048, ncpr=1, threads=1, Aj=np.zeros(2048, np.uint64),
+             Ax=np.zeros(2048, dtype), Arl=np.zeros(2048, np.uint64))
+    MZ = hip.EllMatrix.from_dict(Z)
+    x1 = hip.DeviceVector.zeros(1, dtype)
+    y0 = hip.DeviceVector.from_numpy(np.full(2048, 3, dtype))
+    for variant in (0, 16, 48):
+        MZ.set_variant(variant)
+        MZ.spmv(x1, y0)
+        assert np.all(y0.get() == 0), variant
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_raw_launcher_empty_range_is_noop(hip, dtype):
+    """row_end <= row_start: y unchanged, status OK."""
+    from honei_amd._lib import lib, u64, vp
+    E = P.q1_to_ell(P.matrix_a_bands(17, dtype=dtype))
+    Aj, Ax, Arl = _ell_dev(hip, E, dtype)
+    xd = hip.DeviceVector.from_numpy(np.ones(E["rows"], dtype))
+    yd = hip.DeviceVector.from_numpy(np.full(E["rows"], 5, dtype))
+    sfx = "double" if dtype == np.float64 else "float"
+    for r0, r1 in ((0, 0), (10, 10), (20, 3)):
+        rc = getattr(lib(), f"hsolve_product_smell_dv_{sfx}")(
+            vp(xd.ptr), vp(yd.ptr), vp(Aj.ptr), vp(Ax.ptr), vp(Arl.ptr), u64(r0), u64(r1),
+            u64(E["ncpr"]), u64(E["stride"]), u64(1), None)
+        assert rc == 0
+        assert np.all(yd.get() == 5)
