@@ -105,6 +105,8 @@ _SIGS = {
     "hsolve_prolongation_axpy_double": [vp, u64, vp, u64, dbl, vp, vp],
     "hsolve_cg_konv_q1_float": [vp, vp, vp, u64, dbl, u64, vp, vp],
     "hsolve_cg_konv_q1_double": [vp, vp, vp, u64, dbl, u64, vp, vp],
+    "hsolve_cg_iters_q1_float": [vp, vp, vp, u64, u64, vp, vp],
+    "hsolve_cg_iters_q1_double": [vp, vp, vp, u64, u64, vp, vp],
     "hsolve_ell_create": [C.c_int, u64, u64, u64, u64, u64, vp, vp, vp, C.POINTER(vp)],
     "hsolve_ell_destroy": [vp],
     "hsolve_ell_info": [vp] + [C.POINTER(u64)] * 7,
@@ -116,6 +118,8 @@ _SIGS = {
     "hsolve_cg_q1": [C.c_int, vp, u64, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_cg_ell": [vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_pcg_ell": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
+    "hsolve_pcg_ell_pmat": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
+    "hsolve_cg_iters_ell": [vp, vp, vp, u64, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_pcg_jac_ell": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_ell_spmm": [vp, u64, vp, vp, vp, vp],
     "hsolve_product_csr_dv_float": [vp, vp, vp, vp, vp, u64, u64, u64, u64, vp],

@@ -360,17 +360,54 @@ namespace k
         cg_konv_body_op<DT>(A, rhs, x, work, n, konv_rad, max_iters, iters_out);
     }
 
+    // One step of the "version 1" cg_kernel (conjugate_gradients.hh:108-151
+    // Q1, :153-200 ELL) on gradient g, iterate x and direction u, with the
+    // work vectors en (energy) and ng (new gradient); sh holds at least 3
+    // scalars.  Called by every thread of the workgroup.
     template <typename DT, typename Op>
-    __device__ __forceinline__ void cg_konv_body_op(const Op & A, const DT * __restrict__ rhs,
-            DT * __restrict__ x, DT * __restrict__ work, int64_t n, double konv_rad, uint64_t max_iters,
-            uint64_t * iters_out)
+    __device__ __forceinline__ void cg_step_v1(const Op & A, const DT * __restrict__ rhs, DT * __restrict__ g,
+            DT * __restrict__ x, DT * __restrict__ u, DT * __restrict__ en, DT * __restrict__ ng, int64_t n, DT * sh)
     {
-        __shared__ DT sh[4];
-        DT * g = work;
-        DT * u = work + n;
-        DT * en = work + 2 * n;
-        DT * ng = work + 3 * n;
         const DT eps = Eps<DT>::v;
+        seq_dot<DT>(g, g, n, &sh[0]);
+        DT upper = sh[0];
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+            en[i] = A.row(u, i);
+        __syncthreads();
+        seq_dot<DT>(en, u, n, &sh[1]);
+        DT lower = sh[1];
+        const DT f1 = (fabs(lower) >= eps) ? DT(upper / lower) : DT(upper / eps);
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        {
+            const DT t = u[i] * f1;
+            en[i] = t + x[i];
+        }
+        __syncthreads();
+        lower = upper;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        {
+            const DT p = A.row(en, i);
+            ng[i] = p - rhs[i];
+        }
+        __syncthreads();
+        seq_dot<DT>(ng, ng, n, &sh[2]);
+        upper = sh[2];
+        const DT f2 = (fabs(lower) >= eps) ? DT(upper / lower) : DT(upper / eps);
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        {
+            const DT t = u[i] * f2;
+            u[i] = t - ng[i];
+            g[i] = ng[i];
+            x[i] = en[i];
+        }
+        __syncthreads();
+    }
+
+    // x = 0, g = A x - rhs, u = g * -1 (the set-up of both fixed forms)
+    template <typename DT, typename Op>
+    __device__ __forceinline__ void cg_v1_start(const Op & A, const DT * __restrict__ rhs, DT * __restrict__ g,
+            DT * __restrict__ x, DT * __restrict__ u, int64_t n)
+    {
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
             x[i] = DT(0);
         __syncthreads();
@@ -382,52 +419,81 @@ namespace k
         __syncthreads();
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
             u[i] = g[i] * DT(-1.);
-        seq_dot<DT>(x, x, n, &sh[0]);
-        DT norm_x = sh[0];
+    }
+
+    template <typename DT, typename Op>
+    __device__ __forceinline__ void cg_konv_body_op(const Op & A, const DT * __restrict__ rhs,
+            DT * __restrict__ x, DT * __restrict__ work, int64_t n, double konv_rad, uint64_t max_iters,
+            uint64_t * iters_out)
+    {
+        __shared__ DT sh[4];
+        DT * g = work;
+        DT * u = work + n;
+        DT * en = work + 2 * n;
+        DT * ng = work + 3 * n;
+        cg_v1_start<DT>(A, rhs, g, x, u, n);
+        seq_dot<DT>(x, x, n, &sh[3]);
+        DT norm_x = sh[3];
         DT norm_x_last = norm_x - DT(1);
         uint64_t it = 0;
         while ((double)(norm_x - norm_x_last) > konv_rad && it < max_iters)
         {
-            seq_dot<DT>(g, g, n, &sh[1]);
-            DT upper = sh[1];
-            for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
-                en[i] = A.row(u, i);
-            __syncthreads();
-            seq_dot<DT>(en, u, n, &sh[2]);
-            DT lower = sh[2];
-            const DT f1 = (fabs(lower) >= eps) ? DT(upper / lower) : DT(upper / eps);
-            for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
-            {
-                const DT t = u[i] * f1;
-                en[i] = t + x[i];
-            }
-            __syncthreads();
-            lower = upper;
-            for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
-            {
-                const DT p = A.row(en, i);
-                ng[i] = p - rhs[i];
-            }
-            __syncthreads();
-            seq_dot<DT>(ng, ng, n, &sh[3]);
-            upper = sh[3];
-            const DT f2 = (fabs(lower) >= eps) ? DT(upper / lower) : DT(upper / eps);
-            for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
-            {
-                const DT t = u[i] * f2;
-                u[i] = t - ng[i];
-                g[i] = ng[i];
-                x[i] = en[i];
-            }
-            __syncthreads();
+            cg_step_v1<DT>(A, rhs, g, x, u, en, ng, n, sh);
             ++it;
             norm_x_last = norm_x;
-            seq_dot<DT>(x, x, n, &sh[0]);
-            norm_x = sh[0];
+            seq_dot<DT>(x, x, n, &sh[3]);
+            norm_x = sh[3];
             __syncthreads();
         }
         if (iters_out && threadIdx.x == 0)
             *iters_out = it;
+    }
+
+    // ConjugateGradients<Tag, NONE>::value(Q1, rhs, x, unsigned long iters)
+    // (conjugate_gradients.hh:526-569) in one workgroup: x = 0, then up to
+    // `iters` cg_kernel steps, each followed by the squared norm of the true
+    // defect rhs - A x; stop when it is below 1e-8 times the squared norm of
+    // the initial gradient (the product formed in double).  Vectors g, u,
+    // energy, new_gradient and the defect live in `work` (5n); sequential
+    // sums, so the iterates are bit-identical to tags::CPU.  *used_out = the
+    // steps taken.
+    template <typename DT, typename Op>
+    __device__ __forceinline__ void cg_iters_body_op(const Op & A, const DT * __restrict__ rhs,
+            DT * __restrict__ x, DT * __restrict__ work, int64_t n, uint64_t iters, uint64_t * used_out)
+    {
+        __shared__ DT sh[5];
+        DT * g = work;
+        DT * u = work + n;
+        DT * en = work + 2 * n;
+        DT * ng = work + 3 * n;
+        DT * r = work + 4 * n;
+        cg_v1_start<DT>(A, rhs, g, x, u, n);
+        seq_dot<DT>(g, g, n, &sh[4]);
+        const double initial = (double)sh[4];
+        uint64_t i = 0;
+        while (i < iters)
+        {
+            ++i;
+            cg_step_v1<DT>(A, rhs, g, x, u, en, ng, n, sh);
+            for (int64_t j = threadIdx.x; j < n; j += blockDim.x)
+                r[j] = rhs[j] - A.row(x, j);
+            __syncthreads();
+            seq_dot<DT>(r, r, n, &sh[3]);
+            const double current = (double)sh[3];
+            __syncthreads();
+            if (current < initial * 1e-08)
+                break;
+        }
+        if (used_out && threadIdx.x == 0)
+            *used_out = i;
+    }
+
+    template <typename DT>
+    __global__ __launch_bounds__(256) void cg_iters_q1_kernel(Bands B, const DT * __restrict__ rhs,
+            DT * __restrict__ x, DT * __restrict__ work, int64_t n, int64_t m, uint64_t iters, uint64_t * used_out)
+    {
+        Q1RowOp<DT> A{B, n, m};
+        cg_iters_body_op<DT>(A, rhs, x, work, n, iters, used_out);
     }
 
     template <typename DT>

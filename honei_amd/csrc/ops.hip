@@ -205,6 +205,19 @@ namespace hsolve
         HIP_CHECK(hipGetLastError());
     }
 
+    // one workgroup with sequential sums: up to 2^20 rows
+    template <typename DT>
+    void launch_cg_iters(const k::Bands & B, const DT * rhs, DT * x, DT * work, uint64_t n, uint64_t iters,
+                         uint64_t * used_dev, hipStream_t st)
+    {
+        const int64_t m = (int64_t)std::sqrt((double)n);
+        require((uint64_t)(m * m) == n && n > 0, HSOLVE_ERR_SIZE, "cg_iters_q1: size is not a square");
+        require(n <= (1u << 20), HSOLVE_ERR_SIZE, "cg_iters_q1: more than 2^20 rows for one workgroup");
+        hipLaunchKernelGGL((k::cg_iters_q1_kernel<DT>), dim3(1), dim3(kBlock), 0, st, B, rhs, x, work, (int64_t)n, m,
+                           iters, used_dev);
+        HIP_CHECK(hipGetLastError());
+    }
+
     template void launch_dot<float>(const float *, const float *, uint64_t, float *, int, hipStream_t);
     template void launch_dot<double>(const double *, const double *, uint64_t, double *, int, hipStream_t);
 }
@@ -756,6 +769,29 @@ int hsolve_cg_konv_q1_double(const void * const * bands9, const void * rhs, void
                                     bands9[7], bands9[8]),
                            (const double *)rhs, (double *)x, work, size, konv_rad,
                            max_iters ? max_iters : 100 * size + 1000, (uint64_t *)iters_dev, nullptr, ST);
+    HSOLVE_API_END
+}
+
+int hsolve_cg_iters_q1_float(const void * const * bands9, const void * rhs, void * x, uint64_t size,
+                             uint64_t iters, void * used_dev, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(bands9 && rhs && x, HSOLVE_ERR_INVALID, "cg_iters_q1: null pointer");
+    float * work = (float *)stream_work(ST, sizeof(float) * 5 * size + 256);
+    launch_cg_iters<float>(bands_of(bands9[0], bands9[1], bands9[2], bands9[3], bands9[4], bands9[5], bands9[6],
+                                    bands9[7], bands9[8]),
+                           (const float *)rhs, (float *)x, work, size, iters, (uint64_t *)used_dev, ST);
+    HSOLVE_API_END
+}
+int hsolve_cg_iters_q1_double(const void * const * bands9, const void * rhs, void * x, uint64_t size,
+                              uint64_t iters, void * used_dev, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(bands9 && rhs && x, HSOLVE_ERR_INVALID, "cg_iters_q1: null pointer");
+    double * work = (double *)stream_work(ST, sizeof(double) * 5 * size + 256);
+    launch_cg_iters<double>(bands_of(bands9[0], bands9[1], bands9[2], bands9[3], bands9[4], bands9[5], bands9[6],
+                                     bands9[7], bands9[8]),
+                            (const double *)rhs, (double *)x, work, size, iters, (uint64_t *)used_dev, ST);
     HSOLVE_API_END
 }
 

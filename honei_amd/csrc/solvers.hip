@@ -50,6 +50,10 @@ namespace
         uint64_t iters, max_iters;
         int run;          // 1 while iterating
         int converged;    // 1 when a stopping test fired
+        // 1: ConjugateGradients<NONE>'s fixed-iteration ELL form
+        // (conjugate_gradients.hh:203-248): no epsilon guards on v.p and
+        // alpha_old, thresholds 1e-8 * ||r0|| and 1e-8 formed in double
+        int plain;
     };
 
     template <typename DT> struct Eps;
@@ -143,7 +147,7 @@ namespace
             if (VAR)
                 s->lambda = s->alpha_new / guard_signed<DT>(temp);
             else
-                s->lambda = s->alpha / guard<DT>(temp);
+                s->lambda = s->alpha / (s->plain ? temp : guard<DT>(temp));
         }
     }
 
@@ -169,7 +173,8 @@ namespace
             if (threadIdx.x == 0)
             {
                 const DT temp = (DT)vp;
-                const DT lam = VAR ? s->alpha_new / guard_signed<DT>(temp) : s->alpha / guard<DT>(temp);
+                const DT lam = VAR ? s->alpha_new / guard_signed<DT>(temp)
+                                   : s->alpha / (s->plain ? temp : guard<DT>(temp));
                 sh_lambda = lam;
                 if (blockIdx.x == 0)
                 {
@@ -227,11 +232,13 @@ namespace
         {
             s.alpha_old = s.alpha;
             s.alpha = (DT)rr;
-            s.talpha = s.alpha / guard<DT>(s.alpha_old);
+            s.talpha = s.alpha / (s.plain ? s.alpha_old : guard<DT>(s.alpha_old));
             ++s.iters;
             const DT cur = (DT)sqrt(s.alpha);
             s.current_defect = cur;
-            if (cur < eps * s.initial_defect || cur < eps)
+            const bool stop = s.plain ? ((double)cur < 1e-08 * (double)s.initial_defect || (double)cur < 1e-08)
+                                      : (cur < eps * s.initial_defect || cur < eps);
+            if (stop)
             {
                 s.converged = 1;
                 s.run = 0;
@@ -544,13 +551,14 @@ namespace
 
     template <typename DT, typename Op, bool VAR>
     void run_cg(const Op & A, uint64_t n, const DT * b, DT * x, const DT * P, uint64_t max_iters, double eps,
-                uint64_t * used_iters, double * defect_out, hipStream_t st, int * converged_out)
+                uint64_t * used_iters, double * defect_out, hipStream_t st, int * converged_out, int plain = 0)
     {
         CgWork<DT> w(n);
         CgScalars<DT> h;
         std::memset(&h, 0, sizeof(h));
         h.eps_relative = (DT)eps;
         h.max_iters = max_iters;
+        h.plain = plain;
         HIP_CHECK(hipMemcpyAsync(w.s, &h, sizeof(h), hipMemcpyHostToDevice, st));
         const unsigned grid = std::min<unsigned>(grid_for(n, kBlock), kSolverBlocks);
         const bool fused = grid <= 512;
@@ -743,15 +751,143 @@ namespace
 
     template <typename DT, bool VAR>
     void run_cg_ell(const hsolve_ell_s * m, const DT * b, DT * x, const DT * P, uint64_t max_iters, double eps,
-                    uint64_t * used, double * defect, hipStream_t st, int * conv)
+                    uint64_t * used, double * defect, hipStream_t st, int * conv, int plain = 0)
     {
         const uint64_t n = m->rows;
         with_ell_types(m, [&](auto Aj, auto Arl) {
             using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
             using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
             EllOp<DT, IT, LT> A{Aj, (const DT *)m->Ax, Arl, m->stride, (uint32_t)m->threads};
-            run_cg<DT, EllOp<DT, IT, LT>, VAR>(A, n, b, x, P, max_iters, eps, used, defect, st, conv);
+            run_cg<DT, EllOp<DT, IT, LT>, VAR>(A, n, b, x, P, max_iters, eps, used, defect, st, conv, plain);
         });
+    }
+
+    // ------------------------- CGSolver<VAR> with a SparseMatrixELL P
+    // (cg.hh:141-219): z = Product(P, r) by the handle's SpMV, then the
+    // partials of r.r and r.z in a separate pass.
+
+    // partials of (r.r, r.z)
+    template <typename DT>
+    __global__ __launch_bounds__(256) void pm_dot_kernel(const DT * __restrict__ r, const DT * __restrict__ z,
+            int64_t n, double * partials, const CgScalars<DT> * s)
+    {
+        if (s && s->run == 0)
+            return;
+        double acc_rr = 0.0, acc_rz = 0.0;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        {
+            const double ri = (double)r[i];
+            acc_rr += ri * ri;
+            acc_rz += ri * (double)z[i];
+        }
+        k::put_partials2(acc_rr, acc_rz, partials, blockIdx.x);
+    }
+
+    // after x += lambda p, r -= lambda v: ||r|| and the stopping tests
+    template <typename DT>
+    __global__ __launch_bounds__(256) void pm_test_finish(const double * partials, unsigned nb, CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        double rr, unused;
+        k::sum_partials2(partials, nb, rr, unused);
+        if (threadIdx.x != 0)
+            return;
+        ++s->iters;
+        const DT cur = (DT)sqrt((DT)rr);
+        s->current_defect = cur;
+        const DT eps = s->eps_relative;
+        if (cur < eps * s->initial_defect || cur < eps)
+        {
+            s->converged = 1;
+            s->run = 0;
+        }
+        else if (s->iters == s->max_iters)
+            s->run = 0;
+    }
+
+    // alpha = alpha_new; alpha_new = r.z; talpha = alpha_new / alpha (guarded)
+    template <typename DT>
+    __global__ __launch_bounds__(256) void pm_dir_finish(const double * partials, unsigned nb, CgScalars<DT> * s)
+    {
+        if (s->run == 0)
+            return;
+        double unused, rz;
+        k::sum_partials2(partials, nb, unused, rz);
+        if (threadIdx.x != 0)
+            return;
+        s->alpha = s->alpha_new;
+        s->alpha_new = (DT)rz;
+        s->talpha = s->alpha_new / guard_signed<DT>(s->alpha);
+    }
+
+    template <typename DT, typename Op>
+    void run_pcg_pmat(const Op & A, hsolve_ell Pm, uint64_t n, const DT * b, DT * x, uint64_t max_iters, double eps,
+                      uint64_t * used_iters, double * defect_out, hipStream_t st, int * converged_out)
+    {
+        CgWork<DT> w(n);
+        DeviceArray<DT> zbuf(n);
+        DT * z = zbuf.p;
+        CgScalars<DT> h;
+        std::memset(&h, 0, sizeof(h));
+        h.eps_relative = (DT)eps;
+        h.max_iters = max_iters;
+        HIP_CHECK(hipMemcpyAsync(w.s, &h, sizeof(h), hipMemcpyHostToDevice, st));
+        const unsigned grid = std::min<unsigned>(grid_for(n, kBlock), kSolverBlocks);
+        auto precon = [&](const DT * in, DT * out) {
+            const int rc = hsolve_ell_spmv(Pm, in, out, nullptr, (void *)st);
+            if (rc != HSOLVE_OK)
+                throw Error(rc, std::string("pcg_ell_pmat: preconditioner product failed: ") + hsolve_last_error());
+        };
+        // r = b - A x; p = P r; initial ||r|| and alpha_new = r.p
+        hipLaunchKernelGGL((cg_init_kernel<DT, Op, false>), dim3(grid), dim3(kBlock), 0, st, A, b, (const DT *)x,
+                           w.r, w.p, (const DT *)nullptr, (int64_t)n, w.partials);
+        precon(w.r, w.p);
+        hipLaunchKernelGGL((pm_dot_kernel<DT>), dim3(grid), dim3(kBlock), 0, st, (const DT *)w.r, (const DT *)w.p,
+                           (int64_t)n, w.partials, (const CgScalars<DT> *)nullptr);
+        hipLaunchKernelGGL((cg_init_finish<DT, true>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials, grid,
+                           w.s);
+        HIP_CHECK(hipGetLastError());
+        uint64_t enqueued = 0, chunk = 8;
+        while (true)
+        {
+            for (uint64_t c = 0; c < chunk && enqueued < max_iters; ++c, ++enqueued)
+            {
+                hipLaunchKernelGGL((cg_spmv_kernel<DT, Op>), dim3(grid), dim3(kBlock), 0, st, A, (const DT *)w.p,
+                                   w.v, (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
+                hipLaunchKernelGGL((cg_spmv_finish<DT, true>), dim3(1), dim3(kBlock), 0, st,
+                                   (const double *)w.partials, grid, w.s);
+                hipLaunchKernelGGL((cg_update_kernel<DT, false>), dim3(grid), dim3(kBlock), 0, st, x, w.r,
+                                   (const DT *)w.p, (const DT *)w.v, (const DT *)nullptr, (int64_t)n, w.partials, w.s,
+                                   (const double *)nullptr, 0u);
+                hipLaunchKernelGGL((pm_test_finish<DT>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials,
+                                   grid, w.s);
+                // z = P r (after a stop this writes z only, which nothing reads)
+                precon(w.r, z);
+                hipLaunchKernelGGL((pm_dot_kernel<DT>), dim3(grid), dim3(kBlock), 0, st, (const DT *)w.r,
+                                   (const DT *)z, (int64_t)n, w.partials, (const CgScalars<DT> *)w.s);
+                hipLaunchKernelGGL((pm_dir_finish<DT>), dim3(1), dim3(kBlock), 0, st, (const double *)w.partials,
+                                   grid, w.s);
+                // p = p * talpha + z
+                hipLaunchKernelGGL((cg_direction_kernel<DT, false>), dim3(grid), dim3(kBlock), 0, st, w.p,
+                                   (const DT *)z, (const DT *)nullptr, (int64_t)n, (const CgScalars<DT> *)w.s);
+            }
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(&h, w.s, sizeof(h), hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+            if (h.run == 0 || enqueued >= max_iters)
+                break;
+            if (chunk < 64)
+                chunk *= 2;
+        }
+        if (used_iters)
+            *used_iters = h.iters;
+        if (defect_out)
+            *defect_out = (h.initial_defect > 0) ? (double)h.current_defect / (double)h.initial_defect
+                                                 : (double)h.current_defect;
+        if (converged_out)
+            *converged_out = h.converged;
     }
 }  // namespace
 }  // namespace hsolve
@@ -839,6 +975,57 @@ int hsolve_pcg_jac_ell(hsolve_ell m, const void * dd_inverted, const void * b, v
         set_last_error("pcg_jac_ell: max_iters reached");
         return HSOLVE_ERR_NOT_CONVERGED;
     }
+    HSOLVE_API_END
+}
+
+int hsolve_pcg_ell_pmat(hsolve_ell m, hsolve_ell P, const void * b, void * x, uint64_t max_iters, double eps,
+                        uint64_t * used_iters, double * defect_out, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(m && P && b && x, HSOLVE_ERR_INVALID, "pcg_ell_pmat: null pointer");
+    require(m->rows == m->cols, HSOLVE_ERR_SIZE, "pcg_ell_pmat: matrix is not square");
+    require(P->rows == m->rows && P->cols == m->rows, HSOLVE_ERR_SIZE,
+            "pcg_ell_pmat: preconditioner size differs from the matrix");
+    require(P->dtype == m->dtype, HSOLVE_ERR_INVALID, "pcg_ell_pmat: preconditioner value type differs");
+    int conv = 0;
+    hipStream_t st = as_stream(stream);
+    with_ell_types(m, [&](auto Aj, auto Arl) {
+        using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
+        using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
+        if (m->dtype)
+        {
+            EllOp<double, IT, LT> A{Aj, (const double *)m->Ax, Arl, m->stride, (uint32_t)m->threads};
+            run_pcg_pmat<double>(A, P, m->rows, (const double *)b, (double *)x, max_iters, eps, used_iters,
+                                 defect_out, st, &conv);
+        }
+        else
+        {
+            EllOp<float, IT, LT> A{Aj, (const float *)m->Ax, Arl, m->stride, (uint32_t)m->threads};
+            run_pcg_pmat<float>(A, P, m->rows, (const float *)b, (float *)x, max_iters, eps, used_iters, defect_out,
+                                st, &conv);
+        }
+    });
+    if (! conv && max_iters > 0)
+    {
+        set_last_error("pcg_ell_pmat: max_iters reached");
+        return HSOLVE_ERR_NOT_CONVERGED;
+    }
+    HSOLVE_API_END
+}
+
+int hsolve_cg_iters_ell(hsolve_ell m, const void * rhs, void * x, uint64_t max_iters, uint64_t * used_iters,
+                        double * defect_out, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(m && rhs && x, HSOLVE_ERR_INVALID, "cg_iters_ell: null pointer");
+    require(m->rows == m->cols, HSOLVE_ERR_SIZE, "cg_iters_ell: matrix is not square");
+    int conv = 0;
+    if (m->dtype)
+        run_cg_ell<double, false>(m, (const double *)rhs, (double *)x, nullptr, max_iters, 1e-8, used_iters,
+                                  defect_out, as_stream(stream), &conv, 1);
+    else
+        run_cg_ell<float, false>(m, (const float *)rhs, (float *)x, nullptr, max_iters, 1e-8, used_iters,
+                                 defect_out, as_stream(stream), &conv, 1);
     HSOLVE_API_END
 }
 

@@ -327,6 +327,14 @@ def cg_konv_q1(A: Q1Matrix, rhs: DeviceVector, x: DeviceVector, konv_rad, max_it
     return int(it.get()[0])
 
 
+def cg_iters_q1(A: Q1Matrix, rhs: DeviceVector, x: DeviceVector, iters, stream=None):
+    """ConjugateGradients<NONE>(Q1, rhs, x, unsigned long iters); returns the steps taken."""
+    used = DeviceVector(1, np.uint64)
+    f = getattr(lib(), f"hsolve_cg_iters_q1_{_sfx(A.dtype)}")
+    check(f(A.ptrs, vp(rhs.ptr), vp(x.ptr), u64(A.size), u64(iters), vp(used.ptr), vp(stream)), "cg_iters_q1")
+    return int(used.get()[0])
+
+
 # ----------------------------------------------------------------- solvers
 def cg_q1(A: Q1Matrix, b: DeviceVector, x: DeviceVector, max_iters=10000, eps=1e-8, stream=None, raise_on_cap=True):
     used, defect = u64(), C.c_double()
@@ -354,6 +362,26 @@ def pcg_ell(A: EllMatrix, P: DeviceVector, b: DeviceVector, x: DeviceVector, max
                               C.byref(used), C.byref(defect), vp(stream))
     if rc != 5 or raise_on_cap:
         check(rc, "pcg_ell")
+    return used.value, defect.value
+
+
+def pcg_ell_pmat(A: EllMatrix, P: EllMatrix, b: DeviceVector, x: DeviceVector, max_iters=10000, eps=1e-8,
+                 stream=None, raise_on_cap=True):
+    """CGSolver<VAR> with a SparseMatrixELL preconditioner P (z = P r)."""
+    used, defect = u64(), C.c_double()
+    rc = lib().hsolve_pcg_ell_pmat(vp(A.handle), vp(P.handle), vp(b.ptr), vp(x.ptr), u64(max_iters), float(eps),
+                                   C.byref(used), C.byref(defect), vp(stream))
+    if rc != 5 or raise_on_cap:
+        check(rc, "pcg_ell_pmat")
+    return used.value, defect.value
+
+
+def cg_iters_ell(A: EllMatrix, rhs: DeviceVector, x: DeviceVector, max_iters, stream=None):
+    """ConjugateGradients<NONE>(ELL, rhs, x, unsigned long max_iters); returns
+    (iterations, final relative defect)."""
+    used, defect = u64(), C.c_double()
+    check(lib().hsolve_cg_iters_ell(vp(A.handle), vp(rhs.ptr), vp(x.ptr), u64(max_iters), C.byref(used),
+                                    C.byref(defect), vp(stream)), "cg_iters_ell")
     return used.value, defect.value
 
 

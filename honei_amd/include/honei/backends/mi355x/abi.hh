@@ -106,6 +106,11 @@ namespace honei
             inline int cg_konv_q1(const double * const * b, const double * rhs, double * x, uint64_t n, double konv,
                                   uint64_t max_iters)
             { return hsolve_cg_konv_q1_double((const void * const *)b, rhs, x, n, konv, max_iters, nullptr, stream()); }
+            inline int cg_iters_q1(const float * const * b, const float * rhs, float * x, uint64_t n, uint64_t iters)
+            { return hsolve_cg_iters_q1_float((const void * const *)b, rhs, x, n, iters, nullptr, stream()); }
+            inline int cg_iters_q1(const double * const * b, const double * rhs, double * x, uint64_t n,
+                                   uint64_t iters)
+            { return hsolve_cg_iters_q1_double((const void * const *)b, rhs, x, n, iters, nullptr, stream()); }
 #undef HS_PAIR
         }
     }

@@ -50,7 +50,28 @@ namespace honei
             DT_ * e = axc.elements();
             for (uint64_t i = 0; i < size; ++i)
                 e[i] = (DT_)ax[i];
-            return SparseMatrixELL<DT_>(hdr[1], hdr[2], hdr[3], hdr[4], aj, axc, 1);
+            SparseMatrixELL<DT_> m(hdr[1], hdr[2], hdr[3], hdr[4], aj, axc, 1);
+            // matrix_io.hh:572-579: with ell::threads != 1 the file's matrix is
+            // laid out again through a SparseMatrix
+            if (Configuration::instance()->get_value("ell::threads", 1) != 1)
+                return SparseMatrixELL<DT_>(sparse_of(m));
+            return m;
+        }
+
+        /// SparseMatrix(SparseMatrixELL) (honei/la/sparse_matrix.hh:157-185):
+        /// every slot of every row (not only the first Arl[row]), zeros skipped.
+        template <typename DT_> static SparseMatrix<DT_> sparse_of(const SparseMatrixELL<DT_> & m)
+        {
+            SparseMatrix<DT_> sm(m.rows(), m.columns());
+            const unsigned long * aj = m.Aj().read_elements();
+            const DT_ * ax = m.Ax().read_elements();
+            const unsigned long size = m.Aj().size();
+            for (unsigned long row(0) ; row < m.rows() ; ++row)
+                for (unsigned long i(row * m.threads()) ; i < size ; i += m.stride())
+                    for (unsigned long t(0) ; t < m.threads() ; ++t)
+                        if (ax[i + t] != DT_(0))
+                            sm(row, aj[i + t], ax[i + t]);
+            return sm;
         }
 
         static void write_matrix(const std::string & output, const SparseMatrixELL<double> & m)

@@ -147,18 +147,16 @@ namespace honei
                 target.push_back(P);
                 return;
             }
+            // SparseMatrix(P) (every slot, zeros skipped), Scale, SparseMatrixELL (mg.hh:336-339)
             SparseMatrix<DataType_> sm(P.rows(), P.columns());
             const unsigned long * aj = P.Aj().read_elements();
             const DataType_ * ax = P.Ax().read_elements();
-            const unsigned long * arl = P.Arl().read_elements();
+            const unsigned long size = P.Aj().size();
             for (unsigned long row(0) ; row < P.rows() ; ++row)
-                for (unsigned long k(0) ; k < arl[row] ; ++k)
+                for (unsigned long pos(row * P.threads()) ; pos < size ; pos += P.stride())
                     for (unsigned long t(0) ; t < P.threads() ; ++t)
-                    {
-                        const unsigned long pos = row * P.threads() + t + k * P.stride();
-                        if (ax[pos] != DataType_(0))
-                            sm(row, aj[pos], ax[pos] * damping_factor);
-                    }
+                        if (ax[pos + t] != DataType_(0))
+                            sm(row, aj[pos + t], ax[pos + t] * damping_factor);
             target.push_back(MatrixType_(sm));
         }
     };

@@ -298,6 +298,27 @@ namespace honei
             used_iters = u;
             return x;
         }
+
+        /// CG preconditioned by a sparse matrix P (z = Product(P, r)) (cg.hh:141-219),
+        /// e.g. the SPAI / SAINV matrices PreconFill reads for the operator-list multigrid
+        template <typename DT>
+        static DenseVector<DT> & value(const SparseMatrixELL<DT> & A, const SparseMatrixELL<DT> & P,
+                                       const DenseVector<DT> & b, DenseVector<DT> & x, unsigned long max_iters,
+                                       unsigned long & used_iters, DT eps_relative = 1e-8)
+        {
+            using namespace mi355x;
+            same_size(b.size(), A.rows());
+            same_size(P.rows(), A.rows());
+            same_size(P.columns(), A.rows());
+            uint64_t u = 0;
+            double def = 0;
+            int st = hsolve_pcg_ell_pmat(A.device_handle(), P.device_handle(), rd(b), rw(x), max_iters,
+                                         (double)eps_relative, &u, &def, stream());
+            if (st != HSOLVE_ERR_NOT_CONVERGED)
+                check(st, "CGSolver VAR");
+            used_iters = u;
+            return x;
+        }
     };
 
     template <> struct ConjugateGradients<tags::GPU::MI355X, methods::NONE>
@@ -325,6 +346,38 @@ namespace honei
             same_size(rhs.size(), A.rows());
             same_size(x.size(), A.rows());
             check(hsolve_cg_konv_ell(A.device_handle(), rd(rhs), wr(x), konv_rad, 0, nullptr, stream()),
+                  "ConjugateGradients ELL");
+        }
+
+        /// At most `iters` steps from x = 0, stopping once the squared true
+        /// defect is below 1e-8 of its initial value (conjugate_gradients.hh:526-569);
+        /// one workgroup with the sequential sums, so bit-identical to tags::CPU.
+        /// Like the reference, reaching `iters` is not an error.
+        template <typename DT>
+        static void value(const BandedMatrixQx<Q1Type, DT> & A, const DenseVector<DT> & rhs, DenseVector<DT> & x,
+                          unsigned long iters)
+        {
+            using namespace mi355x;
+            same_size(rhs.size(), A.rows());
+            same_size(x.size(), A.rows());
+            const DT * bands[9];
+            A.device_bands(bands);
+            check(abi::cg_iters_q1(bands, rd(rhs), wr(x), A.size(), iters), "ConjugateGradients");
+        }
+
+        /// Plain CG from the given x for at most max_iters steps, stopping at
+        /// ||r|| < 1e-8 ||r0|| or ||r|| < 1e-8 (conjugate_gradients.hh:571-586,
+        /// kernel :203-248); reaching max_iters is not an error.
+        template <typename DT>
+        static void value(const SparseMatrixELL<DT> & A, const DenseVector<DT> & rhs, DenseVector<DT> & x,
+                          unsigned long max_iters)
+        {
+            using namespace mi355x;
+            same_size(rhs.size(), A.rows());
+            same_size(x.size(), A.rows());
+            uint64_t u = 0;
+            double def = 0;
+            check(hsolve_cg_iters_ell(A.device_handle(), rd(rhs), rw(x), max_iters, &u, &def, stream()),
                   "ConjugateGradients ELL");
         }
     };

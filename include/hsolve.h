@@ -248,6 +248,17 @@ int hsolve_cg_konv_q1_float(const void *const *bands9, const void *rhs, void *x,
 int hsolve_cg_konv_q1_double(const void *const *bands9, const void *rhs, void *x,
                              uint64_t size, double konv_rad, uint64_t max_iters, void *iters_dev,
                              void *stream);
+/* ConjugateGradients<Tag, NONE>::value(Q1, rhs, x, unsigned long iters)
+ * (honei/math/conjugate_gradients.hh:526-569) in one workgroup with the
+ * sequential sums of tags::CPU, so the iterate is bit-identical to it:
+ * x = 0, then up to `iters` steps, stopping once the squared norm of the
+ * true defect falls below 1e-8 times its initial value.  size <= 2^20.
+ * used_dev (uint64 device scalar, may be NULL) receives the steps taken;
+ * like the reference, reaching `iters` is not an error. */
+int hsolve_cg_iters_q1_float(const void *const *bands9, const void *rhs, void *x, uint64_t size,
+                             uint64_t iters, void *used_dev, void *stream);
+int hsolve_cg_iters_q1_double(const void *const *bands9, const void *rhs, void *x, uint64_t size,
+                              uint64_t iters, void *used_dev, void *stream);
 
 /* ---------------- device-resident CSR handle -------------------------------- */
 typedef struct hsolve_csr_s *hsolve_csr;
@@ -336,6 +347,21 @@ int hsolve_cg_ell(hsolve_ell m, const void *b, void *x, uint64_t max_iters, doub
  * vector, z = P * r elementwise), honei/math/cg.hh:141-219. */
 int hsolve_pcg_ell(hsolve_ell m, const void *P, const void *b, void *x, uint64_t max_iters,
                    double eps, uint64_t *used_iters, double *defect_out, void *stream);
+/* CGSolver<Tag, methods::VAR> with a SparseMatrixELL preconditioner P (an
+ * ELL handle of the same value type, rows = columns = rows of A; z =
+ * Product(P, r)), honei/math/cg.hh:141-219: the preconditioner form of the
+ * operator-list multigrid's file-read "spai"/"sainv" matrices. */
+int hsolve_pcg_ell_pmat(hsolve_ell m, hsolve_ell P, const void *b, void *x, uint64_t max_iters,
+                        double eps, uint64_t *used_iters, double *defect_out, void *stream);
+/* ConjugateGradients<Tag, NONE>::value(ELL, rhs, x, unsigned long
+ * max_iters) (honei/math/conjugate_gradients.hh:571-586, cg_kernel
+ * :203-248): CGSolver<NONE>'s steps from the given x without its epsilon
+ * guards (lambda = alpha / (v.p), p = p * (alpha / alpha_old) + r),
+ * stopping when ||r|| < 1e-8 ||r0|| or ||r|| < 1e-8 (both formed in
+ * double) or after max_iters.  Like the reference, reaching max_iters is
+ * not an error; used_iters and the final relative defect on output. */
+int hsolve_cg_iters_ell(hsolve_ell m, const void *rhs, void *x, uint64_t max_iters, uint64_t *used_iters,
+                        double *defect_out, void *stream);
 
 /* ConjugateGradients<Tag, methods::JAC>::value(ELL A, rhs, x, dd_inverted,
  * max_iters, used_iters, eps) (honei/math/conjugate_gradients.hh:1110-1155,

@@ -4,8 +4,10 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Used in the development container to check the
  * C restatement (oracle.c) against the reference itself and to write golden
- * fixtures under tests/golden/.  It never ships: nothing in the product
- * library, and nothing that runs on a GPU host, loads it.
+ * fixtures under tests/golden/.  The built library sits in oracle/_ref/,
+ * outside version control; it is copied along with the working tree like
+ * every built library, but nothing in the product library, and nothing
+ * that runs on a GPU host (GPU tests, smoke, benchmark), loads it.
  */
 #include <honei/la/dense_vector.hh>
 #include <honei/la/banded_matrix_qx.hh>

@@ -974,6 +974,229 @@ static void test_ir_mgop()
     TEST_CHECK(std::sqrt(num / den) <= 1e-6);
 }
 
+// ------------------------------------------------------------------------
+// ConjugateGradients<NONE> with a fixed iteration count, CGSolver<VAR> with a
+// matrix preconditioner, MatrixIO with ell::threads, MG with SPAI matrices
+// ------------------------------------------------------------------------
+template <typename DT> static double rel_diff(const DenseVector<DT> & x, const std::vector<DT> & r)
+{
+    double num = 0, den = 0;
+    for (unsigned long i = 0; i < x.size(); ++i)
+    {
+        num += ((double)x[i] - (double)r[i]) * ((double)x[i] - (double)r[i]);
+        den += (double)r[i] * (double)r[i];
+    }
+    return std::sqrt(num / (den > 0 ? den : 1e-300));
+}
+
+template <typename OE, typename DT> static OE or_view(const SparseMatrixELL<DT> & m);
+
+static void or_cg_iters_q1(const double * const * b, const double * rhs, double * x, uint64_t n, uint64_t it,
+                           uint64_t * used)
+{ or_cg_iters_q1_f64(b, rhs, x, n, it, used); }
+static void or_cg_iters_q1(const float * const * b, const float * rhs, float * x, uint64_t n, uint64_t it,
+                           uint64_t * used)
+{ or_cg_iters_q1_f32(b, rhs, x, n, it, used); }
+
+template <typename DT> static void test_cg_iters_q1()
+{
+    // poisson_cg_double_banded_q1_TEST.cc:50-72: FillMatrix / FillVector
+    // DIRICHLET_NEUMANN on 5 x 5 nodes, value(A, b, x, 20ul)
+    {
+        const unsigned long n = 25;
+        BandedMatrixQx<Q1Type, DT> A(poisson<DT>(n));
+        FillMatrix<Tag, applications::POISSON, boundary_types::DIRICHLET_NEUMANN>::value(A);
+        DenseVector<DT> b(n, DT(0));
+        FillVector<Tag, applications::POISSON, boundary_types::DIRICHLET_NEUMANN>::value(b);
+        DenseVector<DT> x(n, DT(1));
+        ConjugateGradients<Tag, methods::NONE>::value(A, b, x, 20ul);
+        std::vector<DT> xr(n);
+        uint64_t used = 0;
+        const DT * bb[9];
+        bands_host(A, bb);
+        or_cg_iters_q1(bb, b.read_elements(), xr.data(), n, 20, &used);
+        TEST_CHECK(bit_equal(x, xr.data()));
+    }
+    // the 4225 fixture: 50ul runs the fixed-iteration solver (not the
+    // konv_rad one), bit for bit, for caps below and above convergence
+    DenseVector<double> b64(4225), rs(4225);
+    BandedMatrixQx<Q1Type, double> A64(read_bin(4225, b64, rs));
+    const unsigned long n = 4225;
+    BandedMatrixQx<Q1Type, DT> A(poisson<DT>(n));
+    DenseVector<DT> b(n);
+    for (unsigned long i = 0; i < n; ++i)
+    {
+        b[i] = (DT)b64[i];
+        for (int k = 0; k < 9; ++k)
+            A.band(k).elements()[i] = (DT)A64.band(k).read_elements()[i];
+    }
+    for (unsigned long cap : {3ul, 50ul, 5000ul})
+    {
+        DenseVector<DT> x(n, DT(0));
+        ConjugateGradients<Tag, methods::NONE>::value(A, b, x, cap);
+        std::vector<DT> xr(n);
+        uint64_t used = 0;
+        const DT * bb[9];
+        bands_host(A, bb);
+        or_cg_iters_q1(bb, b.read_elements(), xr.data(), n, cap, &used);
+        std::printf("  Q1 fixed-iteration CG, cap %lu: %lu steps (oracle)\n", cap, (unsigned long)used);
+        TEST_CHECK(bit_equal(x, xr.data()));
+    }
+}
+
+static void or_cg_iters_ell(const or_ell_f64 * m, const double * rhs, double * x, uint64_t it, uint64_t * used)
+{ or_cg_iters_ell_f64(m, rhs, x, it, used); }
+static void or_cg_iters_ell(const or_ell_f32 * m, const float * rhs, float * x, uint64_t it, uint64_t * used)
+{ or_cg_iters_ell_f32(m, rhs, x, it, used); }
+
+template <typename DT> static void test_cg_iters_ell()
+{
+    typedef typename std::conditional<sizeof(DT) == 8, or_ell_f64, or_ell_f32>::type OE;
+    // conjugate_gradients_TEST.cc:520-600: area51, value(A, rhs, x, 10000ul),
+    // against FEATFLOW2's solution
+    SparseMatrixELL<DT> A(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/l2/area51_full_0.ell", DT(0)));
+    DenseVector<DT> rhs(VectorIO<io_formats::EXP>::read_vector(g_dir + "/l2/area51_rhs_0", DT(0)));
+    DenseVector<DT> init(VectorIO<io_formats::EXP>::read_vector(g_dir + "/l2/area51_init_0", DT(0)));
+    DenseVector<double> sol(VectorIO<io_formats::EXP>::read_vector(g_dir + "/l2/area51_sol_0", double(0)));
+    DenseVector<DT> x(init.copy());
+    ConjugateGradients<Tag, methods::NONE>::value(A, rhs, x, 10000ul);
+    std::vector<DT> xr(init.read_elements(), init.read_elements() + init.size());
+    uint64_t used = 0;
+    const OE e = or_view<OE>(A);
+    or_cg_iters_ell(&e, rhs.read_elements(), xr.data(), 10000, &used);
+    const double base_eps = 1e-3, add_eps = base_eps / 100.0;
+    const double m = (add_eps - base_eps) / 4.0, bb = base_eps - 4.0 * m;
+    const double eps = 3.0 * (m * sizeof(DT) + bb);
+    double maxdev = 0;
+    for (unsigned long i = 0; i < x.size(); ++i)
+        maxdev = std::max(maxdev, std::fabs((double)x[i] - sol[i]));
+    std::printf("  ELL fixed-iteration CG: oracle %lu iterations, rel diff %.2e, max |x - sol| %.2e (eps %.2e)\n",
+                (unsigned long)used, rel_diff(x, xr), maxdev, eps);
+    TEST_CHECK(rel_diff(x, xr) <= (sizeof(DT) == 8 ? 1e-6 : 1e-4));
+    TEST_CHECK(maxdev <= eps);
+}
+
+static void test_cg_var_pmat()
+{
+    // CGSolver<VAR> with the SPAI matrix of the 5-point 10 x 10 Laplacian
+    SparseMatrixELL<double> A(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/5pt_10x10.ell", double(0)));
+    SparseMatrixELL<double> P(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/5pt_10x10_spai.ell", double(0)));
+    DenseVector<double> b(random_vector<double>(A.rows(), 17)), x(A.rows(), 0.0);
+    unsigned long used = 0;
+    CGSolver<Tag, methods::VAR>::value(A, P, b, x, 500ul, used, 1e-8);
+    const or_ell_f64 ea = or_view<or_ell_f64>(A), ep = or_view<or_ell_f64>(P);
+    std::vector<double> xr(A.rows(), 0.0);
+    uint64_t used_ref = 0;
+    or_cg_var_ell_pm_f64(&ea, &ep, b.read_elements(), xr.data(), 500, 1e-8, &used_ref);
+    std::printf("  CGSolver<VAR, SPAI matrix>: %lu iterations (oracle %lu), rel diff %.2e\n", used,
+                (unsigned long)used_ref, rel_diff(x, xr));
+    TEST_CHECK(used + 1 >= used_ref && used <= used_ref + 1);
+    TEST_CHECK(rel_diff(x, xr) <= 1e-6);
+}
+
+static void test_ell_threads_io()
+{
+    // matrix_io.hh:572-579: with ell::threads = 4 the file's matrix is laid
+    // out again with four lanes per row; Product on it equals the oracle's
+    Configuration::instance()->set_value("ell::threads", 4);
+    SparseMatrixELL<double> A(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/poisson/A_6.ell", double(0)));
+    Configuration::instance()->set_value("ell::threads", 1);
+    SparseMatrixELL<double> A1(MatrixIO<io_formats::ELL>::read_matrix(g_dir + "/poisson/A_6.ell", double(0)));
+    TEST_CHECK(A.threads() == 4 && A1.threads() == 1);
+    // the layout the oracle builds from the same rows (sparse_matrix_ell-impl.hh:78-117)
+    const unsigned long n = A1.rows();
+    std::vector<uint64_t> rp(1, 0), col;
+    std::vector<double> val;
+    for (unsigned long r = 0; r < n; ++r)
+    {
+        for (unsigned long k = 0; k < A1.num_cols_per_row(); ++k)
+        {
+            const unsigned long pos = r + k * A1.stride();
+            if (A1.Ax().read_elements()[pos] != 0)
+            {
+                col.push_back(A1.Aj().read_elements()[pos]);
+                val.push_back(A1.Ax().read_elements()[pos]);
+            }
+        }
+        rp.push_back(col.size());
+    }
+    uint64_t stride = 0, ncpr = 0;
+    or_ell_shape(n, rp.data(), 4, &stride, &ncpr);
+    TEST_CHECK(stride == A.stride() && ncpr == A.num_cols_per_row());
+    std::vector<uint64_t> aj(stride * ncpr, 0), arl(n, 0);
+    std::vector<double> ax(stride * ncpr, 0.0);
+    or_ell_from_csr_f64(n, rp.data(), col.data(), val.data(), 4, stride, ncpr, aj.data(), ax.data(), arl.data());
+    TEST_CHECK(std::memcmp(aj.data(), A.Aj().read_elements(), sizeof(uint64_t) * aj.size()) == 0);
+    TEST_CHECK(std::memcmp(ax.data(), A.Ax().read_elements(), sizeof(double) * ax.size()) == 0);
+    DenseVector<double> x(random_vector<double>(n, 5)), y(n, 0.0);
+    Product<Tag>::value(y, A, x);
+    std::vector<double> yr(n);
+    or_ell_product_f64(yr.data(), aj.data(), ax.data(), arl.data(), x.read_elements(), 0, n, stride, 4);
+    TEST_CHECK(bit_equal(y, yr.data()));
+}
+
+static void test_mgop_spai()
+{
+    // MGData with PreconContType_ = SparseMatrixELL: PreconFill reads
+    // A_<l>_spai.ell; RISmoother applies Product(P, t0); the coarse solve
+    // is CGSolver<VAR> with the matrix
+    typedef MGUtil<Tag, SparseMatrixELL<double>, DenseVector<double>, SparseMatrixELL<double>,
+                   SparseMatrixELL<double>, MatrixIO<io_formats::ELL>, VectorIO<io_formats::EXP>, double> Util;
+    const std::string base = g_dir + "/poisson_advanced/sort_0/";
+    for (int shape = 0; shape < 2; ++shape)
+        for (int var = 0; var < 2; ++var)
+        {
+            auto data = Util::load_data(base, 4, 1.0, "spai");
+            Util::configure(data, 100, 100, 4, 4, 1, 1e-8);
+            const std::vector<double> x0 = host_copy(data.x.back());
+            OperatorList ol;
+            if (shape == 0 && var == 0)
+                ol = MGCycleCreation<Tag, methods::CYCLE::V::STATIC, CGSolver<Tag, methods::NONE>, RISmoother<Tag>,
+                        Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>, double>::value(data);
+            else if (shape == 0)
+                ol = MGCycleCreation<Tag, methods::CYCLE::V::STATIC, CGSolver<Tag, methods::VAR>, RISmoother<Tag>,
+                        Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>, double>::value(data);
+            else if (var == 0)
+                ol = MGCycleCreation<Tag, methods::CYCLE::W::STATIC, CGSolver<Tag, methods::NONE>, RISmoother<Tag>,
+                        Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>, double>::value(data);
+            else
+                ol = MGCycleCreation<Tag, methods::CYCLE::W::STATIC, CGSolver<Tag, methods::VAR>, RISmoother<Tag>,
+                        Restriction<Tag, methods::PROLMAT>, Prolongation<Tag, methods::PROLMAT>, double>::value(data);
+            MGSolver<Tag, Norm<vnt_l_two, true, Tag>>::value(data, ol);
+            const std::vector<double> x = host_copy(data.x.back());
+            std::vector<or_ell_f64> A, Pm, prol, res;
+            for (auto & m : data.A) A.push_back(or_view<or_ell_f64>(m));
+            for (auto & m : data.P) Pm.push_back(or_view<or_ell_f64>(m));
+            for (auto & m : data.prolmat) prol.push_back(or_view<or_ell_f64>(m));
+            for (auto & m : data.resmat) res.push_back(or_view<or_ell_f64>(m));
+            or_mgop_params p;
+            p.cycle = shape;
+            p.coarse_var = var;
+            p.levels = 4;
+            p.min_level = 1;
+            p.max_iters = 100;
+            p.max_iters_coarse = 100;
+            p.n_pre_smooth = 4;
+            p.n_post_smooth = 4;
+            p.eps_relative = 1e-8;
+            std::vector<double> xr(x0);
+            uint64_t used = 0, used_c = 0;
+            double fin = 0;
+            or_mgop_solve_pm_f64(&p, A.data(), Pm.data(), prol.data(), res.data(), data.b.back().read_elements(),
+                                 xr.data(), &used, &used_c, &fin);
+            std::printf("  SPAI cycle %d coarse %s: %lu cycles (oracle %lu)\n", shape, var ? "VAR" : "NONE",
+                        data.used_iters, (unsigned long)used);
+            TEST_CHECK((long)data.used_iters - (long)used <= 1 && (long)used - (long)data.used_iters <= 1);
+            double num = 0, den = 0;
+            for (size_t i = 0; i < x.size(); ++i)
+            {
+                num += (x[i] - xr[i]) * (x[i] - xr[i]);
+                den += xr[i] * xr[i];
+            }
+            TEST_CHECK(std::sqrt(num / den) <= 1e-6);
+        }
+}
+
 int main(int argc, char ** argv)
 {
     g_dir = argc > 1 ? argv[1] : "tests/golden/honei_testdata";
@@ -998,6 +1221,13 @@ int main(int argc, char ** argv)
         {"multigrid_ell", test_multigrid_ell},
         {"mgop_cycles", test_mgop_cycles},
         {"ir_mgop", test_ir_mgop},
+        {"cg_iters_q1<double>", test_cg_iters_q1<double>},
+        {"cg_iters_q1<float>", test_cg_iters_q1<float>},
+        {"cg_iters_ell<double>", test_cg_iters_ell<double>},
+        {"cg_iters_ell<float>", test_cg_iters_ell<float>},
+        {"cg_var_pmat", test_cg_var_pmat},
+        {"ell_threads_io", test_ell_threads_io},
+        {"mgop_spai", test_mgop_spai},
     };
     for (auto & c : cases)
     {

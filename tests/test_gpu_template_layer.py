@@ -20,5 +20,7 @@ def test_template_layer_against_oracle(hip):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
     for name in ("ell_product_defect<double>", "q1_and_blas1<float>", "cg", "multigrid_mixed", "lock_semantics",
-                 "multigrid_fixed", "multigrid_ell", "mgop_cycles", "ir_mgop"):
+                 "multigrid_fixed", "multigrid_ell", "mgop_cycles", "ir_mgop", "cg_iters_q1<double>",
+                 "cg_iters_q1<float>", "cg_iters_ell<double>", "cg_iters_ell<float>", "cg_var_pmat", "ell_threads_io",
+                 "mgop_spai"):
         assert f"PASS {name}" in r.stdout
