@@ -149,6 +149,20 @@ def q1_case(hip, P, side, dtype, steps, warmup):
                 frac_hbm=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
 
 
+def mg_solve_bytes(level, min_level, inner, outer):
+    """Algorithmic HBM bytes of one MG-IR solve on Q1 bands: per FP32 V-cycle
+    (2 + 2 sweeps) the sum over its levels l of 5 M_l + 18 n_l 4 + 2 n_(l-1) 4
+    with M_l = 36 n_l (four sweeps and one defect over the nine bands,
+    the vectors they stream, restriction, prolongation with correction);
+    per FP64 outer step 128 n (fused defect + norm 72n, scale + cast 24n,
+    cast + update 12n + 20n).  5.69 GB per V-cycle and 2.15 GB per outer
+    step at 4097^2."""
+    n = lambda l: ((1 << l) + 1) ** 2
+    cycle = sum(5 * 36 * n(l) + 18 * 4 * n(l) + (2 * 4 * n(l - 1) if l > min_level else 0)
+                for l in range(min_level, level + 1))
+    return inner * cycle + outer * 128 * n(level)
+
+
 def mg_case(hip, P, level, min_level, reps, tol=1e-8):
     n = P.side_of_level(level) ** 2
     mg = hip.MGSolver(hip.mg_params(min_level, level, n_max_iter=2, tolerance=tol))
@@ -165,10 +179,15 @@ def mg_case(hip, P, level, min_level, reps, tol=1e-8):
         if r > 0:
             lat.append((t1 - t0) * 1e3)
     lat = np.asarray(lat)
+    p50 = float(np.percentile(lat, 50))
+    nbytes = mg_solve_bytes(level, min_level, info["inner"], info["outer"])
+    gbps = nbytes / (p50 * 1e-3) / 1e9
     return dict(n=n, levels=f"{min_level}..{level}", tol=tol, outer=info["outer"], inner=info["inner"],
                 rel_defect=info["final_defect"] / info["initial_defect"],
-                ms_p50=float(np.percentile(lat, 50)), ms_p99=float(np.percentile(lat, 99)),
-                solves_per_s=float(1e3 / lat.mean()), device_bytes=mg.device_bytes), x
+                ms_p50=p50, ms_p99=float(np.percentile(lat, 99)),
+                solves_per_s=float(1e3 / lat.mean()), device_bytes=mg.device_bytes,
+                roofline=dict(bound="hbm", bytes_per_solve=nbytes, achieved=round(gbps, 1), peak=HBM_PEAK_GBS,
+                              unit="GB/s", frac=round(gbps / HBM_PEAK_GBS, 4))), x
 
 
 def serve_trace(count, rate, seed):
@@ -378,25 +397,47 @@ def cpu_threads():
 
 
 def cpu_baseline_spmv(E, budget_s=10.0):
-    """The oracle (C restatement of Product<tags::CPU> on ELL) over the same
-    Matrix A and x, repeated until about budget_s seconds: single-threaded,
-    and with the rows split into one equal range per thread as
-    Product<tags::CPU::MultiCore> does (product.hh:2087-2106).  Returns
-    (multi-thread result, single-thread result, the oracle's y)."""
+    """CPU ELL SpMV over the same Matrix A and x, repeated for about budget_s
+    seconds: on one thread (Product<tags::CPU::SSE>), and with the rows split
+    into one equal range per thread as Product<tags::CPU::MultiCore::SSE>
+    does (product.hh:2087-2106).  The kernel is the reference's own SSE
+    product_smell_dv (honei/backends/sse/product.cc), compiled from its sources
+    into oracle/_ref, when that library is present ("reference"); else the
+    oracle's C restatement of the same loop ("port").  Returns (multi-thread
+    result, single-thread result, the product y), y checked against the
+    oracle's."""
     import ctypes as C
     from concurrent.futures import ThreadPoolExecutor
     from honei_amd import problems as P
-    from oracle.pyoracle import Oracle
-    O = Oracle()
-    fn = O.lib.or_ell_product_f64      # ctypes releases the GIL during the call
+    from oracle.pyoracle import REF_SO, Oracle, have_reference
     n = E["rows"]
     x = P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1)
     y = np.zeros(n)
     Aj, Ax, Arl = E["Aj"], E["Ax"], np.ascontiguousarray(E["Arl"], np.uint64)
-    ptr = [a.ctypes.data_as(C.c_void_p) for a in (y, Aj, Ax, Arl, x)]
+    stride, threads = E["stride"], E.get("threads", 1)
+    O = Oracle()
+    kind = "port"
+    if have_reference():
+        try:
+            fn = C.CDLL(REF_SO).ref_sse_product_smell_dv_f64   # ctypes releases the GIL during the call
+            kind = "reference"
+        except (OSError, AttributeError):
+            kind = "port"
+    vp_ = C.c_void_p
+    if kind == "reference":
+        ptr = [a.ctypes.data_as(vp_) for a in (y, Aj, Ax, Arl, x)]
 
-    def rows(r0, r1):
-        fn(*ptr, C.c_uint64(r0), C.c_uint64(r1), C.c_uint64(E["stride"]), C.c_uint64(E.get("threads", 1)))
+        def rows(r0, r1):
+            fn(*ptr, C.c_uint64(stride), C.c_uint64(n), C.c_uint64(E["ncpr"]), C.c_uint64(r0), C.c_uint64(r1),
+               C.c_uint64(threads))
+        what = "the reference's SSE product_smell_dv (Product<tags::CPU::SSE>)"
+    else:
+        ofn = O.lib.or_ell_product_f64
+        ptr = [a.ctypes.data_as(vp_) for a in (y, Aj, Ax, Arl, x)]
+
+        def rows(r0, r1):
+            ofn(*ptr, C.c_uint64(r0), C.c_uint64(r1), C.c_uint64(stride), C.c_uint64(threads))
+        what = "the oracle's Product<tags::CPU> ELL"
 
     nbytes = P.ell_spmv_bytes(n, P.ell_nnz(E), 8)
     out = []
@@ -411,9 +452,16 @@ def cpu_baseline_spmv(E, budget_s=10.0):
                 if time.perf_counter() - t0 > budget_s:
                     break
             dt = (time.perf_counter() - t0) / reps
-        out.append(dict(value=nbytes / dt / 1e9, unit="GB/s", cores=T, kind="port",
-                        sample=f"{reps} full Product<tags::CPU> ELL SpMVs of Matrix A FP64 ({n} rows), "
+        out.append(dict(value=nbytes / dt / 1e9, unit="GB/s", cores=T, kind=kind,
+                        sample=f"{reps} full SpMVs of Matrix A FP64 ({n} rows) by {what}, "
                                f"{dt * 1e3:.1f} ms each, {T} thread(s) over equal row ranges, on {cpu_model()}"))
+    if kind == "reference":
+        y_or = np.zeros(n)
+        O.lib.or_ell_product_f64(y_or.ctypes.data_as(vp_), Aj.ctypes.data_as(vp_), Ax.ctypes.data_as(vp_),
+                                 Arl.ctypes.data_as(vp_), x.ctypes.data_as(vp_), C.c_uint64(0), C.c_uint64(n),
+                                 C.c_uint64(stride), C.c_uint64(threads))
+        if not np.array_equal(y, y_or):
+            raise AssertionError("reference SSE product differs from the oracle's")
     return out[1], out[0], y
 
 

@@ -99,6 +99,8 @@ _SIGS = {
     "hsolve_apply_dirichlet_double": [vp, u64, vp, vp],
     "hsolve_convert_float_double": [vp, vp, u64, vp],
     "hsolve_convert_double_float": [vp, vp, u64, vp],
+    "hsolve_fill_poisson_q1_float": [vp, u64, vp],
+    "hsolve_fill_poisson_q1_double": [vp, u64, vp],
     "hsolve_jacobi_q1_float": [vp, vp, vp, vp, vp, u64, vp],
     "hsolve_jacobi_q1_double": [vp, vp, vp, vp, vp, u64, vp],
     "hsolve_prolongation_axpy_float": [vp, u64, vp, u64, flt, vp, vp],

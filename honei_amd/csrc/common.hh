@@ -63,6 +63,14 @@ namespace hsolve
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     }
 
+    // The same pointer in the global address space.  Device pointers that a
+    // kernel reads from memory (slot tables, pointer arrays) are generic, and
+    // the compiler then uses flat instructions, which also count against the
+    // LDS counter: every wait on them drains LDS traffic as well and the
+    // compiler falls back to waiting for all outstanding loads.
+    template <typename T> using gptr = __attribute__((address_space(1))) T *;
+    template <typename T> __device__ __forceinline__ gptr<T> as_global(T * p) { return (gptr<T>)p; }
+
     inline unsigned grid_for(uint64_t work, unsigned per_block)
     {
         uint64_t g = (work + per_block - 1) / per_block;

@@ -94,6 +94,24 @@ namespace
         return true;
     }
 
+    // ell_spmv_lds_p on a grid of as many workgroups as are resident on the
+    // device at once (at most one per row block)
+    template <typename DT, typename IT, typename LT, int K, int MODE, bool SORTED>
+    void launch_lds_p(hipStream_t st, size_t lds, uint32_t nblk, const DT * x, DT * y, const IT * Aj, const DT * Ax,
+                      const LT * Arl, const DT * rhs, const DT * dinv, const hsolve_ell_s * m, const uint16_t * perm)
+    {
+        auto kern = k::ell_spmv_lds_p<DT, IT, LT, K, MODE, SORTED>;
+        int dev = 0, cus = 0, per_cu = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, k::lds_rows<DT>() / 2, lds));
+        uint32_t G = (uint32_t)std::max(1, cus * std::max(1, per_cu));
+        if (G > nblk)
+            G = nblk;
+        hipLaunchKernelGGL(kern, dim3(G), dim3(k::lds_rows<DT>() / 2), lds, st, x, y, Aj, Ax, Arl, rhs, dinv, m->rows,
+                           m->cols, m->stride, (uint32_t)m->ncpr, m->max_offset, perm, nblk);
+    }
+
     template <typename DT, int MODE>
     void ell_launch(const hsolve_ell_s * m, const DT * x, DT * y, const DT * rhs, const DT * dinv, hipStream_t st)
     {
@@ -115,6 +133,32 @@ namespace
                 const LT * sArl = (const LT *)m->sArl;
                 const DT * sAx = (const DT *)m->sAx;
                 const uint16_t * perm = (const uint16_t *)m->sperm;
+                if (v & HSOLVE_ELL_PERSIST)
+                {
+                    // a fixed grid: as many workgroups as fit on the device at once
+                    const uint32_t nblk = (uint32_t)g;
+#define HS_LP(K)                                                                                              \
+                    if (sorted)                                                                               \
+                        launch_lds_p<DT, IT, LT, K, MODE, true>(st, lds, nblk, x, y, sAj, sAx, sArl, rhs, dinv, m, perm); \
+                    else                                                                                      \
+                        launch_lds_p<DT, IT, LT, K, MODE, false>(st, lds, nblk, x, y, Aj, Ax, Arl, rhs, dinv, m,  \
+                                                                 (const uint16_t *)nullptr)
+                    if (m->ncpr <= 9)
+                    {
+                        HS_LP(9);
+                    }
+                    else if (m->ncpr <= 13)
+                    {
+                        HS_LP(13);
+                    }
+                    else
+                    {
+                        HS_LP(16);
+                    }
+#undef HS_LP
+                    HIP_CHECK(hipGetLastError());
+                    return;
+                }
 #define HS_L(K)                                                                                               \
                 if (sorted)                                                                                   \
                     hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE, true>), dim3(g), dim3(k::lds_rows<DT>() / 2), lds, \
@@ -416,7 +460,7 @@ int hsolve_ell_set_variant(hsolve_ell m, int variant)
 {
     HSOLVE_API_BEGIN
     require(m != nullptr, HSOLVE_ERR_INVALID, "ell_set_variant: null handle");
-    require(variant >= 0 && variant < 64 && (variant & HSOLVE_ELL_SET) == 0, HSOLVE_ERR_INVALID,
+    require(variant >= 0 && variant < 128 && (variant & HSOLVE_ELL_SET) == 0, HSOLVE_ERR_INVALID,
             "ell_set_variant: variant bits out of range");
     m->variant = variant ? (variant | HSOLVE_ELL_SET) : 0;
     HSOLVE_API_END

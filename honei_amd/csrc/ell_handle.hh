@@ -37,7 +37,8 @@ enum
     HSOLVE_ELL_XCD = 4,        // XCD-contiguous block order
     HSOLVE_ELL_SET = 8,        // marks an explicit choice
     HSOLVE_ELL_LDS = 16,       // x window in LDS (banded matrices; else ignored)
-    HSOLVE_ELL_SORT = 32       // with LDS: rows sorted by length inside each block (when the copy exists)
+    HSOLVE_ELL_SORT = 32,      // with LDS: rows sorted by length inside each block (when the copy exists)
+    HSOLVE_ELL_PERSIST = 64    // with LDS: a fixed grid walks the row blocks, next lengths loaded ahead
 };
 
 namespace hsolve

@@ -384,6 +384,160 @@ namespace k
         }
     }
 
+    // Persistent form of ell_spmv_lds (same sums, same order, same results):
+    // a fixed grid of workgroups walks the row blocks, each XCD a contiguous
+    // range of them (consecutive blocks, whose windows overlap, run on one
+    // L2 at about the same time).  While a workgroup works on one block, the
+    // row lengths (and, SORTED, the original rows) of its next block are
+    // already loading, so that block's slot and window loads issue at once
+    // instead of after a round trip for the lengths.
+    template <typename DT, typename IT, typename LT, int MAXK, int MODE, bool SORTED = false>
+    __global__ __launch_bounds__(lds_rows<DT>() / 2) __attribute__((amdgpu_waves_per_eu(4, 8))) void ell_spmv_lds_p(
+            const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
+            const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
+            const DT * __restrict__ dinv, uint64_t rows, uint64_t cols, uint64_t stride, uint32_t ncpr,
+            uint64_t maxoff, const uint16_t * __restrict__ perm, uint32_t nblk)
+    {
+        constexpr int kLdsRows = lds_rows<DT>();
+        constexpr int kLdsThreads = kLdsRows / 2;
+        extern __shared__ __align__(16) unsigned char lds_raw[];
+        DT * xw = reinterpret_cast<DT *>(lds_raw);
+        typedef typename Vec2<DT>::type V;
+        typedef typename Vec2<IT>::type VI;
+        // this workgroup's XCD, its place among that XCD's workgroups, and
+        // the XCD's range of blocks
+        const uint32_t G = gridDim.x;
+        const uint32_t xcd = blockIdx.x & 7u, pos = blockIdx.x >> 3;
+        const uint32_t per_xcd = (G >> 3) + (xcd < (G & 7u) ? 1u : 0u);
+        const uint32_t q = nblk >> 3, rem = nblk & 7u;
+        const uint32_t bbeg = xcd * q + (xcd < rem ? xcd : rem);
+        const uint32_t bcnt = q + (xcd < rem ? 1u : 0u);
+        // lengths and block-local storage rows of the original rows of block
+        // bbeg + i for this lane (lengths are at most 16: one byte each)
+        auto meta = [&](uint32_t i, uint32_t & lens, uint32_t & loc) {
+            const uint64_t rb0 = (uint64_t)(bbeg + i) * kLdsRows;
+            const uint64_t r0 = rb0 + (uint64_t)threadIdx.x * 2;
+            const uint64_t rr = r0 < rows ? r0 : rb0;
+            lens = (uint32_t)Arl[rr] | ((uint32_t)Arl[rr + 1] << 8);
+            loc = (uint32_t)(rr - rb0) | ((uint32_t)(rr + 1 - rb0) << 16);
+            if (SORTED)
+                loc = (uint32_t)perm[rr] | ((uint32_t)perm[rr + 1] << 16);
+        };
+        uint32_t nlens = 0, nloc = 0;
+        if (pos < bcnt)
+            meta(pos, nlens, nloc);
+        for (uint32_t i = pos; i < bcnt; i += per_xcd)
+        {
+            const uint32_t lens = nlens, loc = nloc;
+            if (i + per_xcd < bcnt)
+                meta(i + per_xcd, nlens, nloc);
+            const uint64_t rb0 = (uint64_t)(bbeg + i) * kLdsRows;
+            const uint64_t w0 = rb0 > maxoff ? rb0 - maxoff : 0;
+            const uint64_t wend = rb0 + kLdsRows + maxoff;
+            const uint64_t w1 = wend < cols ? wend : cols;
+            const uint32_t wn = w1 > w0 ? (uint32_t)(w1 - w0) : 0u;
+            const uint64_t r0 = rb0 + (uint64_t)threadIdx.x * 2;
+            const bool live = r0 < rows;
+            const bool two = r0 + 1 < rows;
+            const uint64_t rr = live ? r0 : rb0;
+            const uint32_t l0 = lens & 0xffu, l1 = lens >> 8;
+            const uint64_t o0 = rb0 + (loc & 0xffffu), o1 = rb0 + (loc >> 16);
+            const uint32_t len0 = live ? l0 : 0u;
+            const uint32_t len1 = two ? l1 : 0u;
+            const uint32_t lmax = l0 > l1 ? l0 : l1;
+            DT a0[MAXK], a1[MAXK];
+            IT c0[MAXK], c1[MAXK];
+#pragma unroll
+            for (int k = 0; k < MAXK; ++k)
+            {
+                const uint64_t kk = (uint32_t)k < lmax ? (uint64_t)k : 0;
+                const V v = __builtin_nontemporal_load((const V *)(Ax + rr + kk * stride));
+                const VI c = __builtin_nontemporal_load((const VI *)(Aj + rr + kk * stride));
+                a0[k] = v.x;
+                a1[k] = v.y;
+                c0[k] = (IT)c.x;
+                c1[k] = (IT)c.y;
+            }
+            // one pass of window loads covers the +-2048 window of a block
+            // (FP64: 5120 values on 512 lanes; FP32: 4608 on 256); wider
+            // windows take the loop below
+            constexpr uint32_t kFill = sizeof(DT) == 8 ? 10 : 18;
+            DT wv[kFill];
+            if (wn != 0)
+            {
+#pragma unroll
+                for (uint32_t f = 0; f < kFill; ++f)
+                {
+                    const uint32_t j = f * kLdsThreads + threadIdx.x;
+                    wv[f] = x[w0 + (j < wn ? j : wn - 1)];
+                }
+            }
+            // every lane is done with the previous block's window
+            lds_barrier();
+            if (wn != 0)
+            {
+#pragma unroll
+                for (uint32_t f = 0; f < kFill; ++f)
+                {
+                    const uint32_t j = f * kLdsThreads + threadIdx.x;
+                    if (j < wn)
+                        xw[j] = wv[f];
+                }
+            }
+            for (uint32_t base = kFill * kLdsThreads; base < wn; base += kFill * kLdsThreads)
+            {
+#pragma unroll
+                for (uint32_t f = 0; f < kFill; ++f)
+                {
+                    const uint32_t j = base + f * kLdsThreads + threadIdx.x;
+                    wv[f] = x[w0 + (j < wn ? j : wn - 1)];
+                }
+#pragma unroll
+                for (uint32_t f = 0; f < kFill; ++f)
+                {
+                    const uint32_t j = base + f * kLdsThreads + threadIdx.x;
+                    if (j < wn)
+                        xw[j] = wv[f];
+                }
+            }
+            lds_barrier();
+            if (live)
+            {
+                DT s0 = DT(0), s1 = DT(0);
+#pragma unroll
+                for (int k = 0; k < MAXK; ++k)
+                {
+                    if ((uint32_t)k < len0)
+                        s0 += a0[k] * xw[col_of<IT>(c0[k], o0) - w0];
+                    if ((uint32_t)k < len1)
+                        s1 += a1[k] * xw[col_of<IT>(c1[k], o1) - w0];
+                }
+                if (MODE == 0)
+                {
+                    y[o0] = s0;
+                    if (two)
+                        y[o1] = s1;
+                }
+                else if (MODE == 1)
+                {
+                    y[o0] = rhs[o0] - s0;
+                    if (two)
+                        y[o1] = rhs[o1] - s1;
+                }
+                else
+                {
+                    const DT t0 = rhs[o0] - s0;
+                    y[o0] = x[o0] + t0 * dinv[o0];
+                    if (two)
+                    {
+                        const DT t1 = rhs[o1] - s1;
+                        y[o1] = x[o1] + t1 * dinv[o1];
+                    }
+                }
+            }
+        }
+    }
+
     // ELL with threads > 1: still one lane per row, walking k then t so the
     // summation order is the reference's (product.hh:192-202).
     template <typename DT, typename IT, typename LT, int MODE>

@@ -58,19 +58,16 @@ namespace k
         return (DT)((double)acc + w * (double)v);
     }
 
-    // Restriction<Tag, NONE>: one lane per coarse node (r, s).  The
-    // reference visits the node in its outer loop iterations i = r, r+1, s,
-    // s+1 (those in [1, Nc-1]) and, inside one iteration, in the statement
-    // order (a)..(h) below.
-    template <typename DT>
-    __device__ __forceinline__ void restriction_body(DT * __restrict__ coarse, const DT * __restrict__ fine,
-                                                     int64_t Nc, const Mask & mk, int64_t r, int64_t s)
+    // Restriction<Tag, NONE> at coarse node (r, s) of a side-Nc grid, with
+    // F(dr, ds) the fine value at (2r + dr, 2s + ds).  The reference visits
+    // the node in its outer loop iterations i = r, r+1, s, s+1 (those in
+    // [1, Nc-1], ascending, each once) and, inside one iteration, in the
+    // statement order (a)..(h) below; F is called only for fine nodes those
+    // statements read.
+    template <typename DT, typename FV>
+    __device__ __forceinline__ DT restrict_node(FV && F, int64_t r, int64_t s, int64_t Nc, const Mask & mk)
     {
-        if (s >= Nc)
-            return;
-        const int64_t idx = r * Nc + s, Nf = 2 * Nc - 1;
-        const DT * f = fine;
-        DT v = f[(2 * r) * Nf + 2 * s];
+        DT v = F(0, 0);
         int64_t c[4] = {r, r + 1, s, s + 1};
         // sort 4 values (r < r+1 and s < s+1 already)
         if (c[2] < c[0]) { int64_t t = c[0]; c[0] = c[2]; c[2] = t; }
@@ -87,18 +84,43 @@ namespace k
             prev = i;
             if (i < 1 || i > Nc - 1)
                 continue;
-            if (r == i)     v = add_d(v, 0.5, f[(2 * r - 1) * Nf + 2 * s]);         // (a)
-            if (r == i - 1) v = add_d(v, 0.5, f[(2 * r + 1) * Nf + 2 * s]);         // (b)
-            if (s == i - 1) v = add_d(v, 0.5, f[(2 * r) * Nf + 2 * s + 1]);         // (c)
-            if (s == i)     v = add_d(v, 0.5, f[(2 * r) * Nf + 2 * s - 1]);         // (d)
-            if (r == i - 1 && s <= Nc - 2) v = add_d(v, 0.25, f[(2 * r + 1) * Nf + 2 * s + 1]); // (e)
-            if (r == i - 1 && s >= 1)      v = add_d(v, 0.25, f[(2 * r + 1) * Nf + 2 * s - 1]); // (f)
-            if (r == i && s <= Nc - 2)     v = add_d(v, 0.25, f[(2 * r - 1) * Nf + 2 * s + 1]); // (g)
-            if (r == i && s >= 1)          v = add_d(v, 0.25, f[(2 * r - 1) * Nf + 2 * s - 1]); // (h)
+            if (r == i)     v = add_d(v, 0.5, F(-1, 0));                        // (a)
+            if (r == i - 1) v = add_d(v, 0.5, F(1, 0));                         // (b)
+            if (s == i - 1) v = add_d(v, 0.5, F(0, 1));                         // (c)
+            if (s == i)     v = add_d(v, 0.5, F(0, -1));                        // (d)
+            if (r == i - 1 && s <= Nc - 2) v = add_d(v, 0.25, F(1, 1));         // (e)
+            if (r == i - 1 && s >= 1)      v = add_d(v, 0.25, F(1, -1));        // (f)
+            if (r == i && s <= Nc - 2)     v = add_d(v, 0.25, F(-1, 1));        // (g)
+            if (r == i && s >= 1)          v = add_d(v, 0.25, F(-1, -1));       // (h)
         }
         if (dirichlet_rs((int)r, (int)s, (int)Nc, mk))
             v = DT(0);
-        coarse[idx] = v;
+        return v;
+    }
+
+    // Restriction<Tag, NONE>: one lane per coarse node (r, s).  The nine
+    // fine values around (2r, 2s) are loaded up front (indices clamped to the
+    // grid; the clamped ones are never among those restrict_node reads).
+    template <typename DT>
+    __device__ __forceinline__ void restriction_body(DT * __restrict__ coarse, const DT * __restrict__ fine,
+                                                     int64_t Nc, const Mask & mk, int64_t r, int64_t s)
+    {
+        if (s >= Nc)
+            return;
+        const int64_t Nf = 2 * Nc - 1;
+        DT fv[3][3];
+#pragma unroll
+        for (int dr = 0; dr < 3; ++dr)
+        {
+            const int64_t row = min(max(2 * r + dr - 1, (int64_t)0), Nf - 1);
+#pragma unroll
+            for (int ds = 0; ds < 3; ++ds)
+            {
+                const int64_t col = min(max(2 * s + ds - 1, (int64_t)0), Nf - 1);
+                fv[dr][ds] = fine[row * Nf + col];
+            }
+        }
+        coarse[r * Nc + s] = restrict_node<DT>([&](int dr, int ds) { return fv[dr + 1][ds + 1]; }, r, s, Nc, mk);
     }
 
     // coarse row blockIdx.y, columns along x
@@ -179,6 +201,85 @@ namespace k
         }
     }
 
+    // Prolongation<Tag, NONE> at one fine node (R, S) of a side-Nf grid: the
+    // same values as prolongation2_body (the reference's addition order in
+    // double from 0, the fine grid's Dirichlet mask applied).
+    template <typename DT>
+    __device__ __forceinline__ DT prolongation_node(const DT * __restrict__ coarse, int Nf, const Mask & mk, int R,
+                                                    int S)
+    {
+        const int Nc = (Nf + 1) / 2;
+        const int jj = S >> 1;
+        DT v;
+        if ((R & 1) == 0)
+        {
+            const DT * c = coarse + (int64_t)(R >> 1) * Nc;
+            if ((S & 1) == 0)
+                v = c[jj];
+            else
+            {
+                v = add_d(DT(0), 0.5, c[jj]);
+                v = add_d(v, 0.5, c[jj + 1]);
+            }
+        }
+        else
+        {
+            const int i = (R + 1) >> 1;
+            const DT * cm = coarse + (int64_t)(i - 1) * Nc;
+            const DT * cp = coarse + (int64_t)i * Nc;
+            if ((S & 1) == 0)
+            {
+                v = add_d(DT(0), 0.5, cm[jj]);
+                v = add_d(v, 0.5, cp[jj]);
+            }
+            else
+            {
+                v = add_d(DT(0), 0.25, cm[jj]);
+                v = add_d(v, 0.25, cm[jj + 1]);
+                v = add_d(v, 0.25, cp[jj]);
+                v = add_d(v, 0.25, cp[jj + 1]);
+            }
+        }
+        if (dirichlet_rs(R, S, Nf, mk))
+            v = DT(0);
+        return v;
+    }
+
+    // prolongation_node from values already in registers: c[0..3] = coarse
+    // (R >> 1, S >> 1), (R >> 1, S >> 1 + 1), ((R + 1) >> 1, S >> 1),
+    // ((R + 1) >> 1, S >> 1 + 1) (the second column clamped to the grid; the
+    // values a node does not use may be anything).  Same additions, same bits.
+    template <typename DT>
+    __device__ __forceinline__ DT prolongation_node_v(const DT (&c)[4], int Nf, const Mask & mk, int R, int S)
+    {
+        DT v;
+        if ((R & 1) == 0)
+        {
+            if ((S & 1) == 0)
+                v = c[0];
+            else
+            {
+                v = add_d(DT(0), 0.5, c[0]);
+                v = add_d(v, 0.5, c[1]);
+            }
+        }
+        else if ((S & 1) == 0)
+        {
+            v = add_d(DT(0), 0.5, c[0]);
+            v = add_d(v, 0.5, c[2]);
+        }
+        else
+        {
+            v = add_d(DT(0), 0.25, c[0]);
+            v = add_d(v, 0.25, c[1]);
+            v = add_d(v, 0.25, c[2]);
+            v = add_d(v, 0.25, c[3]);
+        }
+        if (dirichlet_rs(R, S, Nf, mk))
+            v = DT(0);
+        return v;
+    }
+
     template <typename DT, bool AXPY>
     __global__ __launch_bounds__(256) void prolongation2_kernel(DT * __restrict__ fine,
             const DT * __restrict__ coarse, int Nf, DT alpha, Mask mk, const int * __restrict__ active)
@@ -197,9 +298,11 @@ namespace k
     // aligned.  Lanes whose rows touch either end of the vector take the
     // guarded per-row form; every row sums in q1_row's order.
     // ORD as q1_row: 0 Product<Q1>'s order, 1 ascending-column (ELL) order.
+    // With ACC the added-to values come from accin (out itself, or another
+    // array: out = accin + c2).
     template <typename DT, int ACC, int ORD = 0>
     __device__ __forceinline__ void jacobi_zero2_body(const Bands & B, const DT * __restrict__ d,
-            const DT * __restrict__ dinv, DT * __restrict__ out, int64_t n, int64_t m, int64_t lane)
+            const DT * __restrict__ dinv, DT * out, int64_t n, int64_t m, int64_t lane, const DT * accin)
     {
         constexpr int V = Vec16<DT>::V;
         typedef typename Vec16<DT>::type VT;
@@ -228,7 +331,7 @@ namespace k
                 const DT c1 = di * d[i];
                 const DT t = d[i] - s;
                 const DT c2 = c1 + t * di;
-                out[i] = ACC ? out[i] + c2 : c2;
+                out[i] = ACC ? accin[i] + c2 : c2;
             }
             return;
         }
@@ -248,7 +351,7 @@ namespace k
         }
         VT o;
         if (ACC)
-            o = *(const VT *)(out + i0);
+            o = *(const VT *)(accin + i0);
 #pragma unroll
         for (int v = 0; v < V; ++v)
         {
@@ -279,7 +382,25 @@ namespace k
     {
         if (active && *active == 0)
             return;
-        jacobi_zero2_body<DT, ACC>(B, d, dinv, out, n, m, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+        jacobi_zero2_body<DT, ACC>(B, d, dinv, out, n, m, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, out);
+    }
+
+    // FillMatrix<Tag, POISSON, DIRICHLET_0>::value (honei/math/fill_matrix.hh:37-76)
+    // on the nine bands of an M x M grid (N = M * M rows): boundary rows are
+    // identity rows (DD 1, every other band 0), interior rows the Q1
+    // stencil DD 8/3, off-diagonals -1/3 (each formed in double, stored as DT).
+    template <typename DT>
+    __global__ void fill_poisson_kernel(Bands B, int64_t N, int64_t M)
+    {
+        const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= N)
+            return;
+        const int64_t i = k + 1;
+        const bool bnd = (i <= M || i > N - M || (i % M) == 1 || (i % M) == 0);
+        const DT off = bnd ? DT(0.0) : DT(-1.0 / 3.0);
+#pragma unroll
+        for (int b = 0; b < 9; ++b)
+            ((DT *)B.b[b])[k] = (b == 4) ? (bnd ? DT(1.0) : DT(8.0 / 3.0)) : off;
     }
 
     template <typename DT>

@@ -17,8 +17,11 @@
 //          the finest level.
 #include "mg_engine.hh"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 namespace hsolve
@@ -154,7 +157,7 @@ namespace
     // result = initial guess (0 or the caller's) with b injected on the first
     // row, the last row and the first column (multigrid.hh:2282-2297);
     // x32 of the finest level = (float)result
-    __global__ __launch_bounds__(256) void start_init_kernel(const Slot * T, int L, int64_t n, int64_t sq)
+    __global__ __launch_bounds__(256) void start_init_kernel(const Slot * T, int L, int64_t n, int64_t sq, int fx)
     {
         const VB vb = vblock();
         const Slot & sl = T[vb.z];
@@ -167,7 +170,7 @@ namespace
         if (i < sq || i >= n - sq || (i % sq) == 0)
             g = sl.b[i];
         sl.result[i] = g;
-        sl.v[L][F_X][i] = (float)g;
+        sl.v[L][fx][i] = (float)g;
     }
 
     // iterates of the coarser levels start at zero
@@ -185,7 +188,7 @@ namespace
 
     // ------------------------------------------------------- FP64 outer step
     // result += scale * (double)x32
-    __global__ __launch_bounds__(256) void outer_accumulate_kernel(const Slot * T, int L, int64_t n)
+    __global__ __launch_bounds__(256) void outer_accumulate_kernel(const Slot * T, int L, int64_t n, int fx)
     {
         const VB vb = vblock();
         const Slot & sl = T[vb.z];
@@ -195,7 +198,7 @@ namespace
         if (i >= n)
             return;
         const double scale = sl.s->scale;
-        sl.result[i] = sl.result[i] + scale * (double)sl.v[L][F_X][i];
+        sl.result[i] = sl.result[i] + scale * (double)sl.v[L][fx][i];
     }
 
     // od = b - A64 result and its norm; opens the next inner solve or ends
@@ -335,7 +338,7 @@ namespace
     // Opens the inner solve: its initial defect d is rhs itself when
     // initial_zero (multigrid.hh:1626-1628), else rhs - A x.
     template <int INITIAL_ZERO, int MK>
-    __global__ __launch_bounds__(256) void open_inner_kernel(LevelArgs a, const Slot * T, int L)
+    __global__ __launch_bounds__(256) void open_inner_kernel(LevelArgs a, const Slot * T, int L, int fx)
     {
         const VB vb = vblock();
         const Slot & sl = T[vb.z];
@@ -356,10 +359,10 @@ namespace
                     d[i] = rhs[i];
         }
         else if (kBands<MK>)
-            k::q1_body_v<float, 1, kOrd<MK>>(a.B, sl.v[L][F_X], d, rhs, nullptr, n, a.m, lane);
+            k::q1_body_v<float, 1, kOrd<MK>>(a.B, sl.v[L][fx], d, rhs, nullptr, n, a.m, lane);
         else
         {
-            const float * x = sl.v[L][F_X];
+            const float * x = sl.v[L][fx];
             for (int64_t i = lane; i < n; i += (int64_t)gridDim.x * blockDim.x)
                 d[i] = rhs[i] - ell_row<float>(a.E, x, i);
         }
@@ -507,18 +510,20 @@ namespace
     // every slot load of a wave is one coalesced access).
 
     // the first two sweeps from zero (c1 = dinv*d, c2 = c1 + dinv*(d - A c1));
-    // ACC: out += c2, else out = c2
+    // ACC: out = (field facc) + c2 (facc may be fout), else out = c2
     template <int ACC, int MK>
-    __global__ __launch_bounds__(256) void v_zero2_kernel(LevelArgs a, const Slot * T, int L, int fsrc, int fout)
+    __global__ __launch_bounds__(256) void v_zero2_kernel(LevelArgs a, const Slot * T, int L, int fsrc, int fout,
+                                                          int facc)
     {
         const VB vb = vblock();
         HS_GATE_RUN
         const int64_t lane = (int64_t)vb.x * blockDim.x + threadIdx.x;
         const float * d = sl.v[L][fsrc];
         float * out = sl.v[L][fout];
+        const float * accin = sl.v[L][facc];
         if (kBands<MK>)
         {
-            k::jacobi_zero2_body<float, ACC, kOrd<MK>>(a.B, d, a.dinv, out, a.n, a.m, lane);
+            k::jacobi_zero2_body<float, ACC, kOrd<MK>>(a.B, d, a.dinv, out, a.n, a.m, lane, accin);
             return;
         }
         for (int64_t i = lane; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
@@ -528,7 +533,433 @@ namespace
             const float c1 = di * d[i];
             const float t = d[i] - s;
             const float c2 = c1 + t * di;
-            out[i] = ACC ? out[i] + c2 : c2;
+            out[i] = ACC ? accin[i] + c2 : c2;
+        }
+    }
+
+    constexpr int kPostW = 250;
+
+    // b[0..8]: the row's nine bands (LL .. UU)
+    template <int MK, int NB>
+    __device__ __forceinline__ float post_row(const float (&b)[NB], const float * __restrict__ up,
+                                              const float * __restrict__ mid, const float * __restrict__ dn, int j,
+                                              int64_t i, int64_t n, int64_t m)
+    {
+        if (kOrd<MK> == 1)
+        {
+            float s = 0.0f;
+            if (i - m - 1 >= 0) s += b[0] * up[j - 1];
+            if (i - m >= 0)     s += b[1] * up[j];
+            if (i - m + 1 >= 0) s += b[2] * up[j + 1];
+            if (i - 1 >= 0)     s += b[3] * mid[j - 1];
+            s += b[4] * mid[j];
+            if (i + 1 < n)      s += b[5] * mid[j + 1];
+            if (i + m - 1 < n)  s += b[6] * dn[j - 1];
+            if (i + m < n)      s += b[7] * dn[j];
+            if (i + m + 1 < n)  s += b[8] * dn[j + 1];
+            return s;
+        }
+        float s = b[4] * mid[j];
+        if (i - m - 1 >= 0) s += b[0] * up[j - 1];
+        if (i - m >= 0)     s += b[1] * up[j];
+        if (i - m + 1 >= 0) s += b[2] * up[j + 1];
+        if (i - 1 >= 0)     s += b[3] * mid[j - 1];
+        if (i + 1 < n)      s += b[5] * mid[j + 1];
+        if (i + m - 1 < n)  s += b[6] * dn[j - 1];
+        if (i + m < n)      s += b[7] * dn[j];
+        if (i + m + 1 < n)  s += b[8] * dn[j + 1];
+        return s;
+    }
+
+    // Tiles of a level of side m with th rows each (per slot).
+    inline unsigned post_tiles(int64_t m, int th)
+    {
+        return (unsigned)(((m + kPostW - 1) / kPostW) * ((m + th - 1) / th));
+    }
+
+    // Tile rows of the one-pass kernels: a tile's steps are sequential (th + 6
+    // of them, each waiting on memory), so the height is the even th <= 128
+    // with the fewest rounds of steps, ceil(tiles / resident) * (th + 6),
+    // where `resident` tiles fit on the device at once (the larger th on a
+    // tie: fewer halo rows).
+    inline int post_rows(int64_t m, unsigned slots, unsigned resident)
+    {
+        int best = 2;
+        uint64_t best_cost = ~(uint64_t)0;
+        for (int th = 2; th <= 128; th += 2)
+        {
+            const uint64_t tiles = (uint64_t)post_tiles(m, th) * slots;
+            const uint64_t rounds = (tiles + resident - 1) / resident;
+            const uint64_t cost = rounds * (uint64_t)((th + 7) & ~1);
+            if (cost <= best_cost)
+            {
+                best_cost = cost;
+                best = th;
+            }
+            if (th >= m)
+                break;
+        }
+        return best;
+    }
+
+    // Workgroups of kernel `fn` (256 lanes) resident on the current device at
+    // once: CUs x the occupancy its registers and LDS allow (cached).
+    inline unsigned post_resident(const void * fn)
+    {
+        static std::mutex mu;
+        static std::map<std::pair<int, const void *>, unsigned> cache;
+        int dev = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(std::make_pair(dev, fn));
+        if (it != cache.end())
+            return it->second;
+        int cus = 0, per_cu = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
+        const unsigned r = (unsigned)std::max(1, cus * std::max(1, per_cu));
+        cache.emplace(std::make_pair(dev, fn), r);
+        return r;
+    }
+
+    // ------------------------------------------- post-smoothing in one pass
+    // Two damped-Jacobi sweeps x1 = x + dinv (rhs - A x), x2 = x1 + dinv (rhs
+    // - A x1) and, with DEFECT, d = rhs - A x2, in one pass over Q1 bands
+    // (MK 0 / 2).  Every value is the one the separate sweep and defect
+    // kernels form: each row sums in q1_row's order over the same linear
+    // neighbours i +- 1, i +- m, i +- m +- 1 with the same index guards.
+    //
+    // A tile is kPostW consecutive columns of th grid rows (the last tiles of
+    // a level are narrower / shorter).  Its 256 lanes each own one column
+    // offset c = c0 - 3 + lane of every row; a row's linear span [R m + c0 -
+    // 3, R m + c0 + W + 3) runs past the grid row's ends into the
+    // neighbouring rows, exactly as the linear index does.  The tile marches
+    // down its rows: step s takes x of row R0 - 3 + s and the bands, dinv and
+    // rhs of row R0 - 4 + s (loaded two steps ahead, in registers), then forms
+    // x1 of row R0 - 4 + s, x2 of row R0 - 5 + s and d of row R0 - 6 + s from
+    // three-row rings of x, x1, x2 in LDS.  Each row's bands are read from
+    // HBM once per tile and kept in registers for its three stages.
+    //
+    // x2 goes to field fout, which must differ from fx: neighbouring tiles
+    // read x around their own rows while this tile writes.
+    //
+    // The steps synchronise on LDS only (lds_barrier: no wait for global
+    // loads), so the loads of the next two steps stay in flight across them;
+    // no lane reads a global value another lane of the tile wrote.
+    //
+    // PROL: x is first corrected by the prolongation of level L-1's field
+    // fcoarse, x + alpha P(coarse) (Prolongation<NONE> then ScaledSum with
+    // the fixed correction factor), formed per loaded node, so the corrected
+    // x is never stored.
+    template <int MK, int DEFECT, int PROL>
+    __global__ __launch_bounds__(256) void v_post2_kernel(LevelArgs a, const Slot * T, int L, int fx, int fout,
+                                                          int fcoarse, float alpha, k::Mask mk, int th)
+    {
+        const VB vb = vblock();
+        HS_GATE_RUN
+        __shared__ float xr[3][256];
+        __shared__ float x1r[3][256];
+        __shared__ float x2r[DEFECT ? 3 : 1][256];
+        const int64_t n = a.n, m = a.m;
+        const int64_t nrb = (m + th - 1) / th;
+        // tiles in column strips, row blocks fastest: vertical neighbours,
+        // which share the most halo, are consecutive and (k::xcd_block) on
+        // one XCD
+        const unsigned t = gridDim.z == 1 ? k::xcd_block(vb.x, gridDim.x) : vb.x;
+        const int64_t c0 = (int64_t)(t / nrb) * kPostW;
+        const int64_t R0 = (int64_t)(t % nrb) * th;
+        const int W = (int)min((int64_t)kPostW, m - c0);
+        const int H = (int)min((int64_t)th, m - R0);
+        const int j = threadIdx.x;
+        const int64_t c = c0 - 3 + j;
+        const bool in_x = j < W + 6;
+        const bool in_1 = j >= 1 && j < W + 5;
+        const bool in_2 = j >= 2 && j < W + 4;
+        const bool own = j >= 3 && j < W + 3;
+        const gptr<const float> x = as_global<const float>(sl.v[L][fx]);
+        const gptr<const float> rhs = as_global<const float>(sl.v[L][F_RHS]);
+        const gptr<const float> dinv = as_global<const float>(a.dinv);
+        const gptr<float> out = as_global<float>(sl.v[L][fout]);
+        const gptr<float> dout = as_global<float>(sl.v[L][F_D]);
+        const gptr<const float> xc = as_global<const float>(PROL ? sl.v[L - 1][fcoarse] : sl.v[L][fx]);
+        gptr<const float> bp[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            bp[k] = as_global<const float>((const float *)a.B.b[k]);
+        const int Nc = (int)((m + 1) / 2);
+
+        // grid position (R, S) of linear index R0 - 3 + s rows down, column
+        // offset c (c lies in [-3, 2m)), clamped to the grid
+        auto grid_pos = [&](int s, int & R, int & S) {
+            int64_t RR = R0 - 3 + s, SS = c;
+            if (SS < 0)
+            {
+                SS += m;
+                RR -= 1;
+            }
+            else if (SS >= m)
+            {
+                SS -= m;
+                RR += 1;
+            }
+            R = (int)min(max(RR, (int64_t)0), m - 1);
+            S = (int)min(max(SS, (int64_t)0), m - 1);
+        };
+        // The loads of step s: x of row R0 - 3 + s (with PROL the four coarse
+        // values around it) and the bands, dinv and rhs of row R0 - 4 + s.
+        // Every lane loads, at indices clamped to the arrays, with no branch
+        // between the loads; step s decides which values count.
+        auto load = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
+            const int64_t ix = (R0 - 3 + s) * m + c;
+            xq = x[min(max(ix, (int64_t)0), n - 1)];
+            if (PROL)
+            {
+                int R, S;
+                grid_pos(s, R, S);
+                const int64_t ra = (int64_t)(R >> 1) * Nc, rb = (int64_t)((R + 1) >> 1) * Nc;
+                const int jj = S >> 1, jj1 = min(jj + 1, Nc - 1);
+                cq[0] = xc[ra + jj];
+                cq[1] = xc[ra + jj1];
+                cq[2] = xc[rb + jj];
+                cq[3] = xc[rb + jj1];
+            }
+            const int64_t ib = min(max(ix - m, (int64_t)0), n - 1);
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                bq[k] = bp[k][ib];
+            bq[9] = dinv[ib];
+            bq[10] = rhs[ib];
+        };
+        // stage registers: row R0 - 4 + s (stage 1), R0 - 5 + s (stage 2),
+        // R0 - 6 + s (stage 3)
+        float b1[11] = {}, b2[11] = {}, b3[11] = {};
+        // H + 6 steps, rounded up to an even count so that the loop body is
+        // two whole steps with no branch around their loads (the extra step's
+        // rows lie outside the tile: nothing of it is stored)
+        const int steps = (H + 7) & ~1;
+        // one step; (xq, cq, bq) hold its loads and then take those of step s + 2
+        auto step = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
+            const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;   // rows r, r - 1, r - 2
+            {
+                const int64_t ix = (R0 - 3 + s) * m + c;
+                float xv = 0.0f;
+                if (in_x && ix >= 0 && ix < n)
+                {
+                    xv = xq;
+                    if (PROL)
+                    {
+                        int R, S;
+                        grid_pos(s, R, S);
+                        xv = xv + alpha * k::prolongation_node_v<float>(cq, (int)m, mk, R, S);
+                    }
+                }
+                xr[cur][j] = xv;
+                const int64_t ib = ix - m;
+                const bool vb1 = in_1 && s >= 2 && ib >= 0 && ib < n;
+#pragma unroll
+                for (int k = 0; k < 11; ++k)
+                {
+                    b3[k] = b2[k];
+                    b2[k] = b1[k];
+                    b1[k] = vb1 ? bq[k] : 0.0f;
+                }
+            }
+            load(s + 2, xq, cq, bq);
+            lds_barrier();
+            // stage 1: x1 of row R0 - 4 + s
+            if (s >= 2)
+            {
+                float v = 0.0f;
+                const int64_t i = (R0 - 4 + s) * m + c;
+                if (in_1 && i >= 0 && i < n)
+                {
+                    const float sum = post_row<MK>(b1, xr[old], xr[prv], xr[cur], j, i, n, m);
+                    const float tt = b1[10] - sum;
+                    v = xr[prv][j] + tt * b1[9];
+                }
+                x1r[cur][j] = v;
+            }
+            lds_barrier();
+            // stage 2: x2 of row R0 - 5 + s
+            if (s >= 4)
+            {
+                float v = 0.0f;
+                const int64_t r2 = R0 - 5 + s;
+                const int64_t i = r2 * m + c;
+                if (in_2 && i >= 0 && i < n)
+                {
+                    const float sum = post_row<MK>(b2, x1r[old], x1r[prv], x1r[cur], j, i, n, m);
+                    const float tt = b2[10] - sum;
+                    v = x1r[prv][j] + tt * b2[9];
+                    if (own && r2 >= R0 && r2 < R0 + H)
+                        out[i] = v;
+                }
+                if (DEFECT)
+                    x2r[DEFECT ? cur : 0][j] = v;
+            }
+            if (DEFECT)
+            {
+                lds_barrier();
+                // stage 3: d of row R0 - 6 + s
+                const int64_t r3 = R0 - 6 + s;
+                const int64_t i = r3 * m + c;
+                if (s >= 6 && own && r3 < R0 + H && i >= 0 && i < n)
+                {
+                    const float sum = post_row<MK>(b3, x2r[old], x2r[prv], x2r[DEFECT ? cur : 0], j, i, n, m);
+                    dout[i] = b3[10] - sum;
+                }
+            }
+        };
+        float xA, xB, cA[4] = {}, cB[4] = {}, bA[11], bB[11];
+        load(0, xA, cA, bA);
+        load(1, xB, cB, bB);
+        for (int s = 0; s < steps; s += 2)
+        {
+            step(s, xA, cA, bA);
+            step(s + 1, xB, cB, bB);
+        }
+    }
+
+    // ---------------------- pre-smoothing, defect and restriction in one pass
+    // The first two damped-Jacobi sweeps from zero (c1 = dinv d, c2 = c1 +
+    // dinv (d - A c1)), x = c2 (ACC: x = (field facc) + c2), the defect d' =
+    // rhs - A x and the coarse right-hand side Restriction<NONE>(d') with the
+    // Dirichlet mask, in one pass over Q1 bands (MK 0 / 2).  Every value is
+    // the one the separate kernels form (q1_row's order and guards,
+    // restrict_node's addition sequence); d' itself is never stored.
+    //
+    // Tiles as v_post2_kernel, with an even th, so a tile's first row is even
+    // and it owns the coarse rows r with 2r in its rows.  Step s takes row R0
+    // - 3 + s (d, dinv, rhs, the bands and facc, loaded two steps ahead; c1
+    // is formed at once), forms x of row R0 - 4 + s and d' of row R0 - 5 + s,
+    // and, when that row is 2r + 1, coarse row r from the three d' rows in
+    // LDS (lanes on even fine columns).  The lanes cover fine columns c0 - 3
+    // .. c0 + W + 2 in linear index order, so the stencil steps are the
+    // linear index's; the restriction reads grid columns 2s - 1 .. 2s + 1 of
+    // owned columns 2s, which all lie in that span.
+    template <int MK, int ACC>
+    __global__ __launch_bounds__(256) void v_pre2r_kernel(LevelArgs a, const Slot * T, int L, int fd, int facc,
+                                                          int64_t Nc, k::Mask mk, int th)
+    {
+        const VB vb = vblock();
+        HS_GATE_RUN
+        __shared__ float c1r[3][256];
+        __shared__ float xr[3][256];
+        __shared__ float dr[3][256];
+        const int64_t n = a.n, m = a.m;
+        const int64_t nrb = (m + th - 1) / th;
+        const unsigned t = gridDim.z == 1 ? k::xcd_block(vb.x, gridDim.x) : vb.x;
+        const int64_t c0 = (int64_t)(t / nrb) * kPostW;
+        const int64_t R0 = (int64_t)(t % nrb) * th;
+        const int W = (int)min((int64_t)kPostW, m - c0);
+        const int H = (int)min((int64_t)th, m - R0);
+        const int j = threadIdx.x;
+        const int64_t c = c0 - 3 + j;
+        const bool in_x = j < W + 6;
+        const bool in_1 = j >= 1 && j < W + 5;
+        const bool in_2 = j >= 2 && j < W + 4;
+        const bool own = j >= 3 && j < W + 3;
+        const bool own_even = own && ((c & 1) == 0);
+        const gptr<const float> d = as_global<const float>(sl.v[L][fd]);
+        const gptr<const float> rhs = as_global<const float>(sl.v[L][F_RHS]);
+        const gptr<const float> dinv = as_global<const float>(a.dinv);
+        const gptr<const float> acc = as_global<const float>(sl.v[L][facc]);
+        const gptr<float> xo = as_global<float>(sl.v[L][F_X]);
+        const gptr<float> co = as_global<float>(sl.v[L - 1][F_RHS]);
+        gptr<const float> bp[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            bp[k] = as_global<const float>((const float *)a.B.b[k]);
+
+        // row R0 - 3 + s for step s: 9 bands, dinv, d, rhs, facc value; every
+        // lane loads, at an index clamped to the arrays, with no branch
+        // between the loads; step s decides which values count
+        auto load = [&](int s, float (&rn)[13]) {
+            const int64_t i = min(max((R0 - 3 + s) * m + c, (int64_t)0), n - 1);
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                rn[k] = bp[k][i];
+            rn[9] = dinv[i];
+            rn[10] = d[i];
+            rn[11] = rhs[i];
+            rn[12] = ACC ? acc[i] : 0.0f;
+        };
+        // stage registers: rc row R0 - 3 + s (its c1 only, yet), ra row R0 - 4
+        // + s (x: bands, dinv, d, facc), rb row R0 - 5 + s (d': bands, rhs)
+        float rc[13] = {}, ra[13] = {}, rb[13] = {};
+        // H + 6 steps, rounded up to an even count (as v_post2_kernel)
+        const int steps = (H + 7) & ~1;
+        auto step = [&](int s, float (&rq)[13]) {
+            const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;   // rows r, r - 1, r - 2
+            {
+                const int64_t i = (R0 - 3 + s) * m + c;
+                const bool valid = in_x && i >= 0 && i < n;
+#pragma unroll
+                for (int k = 0; k < 13; ++k)
+                {
+                    rb[k] = ra[k];
+                    ra[k] = rc[k];
+                    rc[k] = valid ? rq[k] : 0.0f;
+                }
+                c1r[cur][j] = rc[9] * rc[10];
+            }
+            load(s + 2, rq);
+            lds_barrier();
+            // x of row R0 - 4 + s
+            if (s >= 2)
+            {
+                float v = 0.0f;
+                const int64_t rho = R0 - 4 + s;
+                const int64_t i = rho * m + c;
+                if (in_1 && i >= 0 && i < n)
+                {
+                    const float sum = post_row<MK>(ra, c1r[old], c1r[prv], c1r[cur], j, i, n, m);
+                    const float tt = ra[10] - sum;
+                    const float c2 = c1r[prv][j] + tt * ra[9];
+                    v = ACC ? ra[12] + c2 : c2;
+                    if (own && rho >= R0 && rho < R0 + H)
+                        xo[i] = v;
+                }
+                xr[cur][j] = v;
+            }
+            lds_barrier();
+            // d' of row R0 - 5 + s
+            if (s >= 4)
+            {
+                float v = 0.0f;
+                const int64_t rho = R0 - 5 + s;
+                const int64_t i = rho * m + c;
+                if (in_2 && i >= 0 && i < n)
+                {
+                    const float sum = post_row<MK>(rb, xr[old], xr[prv], xr[cur], j, i, n, m);
+                    v = rb[11] - sum;
+                }
+                dr[cur][j] = v;
+                lds_barrier();
+                // coarse row r when this d' row is 2r + 1
+                if (((rho - 1) & 1) == 0)
+                {
+                    const int64_t r = (rho - 1) >> 1;
+                    if (own_even && 2 * r >= R0 && 2 * r < R0 + H && c < m)
+                    {
+                        const int64_t sc = c >> 1;
+                        const float (*ring)[256] = dr;
+                        co[r * Nc + sc] = k::restrict_node<float>(
+                            [&](int ddr, int dds) {
+                                return ring[ddr < 0 ? old : (ddr == 0 ? prv : cur)][j + dds];
+                            },
+                            r, sc, Nc, mk);
+                    }
+                }
+            }
+        };
+        float qA[13], qB[13];
+        load(0, qA);
+        load(1, qB);
+        for (int s = 0; s < steps; s += 2)
+        {
+            step(s, qA);
+            step(s + 1, qB);
         }
     }
 
@@ -652,11 +1083,11 @@ namespace
     // level L's fout (+)= alpha * P (x of level L-1)
     template <bool AXPY>
     __global__ __launch_bounds__(256) void v_prolong_kernel(const Slot * T, int L, int Nf, float alpha, k::Mask mk,
-                                                            int fout)
+                                                            int fout, int fcoarse)
     {
         const VB vb = vblock();
         HS_GATE_RUN
-        k::prolongation2_body<float, AXPY>(sl.v[L][fout], sl.v[L - 1][F_X], Nf, alpha, mk, vb.y,
+        k::prolongation2_body<float, AXPY>(sl.v[L][fout], sl.v[L - 1][fcoarse], Nf, alpha, mk, vb.y,
                                            vb.x * blockDim.x + threadIdx.x);
     }
 
@@ -752,20 +1183,6 @@ namespace
         b[i] = -1.0 + 2.0 * u;
     }
 
-    // FillMatrix<POISSON, DIRICHLET_0> (fill_matrix.hh:37-76)
-    template <typename DT>
-    __global__ void fill_poisson_kernel(DT * b0, uint64_t bs, int64_t N, int64_t M)
-    {
-        const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (k >= N)
-            return;
-        const int64_t i = k + 1;
-        const bool bnd = (i <= M || i > N - M || (i % M) == 1 || (i % M) == 0);
-        const DT off = bnd ? DT(0.0) : DT(-1.0 / 3.0);
-        for (int b = 0; b < 9; ++b)
-            b0[(uint64_t)b * bs + k] = (b == 4) ? (bnd ? DT(1.0) : DT(8.0 / 3.0)) : off;
-    }
-
     // dinv = (float)((double)(1/dd) * 0.7): ElementInverse then Scale by a
     // double factor, as the reference tests build MGInfo::diags_inverted
     __global__ void dinv_kernel(const float * __restrict__ dd, float * __restrict__ dinv, int64_t n)
@@ -816,8 +1233,27 @@ Params make_params(const hsolve_mg_params & p)
     return q;
 }
 
+namespace
+{
+    // The occupancy of every one-pass kernel, looked up at set-up, never
+    // under stream capture.
+    void warm_post_kernels()
+    {
+        const void * fns[] = {
+            (const void *)v_pre2r_kernel<0, 0>,     (const void *)v_pre2r_kernel<0, 1>,
+            (const void *)v_pre2r_kernel<2, 0>,     (const void *)v_pre2r_kernel<2, 1>,
+            (const void *)v_post2_kernel<0, 0, 0>, (const void *)v_post2_kernel<0, 0, 1>,
+            (const void *)v_post2_kernel<0, 1, 0>, (const void *)v_post2_kernel<0, 1, 1>,
+            (const void *)v_post2_kernel<2, 0, 0>, (const void *)v_post2_kernel<2, 0, 1>,
+            (const void *)v_post2_kernel<2, 1, 0>, (const void *)v_post2_kernel<2, 1, 1>};
+        for (const void * f : fns)
+            (void)post_resident(f);
+    }
+}
+
 void hierarchy_alloc(Hierarchy & h, const Params & p, uint64_t max_level)
 {
+    warm_post_kernels();
     require(p.min_level >= 1 && p.min_level <= max_level && max_level < (uint64_t)kMaxLevels, HSOLVE_ERR_INVALID,
             "mg: need 1 <= min_level <= max_level <= 14");
     require(p.n_pre_smooth >= 1, HSOLVE_ERR_INVALID, "mg: n_pre_smooth must be >= 1");
@@ -858,12 +1294,12 @@ void hierarchy_fill_poisson(Hierarchy & h, hipStream_t st)
     for (uint64_t L = h.p.min_level; L <= h.max_level; ++L)
     {
         HierLevel & l = h.lv[L];
-        hipLaunchKernelGGL((fill_poisson_kernel<float>), dim3(grid_for(l.n, kBlock)), dim3(kBlock), 0, st, l.bands,
-                           l.bs, (int64_t)l.n, l.side);
+        hipLaunchKernelGGL((k::fill_poisson_kernel<float>), dim3(grid_for(l.n, kBlock)), dim3(kBlock), 0, st, l.B(),
+                           (int64_t)l.n, l.side);
         hierarchy_dinv(h, L, st);
         if (h.bands64[L])
-            hipLaunchKernelGGL((fill_poisson_kernel<double>), dim3(grid_for(l.n, kBlock)), dim3(kBlock), 0, st,
-                               h.bands64[L], h.bs64[L], (int64_t)l.n, l.side);
+            hipLaunchKernelGGL((k::fill_poisson_kernel<double>), dim3(grid_for(l.n, kBlock)), dim3(kBlock), 0, st,
+                               h.B64(L), (int64_t)l.n, l.side);
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -1145,11 +1581,23 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     auto norm_grid = [&](uint64_t nn) { return dim3(std::min<unsigned>(lanes4(nn), kNormBlocks), 1, S); };
     // row kernels: 4 rows per lane on Q1 bands, one row per lane on ELL
     auto rgrid = [&](uint64_t nn) { return kBands<MK> ? lanes4(nn) : grid_for(nn, kBlock); };
+    // Post-smoothing in one pass (v_post2_kernel) on Q1 bands with two
+    // sweeps: a level's smoothed iterate then lands in F_TMP, so the
+    // prolongation out of a level above the coarsest reads F_TMP, and the
+    // finest level's iterate lives in F_TMP between V-cycles (the start
+    // writes it there, the outer step and the next cycle's first sweeps
+    // read it there).
+    const bool post2 = kBands<MK> && p.n_pre_smooth == 2 && maxl > minl;
+    const int fx_top = post2 ? F_TMP : F_X;
+    // Pre-smoothing, defect and restriction in one pass (v_pre2r_kernel)
+    // when the transfers are the structured ones and the correction factor
+    // is fixed (the adaptive one reads the level's defect again).
+    const bool pre2r = post2 && p.adapt != 0.0f && ! h.prolmat;
 
     // ---- gates, start of new solves, FP64 outer step, opening of inner solves
     hipLaunchKernelGGL(plan_kernel, dim3(grid_for(S, 64)), dim3(64), 0, st, T, S);
     hipLaunchKernelGGL(start_init_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n,
-                       top.side);
+                       top.side, fx_top);
     if (maxl > minl)
     {
         LevelSizes sz;
@@ -1158,7 +1606,8 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         hipLaunchKernelGGL(start_zero_kernel, dim3(grid_for(h.lv[maxl - 1].n, kBlock), 1, S), blk, 0, st, T,
                            (int)minl, (int)maxl, sz);
     }
-    hipLaunchKernelGGL(outer_accumulate_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
+    hipLaunchKernelGGL(outer_accumulate_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n,
+                       fx_top);
     const unsigned od_blocks = std::min<unsigned>(grid_for((top.n + 1) / 2, kBlock), kNormBlocks);
     hipLaunchKernelGGL((outer_defect_kernel<MK>), dim3(od_blocks, 1, S), blk, 0, st,
                        kBands<MK> ? h.B64(maxl) : k::Bands{}, h.ell64[maxl], T, n, top.side);
@@ -1167,10 +1616,10 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     hipLaunchKernelGGL(to_inner_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
     if (p.initial_zero)
         hipLaunchKernelGGL((open_inner_kernel<1, MK>), dim3(lanes4(top.n), 1, S), blk, 0, st, largs(maxl), T,
-                           (int)maxl);
+                           (int)maxl, fx_top);
     else
         hipLaunchKernelGGL((open_inner_kernel<0, MK>), dim3(rgrid(top.n), 1, S), blk, 0, st, largs(maxl), T,
-                           (int)maxl);
+                           (int)maxl, fx_top);
     hipLaunchKernelGGL((inner_norm_kernel<0>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n);
     hipLaunchKernelGGL((inner_norm_finish_kernel<0>), dim3(1, 1, S), blk, 0, st, T, norm_grid(top.n).x, p.tolerance,
                        p.convergence_check, p.n_max_iter);
@@ -1205,6 +1654,23 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
             // the finest level's defect is d; coarser levels' defect is the
             // restricted one in rhs
             const int dsrc = (L == maxl) ? F_D : F_RHS;
+            if constexpr (kBands<MK>)
+            {
+                if (pre2r)
+                {
+                    const void * fn = L == maxl ? (const void *)v_pre2r_kernel<MK, 1> : (const void *)v_pre2r_kernel<MK, 0>;
+                    const int th = post_rows(a.m, S, post_resident(fn));
+                    const dim3 tg(post_tiles(a.m, th), 1, S);
+                    const int64_t Nc = h.lv[L - 1].side;
+                    if (L == maxl)
+                        hipLaunchKernelGGL((v_pre2r_kernel<MK, 1>), tg, blk, 0, st, a, T, (int)L, dsrc, fx_top, Nc,
+                                           p.mask, th);
+                    else
+                        hipLaunchKernelGGL((v_pre2r_kernel<MK, 0>), tg, blk, 0, st, a, T, (int)L, dsrc, (int)F_X, Nc,
+                                           p.mask, th);
+                    continue;
+                }
+            }
             if (L == maxl)
             {
                 // c = dinv*d, (npre-1) sweeps on c with rhs d, x += c
@@ -1213,11 +1679,11 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                                        (int)L, dsrc, (int)F_X);
                 else if (npre == 2)
                     hipLaunchKernelGGL((v_zero2_kernel<1, MK>), dim3(rgrid(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
-                                       (int)F_X);
+                                       (int)F_X, fx_top);
                 else
                 {
                     hipLaunchKernelGGL((v_zero2_kernel<0, MK>), dim3(rgrid(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
-                                       (int)F_C);
+                                       (int)F_C, (int)F_C);
                     int cc = F_C, ca = F_TMP;
                     sweeps(L, cc, ca, dsrc, npre - 2);
                     move(L, cc, F_X, true);
@@ -1232,7 +1698,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                 else
                 {
                     hipLaunchKernelGGL((v_zero2_kernel<0, MK>), dim3(rgrid(a.n), 1, S), blk, 0, st, a, T, (int)L, dsrc,
-                                       (int)F_X);
+                                       (int)F_X, (int)F_X);
                     int xc = F_X, xa = F_TMP;
                     sweeps(L, xc, xa, dsrc, npre - 2);
                     if (xc != F_X)
@@ -1272,19 +1738,41 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         const int Nf = (int)h.lv[L].side;
         const int Nc = (int)h.lv[L - 1].side;
         const dim3 pg(grid_for((uint64_t)Nc, kBlock), (unsigned)Nf, S);
+        // the coarse level's iterate: the coarse solve's F_X, or F_TMP
+        // after that level's one-pass post-smoothing
+        const int fcoarse = (post2 && L - 1 > minl) ? F_TMP : F_X;
+        if constexpr (kBands<MK>)
+        {
+            if (pre2r)
+            {
+                // prolongation, correction and post-smoothing in one pass
+                const void * fn = L == maxl ? (const void *)v_post2_kernel<MK, 1, 1> : (const void *)v_post2_kernel<MK, 0, 1>;
+                const int th = post_rows(a.m, S, post_resident(fn));
+                const dim3 tg(post_tiles(a.m, th), 1, S);
+                if (L == maxl)
+                    hipLaunchKernelGGL((v_post2_kernel<MK, 1, 1>), tg, blk, 0, st, a, T, (int)L, (int)F_X,
+                                       (int)F_TMP, fcoarse, p.adapt, p.mask, th);
+                else
+                    hipLaunchKernelGGL((v_post2_kernel<MK, 0, 1>), tg, blk, 0, st, a, T, (int)L, (int)F_X,
+                                       (int)F_TMP, fcoarse, p.adapt, p.mask, th);
+                continue;
+            }
+        }
         if (h.prolmat)
         {
             // c = P x_coarse, then x += alpha c
             hipLaunchKernelGGL(v_apply_ell_kernel, dim3(grid_for(h.lv[L].prol.rows, kBlock), 1, S), blk, 0, st, h.lv[L].prol, T,
-                               (int)(L - 1), (int)F_X, (int)L, (int)F_C);
+                               (int)(L - 1), fcoarse, (int)L, (int)F_C);
             if (p.adapt != 0.0f)
                 hipLaunchKernelGGL(v_axpy_const_kernel, dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, T, (int)L, a.n,
                                    p.adapt);
         }
         else if (p.adapt != 0.0f)
-            hipLaunchKernelGGL((v_prolong_kernel<true>), pg, blk, 0, st, T, (int)L, Nf, p.adapt, p.mask, (int)F_X);
+            hipLaunchKernelGGL((v_prolong_kernel<true>), pg, blk, 0, st, T, (int)L, Nf, p.adapt, p.mask, (int)F_X,
+                               fcoarse);
         else
-            hipLaunchKernelGGL((v_prolong_kernel<false>), pg, blk, 0, st, T, (int)L, Nf, 0.0f, p.mask, (int)F_C);
+            hipLaunchKernelGGL((v_prolong_kernel<false>), pg, blk, 0, st, T, (int)L, Nf, 0.0f, p.mask, (int)F_C,
+                               fcoarse);
         if (p.adapt == 0.0f)
         {
             const unsigned ab = std::min<unsigned>(grid_for(a.n, kBlock), kNormBlocks);
@@ -1293,6 +1781,22 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
             hipLaunchKernelGGL(v_axpy_alpha_kernel, dim3(grid_for(a.n, kBlock), 1, S), blk, 0, st, T, (int)L, a.n);
         }
         // post-smoothing: n_pre_smooth sweeps (multigrid.hh:1851)
+        if constexpr (kBands<MK>)
+        {
+            if (post2)
+            {
+                const void * fn = L == maxl ? (const void *)v_post2_kernel<MK, 1, 0> : (const void *)v_post2_kernel<MK, 0, 0>;
+                const int th = post_rows(a.m, S, post_resident(fn));
+                const dim3 tg(post_tiles(a.m, th), 1, S);
+                if (L == maxl)
+                    hipLaunchKernelGGL((v_post2_kernel<MK, 1, 0>), tg, blk, 0, st, a, T, (int)L, (int)F_X,
+                                       (int)F_TMP, fcoarse, 0.0f, p.mask, th);
+                else
+                    hipLaunchKernelGGL((v_post2_kernel<MK, 0, 0>), tg, blk, 0, st, a, T, (int)L, (int)F_X,
+                                       (int)F_TMP, fcoarse, 0.0f, p.mask, th);
+                continue;
+            }
+        }
         int xc = F_X, xa = F_TMP;
         sweeps(L, xc, xa, F_RHS, npre);
         if (xc != F_X)

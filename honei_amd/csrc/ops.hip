@@ -218,6 +218,25 @@ namespace hsolve
         HIP_CHECK(hipGetLastError());
     }
 
+    template <typename DT>
+    void fill_poisson_q1(void * const * bands9, uint64_t size, hipStream_t st)
+    {
+        require(bands9 != nullptr, HSOLVE_ERR_INVALID, "fill_poisson_q1: null band array");
+        k::Bands B;
+        for (int i = 0; i < 9; ++i)
+        {
+            require(bands9[i] != nullptr, HSOLVE_ERR_INVALID, "fill_poisson_q1: null band pointer");
+            B.b[i] = bands9[i];
+        }
+        const int64_t m = (int64_t)std::sqrt((double)size);
+        require((uint64_t)(m * m) == size, HSOLVE_ERR_SIZE, "fill_poisson_q1: size is not a square");
+        if (size == 0)
+            return;
+        hipLaunchKernelGGL((k::fill_poisson_kernel<DT>), dim3(grid_for(size, kBlock)), dim3(kBlock), 0, st, B,
+                           (int64_t)size, m);
+        HIP_CHECK(hipGetLastError());
+    }
+
     template void launch_dot<float>(const float *, const float *, uint64_t, float *, int, hipStream_t);
     template void launch_dot<double>(const double *, const double *, uint64_t, double *, int, hipStream_t);
 }
@@ -724,6 +743,20 @@ int hsolve_apply_dirichlet_double(void * grid, uint64_t size, const uint64_t * m
 }
 
 // ------------------------------------------------------- fused smoothers
+int hsolve_fill_poisson_q1_float(void * const * bands9, uint64_t size, void * stream)
+{
+    HSOLVE_API_BEGIN
+    fill_poisson_q1<float>(bands9, size, ST);
+    HSOLVE_API_END
+}
+
+int hsolve_fill_poisson_q1_double(void * const * bands9, uint64_t size, void * stream)
+{
+    HSOLVE_API_BEGIN
+    fill_poisson_q1<double>(bands9, size, ST);
+    HSOLVE_API_END
+}
+
 int hsolve_jacobi_q1_float(const void * rhs, const void * const * bands9, const void * dinv, const void * x_old,
                            void * x_new, uint64_t size, void * stream)
 {

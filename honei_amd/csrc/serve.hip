@@ -370,15 +370,21 @@ int hsolve_serve_create(const hsolve_serve_params * params, hsolve_serve * out)
         b.slot_bytes = mg::slot_bytes(srv->h, L, true, true);
         HIP_CHECK(hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
-        HIP_CHECK(hipHostMalloc((void **)&b.table_host, sizeof(mg::Slot) * p.max_slots, hipHostMallocDefault));
-        HIP_CHECK(hipHostMalloc((void **)&b.rep_host, sizeof(mg::Report) * p.max_slots, hipHostMallocDefault));
-        std::memset(b.table_host, 0, sizeof(mg::Slot) * p.max_slots);
-        b.table = (mg::Slot *)srv->h.alloc(sizeof(mg::Slot) * p.max_slots);
-        b.rep = (mg::Report *)srv->h.alloc(sizeof(mg::Report) * p.max_slots);
-        HIP_CHECK(hipMemcpy(b.table, b.table_host, sizeof(mg::Slot) * p.max_slots, hipMemcpyHostToDevice));
-        b.mem.assign(p.max_slots, nullptr);
-        b.req.resize(p.max_slots);
-        b.used.assign(p.max_slots, 0);
+        // a tick covers the table entries up to the high-water entry rounded
+        // up to a power of two, so every per-entry array holds that many
+        uint64_t entries = 1;
+        while (entries < p.max_slots)
+            entries <<= 1;
+        HIP_CHECK(hipHostMalloc((void **)&b.table_host, sizeof(mg::Slot) * entries, hipHostMallocDefault));
+        HIP_CHECK(hipHostMalloc((void **)&b.rep_host, sizeof(mg::Report) * entries, hipHostMallocDefault));
+        std::memset(b.table_host, 0, sizeof(mg::Slot) * entries);
+        std::memset(b.rep_host, 0, sizeof(mg::Report) * entries);
+        b.table = (mg::Slot *)srv->h.alloc(sizeof(mg::Slot) * entries);
+        b.rep = (mg::Report *)srv->h.alloc(sizeof(mg::Report) * entries);
+        HIP_CHECK(hipMemcpy(b.table, b.table_host, sizeof(mg::Slot) * entries, hipMemcpyHostToDevice));
+        b.mem.assign(entries, nullptr);
+        b.req.resize(entries);
+        b.used.assign(entries, 0);
         b.cap = (unsigned)p.max_slots;
         if (p.rows_per_tick)
             b.cap = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(p.max_slots, p.rows_per_tick / mg::size_of(L)));

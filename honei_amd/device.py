@@ -117,6 +117,12 @@ class Q1Matrix:
         return [vp(b.ptr) for b in self.bands]
 
 
+def fill_poisson_q1(A: Q1Matrix, stream=None):
+    """FillMatrix<Tag, POISSON, DIRICHLET_0>::value(A) on A's device bands."""
+    fn = lib().hsolve_fill_poisson_q1_double if A.dtype == np.float64 else lib().hsolve_fill_poisson_q1_float
+    check(fn(A.ptrs, u64(A.size), vp(stream)), "fill_poisson_q1")
+
+
 class EllMatrix:
     """SparseMatrixELL<DT> uploaded into a device handle (uint32 columns)."""
 

@@ -1,10 +1,12 @@
 // honei/math/multigrid_kernel.hh -- Multigrid<…, JAC, CYCLE::V, FIXED> on Q1
 // bands or ELL matrices in either precision, and Multigrid<…, MIXED> on ELL
 // matrices (HONEI's honei/math/multigrid.hh:156-872 FIXED, 1936-2543 MIXED
-// ELL), driven from the host: each step of the reference's
-// _multigrid_kernel is one backend operation on device-resident data, and
-// the host reads one norm per V-cycle.  (Multigrid<…, MIXED> on Q1 bands
-// runs entirely on the device, see solvers.hh.)
+// ELL).  MIXED on ELL runs on the device engine (hsolve_mg_create_ell) when
+// every matrix has one lane per row; otherwise, and for FIXED, it is driven
+// from the host: each step of the reference's _multigrid_kernel is one
+// backend operation on device-resident data, and the host reads one norm per
+// V-cycle.  (Multigrid<…, MIXED> on Q1 bands runs on the device engine, see
+// solvers.hh.)
 //
 // Steps per V-cycle, as in the reference: per level down, c = dinv*d and
 // n_pre_smooth-1 Jacobi sweeps (finest: x += c; coarser: x = c), defect,
@@ -21,6 +23,9 @@
 
 #include <cmath>
 #include <limits>
+#include <memory>
+#include <type_traits>
+#include <vector>
 
 namespace honei
 {
@@ -140,13 +145,102 @@ namespace honei
             copy<Tag>(info.x[info.max_level], x);
         }
 
-        /// Multigrid<..., MIXED>::value on ELL matrices (multigrid.hh:2400-2543)
+        /// The engine's view (hsolve_ell_host) of a one-lane-per-row SparseMatrixELL.
+        template <typename DT> hsolve_ell_host ell_host(const SparseMatrixELL<DT> & A)
+        {
+            hsolve_ell_host e;
+            e.rows = A.rows();
+            e.cols = A.columns();
+            e.stride = A.stride();
+            e.num_cols_per_row = A.num_cols_per_row();
+            e.Aj = (const uint64_t *)A.Aj().read_elements();
+            e.Ax = A.Ax().read_elements();
+            e.Arl = (const uint64_t *)A.Arl().read_elements();
+            return e;
+        }
+
+        /// Does the device engine take this hierarchy: every matrix with one
+        /// lane per row, and (structured transfers) level L of (2^L+1)^2 rows?
+        template <typename ProlType_>
+        bool engine_takes(const SparseMatrixELL<double> & system, const MGInfo<float, SparseMatrixELL<float>> & info)
+        {
+            const unsigned long minl = info.min_level, maxl = info.max_level;
+            if (system.threads() != 1 || info.a.size() <= maxl || info.diags_inverted.size() <= maxl || minl < 1 ||
+                minl > maxl || maxl > 14)
+                return false;
+            const bool prolmat = ! std::is_same<ProlType_, methods::NONE>::value;
+            if (prolmat && (info.prolmats.size() <= maxl || info.resmats.size() <= maxl))
+                return false;
+            for (unsigned long L = minl; L <= maxl; ++L)
+            {
+                if (info.a[L].threads() != 1 || info.diags_inverted[L].size() != info.a[L].rows())
+                    return false;
+                const unsigned long side = (1ul << L) + 1;
+                if (! prolmat && info.a[L].rows() != side * side)
+                    return false;
+                if (prolmat && L > minl && (info.prolmats[L].threads() != 1 || info.resmats[L].threads() != 1))
+                    return false;
+            }
+            return info.a[maxl].rows() == system.rows();
+        }
+
+        /// Multigrid<..., MIXED>::value on ELL matrices on the device engine:
+        /// the hierarchy (level matrices, inverted diagonals, transfer
+        /// matrices, the FP64 finest matrix) is uploaded on first use and kept
+        /// in info.device_hierarchy; each solve is then device resident, the
+        /// host waiting once per chunk of ticks.
+        template <typename ProlType_>
+        void mixed_ell_engine(const SparseMatrixELL<double> & system, const DenseVector<double> & rhs,
+                              DenseVector<double> & x, MGInfo<float, SparseMatrixELL<float>> & info)
+        {
+            using namespace mi355x;
+            const unsigned long minl = info.min_level, maxl = info.max_level;
+            if (! info.device_hierarchy)
+            {
+                hsolve_mg_params p = engine_params(info);
+                const bool prolmat = ! std::is_same<ProlType_, methods::NONE>::value;
+                std::vector<hsolve_ell_host> a32(maxl + 1), prol(maxl + 1), res(maxl + 1);
+                std::vector<const float *> d32(maxl + 1, nullptr);
+                for (unsigned long L = minl; L <= maxl; ++L)
+                {
+                    a32[L] = ell_host(info.a[L]);
+                    d32[L] = info.diags_inverted[L].read_elements();
+                    if (prolmat && L > minl)
+                    {
+                        prol[L] = ell_host(info.prolmats[L]);
+                        res[L] = ell_host(info.resmats[L]);
+                    }
+                }
+                const hsolve_ell_host a64 = ell_host(system);
+                hsolve_mg h = nullptr;
+                check(hsolve_mg_create_ell(&p, &a64, a32.data(), d32.data(), prolmat ? prol.data() : nullptr,
+                                           prolmat ? res.data() : nullptr, 1, &h),
+                      "Multigrid setup");
+                info.device_hierarchy = std::shared_ptr<void>(h, [](void * q) { hsolve_mg_destroy((hsolve_mg)q); });
+            }
+            uint64_t outer = 0, inner = 0;
+            double fd = 0, idf = 0;
+            check(hsolve_mg_solve((hsolve_mg)info.device_hierarchy.get(), rd(rhs), rw(x), &outer, &inner, &fd, &idf,
+                                  stream()),
+                  "Multigrid");
+            info.outer_iterations = outer;
+            info.inner_iterations = inner;
+        }
+
+        /// Multigrid<..., MIXED>::value on ELL matrices (multigrid.hh:2400-2543):
+        /// on the device engine when it takes the hierarchy, else driven from
+        /// the host one backend operation per step
         template <typename ProlType_>
         void mixed_ell(const SparseMatrixELL<double> & system, const DenseVector<double> & rhs, DenseVector<double> & x,
                        MGInfo<float, SparseMatrixELL<float>> & info)
         {
             mi355x::same_size(rhs.size(), system.rows());
             mi355x::same_size(x.size(), system.rows());
+            if (engine_takes<ProlType_>(system, info))
+            {
+                mixed_ell_engine<ProlType_>(system, rhs, x, info);
+                return;
+            }
             const unsigned long n = rhs.size(), maxl = info.max_level;
             DenseVector<double> initial_guess(inject_boundary(x, rhs));
             convert<Tag>(info.x[maxl], initial_guess);

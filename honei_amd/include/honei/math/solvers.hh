@@ -445,14 +445,34 @@ namespace honei
 
     namespace mg_kernel_detail
     {
-        // host-driven MIXED solve on ELL matrices (multigrid_kernel.hh)
+        /// The engine's parameters from an MGInfo (no cap on the outer loop,
+        /// as in the reference).
+        template <typename Layout_> hsolve_mg_params engine_params(const MGInfo<float, Layout_> & info)
+        {
+            hsolve_mg_params p;
+            p.min_level = info.min_level;
+            p.max_level = info.max_level;
+            p.n_max_iter = info.n_max_iter;
+            p.initial_zero = info.initial_zero ? 1 : 0;
+            p.convergence_check = info.convergence_check ? 1 : 0;
+            p.tolerance = (double)info.tolerance;
+            p.n_pre_smooth = info.n_pre_smooth;
+            p.adapt_correction_factor = (double)info.adapt_correction_factor;
+            for (int i = 0; i < 8; ++i)
+                p.macro_border_mask[i] = info.macro_border_mask[i];
+            p.max_outer = 0;
+            return p;
+        }
+
+        // MIXED solve on ELL matrices: the device engine, or host-driven (multigrid_kernel.hh)
         template <typename ProlType_>
         void mixed_ell(const SparseMatrixELL<double> & system, const DenseVector<double> & rhs, DenseVector<double> & x,
                        MGInfo<float, SparseMatrixELL<float>> & info);
     }
 
     /// FP32 V-cycles inside FP64 iterative refinement on Q1 bands (multigrid.hh:1616-2393)
-    /// on the device-resident engine; on ELL matrices (multigrid.hh:1936-2543) host-driven.
+    /// and on ELL matrices (multigrid.hh:1936-2543) on the device-resident engine (ELL
+    /// hierarchies with more than one lane per row are driven from the host).
     template <> struct Multigrid<tags::GPU::MI355X, tags::GPU::MI355X, methods::NONE, methods::JAC, methods::CYCLE::V,
                                  methods::MIXED>
     {
@@ -472,18 +492,7 @@ namespace honei
             same_size(x.size(), system.rows());
             if (! info.device_hierarchy)
             {
-                hsolve_mg_params p;
-                p.min_level = info.min_level;
-                p.max_level = info.max_level;
-                p.n_max_iter = info.n_max_iter;
-                p.initial_zero = info.initial_zero ? 1 : 0;
-                p.convergence_check = info.convergence_check ? 1 : 0;
-                p.tolerance = (double)info.tolerance;
-                p.n_pre_smooth = info.n_pre_smooth;
-                p.adapt_correction_factor = (double)info.adapt_correction_factor;
-                for (int i = 0; i < 8; ++i)
-                    p.macro_border_mask[i] = info.macro_border_mask[i];
-                p.max_outer = 0;
+                hsolve_mg_params p = mg_kernel_detail::engine_params(info);
                 if (info.a.size() <= info.max_level || info.diags_inverted.size() <= info.max_level)
                     throw InternalError("Multigrid: MGInfo holds fewer levels than max_level");
                 if (system.size() != info.a[info.max_level].size())

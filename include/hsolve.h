@@ -222,6 +222,13 @@ int hsolve_convert_float_double(const void *src_float, void *dst_double, uint64_
                                 void *stream);
 int hsolve_convert_double_float(const void *src_double, void *dst_float, uint64_t size,
                                 void *stream);
+/* FillMatrix<Tag, POISSON, DIRICHLET_0>::value on a BandedMatrixQx<Q1>
+ * (honei/math/fill_matrix.hh:37-76), on the device: bands9 holds the nine
+ * device band pointers (LL..UU) of a size = M * M matrix.  Boundary rows
+ * become identity rows, interior rows the stencil DD 8/3, others -1/3.
+ * The multigrid engine assembles every level with the same kernel. */
+int hsolve_fill_poisson_q1_float(void *const *bands9, uint64_t size, void *stream);
+int hsolve_fill_poisson_q1_double(void *const *bands9, uint64_t size, void *stream);
 
 /* ---------------- new fused kernels ---------------------------------------- */
 /* One damped-Jacobi sweep in one pass: x_new = x_old + dinv * (rhs - A x_old)
@@ -309,7 +316,9 @@ int hsolve_ell_info(hsolve_ell m, uint64_t *rows, uint64_t *columns, uint64_t *s
  * unless they fill 90% of their slots), 32 = with 16: read a copy whose rows are sorted by length inside
  * each such block, so slots past a row's length are not fetched (the
  * copy is built at upload when the LDS form is the automatic choice, and
- * the automatic choice then uses it).  Results are identical for every
+ * the automatic choice then uses it), 64 = with 16: a fixed grid of
+ * workgroups walks the LDS blocks and loads the next block's row lengths
+ * while it works on the current one.  Results are identical for every
  * variant. */
 int hsolve_ell_set_variant(hsolve_ell m, int variant);
 /* Grid side M when the handle holds a 9-point structured matrix (rows =

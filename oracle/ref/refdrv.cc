@@ -6,8 +6,9 @@
  * C restatement (oracle.c) against the reference itself and to write golden
  * fixtures under tests/golden/.  The built library sits in oracle/_ref/,
  * outside version control; it is copied along with the working tree like
- * every built library, but nothing in the product library, and nothing
- * that runs on a GPU host (GPU tests, smoke, benchmark), loads it.
+ * every built library.  Nothing in the product library, the GPU tests or
+ * smoke loads it; the benchmark times ref_sse_product_smell_dv_f64 from it
+ * as the CPU baseline (HONEI's own SSE kernel on the GPU host's cores).
  */
 #include <honei/la/dense_vector.hh>
 #include <honei/la/banded_matrix_qx.hh>
@@ -34,6 +35,7 @@
 #include <honei/math/matrix_io.hh>
 #include <honei/math/vector_io.hh>
 #include <honei/util/configuration.hh>
+#include <honei/backends/sse/operations.hh>
 
 #include <cstring>
 #include <iostream>
@@ -829,6 +831,19 @@ int ref_mg_fixed_f64(int kind, int prolmat, const void * const * a_q1, const ref
 void ref_set_ell_threads(int threads)
 {
     Configuration::instance()->set_value("ell::threads", threads);
+}
+
+/* The reference's own SSE ELL product kernel (honei/backends/sse/product.cc,
+ * product_smell_dv) on rows [row_start, row_end): what Product<tags::CPU::SSE>
+ * runs, and what each thread of Product<tags::CPU::MultiCore::SSE> runs on
+ * its equal share of the rows (honei/la/product.hh:2087-2106).  Timed as the
+ * CPU baseline of the ELL SpMV. */
+void ref_sse_product_smell_dv_f64(double * y, const uint64_t * Aj, const double * Ax, const uint64_t * Arl,
+                                  const double * x, uint64_t stride, uint64_t rows, uint64_t ncpr,
+                                  uint64_t row_start, uint64_t row_end, uint64_t threads)
+{
+    honei::sse::product_smell_dv(y, (const unsigned long *)Aj, Ax, (const unsigned long *)Arl, x, stride, rows,
+                                 ncpr, row_start, row_end, threads);
 }
 
 }

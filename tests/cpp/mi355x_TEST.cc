@@ -32,6 +32,7 @@
 
 #include <oracle.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -767,12 +768,32 @@ static void test_multigrid_ell()
         for (long r = 1; r < 32; ++r)
             for (long s = 1; s < 32; ++s)
                 b[r * 33 + s] = std::sin(0.1 * (double)(r * 33 + s));
-        if (prolmat)
-            Multigrid<Tag, Tag, methods::PROLMAT, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
-                    A64, b, x, 11ul, std::numeric_limits<double>::epsilon(), info);
-        else
-            Multigrid<Tag, Tag, methods::NONE, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
-                    A64, b, x, 11ul, std::numeric_limits<double>::epsilon(), info);
+        auto solve = [&]() {
+            if (prolmat)
+                Multigrid<Tag, Tag, methods::PROLMAT, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
+                        A64, b, x, 11ul, std::numeric_limits<double>::epsilon(), info);
+            else
+                Multigrid<Tag, Tag, methods::NONE, methods::JAC, methods::CYCLE::V, methods::MIXED>::value(
+                        A64, b, x, 11ul, std::numeric_limits<double>::epsilon(), info);
+        };
+        const auto t0 = std::chrono::steady_clock::now();
+        solve();
+        const auto t1 = std::chrono::steady_clock::now();
+        // the hierarchy stays on the device (MGInfo::device_hierarchy): a second
+        // solve from the same start gives the same bits
+        const DenseVector<double> x1(x.copy());
+        for (unsigned long i = 0; i < n; ++i)
+            x[i] = 0.0;
+        const auto t2 = std::chrono::steady_clock::now();
+        solve();
+        const auto t3 = std::chrono::steady_clock::now();
+        TEST_CHECK(info.device_hierarchy != nullptr);
+        for (unsigned long i = 0; i < n; ++i)
+            TEST_CHECK(x[i] == x1[i]);
+        std::printf("  MIXED ELL %s through Multigrid<tags::GPU::MI355X>: first solve (with upload) %.2f ms, "
+                    "second %.2f ms\n", prolmat ? "PROLMAT" : "NONE",
+                    std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                    std::chrono::duration<double, std::milli>(t3 - t2).count());
         or_mg_params p;
         std::memset(&p, 0, sizeof(p));
         p.min_level = minl; p.max_level = maxl; p.n_max_iter = 2; p.initial_zero = 1; p.tolerance = 1e-8;

@@ -111,8 +111,9 @@ def test_ell_matrix_b_and_a_small(hip, oracle, dtype):
         M = hip.EllMatrix.from_dict(E)
         yd = hip.DeviceVector.zeros(n, dtype)
         xd, rd = hip.DeviceVector.from_numpy(x), hip.DeviceVector.from_numpy(rhs)
-        # every kernel variant gives the same bits (48, 52: rows sorted by length per LDS block)
-        for variant in list(range(8)) + [16, 20, 22, 48, 52]:
+        # every kernel variant gives the same bits (48, 52: rows sorted by length per LDS block;
+        # 80, 112, 116: the persistent LDS-window kernel)
+        for variant in list(range(8)) + [16, 20, 22, 48, 52, 80, 112, 116]:
             M.set_variant(variant)
             M.spmv(xd, yd)
             assert np.array_equal(yd.get(), ref), variant
@@ -122,7 +123,7 @@ def test_ell_matrix_b_and_a_small(hip, oracle, dtype):
         dinv = (1.0 / (np.abs(x) + 1.0)).astype(dtype)
         ref_j = oracle.jacobi_ell(x, E, rhs, dinv, 1)
         dd = hip.DeviceVector.from_numpy(dinv)
-        for variant in (0, 16, 48):
+        for variant in (0, 16, 48, 80, 112):
             M.set_variant(variant)
             M.jacobi(rd, dd, xd, yd)
             assert np.array_equal(yd.get(), ref_j), variant
@@ -298,7 +299,7 @@ def test_ell_lds_empty_tail_and_zero_columns(hip, oracle, rng, dtype):
     M = hip.EllMatrix.from_dict(E)
     xd = hip.DeviceVector.from_numpy(x)
     yd = hip.DeviceVector.from_numpy(np.full(E["rows"], 7, dtype))
-    for variant in (0, 16, 20, 48, 52):
+    for variant in (0, 16, 20, 48, 52, 80, 112, 116):
         M.set_variant(variant)
         M.spmv(xd, yd)
         assert np.array_equal(yd.get(), ref), variant
@@ -308,7 +309,7 @@ def test_ell_lds_empty_tail_and_zero_columns(hip, oracle, rng, dtype):
     MZ = hip.EllMatrix.from_dict(Z)
     x1 = hip.DeviceVector.zeros(1, dtype)
     y0 = hip.DeviceVector.from_numpy(np.full(2048, 3, dtype))
-    for variant in (0, 16, 48):
+    for variant in (0, 16, 48, 112):
         MZ.set_variant(variant)
         MZ.spmv(x1, y0)
         assert np.all(y0.get() == 0), variant

@@ -46,11 +46,11 @@ def _solve_alone(hip, srv, level, seed):
     return x.get(), res
 
 
-@pytest.mark.parametrize("level", [6, 8])
+@pytest.mark.parametrize("level", [6, 8, 10, 11])
 def test_served_solve_matches_single_solve_and_oracle(hip, oracle, level):
     n = P.side_of_level(level) ** 2
     b = P.rhs_uniform(n, seed=SEED + level)
-    srv = _server(hip)
+    srv = _server(hip, max_served_level=max(level, 9), memory_budget=8 << 30)
     x_srv, res = _solve_alone(hip, srv, level, SEED + level)
     assert res["status"] == 0
     # the single-solve handle on the same b from x0 = 0
@@ -159,3 +159,22 @@ def test_run_trace_latencies(hip):
         assert abs(r["t_submit"] - arrivals[i]) < 1e-9
         assert r["t_done"] > r["t_admit"] >= r["t_submit"] - 1e-9
         assert r["final_defect"] < 1e-8 * r["initial_defect"]
+
+
+def test_slot_count_not_a_power_of_two(hip):
+    """A tick covers the table entries up to the high-water entry rounded up
+    to a power of two: with 5 slots per level that is 8 entries, past the
+    configured count.  Seven concurrent requests of one level (5 running, 2
+    waiting) each give the bits of the same request solved alone."""
+    srv = _server(hip, max_slots=5)
+    seeds = [300 + s for s in range(7)]
+    alone = {s: _solve_alone(hip, srv, 6, s) for s in seeds}
+    outs = {}
+    for s in seeds:
+        x = hip.DeviceVector.zeros(P.side_of_level(6) ** 2)
+        outs[srv.submit(6, seed=s, x=x)] = (s, x)
+    got = srv.drain(len(seeds))
+    assert srv.stats()["peak_running"] == 5
+    for rid, (s, x) in outs.items():
+        assert np.array_equal(x.get(), alone[s][0]), s
+        assert (got[rid]["outer"], got[rid]["inner"]) == (alone[s][1]["outer"], alone[s][1]["inner"])

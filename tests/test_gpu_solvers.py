@@ -47,15 +47,26 @@ def test_cg_ell_poisson_257(hip, oracle):
     assert _rel(xd.get(), x_ref) <= 1e-6
 
 
-def test_cg_fp32_q1(hip, oracle):
-    n = 65 * 65
+@pytest.mark.parametrize("side,seed", [(33, 0), (65, 0), (65, 3), (129, 1)])
+def test_cg_fp32_q1(hip, oracle, side, seed):
+    """Pure FP32 CGSolver<NONE> (eps 1e-5): the oracle sums its dot products
+    sequentially in float (dot_product.hh:72-87), the device in a fixed tree
+    in double; iteration counts within +-1, solutions within 1e-4."""
+    n = side * side
     bands = [b.astype(np.float32) for b in P.poisson_q1_bands(n)]
-    b = P.rhs_uniform(n, dtype=np.float32)
+    b = P.rhs_uniform(n, seed=P.SEED + seed, dtype=np.float32)
     x_ref, it_ref, _ = oracle.cg_none_q1(bands, b, np.zeros(n, np.float32), 2000, np.float32(1e-5))
     A = hip.Q1Matrix(bands)
     bd, xd = hip.DeviceVector.from_numpy(b), hip.DeviceVector.zeros(n, np.float32)
     used, _ = hip.cg_q1(A, bd, xd, 2000, 1e-5, raise_on_cap=False)
-    assert abs(used - it_ref) <= 2, (used, it_ref)
+    assert abs(used - it_ref) <= 1, (used, it_ref)
+    assert _rel(xd.get().astype(np.float64), x_ref.astype(np.float64)) <= 1e-4
+    E = P.q1_to_ell(bands)
+    x_ref, it_ref, _ = oracle.cg_none_ell(E, b, np.zeros(n, np.float32), 2000, np.float32(1e-5))
+    M = hip.EllMatrix.from_dict(E)
+    xd = hip.DeviceVector.zeros(n, np.float32)
+    used, _ = hip.cg_ell(M, bd, xd, 2000, 1e-5, raise_on_cap=False)
+    assert abs(used - it_ref) <= 1, (used, it_ref)
     assert _rel(xd.get().astype(np.float64), x_ref.astype(np.float64)) <= 1e-4
 
 
