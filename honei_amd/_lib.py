@@ -120,6 +120,7 @@ _SIGS = {
     "hsolve_cg_q1": [C.c_int, vp, u64, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_cg_ell": [vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_pcg_ell": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
+    "hsolve_cg_ell_noalloc": [vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_pcg_ell_pmat": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_cg_iters_ell": [vp, vp, vp, u64, C.POINTER(u64), C.POINTER(dbl), vp],
     "hsolve_pcg_jac_ell": [vp, vp, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],

@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace hsolve
@@ -516,7 +518,8 @@ namespace
         }
     };
 
-    // Device workspace of one CG solve.
+    // Device workspace of one CG solve: allocated here, or carved from a
+    // caller's block of at least bytes_for(n) bytes (256-byte aligned).
     template <typename DT> struct CgWork
     {
         DT *r = nullptr, *p = nullptr, *v = nullptr;
@@ -525,14 +528,26 @@ namespace
         double * partials2 = nullptr;     // a second set, for kernels that read one set and write the other
         CgScalars<DT> * s = nullptr;
         void * block = nullptr;
+        bool owned = false;
 
-        explicit CgWork(uint64_t n)
+        static size_t vector_bytes(uint64_t n) { return ((sizeof(DT) * n + 255) / 256) * 256; }
+        static size_t bytes_for(uint64_t n)
         {
-            const size_t vb = ((sizeof(DT) * n + 255) / 256) * 256;
-            const size_t bytes = 4 * vb + sizeof(double) * 4 * kSolverBlocks + sizeof(CgScalars<DT>) + 256;
-            hipError_t e = hipMalloc(&block, bytes);
-            if (e != hipSuccess)
-                throw Error(HSOLVE_ERR_ALLOC, "CG workspace allocation failed");
+            return 4 * vector_bytes(n) + sizeof(double) * 4 * kSolverBlocks + sizeof(CgScalars<DT>) + 256;
+        }
+
+        explicit CgWork(uint64_t n, void * external = nullptr)
+        {
+            const size_t vb = vector_bytes(n);
+            if (external)
+                block = external;
+            else
+            {
+                hipError_t e = hipMalloc(&block, bytes_for(n));
+                if (e != hipSuccess)
+                    throw Error(HSOLVE_ERR_ALLOC, "CG workspace allocation failed");
+                owned = true;
+            }
             char * c = (char *)block;
             r = (DT *)c; c += vb;
             p = (DT *)c; c += vb;
@@ -544,16 +559,19 @@ namespace
         }
         ~CgWork()
         {
-            if (block)
+            if (owned && block)
                 (void)hipFree(block);
         }
     };
 
+    // work: NULL (the workspace is allocated for this solve) or a device
+    // block of at least CgWork<DT>::bytes_for(n) bytes
     template <typename DT, typename Op, bool VAR>
     void run_cg(const Op & A, uint64_t n, const DT * b, DT * x, const DT * P, uint64_t max_iters, double eps,
-                uint64_t * used_iters, double * defect_out, hipStream_t st, int * converged_out, int plain = 0)
+                uint64_t * used_iters, double * defect_out, hipStream_t st, int * converged_out, int plain = 0,
+                void * work = nullptr)
     {
-        CgWork<DT> w(n);
+        CgWork<DT> w(n, work);
         CgScalars<DT> h;
         std::memset(&h, 0, sizeof(h));
         h.eps_relative = (DT)eps;
@@ -751,14 +769,15 @@ namespace
 
     template <typename DT, bool VAR>
     void run_cg_ell(const hsolve_ell_s * m, const DT * b, DT * x, const DT * P, uint64_t max_iters, double eps,
-                    uint64_t * used, double * defect, hipStream_t st, int * conv, int plain = 0)
+                    uint64_t * used, double * defect, hipStream_t st, int * conv, int plain = 0,
+                    void * work = nullptr)
     {
         const uint64_t n = m->rows;
         with_ell_types(m, [&](auto Aj, auto Arl) {
             using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
             using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
             EllOp<DT, IT, LT> A{Aj, (const DT *)m->Ax, Arl, m->stride, (uint32_t)m->threads};
-            run_cg<DT, EllOp<DT, IT, LT>, VAR>(A, n, b, x, P, max_iters, eps, used, defect, st, conv, plain);
+            run_cg<DT, EllOp<DT, IT, LT>, VAR>(A, n, b, x, P, max_iters, eps, used, defect, st, conv, plain, work);
         });
     }
 
@@ -888,6 +907,40 @@ namespace
                                                  : (double)h.current_defect;
         if (converged_out)
             *converged_out = h.converged;
+    }
+
+    // ------------------------------------------ allocation-free CG workspace
+    // hsolve_cg_ell_noalloc carves its CG workspace (four vectors, two sets
+    // of reduction partials, the scalars) from this fixed array instead of
+    // allocating it.  Every device the library runs on holds its own copy
+    // until the process ends; every call on a device shares that copy, so
+    // calls that overlap on one device (other threads, other streams) race.
+    constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+    static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+    constexpr int kMaxDevices = 64;
+    std::mutex g_scratch_mutex;
+    void * g_scratch_ptr[kMaxDevices] = {};
+
+    // The device address of the current device's scratch, looked up once per
+    // device with that device current.  (The array's name in host code is a
+    // host-side stand-in, never a device address.)
+    void * noalloc_scratch()
+    {
+        int dev = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        require(dev >= 0 && dev < kMaxDevices, HSOLVE_ERR_INVALID, "cg_ell_noalloc: device index out of range");
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        if (g_scratch_ptr[dev] == nullptr)
+        {
+            void * ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || ptr == nullptr)
+                throw Error(HSOLVE_ERR_HIP, std::string("cg_ell_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                                                hipGetErrorString(e));
+            g_scratch_ptr[dev] = ptr;
+        }
+        return g_scratch_ptr[dev];
     }
 }  // namespace
 }  // namespace hsolve
@@ -1045,6 +1098,34 @@ int hsolve_pcg_ell(hsolve_ell m, const void * P, const void * b, void * x, uint6
     if (! conv && max_iters > 0)
     {
         set_last_error("pcg_ell: max_iters reached");
+        return HSOLVE_ERR_NOT_CONVERGED;
+    }
+    HSOLVE_API_END
+}
+
+int hsolve_cg_ell_noalloc(hsolve_ell m, const void * b, void * x, uint64_t max_iters, double eps,
+                          uint64_t * used_iters, double * defect_out, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(m && b && x, HSOLVE_ERR_INVALID, "cg_ell_noalloc: null pointer");
+    require(m->rows == m->cols, HSOLVE_ERR_SIZE, "cg_ell_noalloc: matrix is not square");
+    const uint64_t n = m->rows;
+    const size_t need = m->dtype ? CgWork<double>::bytes_for(n) : CgWork<float>::bytes_for(n);
+    if (need > kNoAllocScratchBytes)
+        throw Error(HSOLVE_ERR_SIZE, "cg_ell_noalloc: " + std::to_string(n) + " rows need " + std::to_string(need) +
+                                         " bytes of workspace, above the fixed scratch of " +
+                                         std::to_string(kNoAllocScratchBytes) + " bytes; use hsolve_cg_ell");
+    void * work = noalloc_scratch();
+    int conv = 0;
+    if (m->dtype)
+        run_cg_ell<double, false>(m, (const double *)b, (double *)x, nullptr, max_iters, eps, used_iters, defect_out,
+                                  as_stream(stream), &conv, 0, work);
+    else
+        run_cg_ell<float, false>(m, (const float *)b, (float *)x, nullptr, max_iters, eps, used_iters, defect_out,
+                                 as_stream(stream), &conv, 0, work);
+    if (! conv && max_iters > 0)
+    {
+        set_last_error("cg_ell_noalloc: max_iters reached");
         return HSOLVE_ERR_NOT_CONVERGED;
     }
     HSOLVE_API_END

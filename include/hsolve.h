@@ -352,6 +352,17 @@ int hsolve_cg_q1(int dtype, const void *const *bands9, uint64_t size, const void
                  void *stream);
 int hsolve_cg_ell(hsolve_ell m, const void *b, void *x, uint64_t max_iters, double eps,
                   uint64_t *used_iters, double *defect_out, void *stream);
+/* hsolve_cg_ell that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels, same results, bit for
+ * bit; the CG workspace (four vectors, reduction partials, scalars) comes
+ * from a fixed 64 MiB device array that each device holds until the process
+ * ends, instead of a per-call allocation.  Rows up to 2,092,000 (FP64) /
+ * 4,184,000 (FP32); larger matrices return HSOLVE_ERR_SIZE (use
+ * hsolve_cg_ell).  Every call on one device shares that array: calls that
+ * overlap on one device (other host threads, other streams) race. */
+int hsolve_cg_ell_noalloc(hsolve_ell m, const void *b, void *x, uint64_t max_iters, double eps,
+                          uint64_t *used_iters, double *defect_out, void *stream);
 /* CGSolver<Tag, methods::VAR> with a diagonal preconditioner P (device
  * vector, z = P * r elementwise), honei/math/cg.hh:141-219. */
 int hsolve_pcg_ell(hsolve_ell m, const void *P, const void *b, void *x, uint64_t max_iters,
