@@ -115,6 +115,7 @@ _SIGS = {
     "hsolve_ell_set_variant": [vp, C.c_int],
     "hsolve_ell_q1_side": [vp, C.POINTER(u64)],
     "hsolve_ell_spmv": [vp, vp, vp, vp, vp],
+    "hsolve_ell_spmv_rows": [vp, vp, vp, vp, u64, u64, vp],
     "hsolve_ell_jacobi": [vp, vp, vp, vp, vp, vp],
     "hsolve_cg_konv_ell": [vp, vp, vp, dbl, u64, vp, vp],
     "hsolve_cg_q1": [C.c_int, vp, u64, vp, vp, u64, dbl, C.POINTER(u64), C.POINTER(dbl), vp],

@@ -64,13 +64,14 @@ namespace
         return (size_t)(w * vs);
     }
 
+    // rows [r0, r1) (r0 even: the rows pair up as in the whole product)
     template <typename DT, typename IT, typename LT, int MODE, int K>
     bool ell_launch_r2(const hsolve_ell_s * m, int v, const DT * x, DT * y, const IT * Aj, const DT * Ax,
-                       const LT * Arl, const DT * rhs, const DT * dinv, hipStream_t st)
+                       const LT * Arl, const DT * rhs, const DT * dinv, hipStream_t st, uint64_t r0, uint64_t r1)
     {
         if (m->ncpr > (uint64_t)K || m->ncpr == 0 || m->cols == 0)
             return false;
-        const unsigned grid = grid_for((m->rows + 1) / 2, kBlock);
+        const unsigned grid = grid_for((r1 - r0 + 1) / 2, kBlock);
         const bool full = v & HSOLVE_ELL_FULL, xcd = v & HSOLVE_ELL_XCD;
         if constexpr (K == 9)
         {
@@ -78,14 +79,14 @@ namespace
             if (m->q1_side != 0 && full && ! xcd && (m->variant & HSOLVE_ELL_SET) == 0)
             {
                 hipLaunchKernelGGL((k::ell_spmv_r2<DT, IT, LT, 9, MODE, true, false, true>), dim3(grid), dim3(kBlock),
-                                   0, st, x, y, Aj, Ax, Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride,
-                                   (uint32_t)m->ncpr, (int64_t)m->q1_side);
+                                   0, st, x, y, Aj, Ax, Arl, rhs, dinv, r0, r1, m->stride, (uint32_t)m->ncpr,
+                                   (int64_t)m->q1_side);
                 return true;
             }
         }
 #define HS_R2(F, X)                                                                                           \
         hipLaunchKernelGGL((k::ell_spmv_r2<DT, IT, LT, K, MODE, F, X>), dim3(grid), dim3(kBlock), 0, st, x, y, Aj, \
-                           Ax, Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride, (uint32_t)m->ncpr)
+                           Ax, Arl, rhs, dinv, r0, r1, m->stride, (uint32_t)m->ncpr)
         if (full && xcd) HS_R2(true, true);
         else if (full) HS_R2(true, false);
         else if (xcd) HS_R2(false, true);
@@ -112,20 +113,32 @@ namespace
                            m->cols, m->stride, (uint32_t)m->ncpr, m->max_offset, perm, nblk);
     }
 
+    // rows [r0, r1) of the product / defect / sweep (the whole matrix by
+    // default); a row range other than the whole matrix takes the row
+    // kernels (the LDS-window kernels work on whole blocks)
     template <typename DT, int MODE>
-    void ell_launch(const hsolve_ell_s * m, const DT * x, DT * y, const DT * rhs, const DT * dinv, hipStream_t st)
+    void ell_launch(const hsolve_ell_s * m, const DT * x, DT * y, const DT * rhs, const DT * dinv, hipStream_t st,
+                    uint64_t r0 = 0, uint64_t r1 = ~(uint64_t)0)
     {
-        const unsigned grid = grid_for(m->rows, kBlock);
+        if (r1 > m->rows)
+            r1 = m->rows;
+        if (r0 >= r1)
+            return;
+        const bool whole = r0 == 0 && r1 == m->rows;
+        const unsigned grid = grid_for(r1 - r0, kBlock);
         const DT * Ax = (const DT *)m->Ax;
         const int v = ell_variant(m);
         // two rows per lane reads the slot pair at row_end - 1: safe when the
-        // stride leaves a padding row or the row count is even
-        const bool r2_ok = m->threads == 1 && ! (v & HSOLVE_ELL_ONE_ROW) && (m->stride > m->rows || m->rows % 2 == 0);
+        // stride leaves a padding row or the row count is even (a range that
+        // ends before the last row reads a row of the matrix); pairs start on
+        // even rows
+        const bool r2_ok = m->threads == 1 && ! (v & HSOLVE_ELL_ONE_ROW) && (r0 % 2 == 0) &&
+                           (r1 < m->rows || m->stride > m->rows || m->rows % 2 == 0);
         with_ell_types(m, [&](auto Aj, auto Arl) {
             using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
             using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
             const size_t lds = lds_window_bytes(m, sizeof(DT));
-            if (r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->cols != 0 && m->ncpr >= 1 && m->ncpr <= 16)
+            if (whole && r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->cols != 0 && m->ncpr >= 1 && m->ncpr <= 16)
             {
                 const unsigned g = grid_for(m->rows, k::lds_rows<DT>());
                 const bool sorted = (v & HSOLVE_ELL_SORT) && m->sAx;
@@ -184,23 +197,23 @@ namespace
                 HIP_CHECK(hipGetLastError());
                 return;
             }
-            if (r2_ok && (ell_launch_r2<DT, IT, LT, MODE, 4>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
-                          ell_launch_r2<DT, IT, LT, MODE, 9>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
-                          ell_launch_r2<DT, IT, LT, MODE, 13>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st) ||
-                          ell_launch_r2<DT, IT, LT, MODE, 16>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st)))
+            if (r2_ok && (ell_launch_r2<DT, IT, LT, MODE, 4>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st, r0, r1) ||
+                          ell_launch_r2<DT, IT, LT, MODE, 9>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st, r0, r1) ||
+                          ell_launch_r2<DT, IT, LT, MODE, 13>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st, r0, r1) ||
+                          ell_launch_r2<DT, IT, LT, MODE, 16>(m, v, x, y, Aj, Ax, Arl, rhs, dinv, st, r0, r1)))
                 return;
             if (m->threads != 1)
                 hipLaunchKernelGGL((k::ell_spmv_tn<DT, IT, LT, MODE>), dim3(grid), dim3(kBlock), 0, st, x, y, Aj, Ax,
-                                   Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride, (uint32_t)m->threads);
+                                   Arl, rhs, dinv, r0, r1, m->stride, (uint32_t)m->threads);
 #define HS_K(K)                                                                                               \
             else if (m->ncpr <= K)                                                                            \
                 hipLaunchKernelGGL((k::ell_spmv_t1<DT, IT, LT, K, MODE>), dim3(grid), dim3(kBlock), 0, st, x, y, Aj, \
-                                   Ax, Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride);
+                                   Ax, Arl, rhs, dinv, r0, r1, m->stride);
             HS_K(4) HS_K(8) HS_K(9) HS_K(12) HS_K(13) HS_K(16) HS_K(24) HS_K(32)
 #undef HS_K
             else
                 hipLaunchKernelGGL((k::ell_spmv_t1<DT, IT, LT, 0, MODE>), dim3(grid), dim3(kBlock), 0, st, x, y, Aj,
-                                   Ax, Arl, rhs, dinv, (uint64_t)0, m->rows, m->stride);
+                                   Ax, Arl, rhs, dinv, r0, r1, m->stride);
         });
         HIP_CHECK(hipGetLastError());
     }
@@ -471,6 +484,33 @@ int hsolve_ell_q1_side(hsolve_ell m, uint64_t * side)
     HSOLVE_API_BEGIN
     require(m && side, HSOLVE_ERR_INVALID, "ell_q1_side: null pointer");
     *side = m->q1_side;
+    HSOLVE_API_END
+}
+
+int hsolve_ell_spmv_rows(hsolve_ell m, const void * x, void * y, const void * rhs, uint64_t row_start,
+                         uint64_t row_end, void * stream)
+{
+    HSOLVE_API_BEGIN
+    require(m && x && y, HSOLVE_ERR_INVALID, "ell_spmv_rows: null pointer");
+    require(x != y, HSOLVE_ERR_INVALID, "ell_spmv_rows: y must not alias x");
+    require(row_end <= m->rows, HSOLVE_ERR_SIZE, "ell_spmv_rows: row_end beyond the matrix");
+    hipStream_t st = as_stream(stream);
+    if (row_start >= row_end)
+        return HSOLVE_OK;
+    if (m->dtype)
+    {
+        if (rhs)
+            ell_launch<double, 1>(m, (const double *)x, (double *)y, (const double *)rhs, nullptr, st, row_start, row_end);
+        else
+            ell_launch<double, 0>(m, (const double *)x, (double *)y, nullptr, nullptr, st, row_start, row_end);
+    }
+    else
+    {
+        if (rhs)
+            ell_launch<float, 1>(m, (const float *)x, (float *)y, (const float *)rhs, nullptr, st, row_start, row_end);
+        else
+            ell_launch<float, 0>(m, (const float *)x, (float *)y, nullptr, nullptr, st, row_start, row_end);
+    }
     HSOLVE_API_END
 }
 

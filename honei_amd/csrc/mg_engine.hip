@@ -22,6 +22,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace hsolve
@@ -539,35 +540,40 @@ namespace
 
     constexpr int kPostW = 250;
 
-    // b[0..8]: the row's nine bands (LL .. UU)
-    template <int MK, int NB>
+    // Row i of A x from three rows of x in LDS (up, mid, dn: rows r - 1, r,
+    // r + 1 of the tile's column span, j the lane's column), b[0..8] the row's
+    // nine bands (LL .. UU), in q1_row's order.  GUARD: test each neighbour's
+    // index against [0, n) as q1_row does; without it every neighbour must
+    // lie in the matrix (rows 1 .. m - 2 of the grid).  Indices fit in 32 bits
+    // (levels up to 16385^2).
+    template <int MK, bool GUARD, int NB>
     __device__ __forceinline__ float post_row(const float (&b)[NB], const float * __restrict__ up,
                                               const float * __restrict__ mid, const float * __restrict__ dn, int j,
-                                              int64_t i, int64_t n, int64_t m)
+                                              int i, int n, int m)
     {
         if (kOrd<MK> == 1)
         {
             float s = 0.0f;
-            if (i - m - 1 >= 0) s += b[0] * up[j - 1];
-            if (i - m >= 0)     s += b[1] * up[j];
-            if (i - m + 1 >= 0) s += b[2] * up[j + 1];
-            if (i - 1 >= 0)     s += b[3] * mid[j - 1];
+            if (! GUARD || i - m - 1 >= 0) s += b[0] * up[j - 1];
+            if (! GUARD || i - m >= 0)     s += b[1] * up[j];
+            if (! GUARD || i - m + 1 >= 0) s += b[2] * up[j + 1];
+            if (! GUARD || i - 1 >= 0)     s += b[3] * mid[j - 1];
             s += b[4] * mid[j];
-            if (i + 1 < n)      s += b[5] * mid[j + 1];
-            if (i + m - 1 < n)  s += b[6] * dn[j - 1];
-            if (i + m < n)      s += b[7] * dn[j];
-            if (i + m + 1 < n)  s += b[8] * dn[j + 1];
+            if (! GUARD || i + 1 < n)      s += b[5] * mid[j + 1];
+            if (! GUARD || i + m - 1 < n)  s += b[6] * dn[j - 1];
+            if (! GUARD || i + m < n)      s += b[7] * dn[j];
+            if (! GUARD || i + m + 1 < n)  s += b[8] * dn[j + 1];
             return s;
         }
         float s = b[4] * mid[j];
-        if (i - m - 1 >= 0) s += b[0] * up[j - 1];
-        if (i - m >= 0)     s += b[1] * up[j];
-        if (i - m + 1 >= 0) s += b[2] * up[j + 1];
-        if (i - 1 >= 0)     s += b[3] * mid[j - 1];
-        if (i + 1 < n)      s += b[5] * mid[j + 1];
-        if (i + m - 1 < n)  s += b[6] * dn[j - 1];
-        if (i + m < n)      s += b[7] * dn[j];
-        if (i + m + 1 < n)  s += b[8] * dn[j + 1];
+        if (! GUARD || i - m - 1 >= 0) s += b[0] * up[j - 1];
+        if (! GUARD || i - m >= 0)     s += b[1] * up[j];
+        if (! GUARD || i - m + 1 >= 0) s += b[2] * up[j + 1];
+        if (! GUARD || i - 1 >= 0)     s += b[3] * mid[j - 1];
+        if (! GUARD || i + 1 < n)      s += b[5] * mid[j + 1];
+        if (! GUARD || i + m - 1 < n)  s += b[6] * dn[j - 1];
+        if (! GUARD || i + m < n)      s += b[7] * dn[j];
+        if (! GUARD || i + m + 1 < n)  s += b[8] * dn[j + 1];
         return s;
     }
 
@@ -660,18 +666,18 @@ namespace
         __shared__ float xr[3][256];
         __shared__ float x1r[3][256];
         __shared__ float x2r[DEFECT ? 3 : 1][256];
-        const int64_t n = a.n, m = a.m;
-        const int64_t nrb = (m + th - 1) / th;
+        const int n = (int)a.n, m = (int)a.m;
+        const int nrb = (m + th - 1) / th;
         // tiles in column strips, row blocks fastest: vertical neighbours,
         // which share the most halo, are consecutive and (k::xcd_block) on
         // one XCD
         const unsigned t = gridDim.z == 1 ? k::xcd_block(vb.x, gridDim.x) : vb.x;
-        const int64_t c0 = (int64_t)(t / nrb) * kPostW;
-        const int64_t R0 = (int64_t)(t % nrb) * th;
-        const int W = (int)min((int64_t)kPostW, m - c0);
-        const int H = (int)min((int64_t)th, m - R0);
+        const int c0 = (int)(t / nrb) * kPostW;
+        const int R0 = (int)(t % nrb) * th;
+        const int W = min(kPostW, m - c0);
+        const int H = min(th, m - R0);
         const int j = threadIdx.x;
-        const int64_t c = c0 - 3 + j;
+        const int c = c0 - 3 + j;
         const bool in_x = j < W + 6;
         const bool in_1 = j >= 1 && j < W + 5;
         const bool in_2 = j >= 2 && j < W + 4;
@@ -686,138 +692,149 @@ namespace
 #pragma unroll
         for (int k = 0; k < 9; ++k)
             bp[k] = as_global<const float>((const float *)a.B.b[k]);
-        const int Nc = (int)((m + 1) / 2);
+        const int Nc = (m + 1) / 2;
 
-        // grid position (R, S) of linear index R0 - 3 + s rows down, column
+        // grid position (R, S) of the node R0 - 3 + s rows down at column
         // offset c (c lies in [-3, 2m)), clamped to the grid
         auto grid_pos = [&](int s, int & R, int & S) {
-            int64_t RR = R0 - 3 + s, SS = c;
-            if (SS < 0)
+            R = R0 - 3 + s;
+            S = c;
+            if (S < 0)
             {
-                SS += m;
-                RR -= 1;
+                S += m;
+                R -= 1;
             }
-            else if (SS >= m)
+            else if (S >= m)
             {
-                SS -= m;
-                RR += 1;
+                S -= m;
+                R += 1;
             }
-            R = (int)min(max(RR, (int64_t)0), m - 1);
-            S = (int)min(max(SS, (int64_t)0), m - 1);
+            R = min(max(R, 0), m - 1);
+            S = min(max(S, 0), m - 1);
         };
         // The loads of step s: x of row R0 - 3 + s (with PROL the four coarse
         // values around it) and the bands, dinv and rhs of row R0 - 4 + s.
         // Every lane loads, at indices clamped to the arrays, with no branch
         // between the loads; step s decides which values count.
         auto load = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
-            const int64_t ix = (R0 - 3 + s) * m + c;
-            xq = x[min(max(ix, (int64_t)0), n - 1)];
+            const int ix = (R0 - 3 + s) * m + c;
+            xq = x[min(max(ix, 0), n - 1)];
             if (PROL)
             {
                 int R, S;
                 grid_pos(s, R, S);
-                const int64_t ra = (int64_t)(R >> 1) * Nc, rb = (int64_t)((R + 1) >> 1) * Nc;
+                const int ra = (R >> 1) * Nc, rb = ((R + 1) >> 1) * Nc;
                 const int jj = S >> 1, jj1 = min(jj + 1, Nc - 1);
                 cq[0] = xc[ra + jj];
                 cq[1] = xc[ra + jj1];
                 cq[2] = xc[rb + jj];
                 cq[3] = xc[rb + jj1];
             }
-            const int64_t ib = min(max(ix - m, (int64_t)0), n - 1);
+            const int ib = min(max(ix - m, 0), n - 1);
 #pragma unroll
             for (int k = 0; k < 9; ++k)
                 bq[k] = bp[k][ib];
             bq[9] = dinv[ib];
             bq[10] = rhs[ib];
         };
-        // stage registers: row R0 - 4 + s (stage 1), R0 - 5 + s (stage 2),
-        // R0 - 6 + s (stage 3)
-        float b1[11] = {}, b2[11] = {}, b3[11] = {};
         // H + 6 steps, rounded up to an even count so that the loop body is
         // two whole steps with no branch around their loads (the extra step's
         // rows lie outside the tile: nothing of it is stored)
         const int steps = (H + 7) & ~1;
-        // one step; (xq, cq, bq) hold its loads and then take those of step s + 2
-        auto step = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
-            const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;   // rows r, r - 1, r - 2
-            {
-                const int64_t ix = (R0 - 3 + s) * m + c;
-                float xv = 0.0f;
-                if (in_x && ix >= 0 && ix < n)
+        // A tile whose computed rows (R0 - 2 .. R0 + H + 1) lie in rows 2 .. m
+        // - 3 has every node and every neighbour in the matrix: no index tests
+        const bool interior = R0 >= 4 && R0 + H + 4 <= m;
+        auto march = [&](auto interior_tag) {
+            constexpr bool G = ! decltype(interior_tag)::value;
+            // stage registers: row R0 - 4 + s (stage 1), R0 - 5 + s (stage 2),
+            // R0 - 6 + s (stage 3)
+            float b1[11] = {}, b2[11] = {}, b3[11] = {};
+            // one step; (xq, cq, bq) hold its loads and then take those of step s + 2
+            auto step = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
+                const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;   // rows r, r - 1, r - 2
                 {
-                    xv = xq;
-                    if (PROL)
+                    const int ix = (R0 - 3 + s) * m + c;
+                    float xv = 0.0f;
+                    if (in_x && (! G || (ix >= 0 && ix < n)))
                     {
-                        int R, S;
-                        grid_pos(s, R, S);
-                        xv = xv + alpha * k::prolongation_node_v<float>(cq, (int)m, mk, R, S);
+                        xv = xq;
+                        if (PROL)
+                        {
+                            int R, S;
+                            grid_pos(s, R, S);
+                            xv = xv + alpha * k::prolongation_node_v<float>(cq, m, mk, R, S);
+                        }
+                    }
+                    xr[cur][j] = xv;
+                    const int ib = ix - m;
+                    const bool vb1 = in_1 && s >= 2 && (! G || (ib >= 0 && ib < n));
+#pragma unroll
+                    for (int k = 0; k < 11; ++k)
+                    {
+                        b3[k] = b2[k];
+                        b2[k] = b1[k];
+                        b1[k] = vb1 ? bq[k] : 0.0f;
                     }
                 }
-                xr[cur][j] = xv;
-                const int64_t ib = ix - m;
-                const bool vb1 = in_1 && s >= 2 && ib >= 0 && ib < n;
-#pragma unroll
-                for (int k = 0; k < 11; ++k)
+                load(s + 2, xq, cq, bq);
+                lds_barrier();
+                // stage 1: x1 of row R0 - 4 + s
+                if (s >= 2)
                 {
-                    b3[k] = b2[k];
-                    b2[k] = b1[k];
-                    b1[k] = vb1 ? bq[k] : 0.0f;
+                    float v = 0.0f;
+                    const int i = (R0 - 4 + s) * m + c;
+                    if (in_1 && (! G || (i >= 0 && i < n)))
+                    {
+                        const float sum = post_row<MK, G>(b1, xr[old], xr[prv], xr[cur], j, i, n, m);
+                        const float tt = b1[10] - sum;
+                        v = xr[prv][j] + tt * b1[9];
+                    }
+                    x1r[cur][j] = v;
                 }
-            }
-            load(s + 2, xq, cq, bq);
-            lds_barrier();
-            // stage 1: x1 of row R0 - 4 + s
-            if (s >= 2)
-            {
-                float v = 0.0f;
-                const int64_t i = (R0 - 4 + s) * m + c;
-                if (in_1 && i >= 0 && i < n)
+                lds_barrier();
+                // stage 2: x2 of row R0 - 5 + s
+                if (s >= 4)
                 {
-                    const float sum = post_row<MK>(b1, xr[old], xr[prv], xr[cur], j, i, n, m);
-                    const float tt = b1[10] - sum;
-                    v = xr[prv][j] + tt * b1[9];
-                }
-                x1r[cur][j] = v;
-            }
-            lds_barrier();
-            // stage 2: x2 of row R0 - 5 + s
-            if (s >= 4)
-            {
-                float v = 0.0f;
-                const int64_t r2 = R0 - 5 + s;
-                const int64_t i = r2 * m + c;
-                if (in_2 && i >= 0 && i < n)
-                {
-                    const float sum = post_row<MK>(b2, x1r[old], x1r[prv], x1r[cur], j, i, n, m);
-                    const float tt = b2[10] - sum;
-                    v = x1r[prv][j] + tt * b2[9];
-                    if (own && r2 >= R0 && r2 < R0 + H)
-                        out[i] = v;
+                    float v = 0.0f;
+                    const int r2 = R0 - 5 + s;
+                    const int i = r2 * m + c;
+                    if (in_2 && (! G || (i >= 0 && i < n)))
+                    {
+                        const float sum = post_row<MK, G>(b2, x1r[old], x1r[prv], x1r[cur], j, i, n, m);
+                        const float tt = b2[10] - sum;
+                        v = x1r[prv][j] + tt * b2[9];
+                        if (own && r2 >= R0 && r2 < R0 + H)
+                            out[i] = v;
+                    }
+                    if (DEFECT)
+                        x2r[DEFECT ? cur : 0][j] = v;
                 }
                 if (DEFECT)
-                    x2r[DEFECT ? cur : 0][j] = v;
-            }
-            if (DEFECT)
-            {
-                lds_barrier();
-                // stage 3: d of row R0 - 6 + s
-                const int64_t r3 = R0 - 6 + s;
-                const int64_t i = r3 * m + c;
-                if (s >= 6 && own && r3 < R0 + H && i >= 0 && i < n)
                 {
-                    const float sum = post_row<MK>(b3, x2r[old], x2r[prv], x2r[DEFECT ? cur : 0], j, i, n, m);
-                    dout[i] = b3[10] - sum;
+                    lds_barrier();
+                    // stage 3: d of row R0 - 6 + s
+                    const int r3 = R0 - 6 + s;
+                    const int i = r3 * m + c;
+                    if (s >= 6 && own && r3 < R0 + H && (! G || (i >= 0 && i < n)))
+                    {
+                        const float sum = post_row<MK, G>(b3, x2r[old], x2r[prv], x2r[DEFECT ? cur : 0], j, i, n, m);
+                        dout[i] = b3[10] - sum;
+                    }
                 }
+            };
+            float xA, xB, cA[4] = {}, cB[4] = {}, bA[11], bB[11];
+            load(0, xA, cA, bA);
+            load(1, xB, cB, bB);
+            for (int s = 0; s < steps; s += 2)
+            {
+                step(s, xA, cA, bA);
+                step(s + 1, xB, cB, bB);
             }
         };
-        float xA, xB, cA[4] = {}, cB[4] = {}, bA[11], bB[11];
-        load(0, xA, cA, bA);
-        load(1, xB, cB, bB);
-        for (int s = 0; s < steps; s += 2)
-        {
-            step(s, xA, cA, bA);
-            step(s + 1, xB, cB, bB);
-        }
+        if (interior)
+            march(std::true_type{});
+        else
+            march(std::false_type{});
     }
 
     // ---------------------- pre-smoothing, defect and restriction in one pass
@@ -839,22 +856,22 @@ namespace
     // owned columns 2s, which all lie in that span.
     template <int MK, int ACC>
     __global__ __launch_bounds__(256) void v_pre2r_kernel(LevelArgs a, const Slot * T, int L, int fd, int facc,
-                                                          int64_t Nc, k::Mask mk, int th)
+                                                          int64_t Nc64, k::Mask mk, int th)
     {
         const VB vb = vblock();
         HS_GATE_RUN
         __shared__ float c1r[3][256];
         __shared__ float xr[3][256];
         __shared__ float dr[3][256];
-        const int64_t n = a.n, m = a.m;
-        const int64_t nrb = (m + th - 1) / th;
+        const int n = (int)a.n, m = (int)a.m, Nc = (int)Nc64;
+        const int nrb = (m + th - 1) / th;
         const unsigned t = gridDim.z == 1 ? k::xcd_block(vb.x, gridDim.x) : vb.x;
-        const int64_t c0 = (int64_t)(t / nrb) * kPostW;
-        const int64_t R0 = (int64_t)(t % nrb) * th;
-        const int W = (int)min((int64_t)kPostW, m - c0);
-        const int H = (int)min((int64_t)th, m - R0);
+        const int c0 = (int)(t / nrb) * kPostW;
+        const int R0 = (int)(t % nrb) * th;
+        const int W = min(kPostW, m - c0);
+        const int H = min(th, m - R0);
         const int j = threadIdx.x;
-        const int64_t c = c0 - 3 + j;
+        const int c = c0 - 3 + j;
         const bool in_x = j < W + 6;
         const bool in_1 = j >= 1 && j < W + 5;
         const bool in_2 = j >= 2 && j < W + 4;
@@ -875,7 +892,7 @@ namespace
         // lane loads, at an index clamped to the arrays, with no branch
         // between the loads; step s decides which values count
         auto load = [&](int s, float (&rn)[13]) {
-            const int64_t i = min(max((R0 - 3 + s) * m + c, (int64_t)0), n - 1);
+            const int i = min(max((R0 - 3 + s) * m + c, 0), n - 1);
 #pragma unroll
             for (int k = 0; k < 9; ++k)
                 rn[k] = bp[k][i];
@@ -884,83 +901,92 @@ namespace
             rn[11] = rhs[i];
             rn[12] = ACC ? acc[i] : 0.0f;
         };
-        // stage registers: rc row R0 - 3 + s (its c1 only, yet), ra row R0 - 4
-        // + s (x: bands, dinv, d, facc), rb row R0 - 5 + s (d': bands, rhs)
-        float rc[13] = {}, ra[13] = {}, rb[13] = {};
         // H + 6 steps, rounded up to an even count (as v_post2_kernel)
         const int steps = (H + 7) & ~1;
-        auto step = [&](int s, float (&rq)[13]) {
-            const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;   // rows r, r - 1, r - 2
-            {
-                const int64_t i = (R0 - 3 + s) * m + c;
-                const bool valid = in_x && i >= 0 && i < n;
+        // computed rows R0 - 2 .. R0 + H + 1 in rows 2 .. m - 3: no index tests
+        const bool interior = R0 >= 4 && R0 + H + 4 <= m;
+        auto march = [&](auto interior_tag) {
+            constexpr bool G = ! decltype(interior_tag)::value;
+            // stage registers: rc row R0 - 3 + s (its c1 only, yet), ra row R0 -
+            // 4 + s (x: bands, dinv, d, facc), rb row R0 - 5 + s (d': bands, rhs)
+            float rc[13] = {}, ra[13] = {}, rb[13] = {};
+            auto step = [&](int s, float (&rq)[13]) {
+                const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;   // rows r, r - 1, r - 2
+                {
+                    const int i = (R0 - 3 + s) * m + c;
+                    const bool valid = in_x && (! G || (i >= 0 && i < n));
 #pragma unroll
-                for (int k = 0; k < 13; ++k)
-                {
-                    rb[k] = ra[k];
-                    ra[k] = rc[k];
-                    rc[k] = valid ? rq[k] : 0.0f;
-                }
-                c1r[cur][j] = rc[9] * rc[10];
-            }
-            load(s + 2, rq);
-            lds_barrier();
-            // x of row R0 - 4 + s
-            if (s >= 2)
-            {
-                float v = 0.0f;
-                const int64_t rho = R0 - 4 + s;
-                const int64_t i = rho * m + c;
-                if (in_1 && i >= 0 && i < n)
-                {
-                    const float sum = post_row<MK>(ra, c1r[old], c1r[prv], c1r[cur], j, i, n, m);
-                    const float tt = ra[10] - sum;
-                    const float c2 = c1r[prv][j] + tt * ra[9];
-                    v = ACC ? ra[12] + c2 : c2;
-                    if (own && rho >= R0 && rho < R0 + H)
-                        xo[i] = v;
-                }
-                xr[cur][j] = v;
-            }
-            lds_barrier();
-            // d' of row R0 - 5 + s
-            if (s >= 4)
-            {
-                float v = 0.0f;
-                const int64_t rho = R0 - 5 + s;
-                const int64_t i = rho * m + c;
-                if (in_2 && i >= 0 && i < n)
-                {
-                    const float sum = post_row<MK>(rb, xr[old], xr[prv], xr[cur], j, i, n, m);
-                    v = rb[11] - sum;
-                }
-                dr[cur][j] = v;
-                lds_barrier();
-                // coarse row r when this d' row is 2r + 1
-                if (((rho - 1) & 1) == 0)
-                {
-                    const int64_t r = (rho - 1) >> 1;
-                    if (own_even && 2 * r >= R0 && 2 * r < R0 + H && c < m)
+                    for (int k = 0; k < 13; ++k)
                     {
-                        const int64_t sc = c >> 1;
-                        const float (*ring)[256] = dr;
-                        co[r * Nc + sc] = k::restrict_node<float>(
-                            [&](int ddr, int dds) {
-                                return ring[ddr < 0 ? old : (ddr == 0 ? prv : cur)][j + dds];
-                            },
-                            r, sc, Nc, mk);
+                        rb[k] = ra[k];
+                        ra[k] = rc[k];
+                        rc[k] = valid ? rq[k] : 0.0f;
+                    }
+                    c1r[cur][j] = rc[9] * rc[10];
+                }
+                load(s + 2, rq);
+                lds_barrier();
+                // x of row R0 - 4 + s
+                if (s >= 2)
+                {
+                    float v = 0.0f;
+                    const int rho = R0 - 4 + s;
+                    const int i = rho * m + c;
+                    if (in_1 && (! G || (i >= 0 && i < n)))
+                    {
+                        const float sum = post_row<MK, G>(ra, c1r[old], c1r[prv], c1r[cur], j, i, n, m);
+                        const float tt = ra[10] - sum;
+                        const float c2 = c1r[prv][j] + tt * ra[9];
+                        v = ACC ? ra[12] + c2 : c2;
+                        if (own && rho >= R0 && rho < R0 + H)
+                            xo[i] = v;
+                    }
+                    xr[cur][j] = v;
+                }
+                lds_barrier();
+                // d' of row R0 - 5 + s
+                if (s >= 4)
+                {
+                    float v = 0.0f;
+                    const int rho = R0 - 5 + s;
+                    const int i = rho * m + c;
+                    if (in_2 && (! G || (i >= 0 && i < n)))
+                    {
+                        const float sum = post_row<MK, G>(rb, xr[old], xr[prv], xr[cur], j, i, n, m);
+                        v = rb[11] - sum;
+                    }
+                    dr[cur][j] = v;
+                    lds_barrier();
+                    // coarse row r when this d' row is 2r + 1
+                    if (((rho - 1) & 1) == 0)
+                    {
+                        const int r = (rho - 1) >> 1;
+                        if (own_even && 2 * r >= R0 && 2 * r < R0 + H && c < m)
+                        {
+                            const int sc = c >> 1;
+                            const float (*ring)[256] = dr;
+                            co[r * Nc + sc] = k::restrict_node<float>(
+                                [&](int ddr, int dds) {
+                                    return ring[ddr < 0 ? old : (ddr == 0 ? prv : cur)][j + dds];
+                                },
+                                (int64_t)r, (int64_t)sc, (int64_t)Nc, mk);
+                        }
                     }
                 }
+            };
+            float qA[13], qB[13];
+            load(0, qA);
+            load(1, qB);
+            for (int s = 0; s < steps; s += 2)
+            {
+                step(s, qA);
+                step(s + 1, qB);
             }
         };
-        float qA[13], qB[13];
-        load(0, qA);
-        load(1, qB);
-        for (int s = 0; s < steps; s += 2)
-        {
-            step(s, qA);
-            step(s + 1, qB);
-        }
+        if (interior)
+            march(std::true_type{});
+        else
+            march(std::false_type{});
     }
 
     // y = x + dinv * (rhs - A x)

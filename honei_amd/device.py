@@ -164,6 +164,11 @@ class EllMatrix:
         check(lib().hsolve_ell_spmv(vp(self.handle), vp(x.ptr), vp(y.ptr), vp(rhs.ptr) if rhs else None,
                                     vp(stream)), "ell_spmv")
 
+    def spmv_rows(self, x, y, row_start, row_end, rhs=None, stream=None):
+        """Rows [row_start, row_end) of y = A x (or rhs - A x); the rest of y untouched."""
+        check(lib().hsolve_ell_spmv_rows(vp(self.handle), vp(x.ptr), vp(y.ptr), vp(rhs.ptr) if rhs else None,
+                                         u64(row_start), u64(row_end), vp(stream)), "ell_spmv_rows")
+
     def spmm(self, nrhs, X, Y, rhs=None, stream=None):
         """Y = A X (or RHS - A X) on column-blocked multivectors of nrhs columns."""
         check(lib().hsolve_ell_spmm(vp(self.handle), u64(nrhs), vp(X.ptr), vp(Y.ptr), vp(rhs.ptr) if rhs else None,

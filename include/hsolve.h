@@ -330,6 +330,14 @@ int hsolve_ell_set_variant(hsolve_ell m, int variant);
 int hsolve_ell_q1_side(hsolve_ell m, uint64_t *side);
 /* y = A x, or y = rhs - A x when rhs != NULL */
 int hsolve_ell_spmv(hsolve_ell m, const void *x, void *y, const void *rhs, void *stream);
+/* The same on rows [row_start, row_end) only (other entries of y untouched;
+ * an empty range is a no-op): HONEI's ELL Product / Defect on a row range
+ * (product.hh:148-206, defect.hh:357-384 loop over [row_start, row_end)),
+ * as its MultiCore row split (product.hh:2087-2106) and the CUDA launcher
+ * (product.cu:356-402) use it; one rank's share of a row-partitioned
+ * product.  Same bits as those rows of hsolve_ell_spmv. */
+int hsolve_ell_spmv_rows(hsolve_ell m, const void *x, void *y, const void *rhs, uint64_t row_start,
+                         uint64_t row_end, void *stream);
 /* ConjugateGradients<Tag, NONE>::value(ELL, rhs, x, konv_rad) in one
  * workgroup (honei/math/conjugate_gradients.hh:588-620, kernel :153-200):
  * x = 0, then CG steps while ||x||^2 grows by more than konv_rad; rows <=

@@ -330,3 +330,43 @@ def test_raw_launcher_empty_range_is_noop(hip, dtype):
             u64(E["ncpr"]), u64(E["stride"]), u64(1), None)
         assert rc == 0
         assert np.all(yd.get() == 5)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_ell_spmv_rows_equals_whole_product(hip, oracle, dtype):
+    """hsolve_ell_spmv_rows on [r0, r1): those rows of the whole product and
+    defect bit for bit, every other entry of y untouched, for even and odd
+    starts and ends, one row, the last row, an eighth of the rows (one
+    rank's share of a row split) and through every kernel variant; an empty
+    range is a no-op and row_end beyond the matrix is refused."""
+    from honei_amd._lib import lib, u64, vp
+    for E in (P.matrix_b_ell(1 << 15, dtype=dtype), P.matrix_b_ell((1 << 15) - 7, dtype=dtype),
+              P.q1_to_ell(P.matrix_a_bands(129, dtype=dtype))):
+        n = E["rows"]
+        x = P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype)
+        rhs = P.rng_uniform(P.SEED, P.S_RHS, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype)
+        ref, ref_d = oracle.ell_product(E, x), oracle.ell_defect(E, rhs, x)
+        M = hip.EllMatrix.from_dict(E)
+        xd, rd = hip.DeviceVector.from_numpy(x), hip.DeviceVector.from_numpy(rhs)
+        eighth = n // 8
+        ranges = ((0, n), (1, n - 1), (3, 1000), (2, 1001), (3 * eighth, 4 * eighth), (7 * eighth, n),
+                  (n - 1, n), (n - 3, n), (17, 18))
+        for variant in (0, 1, 2, 4, 16, 48, 80, 112):
+            M.set_variant(variant)
+            for r0, r1 in ranges:
+                fill = np.full(n, 7, dtype)
+                yd = hip.DeviceVector.from_numpy(fill)
+                M.spmv_rows(xd, yd, r0, r1)
+                got = yd.get()
+                assert np.array_equal(got[r0:r1], ref[r0:r1]), (variant, r0, r1)
+                assert np.all(got[:r0] == 7) and np.all(got[r1:] == 7), (variant, r0, r1)
+                M.spmv_rows(xd, yd, r0, r1, rhs=rd)
+                got = yd.get()
+                assert np.array_equal(got[r0:r1], ref_d[r0:r1]), (variant, r0, r1)
+                assert np.all(got[:r0] == 7) and np.all(got[r1:] == 7), (variant, r0, r1)
+        yd = hip.DeviceVector.from_numpy(np.full(n, 5, dtype))
+        for r0, r1 in ((0, 0), (10, 10), (20, 3)):
+            M.spmv_rows(xd, yd, r0, r1)
+            assert np.all(yd.get() == 5)
+        rc = lib().hsolve_ell_spmv_rows(vp(M.handle), vp(xd.ptr), vp(yd.ptr), None, u64(0), u64(n + 1), None)
+        assert rc == 2 and np.all(yd.get() == 5)
