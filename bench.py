@@ -109,6 +109,31 @@ def spmv_case(hip, P, E, steps, warmup, defect=False, keep_y=None, variant=None)
     return out
 
 
+def shard_case(hip, P, E, steps, warmup, parts=8):
+    """One rank's share of a row-partitioned product on one GPU: y = A x on
+    rows [k n / parts, (k + 1) n / parts) of the ELL handle
+    (hsolve_ell_spmv_rows), each of the `parts` row blocks timed on its
+    own; bytes by the same model on the block's rows and entries."""
+    n = E["rows"]
+    dtype = E["Ax"].dtype
+    s = dtype.itemsize
+    lens = np.asarray(E["Arl"], np.int64) * int(E.get("threads", 1))
+    M = hip.EllMatrix.from_dict(E)
+    x = hip.DeviceVector.from_numpy(
+        P.rng_uniform(P.SEED, P.S_X, np.arange(n, dtype=np.uint64), -1, 1).astype(dtype))
+    y = hip.DeviceVector.zeros(n, dtype)
+    blocks = []
+    for k in range(parts):
+        r0, r1 = k * n // parts, (k + 1) * n // parts
+        ms = timed(lambda: M.spmv_rows(x, y, r0, r1), steps, warmup)
+        nb = P.ell_spmv_bytes(r1 - r0, int(lens[r0:r1].sum()), s)
+        blocks.append(dict(rows=[r0, r1], ms=ms, bytes=nb, GBps=nb / (ms * 1e-3) / 1e9))
+    del M, x, y
+    worst = min(b["GBps"] for b in blocks)
+    return dict(parts=parts, blocks=blocks, ms_max=max(b["ms"] for b in blocks), GBps_min=worst,
+                frac_hbm_min=worst / HBM_PEAK_GBS)
+
+
 def csr_case(hip, P, E, steps, warmup, defect=False, variant=0):
     """Product / Defect of the same matrix stored as SparseMatrixCSR
     (blocksize 1) through a CSR handle: variant 0 lets the handle use its
@@ -516,6 +541,8 @@ def main():
     ap.add_argument("--no-pcg", action="store_true", help="with --diffusion-only: MG-IR leg only")
     ap.add_argument("--cg257-only", action="store_true",
                     help="plain FP64 CG on the 257x257 Q1 Poisson problem (banded and ELL) with the oracle beside it")
+    ap.add_argument("--shard-only", action="store_true",
+                    help="the row-block legs only: Matrix A and Matrix B products on each eighth of the rows")
     ap.add_argument("--matb-only", action="store_true",
                     help="ELL product / defect on Matrix B (FP64, FP32) only")
     ap.add_argument("--csr-only", action="store_true",
@@ -527,6 +554,18 @@ def main():
         from honei_amd import device as hip
         from honei_amd import problems as P
         print(json.dumps({"cg_fp64_poisson_257": cg257_case(hip, P)}))
+        return
+    if args.shard_only:
+        from honei_amd import device as hip
+        from honei_amd import problems as P
+        E = P.q1_to_ell(P.matrix_a_bands(args.side, dtype=np.float64))
+        out = {"ell_spmv_matrix_a_f64": spmv_case(hip, P, E, args.steps, args.warmup),
+               "ell_spmv_matrix_a_f64_row_blocks_of_8": shard_case(hip, P, E, args.steps, args.warmup)}
+        del E
+        EB = P.matrix_b_ell(1 << 24)
+        out["ell_spmv_matrix_b_f64"] = spmv_case(hip, P, EB, args.steps, args.warmup)
+        out["ell_spmv_matrix_b_f64_row_blocks_of_8"] = shard_case(hip, P, EB, args.steps, args.warmup)
+        print(json.dumps(out))
         return
     if args.matb_only:
         from honei_amd import device as hip
@@ -622,6 +661,7 @@ def main():
         extra["ell_spmv_matrix_a_f32"] = spmv_case(hip, P, E32, args.steps, args.warmup)
         extra["ell_defect_matrix_a_f32"] = spmv_case(hip, P, E32, args.steps, args.warmup, defect=True)
         del E32
+        extra["ell_spmv_matrix_a_f64_row_blocks_of_8"] = shard_case(hip, P, E64, args.steps, args.warmup)
         extra["csr_spmv_matrix_a_f64"] = csr_case(hip, P, E64, args.steps, args.warmup)
         extra["csr_spmv_matrix_a_f64_csr_kernel"] = csr_case(hip, P, E64, args.steps, args.warmup, variant=1)
         log("matrix A extras done")
@@ -629,6 +669,7 @@ def main():
         EB = P.matrix_b_ell(1 << 24)
         log(f"Matrix B built in {time.perf_counter() - t1:.1f}s")
         extra["ell_spmv_matrix_b_f64"] = spmv_case(hip, P, EB, args.steps, args.warmup)
+        extra["ell_spmv_matrix_b_f64_row_blocks_of_8"] = shard_case(hip, P, EB, args.steps, args.warmup)
         extra["csr_spmv_matrix_b_f64"] = csr_case(hip, P, EB, args.steps, args.warmup)
         extra["csr_spmv_matrix_b_f64_csr_kernel"] = csr_case(hip, P, EB, args.steps, args.warmup, variant=1)
         EB["Ax"] = EB["Ax"].astype(np.float32)
@@ -690,6 +731,12 @@ def main():
             "unit": "GB/s",
             "frac": round(head["GBps"] / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            # measured HBM bytes per launch over the launch time: the physical
+            # rate (the stencil-form kernel reads no column indices of full rows)
+            "frac_traffic": (round(traffic / (head["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
+            # the general-ELL case (Matrix B, 2^24 rows) on the same byte model
+            "frac_matrix_b": (round(extra["ell_spmv_matrix_b_f64"]["frac_hbm"], 4)
+                              if "ell_spmv_matrix_b_f64" in extra else None),
         },
         "cpu_baseline": cpu,
         "extra": extra,

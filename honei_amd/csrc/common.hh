@@ -70,6 +70,18 @@ namespace hsolve
     // compiler falls back to waiting for all outstanding loads.
     template <typename T> using gptr = __attribute__((address_space(1))) T *;
     template <typename T> __device__ __forceinline__ gptr<T> as_global(T * p) { return (gptr<T>)p; }
+    // base[i] and base[i] = v through a 32-bit byte offset (i * sizeof(T) <
+    // 2^32): the address is a wave-uniform base plus a 32-bit per-lane
+    // offset, which the load and store instructions take as they are, with
+    // no 64-bit address arithmetic per lane
+    template <typename T> __device__ __forceinline__ T ld32(gptr<const T> base, uint32_t i)
+    {
+        return *(gptr<const T>)((gptr<const char>)base + (uint32_t)(i * (uint32_t)sizeof(T)));
+    }
+    template <typename T> __device__ __forceinline__ void st32(gptr<T> base, uint32_t i, T v)
+    {
+        *(gptr<T>)((gptr<char>)base + (uint32_t)(i * (uint32_t)sizeof(T))) = v;
+    }
 
     inline unsigned grid_for(uint64_t work, unsigned per_block)
     {

@@ -132,21 +132,27 @@ namespace
         // stride leaves a padding row or the row count is even (a range that
         // ends before the last row reads a row of the matrix); pairs start on
         // even rows
-        const bool r2_ok = m->threads == 1 && ! (v & HSOLVE_ELL_ONE_ROW) && (r0 % 2 == 0) &&
-                           (r1 < m->rows || m->stride > m->rows || m->rows % 2 == 0);
+        const bool pairs_ok = m->threads == 1 && ! (v & HSOLVE_ELL_ONE_ROW);
+        const bool r2_ok = pairs_ok && (r0 % 2 == 0) && (r1 < m->rows || m->stride > m->rows || m->rows % 2 == 0);
+        // the LDS-window kernel reads the slot pairs of whole blocks: the
+        // last pair of the matrix needs a padding row or an even row count
+        const bool lds_pairs_ok = pairs_ok && (m->stride > m->rows || m->rows % 2 == 0);
         with_ell_types(m, [&](auto Aj, auto Arl) {
             using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
             using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
             const size_t lds = lds_window_bytes(m, sizeof(DT));
-            if (whole && r2_ok && (v & HSOLVE_ELL_LDS) && lds && m->cols != 0 && m->ncpr >= 1 && m->ncpr <= 16)
+            if (lds_pairs_ok && (v & HSOLVE_ELL_LDS) && lds && m->cols != 0 && m->ncpr >= 1 && m->ncpr <= 16)
             {
-                const unsigned g = grid_for(m->rows, k::lds_rows<DT>());
+                // a row range takes the blocks that hold its rows
+                const uint64_t R = (uint64_t)k::lds_rows<DT>();
+                const uint64_t b0 = r0 / R;
+                const unsigned g = (unsigned)((r1 + R - 1) / R - b0);
                 const bool sorted = (v & HSOLVE_ELL_SORT) && m->sAx;
                 const IT * sAj = (const IT *)m->sAj;
                 const LT * sArl = (const LT *)m->sArl;
                 const DT * sAx = (const DT *)m->sAx;
                 const uint16_t * perm = (const uint16_t *)m->sperm;
-                if (v & HSOLVE_ELL_PERSIST)
+                if ((v & HSOLVE_ELL_PERSIST) && whole)
                 {
                     // a fixed grid: as many workgroups as fit on the device at once
                     const uint32_t nblk = (uint32_t)g;
@@ -176,11 +182,11 @@ namespace
                 if (sorted)                                                                                   \
                     hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE, true>), dim3(g), dim3(k::lds_rows<DT>() / 2), lds, \
                                        st, x, y, sAj, sAx, sArl, rhs, dinv, m->rows, m->cols, m->stride,        \
-                                       (uint32_t)m->ncpr, m->max_offset, perm);                               \
+                                       (uint32_t)m->ncpr, m->max_offset, perm, b0, r0, r1);                   \
                 else                                                                                          \
                     hipLaunchKernelGGL((k::ell_spmv_lds<DT, IT, LT, K, MODE, false>), dim3(g), dim3(k::lds_rows<DT>() / 2), lds, \
                                        st, x, y, Aj, Ax, Arl, rhs, dinv, m->rows, m->cols, m->stride,          \
-                                       (uint32_t)m->ncpr, m->max_offset, (const uint16_t *)nullptr)
+                                       (uint32_t)m->ncpr, m->max_offset, (const uint16_t *)nullptr, b0, r0, r1)
                 if (m->ncpr <= 9)
                 {
                     HS_L(9);

@@ -263,7 +263,7 @@ namespace k
             const DT * __restrict__ x, DT * __restrict__ y, const IT * __restrict__ Aj,
             const DT * __restrict__ Ax, const LT * __restrict__ Arl, const DT * __restrict__ rhs,
             const DT * __restrict__ dinv, uint64_t rows, uint64_t cols, uint64_t stride, uint32_t ncpr,
-            uint64_t maxoff, const uint16_t * __restrict__ perm)
+            uint64_t maxoff, const uint16_t * __restrict__ perm, uint64_t blk0, uint64_t row_lo, uint64_t row_hi)
     {
         constexpr int kLdsRows = lds_rows<DT>();
         constexpr int kLdsThreads = kLdsRows / 2;
@@ -271,7 +271,9 @@ namespace k
         DT * xw = reinterpret_cast<DT *>(lds_raw);
         typedef typename Vec2<DT>::type V;
         typedef typename Vec2<IT>::type VI;
-        const uint64_t rb0 = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kLdsRows;
+        // blocks blk0 .. blk0 + gridDim.x - 1; rows outside [row_lo, row_hi)
+        // are computed with their block but not written
+        const uint64_t rb0 = (blk0 + (uint64_t)xcd_block(blockIdx.x, gridDim.x)) * kLdsRows;
         const uint64_t w0 = rb0 > maxoff ? rb0 - maxoff : 0;
         const uint64_t wend = rb0 + kLdsRows + maxoff;
         const uint64_t w1 = wend < cols ? wend : cols;
@@ -360,23 +362,30 @@ namespace k
             if ((uint32_t)k < len1)
                 s1 += a1[k] * xw[col_of<IT>(c1[k], o1) - w0];
         }
+        const bool k0 = o0 >= row_lo && o0 < row_hi;
+        const bool k1 = two && o1 >= row_lo && o1 < row_hi;
         if (MODE == 0)
         {
-            y[o0] = s0;
-            if (two)
+            if (k0)
+                y[o0] = s0;
+            if (k1)
                 y[o1] = s1;
         }
         else if (MODE == 1)
         {
-            y[o0] = rhs[o0] - s0;
-            if (two)
+            if (k0)
+                y[o0] = rhs[o0] - s0;
+            if (k1)
                 y[o1] = rhs[o1] - s1;
         }
         else
         {
-            const DT t0 = rhs[o0] - s0;
-            y[o0] = x[o0] + t0 * dinv[o0];
-            if (two)
+            if (k0)
+            {
+                const DT t0 = rhs[o0] - s0;
+                y[o0] = x[o0] + t0 * dinv[o0];
+            }
+            if (k1)
             {
                 const DT t1 = rhs[o1] - s1;
                 y[o1] = x[o1] + t1 * dinv[o1];

@@ -40,67 +40,90 @@ namespace k
     // division (the transfer kernels run on a 2-D grid of rows x columns).
     __device__ __forceinline__ bool dirichlet_rs(int r, int s, int M, const Mask & mk)
     {
+        // corners first (0: r = 0, s = 0; 1: r = 0, s = M-1; 2: r = M-1, s =
+        // M-1; 3: r = M-1, s = 0), then the edges r = 0, s = M-1, r = M-1,
+        // s = 0 in that priority; as selects, so lanes do not diverge
         const bool r0 = r == 0, rM = r == M - 1, s0 = s == 0, sM = s == M - 1;
-        if (r0 && s0) return mk.dir[0];
-        if (r0 && sM) return mk.dir[1];
-        if (rM && sM) return mk.dir[2];
-        if (rM && s0) return mk.dir[3];
-        if (r0) return mk.dir[4];
-        if (sM) return mk.dir[5];
-        if (rM) return mk.dir[6];
-        if (s0) return mk.dir[7];
-        return false;
+        const bool corner = (r0 || rM) && (s0 || sM);
+        const bool c = r0 ? (s0 ? mk.dir[0] : mk.dir[1]) : (sM ? mk.dir[2] : mk.dir[3]);
+        const bool e = r0 ? mk.dir[4] : sM ? mk.dir[5] : rM ? mk.dir[6] : s0 ? mk.dir[7] : false;
+        return corner ? c : e;
     }
 
+    // (DT)(acc + w v) evaluated in double, as the reference's transfers do
+    // (a DT value times a double weight, added to a DT accumulator).  Every
+    // caller's weight is 0.5 or 0.25.
     template <typename DT>
     __device__ __forceinline__ DT add_d(DT acc, double w, DT v)
     {
         return (DT)((double)acc + w * (double)v);
     }
 
+    // For float the same value in one fused multiply-add.  With w a power of
+    // two, w v is exact in double, so the double sum is acc + w v rounded
+    // once to 53 bits and then to 24.  If the addends' exponents differ by at
+    // most 29 the sum fits 53 bits and that first rounding is exact; if they
+    // differ by more, the smaller addend lies below the bit under acc's last
+    // float bit, so the 53-bit result is no float rounding midpoint.  Either
+    // way the two roundings give what rounding the exact sum once to float
+    // gives, which is what fma(w, v, acc) returns (signed zeros and
+    // overflow included).
+    template <>
+    __device__ __forceinline__ float add_d<float>(float acc, double w, float v)
+    {
+        return __builtin_fmaf((float)w, v, acc);
+    }
+
     // Restriction<Tag, NONE> at coarse node (r, s) of a side-Nc grid, with
     // F(dr, ds) the fine value at (2r + dr, 2s + ds).  The reference visits
     // the node in its outer loop iterations i = r, r+1, s, s+1 (those in
-    // [1, Nc-1], ascending, each once) and, inside one iteration, in the
-    // statement order (a)..(h) below; F is called only for fine nodes those
-    // statements read.
+    // [1, Nc-1], ascending, each distinct value once) and, inside one
+    // iteration, runs the statements
+    //   (a) r == i:     0.5 F(-1, 0)      (e) r == i-1, s <= Nc-2: 0.25 F(1, 1)
+    //   (b) r == i-1:   0.5 F(1, 0)       (f) r == i-1, s >= 1:    0.25 F(1, -1)
+    //   (c) s == i-1:   0.5 F(0, 1)       (g) r == i, s <= Nc-2:   0.25 F(-1, 1)
+    //   (d) s == i:     0.5 F(0, -1)      (h) r == i, s >= 1:      0.25 F(-1, -1)
+    // in that order.  Each statement's condition fixes its iteration (i = r
+    // for a, g, h; r + 1 for b, e, f; s + 1 for c; s for d), so a statement
+    // runs iff that i lies in [1, Nc-1] and its own test holds, and the
+    // sequence is the statements sorted by (i, letter): a g h b e f with d
+    // and c placed by how s compares to r:
+    //   s < r-1:  d c a g h b e f        s = r+1:  a g h b d e f c
+    //   s = r-1:  d a c g h b e f        s > r+1:  a g h b e f d c
+    //   s = r:    a d g h b c e f
+    // Below, twelve fixed slots in that order, each added when it applies
+    // (a select, no branch), cover all five cases.
     template <typename DT, typename FV>
-    __device__ __forceinline__ DT restrict_node(FV && F, int64_t r, int64_t s, int64_t Nc, const Mask & mk)
+    __device__ __forceinline__ DT restrict_node(FV && F, int r, int s, int Nc, const Mask & mk)
     {
+        const DT fa = F(-1, 0), fb = F(1, 0), fc = F(0, 1), fd = F(0, -1);
+        const DT fe = F(1, 1), ff = F(1, -1), fg = F(-1, 1), fh = F(-1, -1);
+        const bool r_lo = r >= 1, r_hi = r <= Nc - 2, s_lo = s >= 1, s_hi = s <= Nc - 2;
         DT v = F(0, 0);
-        int64_t c[4] = {r, r + 1, s, s + 1};
-        // sort 4 values (r < r+1 and s < s+1 already)
-        if (c[2] < c[0]) { int64_t t = c[0]; c[0] = c[2]; c[2] = t; }
-        if (c[3] < c[1]) { int64_t t = c[1]; c[1] = c[3]; c[3] = t; }
-        if (c[1] < c[0]) { int64_t t = c[0]; c[0] = c[1]; c[1] = t; }
-        if (c[3] < c[2]) { int64_t t = c[2]; c[2] = c[3]; c[3] = t; }
-        if (c[2] < c[1]) { int64_t t = c[1]; c[1] = c[2]; c[2] = t; }
-        int64_t prev = -1;
-        for (int q = 0; q < 4; ++q)
-        {
-            const int64_t i = c[q];
-            if (i == prev)
-                continue;
-            prev = i;
-            if (i < 1 || i > Nc - 1)
-                continue;
-            if (r == i)     v = add_d(v, 0.5, F(-1, 0));                        // (a)
-            if (r == i - 1) v = add_d(v, 0.5, F(1, 0));                         // (b)
-            if (s == i - 1) v = add_d(v, 0.5, F(0, 1));                         // (c)
-            if (s == i)     v = add_d(v, 0.5, F(0, -1));                        // (d)
-            if (r == i - 1 && s <= Nc - 2) v = add_d(v, 0.25, F(1, 1));         // (e)
-            if (r == i - 1 && s >= 1)      v = add_d(v, 0.25, F(1, -1));        // (f)
-            if (r == i && s <= Nc - 2)     v = add_d(v, 0.25, F(-1, 1));        // (g)
-            if (r == i && s >= 1)          v = add_d(v, 0.25, F(-1, -1));       // (h)
-        }
-        if (dirichlet_rs((int)r, (int)s, (int)Nc, mk))
+        auto put = [&](bool on, double w, DT f) {
+            const DT t = add_d(v, w, f);
+            v = on ? t : v;
+        };
+        put(s < r && s_lo, 0.5, fd);                                             // d      (s < r)
+        put(s < r - 1 && s_hi, 0.5, fc);                                         // c      (s < r-1)
+        put(r_lo, 0.5, fa);                                                      // a
+        put((s == r - 1 && s_hi) || (s == r && s_lo), 0.5, s == r - 1 ? fc : fd); // c | d  (s = r-1 | r)
+        put(r_lo && s_hi, 0.25, fg);                                             // g
+        put(r_lo && s_lo, 0.25, fh);                                             // h
+        put(r_hi, 0.5, fb);                                                      // b
+        put((s == r && s_hi) || (s == r + 1 && s_lo), 0.5, s == r ? fc : fd);     // c | d  (s = r | r+1)
+        put(r_hi && s_hi, 0.25, fe);                                             // e
+        put(r_hi && s_lo, 0.25, ff);                                             // f
+        put((s > r + 1 && s_lo) || (s == r + 1 && s_hi), 0.5, s > r + 1 ? fd : fc); // d | c  (s > r+1 | r+1)
+        put(s > r + 1 && s_hi, 0.5, fc);                                         // c      (s > r+1)
+        if (dirichlet_rs(r, s, Nc, mk))
             v = DT(0);
         return v;
     }
 
     // Restriction<Tag, NONE>: one lane per coarse node (r, s).  The nine
     // fine values around (2r, 2s) are loaded up front (indices clamped to the
-    // grid; the clamped ones are never among those restrict_node reads).
+    // grid; the clamped ones are never among those restrict_node adds).
     template <typename DT>
     __device__ __forceinline__ void restriction_body(DT * __restrict__ coarse, const DT * __restrict__ fine,
                                                      int64_t Nc, const Mask & mk, int64_t r, int64_t s)
@@ -120,7 +143,8 @@ namespace k
                 fv[dr][ds] = fine[row * Nf + col];
             }
         }
-        coarse[r * Nc + s] = restrict_node<DT>([&](int dr, int ds) { return fv[dr + 1][ds + 1]; }, r, s, Nc, mk);
+        coarse[r * Nc + s] = restrict_node<DT>([&](int dr, int ds) { return fv[dr + 1][ds + 1]; }, (int)r, (int)s,
+                                               (int)Nc, mk);
     }
 
     // coarse row blockIdx.y, columns along x
@@ -252,29 +276,16 @@ namespace k
     template <typename DT>
     __device__ __forceinline__ DT prolongation_node_v(const DT (&c)[4], int Nf, const Mask & mk, int R, int S)
     {
-        DT v;
-        if ((R & 1) == 0)
-        {
-            if ((S & 1) == 0)
-                v = c[0];
-            else
-            {
-                v = add_d(DT(0), 0.5, c[0]);
-                v = add_d(v, 0.5, c[1]);
-            }
-        }
-        else if ((S & 1) == 0)
-        {
-            v = add_d(DT(0), 0.5, c[0]);
-            v = add_d(v, 0.5, c[2]);
-        }
-        else
-        {
-            v = add_d(DT(0), 0.25, c[0]);
-            v = add_d(v, 0.25, c[1]);
-            v = add_d(v, 0.25, c[2]);
-            v = add_d(v, 0.25, c[3]);
-        }
+        // the cases of prolongation_node as one chain with selects (lanes of
+        // both column parities share a wave): the weight is 0.5 with one
+        // odd coordinate and 0.25 with two, the second term is c[1] for an
+        // odd column and c[2] for an odd row
+        const bool ro = (R & 1) != 0, so = (S & 1) != 0;
+        const DT w = (ro && so) ? DT(0.25) : DT(0.5);
+        const DT v1 = add_d(DT(0), (double)w, c[0]);
+        const DT v2 = add_d(v1, (double)w, so ? c[1] : c[2]);
+        const DT v4 = add_d(add_d(v2, (double)w, c[2]), (double)w, c[3]);
+        DT v = (ro && so) ? v4 : ((ro || so) ? v2 : c[0]);
         if (dirichlet_rs(R, S, Nf, mk))
             v = DT(0);
         return v;

@@ -107,10 +107,11 @@ namespace
     // grid side) or an ELL matrix (E), chosen by the kernels' MK argument.
     struct LevelArgs
     {
-        k::Bands B;
+        k::Bands B;          // B.b[k] = B.b[0] + k * bs
         EllMat32 E;
         const float * dinv;
         int64_t n, m;
+        int64_t bs;          // band stride in elements
     };
 
     // Matrix kinds of the level kernels: MK 0 Q1 bands (Product<Q1>'s sum
@@ -538,7 +539,8 @@ namespace
         }
     }
 
-    constexpr int kPostW = 250;
+    constexpr int kPostLanes = 256;
+    constexpr int kPostW = kPostLanes - 6;
 
     // Row i of A x from three rows of x in LDS (up, mid, dn: rows r - 1, r,
     // r + 1 of the tile's column span, j the lane's column), b[0..8] the row's
@@ -622,7 +624,7 @@ namespace
             return it->second;
         int cus = 0, per_cu = 0;
         HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kPostLanes, 0));
         const unsigned r = (unsigned)std::max(1, cus * std::max(1, per_cu));
         cache.emplace(std::make_pair(dev, fn), r);
         return r;
@@ -636,7 +638,7 @@ namespace
     // neighbours i +- 1, i +- m, i +- m +- 1 with the same index guards.
     //
     // A tile is kPostW consecutive columns of th grid rows (the last tiles of
-    // a level are narrower / shorter).  Its 256 lanes each own one column
+    // a level are narrower / shorter).  Its kPostLanes lanes each own one column
     // offset c = c0 - 3 + lane of every row; a row's linear span [R m + c0 -
     // 3, R m + c0 + W + 3) runs past the grid row's ends into the
     // neighbouring rows, exactly as the linear index does.  The tile marches
@@ -658,22 +660,24 @@ namespace
     // the fixed correction factor), formed per loaded node, so the corrected
     // x is never stored.
     template <int MK, int DEFECT, int PROL>
-    __global__ __launch_bounds__(256) void v_post2_kernel(LevelArgs a, const Slot * T, int L, int fx, int fout,
+    __global__ __launch_bounds__(kPostLanes) void v_post2_kernel(LevelArgs a, const Slot * T, int L, int fx, int fout,
                                                           int fcoarse, float alpha, k::Mask mk, int th)
     {
         const VB vb = vblock();
         HS_GATE_RUN
-        __shared__ float xr[3][256];
-        __shared__ float x1r[3][256];
-        __shared__ float x2r[DEFECT ? 3 : 1][256];
+        __shared__ float xr[3][kPostLanes];
+        __shared__ float x1r[3][kPostLanes];
+        __shared__ float x2r[DEFECT ? 3 : 1][kPostLanes];
         const int n = (int)a.n, m = (int)a.m;
-        const int nrb = (m + th - 1) / th;
-        // tiles in column strips, row blocks fastest: vertical neighbours,
-        // which share the most halo, are consecutive and (k::xcd_block) on
-        // one XCD
+        // tiles in row blocks, column strips fastest: horizontal neighbours
+        // march through the same rows at the same time on one XCD
+        // (k::xcd_block), so the cache lines they share (a row chunk's first
+        // and last, which the grid's odd side leaves unaligned) come from HBM
+        // once
         const unsigned t = gridDim.z == 1 ? k::xcd_block(vb.x, gridDim.x) : vb.x;
-        const int c0 = (int)(t / nrb) * kPostW;
-        const int R0 = (int)(t % nrb) * th;
+        const int nst = (m + kPostW - 1) / kPostW;
+        const int c0 = (int)(t % nst) * kPostW;
+        const int R0 = (int)(t / nst) * th;
         const int W = min(kPostW, m - c0);
         const int H = min(th, m - R0);
         const int j = threadIdx.x;
@@ -688,10 +692,10 @@ namespace
         const gptr<float> out = as_global<float>(sl.v[L][fout]);
         const gptr<float> dout = as_global<float>(sl.v[L][F_D]);
         const gptr<const float> xc = as_global<const float>(PROL ? sl.v[L - 1][fcoarse] : sl.v[L][fx]);
-        gptr<const float> bp[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-            bp[k] = as_global<const float>((const float *)a.B.b[k]);
+        // the nine bands from one base: band k of node i at element i + k bs
+        // (9 bs 4 < 2^32, checked on the host)
+        const gptr<const float> bands = as_global<const float>((const float *)a.B.b[0]);
+        const uint32_t bs = (uint32_t)a.bs;
         const int Nc = (m + 1) / 2;
 
         // grid position (R, S) of the node R0 - 3 + s rows down at column
@@ -718,24 +722,24 @@ namespace
         // between the loads; step s decides which values count.
         auto load = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
             const int ix = (R0 - 3 + s) * m + c;
-            xq = x[min(max(ix, 0), n - 1)];
+            xq = ld32(x, (uint32_t)min(max(ix, 0), n - 1));
             if (PROL)
             {
                 int R, S;
                 grid_pos(s, R, S);
                 const int ra = (R >> 1) * Nc, rb = ((R + 1) >> 1) * Nc;
                 const int jj = S >> 1, jj1 = min(jj + 1, Nc - 1);
-                cq[0] = xc[ra + jj];
-                cq[1] = xc[ra + jj1];
-                cq[2] = xc[rb + jj];
-                cq[3] = xc[rb + jj1];
+                cq[0] = ld32(xc, (uint32_t)(ra + jj));
+                cq[1] = ld32(xc, (uint32_t)(ra + jj1));
+                cq[2] = ld32(xc, (uint32_t)(rb + jj));
+                cq[3] = ld32(xc, (uint32_t)(rb + jj1));
             }
-            const int ib = min(max(ix - m, 0), n - 1);
+            const uint32_t ib = (uint32_t)min(max(ix - m, 0), n - 1);
 #pragma unroll
             for (int k = 0; k < 9; ++k)
-                bq[k] = bp[k][ib];
-            bq[9] = dinv[ib];
-            bq[10] = rhs[ib];
+                bq[k] = ld32(bands, ib + (uint32_t)k * bs);
+            bq[9] = ld32(dinv, ib);
+            bq[10] = ld32(rhs, ib);
         };
         // H + 6 steps, rounded up to an even count so that the loop body is
         // two whole steps with no branch around their loads (the extra step's
@@ -804,7 +808,7 @@ namespace
                         const float tt = b2[10] - sum;
                         v = x1r[prv][j] + tt * b2[9];
                         if (own && r2 >= R0 && r2 < R0 + H)
-                            out[i] = v;
+                            st32(out, (uint32_t)i, v);
                     }
                     if (DEFECT)
                         x2r[DEFECT ? cur : 0][j] = v;
@@ -818,7 +822,7 @@ namespace
                     if (s >= 6 && own && r3 < R0 + H && (! G || (i >= 0 && i < n)))
                     {
                         const float sum = post_row<MK, G>(b3, x2r[old], x2r[prv], x2r[DEFECT ? cur : 0], j, i, n, m);
-                        dout[i] = b3[10] - sum;
+                        st32(dout, (uint32_t)i, b3[10] - sum);
                     }
                 }
             };
@@ -855,19 +859,19 @@ namespace
     // linear index's; the restriction reads grid columns 2s - 1 .. 2s + 1 of
     // owned columns 2s, which all lie in that span.
     template <int MK, int ACC>
-    __global__ __launch_bounds__(256) void v_pre2r_kernel(LevelArgs a, const Slot * T, int L, int fd, int facc,
+    __global__ __launch_bounds__(kPostLanes) void v_pre2r_kernel(LevelArgs a, const Slot * T, int L, int fd, int facc,
                                                           int64_t Nc64, k::Mask mk, int th)
     {
         const VB vb = vblock();
         HS_GATE_RUN
-        __shared__ float c1r[3][256];
-        __shared__ float xr[3][256];
-        __shared__ float dr[3][256];
+        __shared__ float c1r[3][kPostLanes];
+        __shared__ float xr[3][kPostLanes];
+        __shared__ float dr[3][kPostLanes];
         const int n = (int)a.n, m = (int)a.m, Nc = (int)Nc64;
-        const int nrb = (m + th - 1) / th;
         const unsigned t = gridDim.z == 1 ? k::xcd_block(vb.x, gridDim.x) : vb.x;
-        const int c0 = (int)(t / nrb) * kPostW;
-        const int R0 = (int)(t % nrb) * th;
+        const int nst = (m + kPostW - 1) / kPostW;
+        const int c0 = (int)(t % nst) * kPostW;
+        const int R0 = (int)(t / nst) * th;
         const int W = min(kPostW, m - c0);
         const int H = min(th, m - R0);
         const int j = threadIdx.x;
@@ -883,23 +887,22 @@ namespace
         const gptr<const float> acc = as_global<const float>(sl.v[L][facc]);
         const gptr<float> xo = as_global<float>(sl.v[L][F_X]);
         const gptr<float> co = as_global<float>(sl.v[L - 1][F_RHS]);
-        gptr<const float> bp[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-            bp[k] = as_global<const float>((const float *)a.B.b[k]);
+        // bands as in v_post2_kernel
+        const gptr<const float> bands = as_global<const float>((const float *)a.B.b[0]);
+        const uint32_t bs = (uint32_t)a.bs;
 
         // row R0 - 3 + s for step s: 9 bands, dinv, d, rhs, facc value; every
         // lane loads, at an index clamped to the arrays, with no branch
         // between the loads; step s decides which values count
         auto load = [&](int s, float (&rn)[13]) {
-            const int i = min(max((R0 - 3 + s) * m + c, 0), n - 1);
+            const uint32_t i = (uint32_t)min(max((R0 - 3 + s) * m + c, 0), n - 1);
 #pragma unroll
             for (int k = 0; k < 9; ++k)
-                rn[k] = bp[k][i];
-            rn[9] = dinv[i];
-            rn[10] = d[i];
-            rn[11] = rhs[i];
-            rn[12] = ACC ? acc[i] : 0.0f;
+                rn[k] = ld32(bands, i + (uint32_t)k * bs);
+            rn[9] = ld32(dinv, i);
+            rn[10] = ld32(d, i);
+            rn[11] = ld32(rhs, i);
+            rn[12] = ACC ? ld32(acc, i) : 0.0f;
         };
         // H + 6 steps, rounded up to an even count (as v_post2_kernel)
         const int steps = (H + 7) & ~1;
@@ -939,7 +942,7 @@ namespace
                         const float c2 = c1r[prv][j] + tt * ra[9];
                         v = ACC ? ra[12] + c2 : c2;
                         if (own && rho >= R0 && rho < R0 + H)
-                            xo[i] = v;
+                            st32(xo, (uint32_t)i, v);
                     }
                     xr[cur][j] = v;
                 }
@@ -964,12 +967,12 @@ namespace
                         if (own_even && 2 * r >= R0 && 2 * r < R0 + H && c < m)
                         {
                             const int sc = c >> 1;
-                            const float (*ring)[256] = dr;
-                            co[r * Nc + sc] = k::restrict_node<float>(
+                            const float (*ring)[kPostLanes] = dr;
+                            st32(co, (uint32_t)(r * Nc + sc), k::restrict_node<float>(
                                 [&](int ddr, int dds) {
                                     return ring[ddr < 0 ? old : (ddr == 0 ? prv : cur)][j + dds];
                                 },
-                                (int64_t)r, (int64_t)sc, (int64_t)Nc, mk);
+                                r, sc, Nc, mk));
                         }
                     }
                 }
@@ -1602,6 +1605,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         a.dinv = l.dinv;
         a.n = (int64_t)l.n;
         a.m = l.side;
+        a.bs = (int64_t)l.bs;
         return a;
     };
     auto norm_grid = [&](uint64_t nn) { return dim3(std::min<unsigned>(lanes4(nn), kNormBlocks), 1, S); };
@@ -1613,7 +1617,9 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     // finest level's iterate lives in F_TMP between V-cycles (the start
     // writes it there, the outer step and the next cycle's first sweeps
     // read it there).
-    const bool post2 = kBands<MK> && p.n_pre_smooth == 2 && maxl > minl;
+    // (the one-pass kernels address a level's nine bands by 32-bit offsets
+    // from the first: levels up to 8193^2)
+    const bool post2 = kBands<MK> && p.n_pre_smooth == 2 && maxl > minl && 36 * h.lv[maxl].bs < (1ull << 32);
     const int fx_top = post2 ? F_TMP : F_X;
     // Pre-smoothing, defect and restriction in one pass (v_pre2r_kernel)
     // when the transfers are the structured ones and the correction factor
@@ -1689,10 +1695,10 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                     const dim3 tg(post_tiles(a.m, th), 1, S);
                     const int64_t Nc = h.lv[L - 1].side;
                     if (L == maxl)
-                        hipLaunchKernelGGL((v_pre2r_kernel<MK, 1>), tg, blk, 0, st, a, T, (int)L, dsrc, fx_top, Nc,
+                        hipLaunchKernelGGL((v_pre2r_kernel<MK, 1>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, dsrc, fx_top, Nc,
                                            p.mask, th);
                     else
-                        hipLaunchKernelGGL((v_pre2r_kernel<MK, 0>), tg, blk, 0, st, a, T, (int)L, dsrc, (int)F_X, Nc,
+                        hipLaunchKernelGGL((v_pre2r_kernel<MK, 0>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, dsrc, (int)F_X, Nc,
                                            p.mask, th);
                     continue;
                 }
@@ -1776,10 +1782,10 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                 const int th = post_rows(a.m, S, post_resident(fn));
                 const dim3 tg(post_tiles(a.m, th), 1, S);
                 if (L == maxl)
-                    hipLaunchKernelGGL((v_post2_kernel<MK, 1, 1>), tg, blk, 0, st, a, T, (int)L, (int)F_X,
+                    hipLaunchKernelGGL((v_post2_kernel<MK, 1, 1>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X,
                                        (int)F_TMP, fcoarse, p.adapt, p.mask, th);
                 else
-                    hipLaunchKernelGGL((v_post2_kernel<MK, 0, 1>), tg, blk, 0, st, a, T, (int)L, (int)F_X,
+                    hipLaunchKernelGGL((v_post2_kernel<MK, 0, 1>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X,
                                        (int)F_TMP, fcoarse, p.adapt, p.mask, th);
                 continue;
             }
@@ -1815,10 +1821,10 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                 const int th = post_rows(a.m, S, post_resident(fn));
                 const dim3 tg(post_tiles(a.m, th), 1, S);
                 if (L == maxl)
-                    hipLaunchKernelGGL((v_post2_kernel<MK, 1, 0>), tg, blk, 0, st, a, T, (int)L, (int)F_X,
+                    hipLaunchKernelGGL((v_post2_kernel<MK, 1, 0>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X,
                                        (int)F_TMP, fcoarse, 0.0f, p.mask, th);
                 else
-                    hipLaunchKernelGGL((v_post2_kernel<MK, 0, 0>), tg, blk, 0, st, a, T, (int)L, (int)F_X,
+                    hipLaunchKernelGGL((v_post2_kernel<MK, 0, 0>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X,
                                        (int)F_TMP, fcoarse, 0.0f, p.mask, th);
                 continue;
             }
