@@ -642,11 +642,14 @@ namespace
     // offset c = c0 - 3 + lane of every row; a row's linear span [R m + c0 -
     // 3, R m + c0 + W + 3) runs past the grid row's ends into the
     // neighbouring rows, exactly as the linear index does.  The tile marches
-    // down its rows: step s takes x of row R0 - 3 + s and the bands, dinv and
-    // rhs of row R0 - 4 + s (loaded two steps ahead, in registers), then forms
-    // x1 of row R0 - 4 + s, x2 of row R0 - 5 + s and d of row R0 - 6 + s from
-    // three-row rings of x, x1, x2 in LDS.  Each row's bands are read from
-    // HBM once per tile and kept in registers for its three stages.
+    // through its rows, down (even row blocks) or up (odd ones): march
+    // position p is grid row R0 + p or R0 + H - 1 - p.  Step s takes x at
+    // position s - 3 and the bands, dinv and rhs at s - 4 (loaded two steps
+    // ahead, in registers), then forms x1 at position s - 4, x2 at s - 5 and
+    // d at s - 6 from three-row rings of x, x1, x2 in LDS.  Each row's bands
+    // are read from HBM once per tile and kept in registers for its three
+    // stages.  A node's value does not depend on the direction: its sum
+    // reads the rows above and below from the ring either way.
     //
     // x2 goes to field fout, which must differ from fx: neighbouring tiles
     // read x around their own rows while this tile writes.
@@ -680,6 +683,12 @@ namespace
         const int R0 = (int)(t / nst) * th;
         const int W = min(kPostW, m - c0);
         const int H = min(th, m - R0);
+        // even row blocks march down, odd ones up, so the halo rows two
+        // vertical neighbours share are read by both at about the same time
+        // (both at their first steps or both at their last); march position
+        // p is grid row Rs + dir p
+        const int dir = ((t / nst) & 1) ? -1 : 1;
+        const int Rs = dir > 0 ? R0 : R0 + H - 1;
         const int j = threadIdx.x;
         const int c = c0 - 3 + j;
         const bool in_x = j < W + 6;
@@ -698,10 +707,10 @@ namespace
         const uint32_t bs = (uint32_t)a.bs;
         const int Nc = (m + 1) / 2;
 
-        // grid position (R, S) of the node R0 - 3 + s rows down at column
+        // grid position (R, S) of the node at march position s - 3, column
         // offset c (c lies in [-3, 2m)), clamped to the grid
         auto grid_pos = [&](int s, int & R, int & S) {
-            R = R0 - 3 + s;
+            R = Rs + dir * (s - 3);
             S = c;
             if (S < 0)
             {
@@ -716,12 +725,12 @@ namespace
             R = min(max(R, 0), m - 1);
             S = min(max(S, 0), m - 1);
         };
-        // The loads of step s: x of row R0 - 3 + s (with PROL the four coarse
-        // values around it) and the bands, dinv and rhs of row R0 - 4 + s.
-        // Every lane loads, at indices clamped to the arrays, with no branch
-        // between the loads; step s decides which values count.
+        // The loads of step s: x at march position s - 3 (with PROL the four
+        // coarse values around it) and the bands, dinv and rhs at position
+        // s - 4.  Every lane loads, at indices clamped to the arrays, with no
+        // branch between the loads; step s decides which values count.
         auto load = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
-            const int ix = (R0 - 3 + s) * m + c;
+            const int ix = (Rs + dir * (s - 3)) * m + c;
             xq = ld32(x, (uint32_t)min(max(ix, 0), n - 1));
             if (PROL)
             {
@@ -734,7 +743,7 @@ namespace
                 cq[2] = ld32(xc, (uint32_t)(rb + jj));
                 cq[3] = ld32(xc, (uint32_t)(rb + jj1));
             }
-            const uint32_t ib = (uint32_t)min(max(ix - m, 0), n - 1);
+            const uint32_t ib = (uint32_t)min(max(ix - dir * m, 0), n - 1);
 #pragma unroll
             for (int k = 0; k < 9; ++k)
                 bq[k] = ld32(bands, ib + (uint32_t)k * bs);
@@ -745,19 +754,23 @@ namespace
         // two whole steps with no branch around their loads (the extra step's
         // rows lie outside the tile: nothing of it is stored)
         const int steps = (H + 7) & ~1;
-        // A tile whose computed rows (R0 - 2 .. R0 + H + 1) lie in rows 2 .. m
-        // - 3 has every node and every neighbour in the matrix: no index tests
+        // A tile whose rows R0 - 4 .. R0 + H + 3 lie in the grid has every
+        // node the stored values depend on, and every neighbour of those, in
+        // the matrix: no index tests
         const bool interior = R0 >= 4 && R0 + H + 4 <= m;
         auto march = [&](auto interior_tag) {
             constexpr bool G = ! decltype(interior_tag)::value;
-            // stage registers: row R0 - 4 + s (stage 1), R0 - 5 + s (stage 2),
-            // R0 - 6 + s (stage 3)
+            // stage registers: march position s - 4 (stage 1), s - 5 (stage
+            // 2), s - 6 (stage 3)
             float b1[11] = {}, b2[11] = {}, b3[11] = {};
             // one step; (xq, cq, bq) hold its loads and then take those of step s + 2
             auto step = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
-                const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;   // rows r, r - 1, r - 2
+                // ring rows: positions p, p - 1, p - 2; the row above a stage's
+                // row is the older one marching down, the newer one marching up
+                const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;
+                const int above = dir > 0 ? old : cur, below = dir > 0 ? cur : old;
                 {
-                    const int ix = (R0 - 3 + s) * m + c;
+                    const int ix = (Rs + dir * (s - 3)) * m + c;
                     float xv = 0.0f;
                     if (in_x && (! G || (ix >= 0 && ix < n)))
                     {
@@ -770,7 +783,7 @@ namespace
                         }
                     }
                     xr[cur][j] = xv;
-                    const int ib = ix - m;
+                    const int ib = ix - dir * m;
                     const bool vb1 = in_1 && s >= 2 && (! G || (ib >= 0 && ib < n));
 #pragma unroll
                     for (int k = 0; k < 11; ++k)
@@ -782,29 +795,29 @@ namespace
                 }
                 load(s + 2, xq, cq, bq);
                 lds_barrier();
-                // stage 1: x1 of row R0 - 4 + s
+                // stage 1: x1 at position s - 4
                 if (s >= 2)
                 {
                     float v = 0.0f;
-                    const int i = (R0 - 4 + s) * m + c;
+                    const int i = (Rs + dir * (s - 4)) * m + c;
                     if (in_1 && (! G || (i >= 0 && i < n)))
                     {
-                        const float sum = post_row<MK, G>(b1, xr[old], xr[prv], xr[cur], j, i, n, m);
+                        const float sum = post_row<MK, G>(b1, xr[above], xr[prv], xr[below], j, i, n, m);
                         const float tt = b1[10] - sum;
                         v = xr[prv][j] + tt * b1[9];
                     }
                     x1r[cur][j] = v;
                 }
                 lds_barrier();
-                // stage 2: x2 of row R0 - 5 + s
+                // stage 2: x2 at position s - 5
                 if (s >= 4)
                 {
                     float v = 0.0f;
-                    const int r2 = R0 - 5 + s;
+                    const int r2 = Rs + dir * (s - 5);
                     const int i = r2 * m + c;
                     if (in_2 && (! G || (i >= 0 && i < n)))
                     {
-                        const float sum = post_row<MK, G>(b2, x1r[old], x1r[prv], x1r[cur], j, i, n, m);
+                        const float sum = post_row<MK, G>(b2, x1r[above], x1r[prv], x1r[below], j, i, n, m);
                         const float tt = b2[10] - sum;
                         v = x1r[prv][j] + tt * b2[9];
                         if (own && r2 >= R0 && r2 < R0 + H)
@@ -816,12 +829,13 @@ namespace
                 if (DEFECT)
                 {
                     lds_barrier();
-                    // stage 3: d of row R0 - 6 + s
-                    const int r3 = R0 - 6 + s;
+                    // stage 3: d at position s - 6
+                    const int r3 = Rs + dir * (s - 6);
                     const int i = r3 * m + c;
-                    if (s >= 6 && own && r3 < R0 + H && (! G || (i >= 0 && i < n)))
+                    if (s >= 6 && own && r3 >= R0 && r3 < R0 + H && (! G || (i >= 0 && i < n)))
                     {
-                        const float sum = post_row<MK, G>(b3, x2r[old], x2r[prv], x2r[DEFECT ? cur : 0], j, i, n, m);
+                        const float sum = post_row<MK, G>(b3, x2r[DEFECT ? above : 0], x2r[DEFECT ? prv : 0],
+                                                          x2r[DEFECT ? below : 0], j, i, n, m);
                         st32(dout, (uint32_t)i, b3[10] - sum);
                     }
                 }
@@ -849,12 +863,13 @@ namespace
     // the one the separate kernels form (q1_row's order and guards,
     // restrict_node's addition sequence); d' itself is never stored.
     //
-    // Tiles as v_post2_kernel, with an even th, so a tile's first row is even
-    // and it owns the coarse rows r with 2r in its rows.  Step s takes row R0
-    // - 3 + s (d, dinv, rhs, the bands and facc, loaded two steps ahead; c1
-    // is formed at once), forms x of row R0 - 4 + s and d' of row R0 - 5 + s,
-    // and, when that row is 2r + 1, coarse row r from the three d' rows in
-    // LDS (lanes on even fine columns).  The lanes cover fine columns c0 - 3
+    // Tiles and march directions as v_post2_kernel, with an even th, so a
+    // tile's first row is even and it owns the coarse rows r with 2r in its
+    // rows.  Step s takes march position s - 3 (d, dinv, rhs, the bands and
+    // facc, loaded two steps ahead; c1 is formed at once), forms x at s - 4
+    // and d' at s - 5, and, when that d' row is 2r + 1 (marching down) or 2r
+    // - 1 (up), coarse row r from the three d' rows in LDS (lanes on even
+    // fine columns).  The lanes cover fine columns c0 - 3
     // .. c0 + W + 2 in linear index order, so the stencil steps are the
     // linear index's; the restriction reads grid columns 2s - 1 .. 2s + 1 of
     // owned columns 2s, which all lie in that span.
@@ -874,6 +889,9 @@ namespace
         const int R0 = (int)(t / nst) * th;
         const int W = min(kPostW, m - c0);
         const int H = min(th, m - R0);
+        // march direction as in v_post2_kernel: position p is row Rs + dir p
+        const int dir = ((t / nst) & 1) ? -1 : 1;
+        const int Rs = dir > 0 ? R0 : R0 + H - 1;
         const int j = threadIdx.x;
         const int c = c0 - 3 + j;
         const bool in_x = j < W + 6;
@@ -891,11 +909,11 @@ namespace
         const gptr<const float> bands = as_global<const float>((const float *)a.B.b[0]);
         const uint32_t bs = (uint32_t)a.bs;
 
-        // row R0 - 3 + s for step s: 9 bands, dinv, d, rhs, facc value; every
+        // march position s - 3 for step s: 9 bands, dinv, d, rhs, facc value; every
         // lane loads, at an index clamped to the arrays, with no branch
         // between the loads; step s decides which values count
         auto load = [&](int s, float (&rn)[13]) {
-            const uint32_t i = (uint32_t)min(max((R0 - 3 + s) * m + c, 0), n - 1);
+            const uint32_t i = (uint32_t)min(max((Rs + dir * (s - 3)) * m + c, 0), n - 1);
 #pragma unroll
             for (int k = 0; k < 9; ++k)
                 rn[k] = ld32(bands, i + (uint32_t)k * bs);
@@ -906,17 +924,19 @@ namespace
         };
         // H + 6 steps, rounded up to an even count (as v_post2_kernel)
         const int steps = (H + 7) & ~1;
-        // computed rows R0 - 2 .. R0 + H + 1 in rows 2 .. m - 3: no index tests
+        // rows R0 - 4 .. R0 + H + 3 in the grid: no index tests (as v_post2_kernel)
         const bool interior = R0 >= 4 && R0 + H + 4 <= m;
         auto march = [&](auto interior_tag) {
             constexpr bool G = ! decltype(interior_tag)::value;
-            // stage registers: rc row R0 - 3 + s (its c1 only, yet), ra row R0 -
-            // 4 + s (x: bands, dinv, d, facc), rb row R0 - 5 + s (d': bands, rhs)
+            // stage registers: rc position s - 3 (its c1 only, yet), ra position
+            // s - 4 (x: bands, dinv, d, facc), rb position s - 5 (d': bands, rhs)
             float rc[13] = {}, ra[13] = {}, rb[13] = {};
             auto step = [&](int s, float (&rq)[13]) {
-                const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;   // rows r, r - 1, r - 2
+                // ring rows: positions p, p - 1, p - 2 (above / below as in v_post2_kernel)
+                const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;
+                const int above = dir > 0 ? old : cur, below = dir > 0 ? cur : old;
                 {
-                    const int i = (R0 - 3 + s) * m + c;
+                    const int i = (Rs + dir * (s - 3)) * m + c;
                     const bool valid = in_x && (! G || (i >= 0 && i < n));
 #pragma unroll
                     for (int k = 0; k < 13; ++k)
@@ -929,15 +949,15 @@ namespace
                 }
                 load(s + 2, rq);
                 lds_barrier();
-                // x of row R0 - 4 + s
+                // x at position s - 4
                 if (s >= 2)
                 {
                     float v = 0.0f;
-                    const int rho = R0 - 4 + s;
+                    const int rho = Rs + dir * (s - 4);
                     const int i = rho * m + c;
                     if (in_1 && (! G || (i >= 0 && i < n)))
                     {
-                        const float sum = post_row<MK, G>(ra, c1r[old], c1r[prv], c1r[cur], j, i, n, m);
+                        const float sum = post_row<MK, G>(ra, c1r[above], c1r[prv], c1r[below], j, i, n, m);
                         const float tt = ra[10] - sum;
                         const float c2 = c1r[prv][j] + tt * ra[9];
                         v = ACC ? ra[12] + c2 : c2;
@@ -947,30 +967,31 @@ namespace
                     xr[cur][j] = v;
                 }
                 lds_barrier();
-                // d' of row R0 - 5 + s
+                // d' at position s - 5
                 if (s >= 4)
                 {
                     float v = 0.0f;
-                    const int rho = R0 - 5 + s;
+                    const int rho = Rs + dir * (s - 5);
                     const int i = rho * m + c;
                     if (in_2 && (! G || (i >= 0 && i < n)))
                     {
-                        const float sum = post_row<MK, G>(rb, xr[old], xr[prv], xr[cur], j, i, n, m);
+                        const float sum = post_row<MK, G>(rb, xr[above], xr[prv], xr[below], j, i, n, m);
                         v = rb[11] - sum;
                     }
                     dr[cur][j] = v;
                     lds_barrier();
-                    // coarse row r when this d' row is 2r + 1
+                    // coarse row r when this d' row is 2r + dir: rows 2r - 1,
+                    // 2r, 2r + 1 are then in the ring
                     if (((rho - 1) & 1) == 0)
                     {
-                        const int r = (rho - 1) >> 1;
+                        const int r = (rho - dir) >> 1;
                         if (own_even && 2 * r >= R0 && 2 * r < R0 + H && c < m)
                         {
                             const int sc = c >> 1;
                             const float (*ring)[kPostLanes] = dr;
                             st32(co, (uint32_t)(r * Nc + sc), k::restrict_node<float>(
                                 [&](int ddr, int dds) {
-                                    return ring[ddr < 0 ? old : (ddr == 0 ? prv : cur)][j + dds];
+                                    return ring[dir * ddr < 0 ? old : (ddr == 0 ? prv : cur)][j + dds];
                                 },
                                 r, sc, Nc, mk));
                         }
