@@ -1130,6 +1130,46 @@ namespace
                                   &sl.s->cg_iters);
     }
 
+    // v_coarse_kernel with the vectors and (Q1 bands) the level's nine bands
+    // copied to LDS, for coarsest levels up to 17^2: the CG's sequential
+    // sums then wait on LDS, not on a memory round trip per step.  It runs
+    // k::cg_konv_body_op itself on the copies, so every operation and its
+    // order are unchanged and so are the bits.  One wave per slot.
+    constexpr int kCoarseLdsMax = 289;
+    template <int MK>
+    __global__ __launch_bounds__(64) void v_coarse_lds_kernel(LevelArgs a, const Slot * T, int L, int frhs,
+                                                              double konv_rad, uint64_t max_iters)
+    {
+        const VB vb = vblock();
+        HS_GATE_RUN
+        __shared__ float vec[6 * kCoarseLdsMax];                  // rhs, x, then the 4n of work
+        __shared__ float bnd[kBands<MK> ? 9 * kCoarseLdsMax : 1];
+        const int n = (int)a.n;
+        float * rhs = vec;
+        float * x = vec + kCoarseLdsMax;
+        float * work = vec + 2 * kCoarseLdsMax;
+        const float * rg = sl.v[L][frhs];
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            rhs[i] = rg[i];
+        if constexpr (kBands<MK>)
+        {
+            for (int k = 0; k < 9; ++k)
+            {
+                const float * b = (const float *)a.B.b[k];
+                for (int i = threadIdx.x; i < n; i += blockDim.x)
+                    bnd[k * kCoarseLdsMax + i] = b[i];
+                a.B.b[k] = bnd + k * kCoarseLdsMax;
+            }
+        }
+        __syncthreads();
+        LevelRowOp<MK> A{a};
+        k::cg_konv_body_op<float>(A, rhs, x, work, a.n, konv_rad, max_iters, &sl.s->cg_iters);
+        __syncthreads();
+        float * xg = sl.v[L][F_X];
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            xg[i] = x[i];
+    }
+
     // level L's fout (+)= alpha * P (x of level L-1)
     template <bool AXPY>
     __global__ __launch_bounds__(256) void v_prolong_kernel(const Slot * T, int L, int Nf, float alpha, k::Mask mk,
@@ -1775,8 +1815,12 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         // with one level it sees the finest defect d
         const LevelArgs a = largs(minl);
         const int crhs = (minl == maxl) ? F_D : F_RHS;
-        hipLaunchKernelGGL((v_coarse_kernel<MK>), dim3(1, 1, S), blk, 0, st, a, T, (int)minl, crhs,
-                           (double)1.19209290e-07F, (uint64_t)(100 * a.n + 1000));
+        if (a.n <= kCoarseLdsMax)
+            hipLaunchKernelGGL((v_coarse_lds_kernel<MK>), dim3(1, 1, S), dim3(64), 0, st, a, T, (int)minl, crhs,
+                               (double)1.19209290e-07F, (uint64_t)(100 * a.n + 1000));
+        else
+            hipLaunchKernelGGL((v_coarse_kernel<MK>), dim3(1, 1, S), blk, 0, st, a, T, (int)minl, crhs,
+                               (double)1.19209290e-07F, (uint64_t)(100 * a.n + 1000));
         if (minl == maxl)
         {
             defect(minl);
