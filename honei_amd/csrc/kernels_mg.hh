@@ -443,10 +443,72 @@ namespace k
             out[i] = c1;
     }
 
-    // Sequential sum of a[i]*b[i] by one thread, as DotProduct<tags::CPU>.
-    template <typename DT>
+    // The threads that run a one-workgroup CG: NT = 0, the whole workgroup
+    // (barriers); NT = 64, the first wave alone (the others wait elsewhere),
+    // whose "barrier" waits for the wave's own memory operations to finish.
+    template <int NT> struct Team
+    {
+        __device__ static int stride() { return NT ? NT : (int)blockDim.x; }
+        __device__ static bool one_wave() { return NT == 64 || blockDim.x == 64; }
+        __device__ static void sync()
+        {
+            if (NT)
+                asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+            else
+                __syncthreads();
+        }
+    };
+
+    // v as held by lane j of the wave (j uniform)
+    __device__ __forceinline__ float lane_value(float v, int j)
+    {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+    }
+    __device__ __forceinline__ double lane_value(double v, int j)
+    {
+        const long long u = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), j);
+        const int hi = __builtin_amdgcn_readlane((int)(u >> 32), j);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+
+    // Sequential sum of a[i]*b[i], as DotProduct<tags::CPU>: by one thread,
+    // or, on one wave, from the lanes' products (below).
+    template <typename DT, int NT = 0>
     __device__ __forceinline__ void seq_dot(const DT * a, const DT * b, int64_t n, DT * dst)
     {
+        typedef Team<NT> T;
+        if (T::one_wave())
+        {
+            // one wave: the lanes form 64 products at once and the sum adds
+            // them in index order, each read from its lane (the same
+            // products and the same order of additions as the loop below,
+            // without a memory round trip per term)
+            // (terms past n, and lanes past the last term, add +0, which
+            // leaves r unchanged: r starts at +0 and a sum that starts at +0
+            // is never -0, the one value +0 would change)
+            DT r = DT(0);
+            for (int64_t c = 0; c < n; c += 64)
+            {
+                const int64_t i = c + threadIdx.x;
+                const DT p = i < n ? a[i] * b[i] : DT(0);
+                const int cnt = (int)(n - c < 64 ? n - c : 64);
+                for (int j0 = 0; j0 < cnt; j0 += 8)
+                {
+                    DT q[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        q[u] = lane_value(p, j0 + u);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        r += q[u];
+                }
+            }
+            if (threadIdx.x == 0)
+                *dst = r;
+            T::sync();
+            return;
+        }
         if (threadIdx.x == 0)
         {
             DT r = DT(0);
@@ -454,7 +516,7 @@ namespace k
                 r += a[i] * b[i];
             *dst = r;
         }
-        __syncthreads();
+        T::sync();
     }
 
     template <typename DT> struct Eps;
@@ -478,7 +540,7 @@ namespace k
     // A x.  Vectors g, u, energy, new_gradient live in `work` (4n);
     // reductions are the sequential CPU sums, so the iteration is
     // bit-identical to tags::CPU.
-    template <typename DT, typename Op>
+    template <typename DT, typename Op, int NT = 0>
     __device__ __forceinline__ void cg_konv_body_op(const Op & A, const DT * __restrict__ rhs,
             DT * __restrict__ x, DT * __restrict__ work, int64_t n, double konv_rad, uint64_t max_iters,
             uint64_t * iters_out);
@@ -495,65 +557,65 @@ namespace k
     // One step of the "version 1" cg_kernel (conjugate_gradients.hh:108-151
     // Q1, :153-200 ELL) on gradient g, iterate x and direction u, with the
     // work vectors en (energy) and ng (new gradient); sh holds at least 3
-    // scalars.  Called by every thread of the workgroup.
-    template <typename DT, typename Op>
+    // scalars.  Called by every thread of the team (Team<NT>).
+    template <typename DT, typename Op, int NT = 0>
     __device__ __forceinline__ void cg_step_v1(const Op & A, const DT * __restrict__ rhs, DT * __restrict__ g,
             DT * __restrict__ x, DT * __restrict__ u, DT * __restrict__ en, DT * __restrict__ ng, int64_t n, DT * sh)
     {
         const DT eps = Eps<DT>::v;
-        seq_dot<DT>(g, g, n, &sh[0]);
+        seq_dot<DT, NT>(g, g, n, &sh[0]);
         DT upper = sh[0];
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        for (int64_t i = threadIdx.x; i < n; i += Team<NT>::stride())
             en[i] = A.row(u, i);
-        __syncthreads();
-        seq_dot<DT>(en, u, n, &sh[1]);
+        Team<NT>::sync();
+        seq_dot<DT, NT>(en, u, n, &sh[1]);
         DT lower = sh[1];
         const DT f1 = (fabs(lower) >= eps) ? DT(upper / lower) : DT(upper / eps);
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        for (int64_t i = threadIdx.x; i < n; i += Team<NT>::stride())
         {
             const DT t = u[i] * f1;
             en[i] = t + x[i];
         }
-        __syncthreads();
+        Team<NT>::sync();
         lower = upper;
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        for (int64_t i = threadIdx.x; i < n; i += Team<NT>::stride())
         {
             const DT p = A.row(en, i);
             ng[i] = p - rhs[i];
         }
-        __syncthreads();
-        seq_dot<DT>(ng, ng, n, &sh[2]);
+        Team<NT>::sync();
+        seq_dot<DT, NT>(ng, ng, n, &sh[2]);
         upper = sh[2];
         const DT f2 = (fabs(lower) >= eps) ? DT(upper / lower) : DT(upper / eps);
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        for (int64_t i = threadIdx.x; i < n; i += Team<NT>::stride())
         {
             const DT t = u[i] * f2;
             u[i] = t - ng[i];
             g[i] = ng[i];
             x[i] = en[i];
         }
-        __syncthreads();
+        Team<NT>::sync();
     }
 
     // x = 0, g = A x - rhs, u = g * -1 (the set-up of both fixed forms)
-    template <typename DT, typename Op>
+    template <typename DT, typename Op, int NT = 0>
     __device__ __forceinline__ void cg_v1_start(const Op & A, const DT * __restrict__ rhs, DT * __restrict__ g,
             DT * __restrict__ x, DT * __restrict__ u, int64_t n)
     {
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        for (int64_t i = threadIdx.x; i < n; i += Team<NT>::stride())
             x[i] = DT(0);
-        __syncthreads();
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        Team<NT>::sync();
+        for (int64_t i = threadIdx.x; i < n; i += Team<NT>::stride())
         {
             const DT p = A.row(x, i);
             g[i] = p - rhs[i];
         }
-        __syncthreads();
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        Team<NT>::sync();
+        for (int64_t i = threadIdx.x; i < n; i += Team<NT>::stride())
             u[i] = g[i] * DT(-1.);
     }
 
-    template <typename DT, typename Op>
+    template <typename DT, typename Op, int NT>
     __device__ __forceinline__ void cg_konv_body_op(const Op & A, const DT * __restrict__ rhs,
             DT * __restrict__ x, DT * __restrict__ work, int64_t n, double konv_rad, uint64_t max_iters,
             uint64_t * iters_out)
@@ -563,19 +625,19 @@ namespace k
         DT * u = work + n;
         DT * en = work + 2 * n;
         DT * ng = work + 3 * n;
-        cg_v1_start<DT>(A, rhs, g, x, u, n);
-        seq_dot<DT>(x, x, n, &sh[3]);
+        cg_v1_start<DT, Op, NT>(A, rhs, g, x, u, n);
+        seq_dot<DT, NT>(x, x, n, &sh[3]);
         DT norm_x = sh[3];
         DT norm_x_last = norm_x - DT(1);
         uint64_t it = 0;
         while ((double)(norm_x - norm_x_last) > konv_rad && it < max_iters)
         {
-            cg_step_v1<DT>(A, rhs, g, x, u, en, ng, n, sh);
+            cg_step_v1<DT, Op, NT>(A, rhs, g, x, u, en, ng, n, sh);
             ++it;
             norm_x_last = norm_x;
-            seq_dot<DT>(x, x, n, &sh[3]);
+            seq_dot<DT, NT>(x, x, n, &sh[3]);
             norm_x = sh[3];
-            __syncthreads();
+            Team<NT>::sync();
         }
         if (iters_out && threadIdx.x == 0)
             *iters_out = it;

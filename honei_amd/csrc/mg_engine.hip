@@ -1170,6 +1170,213 @@ namespace
             xg[i] = x[i];
     }
 
+    // ------------------------------------------------ the bottom levels in LDS
+    // Levels minl .. top (top <= 5, below the finest) of a V-cycle on Q1
+    // bands, in one workgroup per slot with every vector and band of those
+    // levels in LDS: the same operations, per node, as the one-pass kernels
+    // and the coarse solve run there (two sweeps from zero, defect,
+    // restriction down to minl, the CG on minl, then prolongation with the
+    // correction factor and two sweeps back up to top), one level pass per
+    // barrier.  It reads the rhs of level top (written by the finest-but-
+    // one-pass kernel above it) and leaves top's smoothed iterate in its
+    // F_TMP, where the one-pass post-smoother above reads it.  It replaces
+    // two launches per level and the coarse solve's, each a chain of
+    // dependent memory round trips.  The coarse CG runs on the first wave
+    // alone, whose sequential sums read their terms from its lanes.
+    constexpr int kBottomMax = 5;
+    struct BottomArgs
+    {
+        LevelArgs lv[kBottomMax + 1];
+        int off[kBottomMax + 1];   // float offset of each level's block in the dynamic LDS
+        int minl, top;
+        float alpha;
+        k::Mask mk;
+        double konv_rad;
+        uint64_t max_iters;
+    };
+
+    // Row i of A x on a level held in LDS (band k at b + k n): q1_row's
+    // order and index guards
+    template <int ORD>
+    __device__ __forceinline__ float lds_row(const float * b, const float * x, int i, int n, int m)
+    {
+        if (ORD == 1)
+        {
+            float s = 0.0f;
+            if (i - m - 1 >= 0) s += b[i] * x[i - m - 1];
+            if (i - m >= 0)     s += b[n + i] * x[i - m];
+            if (i - m + 1 >= 0) s += b[2 * n + i] * x[i - m + 1];
+            if (i - 1 >= 0)     s += b[3 * n + i] * x[i - 1];
+            s += b[4 * n + i] * x[i];
+            if (i + 1 < n)      s += b[5 * n + i] * x[i + 1];
+            if (i + m - 1 < n)  s += b[6 * n + i] * x[i + m - 1];
+            if (i + m < n)      s += b[7 * n + i] * x[i + m];
+            if (i + m + 1 < n)  s += b[8 * n + i] * x[i + m + 1];
+            return s;
+        }
+        float s = b[4 * n + i] * x[i];
+        if (i - m - 1 >= 0) s += b[i] * x[i - m - 1];
+        if (i - m >= 0)     s += b[n + i] * x[i - m];
+        if (i - m + 1 >= 0) s += b[2 * n + i] * x[i - m + 1];
+        if (i - 1 >= 0)     s += b[3 * n + i] * x[i - 1];
+        if (i + 1 < n)      s += b[5 * n + i] * x[i + 1];
+        if (i + m - 1 < n)  s += b[6 * n + i] * x[i + m - 1];
+        if (i + m < n)      s += b[7 * n + i] * x[i + m];
+        if (i + m + 1 < n)  s += b[8 * n + i] * x[i + m + 1];
+        return s;
+    }
+
+    // A level held in LDS as the operator of the one-workgroup CG
+    template <int ORD> struct LdsRowOp
+    {
+        const float * b;
+        int n, m;
+        __device__ __forceinline__ float row(const float * __restrict__ x, int64_t i) const
+        {
+            return lds_row<ORD>(b, x, (int)i, n, m);
+        }
+    };
+
+    // Block of level L in LDS: bands (9n), dinv, rhs, x, then tmp (levels
+    // above minl) or the CG's 4n of work (minl).
+    inline int bottom_floats(int n, bool coarsest) { return (coarsest ? 16 : 13) * n; }
+
+    template <int MK>
+    __global__ __launch_bounds__(1024) void v_bottom_kernel(BottomArgs ba, const Slot * T)
+    {
+        const VB vb = vblock();
+        HS_GATE_RUN
+        extern __shared__ float lds[];
+        const int tid = threadIdx.x, nt = blockDim.x;
+        // bands and dinv of every level: each lane issues the loads of two
+        // rows of all ten arrays before it stores any, so a level costs one
+        // or two memory round trips, not one per array and row
+        for (int L = ba.minl; L <= ba.top; ++L)
+        {
+            const int n = (int)ba.lv[L].n;
+            float * b = lds + ba.off[L];
+            const float * src[10];
+            for (int k = 0; k < 9; ++k)
+                src[k] = (const float *)ba.lv[L].B.b[k];
+            src[9] = ba.lv[L].dinv;
+            for (int i0 = tid; i0 < n; i0 += 2 * nt)
+            {
+                float v[2][10];
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                {
+                    const int i = min(i0 + h * nt, n - 1);
+#pragma unroll
+                    for (int k = 0; k < 10; ++k)
+                        v[h][k] = src[k][i];
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                {
+                    const int i = i0 + h * nt;
+                    if (i < n)
+                    {
+#pragma unroll
+                        for (int k = 0; k < 10; ++k)
+                            b[k * n + i] = v[h][k];
+                    }
+                }
+            }
+        }
+        {
+            const int n = (int)ba.lv[ba.top].n;
+            const float * g = sl.v[ba.top][F_RHS];
+            float * rhs = lds + ba.off[ba.top] + 10 * n;
+            for (int i = tid; i < n; i += nt)
+                rhs[i] = g[i];
+        }
+        __syncthreads();
+        // down: x = c1 + dinv (rhs - A c1) with c1 = dinv rhs, d' = rhs - A x,
+        // rhs of L - 1 = R d'
+        for (int L = ba.top; L > ba.minl; --L)
+        {
+            const int n = (int)ba.lv[L].n, m = (int)ba.lv[L].m;
+            const float * b = lds + ba.off[L];
+            const float * dinv = b + 9 * n;
+            const float * rhs = b + 10 * n;
+            float * x = lds + ba.off[L] + 11 * n;
+            float * tmp = lds + ba.off[L] + 12 * n;
+            for (int i = tid; i < n; i += nt)
+                tmp[i] = dinv[i] * rhs[i];
+            __syncthreads();
+            for (int i = tid; i < n; i += nt)
+            {
+                const float t = rhs[i] - lds_row<kOrd<MK>>(b, tmp, i, n, m);
+                x[i] = tmp[i] + t * dinv[i];
+            }
+            __syncthreads();
+            for (int i = tid; i < n; i += nt)
+                tmp[i] = rhs[i] - lds_row<kOrd<MK>>(b, x, i, n, m);
+            __syncthreads();
+            const int Nc = (int)ba.lv[L - 1].m;
+            float * crhs = lds + ba.off[L - 1] + 10 * Nc * Nc;
+            for (int q = tid; q < Nc * Nc; q += nt)
+            {
+                const int r = q / Nc, sc = q - r * Nc;
+                crhs[q] = k::restrict_node<float>(
+                    [&](int dr, int ds) {
+                        const int idx = (2 * r + dr) * m + 2 * sc + ds;
+                        return tmp[min(max(idx, 0), n - 1)];
+                    },
+                    r, sc, Nc, ba.mk);
+            }
+            __syncthreads();
+        }
+        // the coarse solve, as v_coarse_lds_kernel, on the first wave
+        {
+            const int L = ba.minl;
+            const int n = (int)ba.lv[L].n;
+            float * b = lds + ba.off[L];
+            const LdsRowOp<kOrd<MK>> A{b, n, (int)ba.lv[L].m};
+            if (threadIdx.x < 64)
+                k::cg_konv_body_op<float, LdsRowOp<kOrd<MK>>, 64>(A, b + 10 * n, b + 11 * n, b + 12 * n, (int64_t)n,
+                                                                 ba.konv_rad, ba.max_iters, &sl.s->cg_iters);
+            __syncthreads();
+        }
+        // up: x_in = x + alpha P (coarse iterate), then two sweeps into tmp
+        for (int L = ba.minl + 1; L <= ba.top; ++L)
+        {
+            const int n = (int)ba.lv[L].n, m = (int)ba.lv[L].m;
+            const int Nc = (int)ba.lv[L - 1].m;
+            const float * cf = lds + ba.off[L - 1] + (L - 1 == ba.minl ? 11 : 12) * Nc * Nc;
+            const float * b = lds + ba.off[L];
+            const float * dinv = b + 9 * n;
+            const float * rhs = b + 10 * n;
+            float * x = lds + ba.off[L] + 11 * n;
+            float * tmp = lds + ba.off[L] + 12 * n;
+            for (int i = tid; i < n; i += nt)
+            {
+                const int R = i / m, S = i - R * m;
+                tmp[i] = x[i] + ba.alpha * k::prolongation_node<float>(cf, m, ba.mk, R, S);
+            }
+            __syncthreads();
+            for (int i = tid; i < n; i += nt)
+            {
+                const float t = rhs[i] - lds_row<kOrd<MK>>(b, tmp, i, n, m);
+                x[i] = tmp[i] + t * dinv[i];
+            }
+            __syncthreads();
+            for (int i = tid; i < n; i += nt)
+            {
+                const float t = rhs[i] - lds_row<kOrd<MK>>(b, x, i, n, m);
+                tmp[i] = x[i] + t * dinv[i];
+            }
+            __syncthreads();
+        }
+        {
+            const int n = (int)ba.lv[ba.top].n;
+            const float * tmp = lds + ba.off[ba.top] + 12 * n;
+            float * g = sl.v[ba.top][F_TMP];
+            for (int i = tid; i < n; i += nt)
+                g[i] = tmp[i];
+        }
+    }
+
     // level L's fout (+)= alpha * P (x of level L-1)
     template <bool AXPY>
     __global__ __launch_bounds__(256) void v_prolong_kernel(const Slot * T, int L, int Nf, float alpha, k::Mask mk,
@@ -1327,6 +1534,35 @@ namespace
 {
     // The occupancy of every one-pass kernel, looked up at set-up, never
     // under stream capture.
+    // Dynamic LDS v_bottom_kernel may take on the current device (cached per
+    // device): the device's per-workgroup limit, once the kernels are allowed
+    // more than 64 KiB, else 60 KiB (4 KiB of either left for its static LDS)
+    inline unsigned bottom_lds_limit()
+    {
+        static std::mutex mu;
+        static std::map<int, unsigned> cache;
+        int dev = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(dev);
+        if (it != cache.end())
+            return it->second;
+        int v = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        // (4 KiB left for the kernel's static LDS)
+        const int want = v - 4096;
+        unsigned r = (unsigned)std::min(std::max(want, 0), 65536 - 4096);
+        if (want > 65536 &&
+            hipFuncSetAttribute((const void *)v_bottom_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, want) ==
+                hipSuccess &&
+            hipFuncSetAttribute((const void *)v_bottom_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, want) ==
+                hipSuccess)
+            r = (unsigned)want;
+        (void)hipGetLastError();
+        cache.emplace(dev, r);
+        return r;
+    }
+
     void warm_post_kernels()
     {
         const void * fns[] = {
@@ -1338,6 +1574,7 @@ namespace
             (const void *)v_post2_kernel<2, 1, 0>, (const void *)v_post2_kernel<2, 1, 1>};
         for (const void * f : fns)
             (void)post_resident(f);
+        (void)bottom_lds_limit();
     }
 }
 
@@ -1686,6 +1923,43 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     // when the transfers are the structured ones and the correction factor
     // is fixed (the adaptive one reads the level's defect again).
     const bool pre2r = post2 && p.adapt != 0.0f && ! h.prolmat;
+    // With the one-pass kernels, levels minl .. btop (btop <= 5, below the
+    // finest, as many as fit the LDS of one workgroup) run in one
+    // v_bottom_kernel launch (one workgroup per slot); btop = 0: none.
+    int btop = 0;
+    BottomArgs ba{};
+    size_t bbytes = 0;
+    if (pre2r)
+    {
+        for (int top = (int)std::min<uint64_t>(kBottomMax, maxl - 1); top > (int)minl; --top)
+        {
+            size_t fl = 0;
+            for (int L = (int)minl; L <= top; ++L)
+                fl += (size_t)bottom_floats((int)h.lv[L].n, L == (int)minl);
+            if (fl * sizeof(float) <= bottom_lds_limit())
+            {
+                btop = top;
+                bbytes = fl * sizeof(float);
+                break;
+            }
+        }
+        if (btop)
+        {
+            int off = 0;
+            for (int L = (int)minl; L <= btop; ++L)
+            {
+                ba.lv[L] = largs(L);
+                ba.off[L] = off;
+                off += bottom_floats((int)h.lv[L].n, L == (int)minl);
+            }
+            ba.minl = (int)minl;
+            ba.top = btop;
+            ba.alpha = p.adapt;
+            ba.mk = p.mask;
+            ba.konv_rad = (double)1.19209290e-07F;
+            ba.max_iters = (uint64_t)(100 * h.lv[minl].n + 1000);
+        }
+    }
 
     // ---- gates, start of new solves, FP64 outer step, opening of inner solves
     hipLaunchKernelGGL(plan_kernel, dim3(grid_for(S, 64)), dim3(64), 0, st, T, S);
@@ -1743,6 +2017,8 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     {
         for (uint64_t L = maxl; L > minl; --L)
         {
+            if (btop && L <= (uint64_t)btop)
+                break;
             const LevelArgs a = largs(L);
             // the finest level's defect is d; coarser levels' defect is the
             // restricted one in rhs
@@ -1815,7 +2091,9 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         // with one level it sees the finest defect d
         const LevelArgs a = largs(minl);
         const int crhs = (minl == maxl) ? F_D : F_RHS;
-        if (a.n <= kCoarseLdsMax)
+        if (btop)
+            hipLaunchKernelGGL((v_bottom_kernel<MK>), dim3(1, 1, S), dim3(1024), bbytes, st, ba, T);
+        else if (a.n <= kCoarseLdsMax)
             hipLaunchKernelGGL((v_coarse_lds_kernel<MK>), dim3(1, 1, S), dim3(64), 0, st, a, T, (int)minl, crhs,
                                (double)1.19209290e-07F, (uint64_t)(100 * a.n + 1000));
         else
@@ -1831,6 +2109,8 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     }
     for (uint64_t L = minl + 1; L <= maxl; ++L)
     {
+        if (btop && L <= (uint64_t)btop)
+            continue;
         const LevelArgs a = largs(L);
         const int Nf = (int)h.lv[L].side;
         const int Nc = (int)h.lv[L - 1].side;
