@@ -541,6 +541,10 @@ namespace
 
     constexpr int kPostLanes = 256;
     constexpr int kPostW = kPostLanes - 6;
+    // slots per workgroup of the one-pass kernels when a tick runs several:
+    // they share each tile's band and dinv loads
+    constexpr int kPostSlots = 2;
+    template <int N> using IntC = std::integral_constant<int, N>;
 
     // Row i of A x from three rows of x in LDS (up, mid, dn: rows r - 1, r,
     // r + 1 of the tile's column span, j the lane's column), b[0..8] the row's
@@ -662,15 +666,29 @@ namespace
     // fcoarse, x + alpha P(coarse) (Prolongation<NONE> then ScaledSum with
     // the fixed correction factor), formed per loaded node, so the corrected
     // x is never stored.
-    template <int MK, int DEFECT, int PROL>
+    template <int MK, int DEFECT, int PROL, int K>
     __global__ __launch_bounds__(kPostLanes) void v_post2_kernel(LevelArgs a, const Slot * T, int L, int fx, int fout,
-                                                          int fcoarse, float alpha, k::Mask mk, int th)
+                                                          int fcoarse, float alpha, k::Mask mk, int th, int nslots)
     {
         const VB vb = vblock();
-        HS_GATE_RUN
-        __shared__ float xr[3][kPostLanes];
-        __shared__ float x1r[3][kPostLanes];
-        __shared__ float x2r[DEFECT ? 3 : 1][kPostLanes];
+        // the workgroup's K slots vb.z K + q: they share the tile's band and
+        // dinv loads; a slot that is not running borrows the arrays of one
+        // that is for its loads and stores nothing
+        bool on[K];
+        int pick = -1;
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+        {
+            const int z = (int)vb.z * K + q;
+            on[q] = z < nslots && live(T[z]) && T[z].s->run != 0;
+            if (on[q] && pick < 0)
+                pick = q;
+        }
+        if (pick < 0)
+            return;
+        __shared__ float xr[K][3][kPostLanes];
+        __shared__ float x1r[K][3][kPostLanes];
+        __shared__ float x2r[K][DEFECT ? 3 : 1][kPostLanes];
         const int n = (int)a.n, m = (int)a.m;
         // tiles in row blocks, column strips fastest: horizontal neighbours
         // march through the same rows at the same time on one XCD
@@ -695,12 +713,19 @@ namespace
         const bool in_1 = j >= 1 && j < W + 5;
         const bool in_2 = j >= 2 && j < W + 4;
         const bool own = j >= 3 && j < W + 3;
-        const gptr<const float> x = as_global<const float>(sl.v[L][fx]);
-        const gptr<const float> rhs = as_global<const float>(sl.v[L][F_RHS]);
+        gptr<const float> x[K], rhs[K], xc[K];
+        gptr<float> out[K], dout[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+        {
+            const Slot & sl = T[(int)vb.z * K + (on[q] ? q : pick)];
+            x[q] = as_global<const float>(sl.v[L][fx]);
+            rhs[q] = as_global<const float>(sl.v[L][F_RHS]);
+            out[q] = as_global<float>(sl.v[L][fout]);
+            dout[q] = as_global<float>(sl.v[L][F_D]);
+            xc[q] = as_global<const float>(PROL ? sl.v[L - 1][fcoarse] : sl.v[L][fx]);
+        }
         const gptr<const float> dinv = as_global<const float>(a.dinv);
-        const gptr<float> out = as_global<float>(sl.v[L][fout]);
-        const gptr<float> dout = as_global<float>(sl.v[L][F_D]);
-        const gptr<const float> xc = as_global<const float>(PROL ? sl.v[L - 1][fcoarse] : sl.v[L][fx]);
         // the nine bands from one base: band k of node i at element i + k bs
         // (9 bs 4 < 2^32, checked on the host)
         const gptr<const float> bands = as_global<const float>((const float *)a.B.b[0]);
@@ -725,30 +750,35 @@ namespace
             R = min(max(R, 0), m - 1);
             S = min(max(S, 0), m - 1);
         };
-        // The loads of step s: x at march position s - 3 (with PROL the four
-        // coarse values around it) and the bands, dinv and rhs at position
-        // s - 4.  Every lane loads, at indices clamped to the arrays, with no
-        // branch between the loads; step s decides which values count.
-        auto load = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
+        // The loads of step s: each slot's x at march position s - 3 (with
+        // PROL the four coarse values around it) and rhs at position s - 4,
+        // and the bands and dinv at s - 4 (shared).  Every lane loads, at
+        // indices clamped to the arrays, with no branch between the loads;
+        // step s decides which values count.
+        auto load = [&](int s, float (&xq)[K], float (&cq)[K][4], float (&rq)[K], float (&bq)[10]) {
             const int ix = (Rs + dir * (s - 3)) * m + c;
-            xq = ld32(x, (uint32_t)min(max(ix, 0), n - 1));
-            if (PROL)
-            {
-                int R, S;
-                grid_pos(s, R, S);
-                const int ra = (R >> 1) * Nc, rb = ((R + 1) >> 1) * Nc;
-                const int jj = S >> 1, jj1 = min(jj + 1, Nc - 1);
-                cq[0] = ld32(xc, (uint32_t)(ra + jj));
-                cq[1] = ld32(xc, (uint32_t)(ra + jj1));
-                cq[2] = ld32(xc, (uint32_t)(rb + jj));
-                cq[3] = ld32(xc, (uint32_t)(rb + jj1));
-            }
             const uint32_t ib = (uint32_t)min(max(ix - dir * m, 0), n - 1);
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+            {
+                xq[q] = ld32(x[q], (uint32_t)min(max(ix, 0), n - 1));
+                if (PROL)
+                {
+                    int R, S;
+                    grid_pos(s, R, S);
+                    const int ra = (R >> 1) * Nc, rb = ((R + 1) >> 1) * Nc;
+                    const int jj = S >> 1, jj1 = min(jj + 1, Nc - 1);
+                    cq[q][0] = ld32(xc[q], (uint32_t)(ra + jj));
+                    cq[q][1] = ld32(xc[q], (uint32_t)(ra + jj1));
+                    cq[q][2] = ld32(xc[q], (uint32_t)(rb + jj));
+                    cq[q][3] = ld32(xc[q], (uint32_t)(rb + jj1));
+                }
+                rq[q] = ld32(rhs[q], ib);
+            }
 #pragma unroll
             for (int k = 0; k < 9; ++k)
                 bq[k] = ld32(bands, ib + (uint32_t)k * bs);
             bq[9] = ld32(dinv, ib);
-            bq[10] = ld32(rhs, ib);
         };
         // H + 6 steps, rounded up to an even count so that the loop body is
         // two whole steps with no branch around their loads (the extra step's
@@ -761,92 +791,121 @@ namespace
         auto march = [&](auto interior_tag) {
             constexpr bool G = ! decltype(interior_tag)::value;
             // stage registers: march position s - 4 (stage 1), s - 5 (stage
-            // 2), s - 6 (stage 3)
-            float b1[11] = {}, b2[11] = {}, b3[11] = {};
-            // one step; (xq, cq, bq) hold its loads and then take those of step s + 2
-            auto step = [&](int s, float & xq, float (&cq)[4], float (&bq)[11]) {
+            // 2), s - 6 (stage 3): bands and dinv (b), each slot's rhs (r)
+            float b1[10] = {}, b2[10] = {}, b3[10] = {};
+            float r1[K] = {}, r2[K] = {}, r3[K] = {};
+            // one step; its loads, then those of step s + 2
+            auto step = [&](int s, float (&xq)[K], float (&cq)[K][4], float (&rq)[K], float (&bq)[10]) {
                 // ring rows: positions p, p - 1, p - 2; the row above a stage's
                 // row is the older one marching down, the newer one marching up
                 const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;
                 const int above = dir > 0 ? old : cur, below = dir > 0 ? cur : old;
                 {
                     const int ix = (Rs + dir * (s - 3)) * m + c;
-                    float xv = 0.0f;
-                    if (in_x && (! G || (ix >= 0 && ix < n)))
+                    const bool vx = in_x && (! G || (ix >= 0 && ix < n));
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
                     {
-                        xv = xq;
-                        if (PROL)
+                        float xv = 0.0f;
+                        if (vx)
                         {
-                            int R, S;
-                            grid_pos(s, R, S);
-                            xv = xv + alpha * k::prolongation_node_v<float>(cq, m, mk, R, S);
+                            xv = xq[q];
+                            if (PROL)
+                            {
+                                int R, S;
+                                grid_pos(s, R, S);
+                                xv = xv + alpha * k::prolongation_node_v<float>(cq[q], m, mk, R, S);
+                            }
                         }
+                        xr[q][cur][j] = xv;
                     }
-                    xr[cur][j] = xv;
                     const int ib = ix - dir * m;
                     const bool vb1 = in_1 && s >= 2 && (! G || (ib >= 0 && ib < n));
 #pragma unroll
-                    for (int k = 0; k < 11; ++k)
+                    for (int k = 0; k < 10; ++k)
                     {
                         b3[k] = b2[k];
                         b2[k] = b1[k];
                         b1[k] = vb1 ? bq[k] : 0.0f;
                     }
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
+                    {
+                        r3[q] = r2[q];
+                        r2[q] = r1[q];
+                        r1[q] = vb1 ? rq[q] : 0.0f;
+                    }
                 }
-                load(s + 2, xq, cq, bq);
+                load(s + 2, xq, cq, rq, bq);
                 lds_barrier();
                 // stage 1: x1 at position s - 4
                 if (s >= 2)
                 {
-                    float v = 0.0f;
                     const int i = (Rs + dir * (s - 4)) * m + c;
-                    if (in_1 && (! G || (i >= 0 && i < n)))
+                    const bool v1 = in_1 && (! G || (i >= 0 && i < n));
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
                     {
-                        const float sum = post_row<MK, G>(b1, xr[above], xr[prv], xr[below], j, i, n, m);
-                        const float tt = b1[10] - sum;
-                        v = xr[prv][j] + tt * b1[9];
+                        float v = 0.0f;
+                        if (v1)
+                        {
+                            const float sum = post_row<MK, G>(b1, xr[q][above], xr[q][prv], xr[q][below], j, i, n, m);
+                            const float tt = r1[q] - sum;
+                            v = xr[q][prv][j] + tt * b1[9];
+                        }
+                        x1r[q][cur][j] = v;
                     }
-                    x1r[cur][j] = v;
                 }
                 lds_barrier();
                 // stage 2: x2 at position s - 5
                 if (s >= 4)
                 {
-                    float v = 0.0f;
-                    const int r2 = Rs + dir * (s - 5);
-                    const int i = r2 * m + c;
-                    if (in_2 && (! G || (i >= 0 && i < n)))
+                    const int r2w = Rs + dir * (s - 5);
+                    const int i = r2w * m + c;
+                    const bool v2 = in_2 && (! G || (i >= 0 && i < n));
+                    const bool st2 = own && r2w >= R0 && r2w < R0 + H;
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
                     {
-                        const float sum = post_row<MK, G>(b2, x1r[above], x1r[prv], x1r[below], j, i, n, m);
-                        const float tt = b2[10] - sum;
-                        v = x1r[prv][j] + tt * b2[9];
-                        if (own && r2 >= R0 && r2 < R0 + H)
-                            st32(out, (uint32_t)i, v);
+                        float v = 0.0f;
+                        if (v2)
+                        {
+                            const float sum = post_row<MK, G>(b2, x1r[q][above], x1r[q][prv], x1r[q][below], j, i, n, m);
+                            const float tt = r2[q] - sum;
+                            v = x1r[q][prv][j] + tt * b2[9];
+                            if (st2 && on[q])
+                                st32(out[q], (uint32_t)i, v);
+                        }
+                        if (DEFECT)
+                            x2r[q][DEFECT ? cur : 0][j] = v;
                     }
-                    if (DEFECT)
-                        x2r[DEFECT ? cur : 0][j] = v;
                 }
                 if (DEFECT)
                 {
                     lds_barrier();
                     // stage 3: d at position s - 6
-                    const int r3 = Rs + dir * (s - 6);
-                    const int i = r3 * m + c;
-                    if (s >= 6 && own && r3 >= R0 && r3 < R0 + H && (! G || (i >= 0 && i < n)))
+                    const int r3w = Rs + dir * (s - 6);
+                    const int i = r3w * m + c;
+                    if (s >= 6 && own && r3w >= R0 && r3w < R0 + H && (! G || (i >= 0 && i < n)))
                     {
-                        const float sum = post_row<MK, G>(b3, x2r[DEFECT ? above : 0], x2r[DEFECT ? prv : 0],
-                                                          x2r[DEFECT ? below : 0], j, i, n, m);
-                        st32(dout, (uint32_t)i, b3[10] - sum);
+#pragma unroll
+                        for (int q = 0; q < K; ++q)
+                        {
+                            const float sum = post_row<MK, G>(b3, x2r[q][DEFECT ? above : 0], x2r[q][DEFECT ? prv : 0],
+                                                              x2r[q][DEFECT ? below : 0], j, i, n, m);
+                            if (on[q])
+                                st32(dout[q], (uint32_t)i, r3[q] - sum);
+                        }
                     }
                 }
             };
-            float xA, xB, cA[4] = {}, cB[4] = {}, bA[11], bB[11];
-            load(0, xA, cA, bA);
-            load(1, xB, cB, bB);
+            float xA[K], xB[K], cA[K][4] = {}, cB[K][4] = {}, rA[K], rB[K], bA[10], bB[10];
+            load(0, xA, cA, rA, bA);
+            load(1, xB, cB, rB, bB);
             for (int s = 0; s < steps; s += 2)
             {
-                step(s, xA, cA, bA);
-                step(s + 1, xB, cB, bB);
+                step(s, xA, cA, rA, bA);
+                step(s + 1, xB, cB, rB, bB);
             }
         };
         if (interior)
@@ -873,15 +932,27 @@ namespace
     // .. c0 + W + 2 in linear index order, so the stencil steps are the
     // linear index's; the restriction reads grid columns 2s - 1 .. 2s + 1 of
     // owned columns 2s, which all lie in that span.
-    template <int MK, int ACC>
+    template <int MK, int ACC, int K>
     __global__ __launch_bounds__(kPostLanes) void v_pre2r_kernel(LevelArgs a, const Slot * T, int L, int fd, int facc,
-                                                          int64_t Nc64, k::Mask mk, int th)
+                                                          int64_t Nc64, k::Mask mk, int th, int nslots)
     {
         const VB vb = vblock();
-        HS_GATE_RUN
-        __shared__ float c1r[3][kPostLanes];
-        __shared__ float xr[3][kPostLanes];
-        __shared__ float dr[3][kPostLanes];
+        // the workgroup's K slots, as in v_post2_kernel
+        bool on[K];
+        int pick = -1;
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+        {
+            const int z = (int)vb.z * K + q;
+            on[q] = z < nslots && live(T[z]) && T[z].s->run != 0;
+            if (on[q] && pick < 0)
+                pick = q;
+        }
+        if (pick < 0)
+            return;
+        __shared__ float c1r[K][3][kPostLanes];
+        __shared__ float xr[K][3][kPostLanes];
+        __shared__ float dr[K][3][kPostLanes];
         const int n = (int)a.n, m = (int)a.m, Nc = (int)Nc64;
         const unsigned t = gridDim.z == 1 ? k::xcd_block(vb.x, gridDim.x) : vb.x;
         const int nst = (m + kPostW - 1) / kPostW;
@@ -899,28 +970,40 @@ namespace
         const bool in_2 = j >= 2 && j < W + 4;
         const bool own = j >= 3 && j < W + 3;
         const bool own_even = own && ((c & 1) == 0);
-        const gptr<const float> d = as_global<const float>(sl.v[L][fd]);
-        const gptr<const float> rhs = as_global<const float>(sl.v[L][F_RHS]);
+        gptr<const float> d[K], rhs[K], acc[K];
+        gptr<float> xo[K], co[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+        {
+            const Slot & sl = T[(int)vb.z * K + (on[q] ? q : pick)];
+            d[q] = as_global<const float>(sl.v[L][fd]);
+            rhs[q] = as_global<const float>(sl.v[L][F_RHS]);
+            acc[q] = as_global<const float>(sl.v[L][facc]);
+            xo[q] = as_global<float>(sl.v[L][F_X]);
+            co[q] = as_global<float>(sl.v[L - 1][F_RHS]);
+        }
         const gptr<const float> dinv = as_global<const float>(a.dinv);
-        const gptr<const float> acc = as_global<const float>(sl.v[L][facc]);
-        const gptr<float> xo = as_global<float>(sl.v[L][F_X]);
-        const gptr<float> co = as_global<float>(sl.v[L - 1][F_RHS]);
         // bands as in v_post2_kernel
         const gptr<const float> bands = as_global<const float>((const float *)a.B.b[0]);
         const uint32_t bs = (uint32_t)a.bs;
 
-        // march position s - 3 for step s: 9 bands, dinv, d, rhs, facc value; every
-        // lane loads, at an index clamped to the arrays, with no branch
-        // between the loads; step s decides which values count
-        auto load = [&](int s, float (&rn)[13]) {
+        // march position s - 3 for step s: the 9 bands and dinv (shared) and
+        // each slot's d, rhs and facc value (pq[q][0..2]); every lane loads,
+        // at an index clamped to the arrays, with no branch between the
+        // loads; step s decides which values count
+        auto load = [&](int s, float (&bq)[10], float (&pq)[K][3]) {
             const uint32_t i = (uint32_t)min(max((Rs + dir * (s - 3)) * m + c, 0), n - 1);
 #pragma unroll
             for (int k = 0; k < 9; ++k)
-                rn[k] = ld32(bands, i + (uint32_t)k * bs);
-            rn[9] = ld32(dinv, i);
-            rn[10] = ld32(d, i);
-            rn[11] = ld32(rhs, i);
-            rn[12] = ACC ? ld32(acc, i) : 0.0f;
+                bq[k] = ld32(bands, i + (uint32_t)k * bs);
+            bq[9] = ld32(dinv, i);
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+            {
+                pq[q][0] = ld32(d[q], i);
+                pq[q][1] = ld32(rhs[q], i);
+                pq[q][2] = ACC ? ld32(acc[q], i) : 0.0f;
+            }
         };
         // H + 6 steps, rounded up to an even count (as v_post2_kernel)
         const int steps = (H + 7) & ~1;
@@ -928,10 +1011,12 @@ namespace
         const bool interior = R0 >= 4 && R0 + H + 4 <= m;
         auto march = [&](auto interior_tag) {
             constexpr bool G = ! decltype(interior_tag)::value;
-            // stage registers: rc position s - 3 (its c1 only, yet), ra position
-            // s - 4 (x: bands, dinv, d, facc), rb position s - 5 (d': bands, rhs)
-            float rc[13] = {}, ra[13] = {}, rb[13] = {};
-            auto step = [&](int s, float (&rq)[13]) {
+            // stage registers: position s - 3 (its c1 only, yet), s - 4 (x:
+            // bands, dinv, d, facc), s - 5 (d': bands, rhs); bands and dinv
+            // (bc, ba, bb) shared, d, rhs, facc per slot (pc, pa, pb)
+            float bc[10] = {}, ba[10] = {}, bb[10] = {};
+            float pc[K][3] = {}, pa[K][3] = {}, pb[K][3] = {};
+            auto step = [&](int s, float (&bq)[10], float (&pq)[K][3]) {
                 // ring rows: positions p, p - 1, p - 2 (above / below as in v_post2_kernel)
                 const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;
                 const int above = dir > 0 ? old : cur, below = dir > 0 ? cur : old;
@@ -939,46 +1024,68 @@ namespace
                     const int i = (Rs + dir * (s - 3)) * m + c;
                     const bool valid = in_x && (! G || (i >= 0 && i < n));
 #pragma unroll
-                    for (int k = 0; k < 13; ++k)
+                    for (int k = 0; k < 10; ++k)
                     {
-                        rb[k] = ra[k];
-                        ra[k] = rc[k];
-                        rc[k] = valid ? rq[k] : 0.0f;
+                        bb[k] = ba[k];
+                        ba[k] = bc[k];
+                        bc[k] = valid ? bq[k] : 0.0f;
                     }
-                    c1r[cur][j] = rc[9] * rc[10];
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
+                    {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+                        {
+                            pb[q][k] = pa[q][k];
+                            pa[q][k] = pc[q][k];
+                            pc[q][k] = valid ? pq[q][k] : 0.0f;
+                        }
+                        c1r[q][cur][j] = bc[9] * pc[q][0];
+                    }
                 }
-                load(s + 2, rq);
+                load(s + 2, bq, pq);
                 lds_barrier();
                 // x at position s - 4
                 if (s >= 2)
                 {
-                    float v = 0.0f;
                     const int rho = Rs + dir * (s - 4);
                     const int i = rho * m + c;
-                    if (in_1 && (! G || (i >= 0 && i < n)))
+                    const bool v1 = in_1 && (! G || (i >= 0 && i < n));
+                    const bool stx = own && rho >= R0 && rho < R0 + H;
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
                     {
-                        const float sum = post_row<MK, G>(ra, c1r[above], c1r[prv], c1r[below], j, i, n, m);
-                        const float tt = ra[10] - sum;
-                        const float c2 = c1r[prv][j] + tt * ra[9];
-                        v = ACC ? ra[12] + c2 : c2;
-                        if (own && rho >= R0 && rho < R0 + H)
-                            st32(xo, (uint32_t)i, v);
+                        float v = 0.0f;
+                        if (v1)
+                        {
+                            const float sum = post_row<MK, G>(ba, c1r[q][above], c1r[q][prv], c1r[q][below], j, i, n, m);
+                            const float tt = pa[q][0] - sum;
+                            const float c2 = c1r[q][prv][j] + tt * ba[9];
+                            v = ACC ? pa[q][2] + c2 : c2;
+                            if (stx && on[q])
+                                st32(xo[q], (uint32_t)i, v);
+                        }
+                        xr[q][cur][j] = v;
                     }
-                    xr[cur][j] = v;
                 }
                 lds_barrier();
                 // d' at position s - 5
                 if (s >= 4)
                 {
-                    float v = 0.0f;
                     const int rho = Rs + dir * (s - 5);
                     const int i = rho * m + c;
-                    if (in_2 && (! G || (i >= 0 && i < n)))
+                    const bool v2 = in_2 && (! G || (i >= 0 && i < n));
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
                     {
-                        const float sum = post_row<MK, G>(rb, xr[above], xr[prv], xr[below], j, i, n, m);
-                        v = rb[11] - sum;
+                        float v = 0.0f;
+                        if (v2)
+                        {
+                            const float sum = post_row<MK, G>(bb, xr[q][above], xr[q][prv], xr[q][below], j, i, n, m);
+                            v = pb[q][1] - sum;
+                        }
+                        dr[q][cur][j] = v;
                     }
-                    dr[cur][j] = v;
                     lds_barrier();
                     // coarse row r when this d' row is 2r + dir: rows 2r - 1,
                     // 2r, 2r + 1 are then in the ring
@@ -988,23 +1095,29 @@ namespace
                         if (own_even && 2 * r >= R0 && 2 * r < R0 + H && c < m)
                         {
                             const int sc = c >> 1;
-                            const float (*ring)[kPostLanes] = dr;
-                            st32(co, (uint32_t)(r * Nc + sc), k::restrict_node<float>(
-                                [&](int ddr, int dds) {
-                                    return ring[dir * ddr < 0 ? old : (ddr == 0 ? prv : cur)][j + dds];
-                                },
-                                r, sc, Nc, mk));
+#pragma unroll
+                            for (int q = 0; q < K; ++q)
+                            {
+                                const float (*ring)[kPostLanes] = dr[q];
+                                const float v = k::restrict_node<float>(
+                                    [&](int ddr, int dds) {
+                                        return ring[dir * ddr < 0 ? old : (ddr == 0 ? prv : cur)][j + dds];
+                                    },
+                                    r, sc, Nc, mk);
+                                if (on[q])
+                                    st32(co[q], (uint32_t)(r * Nc + sc), v);
+                            }
                         }
                     }
                 }
             };
-            float qA[13], qB[13];
-            load(0, qA);
-            load(1, qB);
+            float bA[10], bB[10], pA[K][3], pB[K][3];
+            load(0, bA, pA);
+            load(1, bB, pB);
             for (int s = 0; s < steps; s += 2)
             {
-                step(s, qA);
-                step(s + 1, qB);
+                step(s, bA, pA);
+                step(s + 1, bB, pB);
             }
         };
         if (interior)
@@ -1565,13 +1678,15 @@ namespace
 
     void warm_post_kernels()
     {
-        const void * fns[] = {
-            (const void *)v_pre2r_kernel<0, 0>,     (const void *)v_pre2r_kernel<0, 1>,
-            (const void *)v_pre2r_kernel<2, 0>,     (const void *)v_pre2r_kernel<2, 1>,
-            (const void *)v_post2_kernel<0, 0, 0>, (const void *)v_post2_kernel<0, 0, 1>,
-            (const void *)v_post2_kernel<0, 1, 0>, (const void *)v_post2_kernel<0, 1, 1>,
-            (const void *)v_post2_kernel<2, 0, 0>, (const void *)v_post2_kernel<2, 0, 1>,
-            (const void *)v_post2_kernel<2, 1, 0>, (const void *)v_post2_kernel<2, 1, 1>};
+#define HS_K(KS)                                                                                               \
+    (const void *)v_pre2r_kernel<0, 0, KS>, (const void *)v_pre2r_kernel<0, 1, KS>,                            \
+        (const void *)v_pre2r_kernel<2, 0, KS>, (const void *)v_pre2r_kernel<2, 1, KS>,                        \
+        (const void *)v_post2_kernel<0, 0, 0, KS>, (const void *)v_post2_kernel<0, 0, 1, KS>,                  \
+        (const void *)v_post2_kernel<0, 1, 0, KS>, (const void *)v_post2_kernel<0, 1, 1, KS>,                  \
+        (const void *)v_post2_kernel<2, 0, 0, KS>, (const void *)v_post2_kernel<2, 0, 1, KS>,                  \
+        (const void *)v_post2_kernel<2, 1, 0, KS>, (const void *)v_post2_kernel<2, 1, 1, KS>
+        const void * fns[] = {HS_K(1), HS_K(kPostSlots)};
+#undef HS_K
         for (const void * f : fns)
             (void)post_resident(f);
         (void)bottom_lds_limit();
@@ -2027,16 +2142,19 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
             {
                 if (pre2r)
                 {
-                    const void * fn = L == maxl ? (const void *)v_pre2r_kernel<MK, 1> : (const void *)v_pre2r_kernel<MK, 0>;
-                    const int th = post_rows(a.m, S, post_resident(fn));
-                    const dim3 tg(post_tiles(a.m, th), 1, S);
                     const int64_t Nc = h.lv[L - 1].side;
+                    auto go = [&](auto acc_tag, auto k_tag) {
+                        constexpr int ACC = decltype(acc_tag)::value, KS = decltype(k_tag)::value;
+                        const unsigned groups = (S + KS - 1) / KS;
+                        const int th = post_rows(a.m, groups, post_resident((const void *)v_pre2r_kernel<MK, ACC, KS>));
+                        hipLaunchKernelGGL((v_pre2r_kernel<MK, ACC, KS>), dim3(post_tiles(a.m, th), 1, groups),
+                                           dim3(kPostLanes), 0, st, a, T, (int)L, dsrc, ACC ? fx_top : (int)F_X, Nc, p.mask,
+                                           th, (int)S);
+                    };
                     if (L == maxl)
-                        hipLaunchKernelGGL((v_pre2r_kernel<MK, 1>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, dsrc, fx_top, Nc,
-                                           p.mask, th);
+                        S > 1 ? go(IntC<1>{}, IntC<kPostSlots>{}) : go(IntC<1>{}, IntC<1>{});
                     else
-                        hipLaunchKernelGGL((v_pre2r_kernel<MK, 0>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, dsrc, (int)F_X, Nc,
-                                           p.mask, th);
+                        S > 1 ? go(IntC<0>{}, IntC<kPostSlots>{}) : go(IntC<0>{}, IntC<1>{});
                     continue;
                 }
             }
@@ -2107,6 +2225,23 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
             return;
         }
     }
+    // v_post2_kernel on level L (DEFECT on the finest), kPostSlots slots per
+    // workgroup when the tick runs several
+    auto post2_launch = [&](uint64_t L, const LevelArgs & a, int fcoarse, float alpha, auto prol_tag) {
+        constexpr int PROL = decltype(prol_tag)::value;
+        auto go = [&](auto def_tag, auto k_tag) {
+            constexpr int DEF = decltype(def_tag)::value, KS = decltype(k_tag)::value;
+            const unsigned groups = (S + KS - 1) / KS;
+            const int th = post_rows(a.m, groups, post_resident((const void *)v_post2_kernel<MK, DEF, PROL, KS>));
+            hipLaunchKernelGGL((v_post2_kernel<MK, DEF, PROL, KS>), dim3(post_tiles(a.m, th), 1, groups),
+                               dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X, (int)F_TMP, fcoarse, alpha, p.mask, th,
+                               (int)S);
+        };
+        if (L == maxl)
+            S > 1 ? go(IntC<1>{}, IntC<kPostSlots>{}) : go(IntC<1>{}, IntC<1>{});
+        else
+            S > 1 ? go(IntC<0>{}, IntC<kPostSlots>{}) : go(IntC<0>{}, IntC<1>{});
+    };
     for (uint64_t L = minl + 1; L <= maxl; ++L)
     {
         if (btop && L <= (uint64_t)btop)
@@ -2123,15 +2258,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
             if (pre2r)
             {
                 // prolongation, correction and post-smoothing in one pass
-                const void * fn = L == maxl ? (const void *)v_post2_kernel<MK, 1, 1> : (const void *)v_post2_kernel<MK, 0, 1>;
-                const int th = post_rows(a.m, S, post_resident(fn));
-                const dim3 tg(post_tiles(a.m, th), 1, S);
-                if (L == maxl)
-                    hipLaunchKernelGGL((v_post2_kernel<MK, 1, 1>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X,
-                                       (int)F_TMP, fcoarse, p.adapt, p.mask, th);
-                else
-                    hipLaunchKernelGGL((v_post2_kernel<MK, 0, 1>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X,
-                                       (int)F_TMP, fcoarse, p.adapt, p.mask, th);
+                post2_launch(L, a, fcoarse, p.adapt, IntC<1>{});
                 continue;
             }
         }
@@ -2162,15 +2289,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         {
             if (post2)
             {
-                const void * fn = L == maxl ? (const void *)v_post2_kernel<MK, 1, 0> : (const void *)v_post2_kernel<MK, 0, 0>;
-                const int th = post_rows(a.m, S, post_resident(fn));
-                const dim3 tg(post_tiles(a.m, th), 1, S);
-                if (L == maxl)
-                    hipLaunchKernelGGL((v_post2_kernel<MK, 1, 0>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X,
-                                       (int)F_TMP, fcoarse, 0.0f, p.mask, th);
-                else
-                    hipLaunchKernelGGL((v_post2_kernel<MK, 0, 0>), tg, dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X,
-                                       (int)F_TMP, fcoarse, 0.0f, p.mask, th);
+                post2_launch(L, a, fcoarse, 0.0f, IntC<0>{});
                 continue;
             }
         }
