@@ -165,14 +165,14 @@ namespace
         const Slot & sl = T[vb.z];
         if (! live(sl) || sl.s->do_start == 0)
             return;
-        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
-        if (i >= n)
-            return;
-        double g = sl.s->zero_guess ? 0.0 : sl.result[i];
-        if (i < sq || i >= n - sq || (i % sq) == 0)
-            g = sl.b[i];
-        sl.result[i] = g;
-        sl.v[L][fx][i] = (float)g;
+        for (int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        {
+            double g = sl.s->zero_guess ? 0.0 : sl.result[i];
+            if (i < sq || i >= n - sq || (i % sq) == 0)
+                g = sl.b[i];
+            sl.result[i] = g;
+            sl.v[L][fx][i] = (float)g;
+        }
     }
 
     // iterates of the coarser levels start at zero
@@ -182,9 +182,8 @@ namespace
         const Slot & sl = T[vb.z];
         if (! live(sl) || sl.s->do_start == 0)
             return;
-        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
         for (int l = minl; l < L; ++l)
-            if (i < sz.n[l])
+            for (int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x; i < sz.n[l]; i += (int64_t)gridDim.x * blockDim.x)
                 sl.v[l][F_X][i] = 0.0f;
     }
 
@@ -196,11 +195,9 @@ namespace
         const Slot & sl = T[vb.z];
         if (! live(sl) || sl.s->do_outer == 0)
             return;
-        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
-        if (i >= n)
-            return;
         const double scale = sl.s->scale;
-        sl.result[i] = sl.result[i] + scale * (double)sl.v[L][fx][i];
+        for (int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+            sl.result[i] = sl.result[i] + scale * (double)sl.v[L][fx][i];
     }
 
     // od = b - A64 result and its norm; opens the next inner solve or ends
@@ -328,13 +325,14 @@ namespace
         const Slot & sl = T[vb.z];
         if (! live(sl) || sl.s->open == 0)
             return;
-        const int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
-        if (i >= n)
-            return;
         const double inv = sl.s->inv;
-        sl.v[L][F_RHS][i] = (float)(sl.od[i] * inv);
-        if (sl.s->do_start)
-            sl.result[i] = 0.0;
+        const bool clear = sl.s->do_start != 0;
+        for (int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        {
+            sl.v[L][F_RHS][i] = (float)(sl.od[i] * inv);
+            if (clear)
+                sl.result[i] = 0.0;
+        }
     }
 
     // Opens the inner solve: its initial defect d is rhs itself when
@@ -347,25 +345,31 @@ namespace
         if (! live(sl) || sl.s->open == 0)
             return;
         const int64_t n = a.n;
-        const int64_t lane = (int64_t)vb.x * blockDim.x + threadIdx.x;
+        const int64_t lane0 = (int64_t)vb.x * blockDim.x + threadIdx.x;
+        const int64_t lanes = (int64_t)gridDim.x * blockDim.x;
         const float * rhs = sl.v[L][F_RHS];
         float * d = sl.v[L][F_D];
         if (INITIAL_ZERO)
         {
             typedef k::Vec16<float>::type F4;
-            const int64_t i0 = lane * 4;
-            if (i0 + 4 <= n)
-                *(F4 *)(d + i0) = *(const F4 *)(rhs + i0);
-            else
-                for (int64_t i = i0; i < n; ++i)
-                    d[i] = rhs[i];
+            for (int64_t i0 = lane0 * 4; i0 < n; i0 += lanes * 4)
+            {
+                if (i0 + 4 <= n)
+                    *(F4 *)(d + i0) = *(const F4 *)(rhs + i0);
+                else
+                    for (int64_t i = i0; i < n; ++i)
+                        d[i] = rhs[i];
+            }
         }
         else if (kBands<MK>)
-            k::q1_body_v<float, 1, kOrd<MK>>(a.B, sl.v[L][fx], d, rhs, nullptr, n, a.m, lane);
+        {
+            for (int64_t lane = lane0; lane * 4 < n; lane += lanes)
+                k::q1_body_v<float, 1, kOrd<MK>>(a.B, sl.v[L][fx], d, rhs, nullptr, n, a.m, lane);
+        }
         else
         {
             const float * x = sl.v[L][fx];
-            for (int64_t i = lane; i < n; i += (int64_t)gridDim.x * blockDim.x)
+            for (int64_t i = lane0; i < n; i += lanes)
                 d[i] = rhs[i] - ell_row<float>(a.E, x, i);
         }
         if (vb.x == 0 && threadIdx.x == 0)
@@ -539,6 +543,8 @@ namespace
         }
     }
 
+    // workgroups (over all slots) of the per-tick gate kernels' capped grids
+    constexpr unsigned kGateBlocks = 4096;
     constexpr int kPostLanes = 256;
     constexpr int kPostW = kPostLanes - 6;
     // slots per workgroup of the one-pass kernels when a tick runs several:
@@ -2078,29 +2084,35 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
 
     // ---- gates, start of new solves, FP64 outer step, opening of inner solves
     hipLaunchKernelGGL(plan_kernel, dim3(grid_for(S, 64)), dim3(64), 0, st, T, S);
-    hipLaunchKernelGGL(start_init_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n,
+    // the per-tick kernels that most ticks skip (start, outer step, opening
+    // of an inner solve) loop over their rows on a capped grid, so a skipped
+    // launch costs a few thousand empty workgroups, not one per 256 rows
+    auto ggrid = [&](uint64_t blocks) {
+        return dim3((unsigned)std::min<uint64_t>(blocks, std::max<unsigned>(64u, kGateBlocks / S)), 1, S);
+    };
+    hipLaunchKernelGGL(start_init_kernel, ggrid(grid_for(top.n, kBlock)), blk, 0, st, T, (int)maxl, n,
                        top.side, fx_top);
     if (maxl > minl)
     {
         LevelSizes sz;
         for (int l = 0; l < kMaxLevels; ++l)
             sz.n[l] = (int64_t)h.lv[l].n;
-        hipLaunchKernelGGL(start_zero_kernel, dim3(grid_for(h.lv[maxl - 1].n, kBlock), 1, S), blk, 0, st, T,
+        hipLaunchKernelGGL(start_zero_kernel, ggrid(grid_for(h.lv[maxl - 1].n, kBlock)), blk, 0, st, T,
                            (int)minl, (int)maxl, sz);
     }
-    hipLaunchKernelGGL(outer_accumulate_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n,
+    hipLaunchKernelGGL(outer_accumulate_kernel, ggrid(grid_for(top.n, kBlock)), blk, 0, st, T, (int)maxl, n,
                        fx_top);
     const unsigned od_blocks = std::min<unsigned>(grid_for((top.n + 1) / 2, kBlock), kNormBlocks);
     hipLaunchKernelGGL((outer_defect_kernel<MK>), dim3(od_blocks, 1, S), blk, 0, st,
                        kBands<MK> ? h.B64(maxl) : k::Bands{}, h.ell64[maxl], T, n, top.side);
     hipLaunchKernelGGL(outer_defect_finish_kernel, dim3(1, 1, S), blk, 0, st, T, od_blocks, (double)p.tolerance,
                        p.max_outer);
-    hipLaunchKernelGGL(to_inner_kernel, dim3(grid_for(top.n, kBlock), 1, S), blk, 0, st, T, (int)maxl, n);
+    hipLaunchKernelGGL(to_inner_kernel, ggrid(grid_for(top.n, kBlock)), blk, 0, st, T, (int)maxl, n);
     if (p.initial_zero)
-        hipLaunchKernelGGL((open_inner_kernel<1, MK>), dim3(lanes4(top.n), 1, S), blk, 0, st, largs(maxl), T,
+        hipLaunchKernelGGL((open_inner_kernel<1, MK>), ggrid(lanes4(top.n)), blk, 0, st, largs(maxl), T,
                            (int)maxl, fx_top);
     else
-        hipLaunchKernelGGL((open_inner_kernel<0, MK>), dim3(rgrid(top.n), 1, S), blk, 0, st, largs(maxl), T,
+        hipLaunchKernelGGL((open_inner_kernel<0, MK>), ggrid(rgrid(top.n)), blk, 0, st, largs(maxl), T,
                            (int)maxl, fx_top);
     hipLaunchKernelGGL((inner_norm_kernel<0>), norm_grid(top.n), blk, 0, st, T, (int)maxl, n);
     hipLaunchKernelGGL((inner_norm_finish_kernel<0>), dim3(1, 1, S), blk, 0, st, T, norm_grid(top.n).x, p.tolerance,
