@@ -121,8 +121,12 @@ def test_mg_mixed_fixture_4225(hip, oracle):
     assert np.max(np.abs(x - F["ref_sol"])) <= 1e-4
 
 
-@pytest.mark.parametrize("maxl,minl", [(7, 1), (9, 2), (10, 2)])
+@pytest.mark.parametrize("maxl,minl", [(7, 1), (9, 2), (10, 2), (6, 4), (7, 5)])
 def test_mg_mixed_poisson_levels(hip, oracle, maxl, minl):
+    """MG-IR on Q1 Poisson against the oracle's Multigrid<MIXED>.  The
+    level pairs cover the engine's bottom-levels kernel with one to five
+    levels below level 6 ((7, 1), (9, 2), (6, 4)) and a coarsest level too
+    large for it ((7, 5): the coarse CG in global memory)."""
     n = P.side_of_level(maxl) ** 2
     A64 = P.poisson_q1_bands(n)
     b = P.rhs_uniform(n)
