@@ -67,12 +67,13 @@ namespace honei
                 throw InternalError("Product: result must not share storage with the operand");
             if (row_end == 0)
                 row_end = a.rows();
+            if (row_start > row_end || row_end > a.rows())
+                throw VectorSizeDoesNotMatch(row_end, a.rows());
             if (row_start == 0 && row_end == a.rows())
                 check(hsolve_ell_spmv(a.device_handle(), rd(b), wr(result), nullptr, stream()), "Product ELL");
-            else
-                check(abi::product_smell(rd(b), rw(result), a.Aj().lock(lm_read_only, tv_gpu_hip), rd(a.Ax()),
-                                         a.Arl().lock(lm_read_only, tv_gpu_hip), row_start, row_end,
-                                         a.num_cols_per_row(), a.stride(), a.threads()),
+            else if (row_start < row_end)
+                // rows outside the range keep their values: the result is read-write here
+                check(hsolve_ell_spmv_rows(a.device_handle(), rd(b), rw(result), nullptr, row_start, row_end, stream()),
                       "Product ELL rows");
             return result;
         }

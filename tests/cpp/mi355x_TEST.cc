@@ -144,10 +144,21 @@ template <typename DT> static void test_ell_product_defect()
     or_ell_product(ref.data(), (const uint64_t *)A.Aj().read_elements(), A.Ax().read_elements(),
                    (const uint64_t *)A.Arl().read_elements(), x.read_elements(), A.rows(), A.stride(), A.threads());
     TEST_CHECK(bit_equal(y, ref.data()));
-    // a row range goes through the reference-layout entry point
-    DenseVector<DT> y2(A.rows(), DT(0));
-    Product<Tag>::value(y2, A, x, 0, A.rows() / 2);
-    TEST_CHECK(std::memcmp(y2.read_elements(), ref.data(), sizeof(DT) * (A.rows() / 2)) == 0);
+    // row ranges (an even and an odd first row, an empty one) write their rows and leave the others
+    const unsigned long n = A.rows();
+    const unsigned long ranges[3][2] = {{0, n / 2}, {n / 3 | 1, 2 * n / 3}, {n / 4, n / 4}};
+    for (const auto & rg : ranges)
+    {
+        DenseVector<DT> y2(n, DT(-3));
+        Product<Tag>::value(y2, A, x, rg[0], rg[1]);
+        bool ok = true;
+        for (unsigned long i = 0; i < n; ++i)
+        {
+            const DT want = (i >= rg[0] && i < rg[1]) ? ref[i] : DT(-3);
+            ok = ok && std::memcmp(&y2.read_elements()[i], &want, sizeof(DT)) == 0;
+        }
+        TEST_CHECK(ok);
+    }
     DenseVector<DT> r(A.rows());
     Defect<Tag>::value(r, rhs, A, x);
     for (unsigned long i = 0; i < A.rows(); ++i)
