@@ -53,15 +53,16 @@ namespace
         return HSOLVE_ELL_XCD | HSOLVE_ELL_LDS | (m->sAx ? HSOLVE_ELL_SORT : 0);
     }
 
-    // LDS bytes of ell_spmv_lds's x window, or 0 when the handle does not
-    // qualify (threads == 1, columns within +-max_offset of the row, a
-    // window of at most 48 KiB so that 3 blocks fit on a CU)
+    // LDS bytes of ell_spmv_lds's x window (16 more for its start rounded
+    // down to a 16-byte boundary), or 0 when the handle does not qualify
+    // (threads == 1, columns within +-max_offset of the row, a window of at
+    // most 48 KiB so that 3 blocks fit on a CU)
     size_t lds_window_bytes(const hsolve_ell_s * m, size_t vs)
     {
         const uint64_t w = (uint64_t)(vs == 8 ? k::lds_rows<double>() : k::lds_rows<float>()) + 2 * m->max_offset;
         if (m->threads != 1 || w * vs > 48 * 1024)
             return 0;
-        return (size_t)(w * vs);
+        return (size_t)(w * vs + 16);
     }
 
     // rows [r0, r1) (r0 even: the rows pair up as in the whole product)
@@ -141,7 +142,9 @@ namespace
             using IT = std::remove_const_t<std::remove_pointer_t<decltype(Aj)>>;
             using LT = std::remove_const_t<std::remove_pointer_t<decltype(Arl)>>;
             const size_t lds = lds_window_bytes(m, sizeof(DT));
-            if (lds_pairs_ok && (v & HSOLVE_ELL_LDS) && lds && m->cols != 0 && m->ncpr >= 1 && m->ncpr <= 16)
+            // (ell_spmv_lds loads its window in 16-byte chunks: x 16-byte aligned)
+            if (lds_pairs_ok && (v & HSOLVE_ELL_LDS) && lds && m->cols != 0 && m->ncpr >= 1 && m->ncpr <= 16 &&
+                (((uintptr_t)x % 16) == 0 || (v & HSOLVE_ELL_PERSIST)))
             {
                 // a row range takes the blocks that hold its rows
                 const uint64_t R = (uint64_t)k::lds_rows<DT>();

@@ -316,6 +316,31 @@ def test_ell_lds_empty_tail_and_zero_columns(hip, oracle, rng, dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+def test_ell_lds_window_alignment(hip, oracle, rng, dtype):
+    """The LDS-window kernel loads x in 16-byte chunks from a window start
+    rounded down to a 16-byte boundary (odd max offset: the rounding moves
+    the start; a window end that is not a whole chunk: the tail loads).  An
+    x that is not 16-byte aligned (a view at an element offset) takes the
+    row kernels; every offset gives the oracle's bits."""
+    E = _banded_ell_tail(5004, 4999, 31, 10, dtype, seed=11)
+    n, c = E["rows"], E["cols"]
+    x = rng.uniform(-1, 1, c).astype(dtype)
+    ref = oracle.ell_product(E, x)
+    M = hip.EllMatrix.from_dict(E)
+    big = hip.DeviceVector.zeros(c + 8, dtype)
+    for off in range(4):
+        xv = big.view(off, c)
+        host = np.zeros(c + 8, dtype)
+        host[off:off + c] = x
+        big.upload(host)
+        for variant in (0, 16, 48):
+            M.set_variant(variant)
+            yd = hip.DeviceVector.from_numpy(np.full(n, 9, dtype))
+            M.spmv(xv, yd)
+            assert np.array_equal(yd.get(), ref), (off, variant)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_raw_launcher_empty_range_is_noop(hip, dtype):
     """row_end <= row_start: y unchanged, status OK."""
     from honei_amd._lib import lib, u64, vp
