@@ -112,6 +112,8 @@ namespace
         const float * dinv;
         int64_t n, m;
         int64_t bs;          // band stride in elements
+        const uint64_t * codes = nullptr;   // value codes of bands and dinv, or null
+        const float * ctab = nullptr;       // their tables (16 per array)
     };
 
     // Matrix kinds of the level kernels: MK 0 Q1 bands (Product<Q1>'s sum
@@ -672,7 +674,7 @@ namespace
     // fcoarse, x + alpha P(coarse) (Prolongation<NONE> then ScaledSum with
     // the fixed correction factor), formed per loaded node, so the corrected
     // x is never stored.
-    template <int MK, int DEFECT, int PROL, int K>
+    template <int MK, int DEFECT, int PROL, int K, bool CB = false>
     __global__ __launch_bounds__(kPostLanes) void v_post2_kernel(LevelArgs a, const Slot * T, int L, int fx, int fout,
                                                           int fcoarse, float alpha, k::Mask mk, int th, int nslots)
     {
@@ -692,6 +694,23 @@ namespace
         }
         if (pick < 0)
             return;
+        // CB: the bands and dinv come as value codes (one 64-bit word per
+        // node) and are decoded through their tables, held in LDS
+        __shared__ float ctab[CB ? 160 : 1];
+        if constexpr (CB)
+        {
+            if (threadIdx.x < 160)
+                ctab[threadIdx.x] = a.ctab[threadIdx.x];
+            __syncthreads();
+        }
+        const gptr<const uint64_t> codes = as_global<const uint64_t>(a.codes);
+        auto band_in = [&](const auto & bq, int k) -> float {
+            if constexpr (CB)
+                return ctab[k * 16 + (int)((bq >> (4 * k)) & 15u)];
+            else
+                return bq[k];
+        };
+        using BandIn = std::conditional_t<CB, uint64_t, float[10]>;
         __shared__ float xr[K][3][kPostLanes];
         __shared__ float x1r[K][3][kPostLanes];
         __shared__ float x2r[K][DEFECT ? 3 : 1][kPostLanes];
@@ -761,7 +780,7 @@ namespace
         // and the bands and dinv at s - 4 (shared).  Every lane loads, at
         // indices clamped to the arrays, with no branch between the loads;
         // step s decides which values count.
-        auto load = [&](int s, float (&xq)[K], float (&cq)[K][4], float (&rq)[K], float (&bq)[10]) {
+        auto load = [&](int s, float (&xq)[K], float (&cq)[K][4], float (&rq)[K], BandIn & bq) {
             const int ix = (Rs + dir * (s - 3)) * m + c;
             const uint32_t ib = (uint32_t)min(max(ix - dir * m, 0), n - 1);
 #pragma unroll
@@ -781,10 +800,15 @@ namespace
                 }
                 rq[q] = ld32(rhs[q], ib);
             }
+            if constexpr (CB)
+                bq = ld32(codes, ib);
+            else
+            {
 #pragma unroll
-            for (int k = 0; k < 9; ++k)
-                bq[k] = ld32(bands, ib + (uint32_t)k * bs);
-            bq[9] = ld32(dinv, ib);
+                for (int k = 0; k < 9; ++k)
+                    bq[k] = ld32(bands, ib + (uint32_t)k * bs);
+                bq[9] = ld32(dinv, ib);
+            }
         };
         // H + 6 steps, rounded up to an even count so that the loop body is
         // two whole steps with no branch around their loads (the extra step's
@@ -801,7 +825,7 @@ namespace
             float b1[10] = {}, b2[10] = {}, b3[10] = {};
             float r1[K] = {}, r2[K] = {}, r3[K] = {};
             // one step; its loads, then those of step s + 2
-            auto step = [&](int s, float (&xq)[K], float (&cq)[K][4], float (&rq)[K], float (&bq)[10]) {
+            auto step = [&](int s, float (&xq)[K], float (&cq)[K][4], float (&rq)[K], BandIn & bq) {
                 // ring rows: positions p, p - 1, p - 2; the row above a stage's
                 // row is the older one marching down, the newer one marching up
                 const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;
@@ -832,7 +856,7 @@ namespace
                     {
                         b3[k] = b2[k];
                         b2[k] = b1[k];
-                        b1[k] = vb1 ? bq[k] : 0.0f;
+                        b1[k] = vb1 ? band_in(bq, k) : 0.0f;
                     }
 #pragma unroll
                     for (int q = 0; q < K; ++q)
@@ -905,7 +929,8 @@ namespace
                     }
                 }
             };
-            float xA[K], xB[K], cA[K][4] = {}, cB[K][4] = {}, rA[K], rB[K], bA[10], bB[10];
+            float xA[K], xB[K], cA[K][4] = {}, cB[K][4] = {}, rA[K], rB[K];
+            BandIn bA, bB;
             load(0, xA, cA, rA, bA);
             load(1, xB, cB, rB, bB);
             for (int s = 0; s < steps; s += 2)
@@ -938,7 +963,7 @@ namespace
     // .. c0 + W + 2 in linear index order, so the stencil steps are the
     // linear index's; the restriction reads grid columns 2s - 1 .. 2s + 1 of
     // owned columns 2s, which all lie in that span.
-    template <int MK, int ACC, int K>
+    template <int MK, int ACC, int K, bool CB = false>
     __global__ __launch_bounds__(kPostLanes) void v_pre2r_kernel(LevelArgs a, const Slot * T, int L, int fd, int facc,
                                                           int64_t Nc64, k::Mask mk, int th, int nslots)
     {
@@ -956,6 +981,22 @@ namespace
         }
         if (pick < 0)
             return;
+        // CB: value-coded bands and dinv, as in v_post2_kernel
+        __shared__ float ctab[CB ? 160 : 1];
+        if constexpr (CB)
+        {
+            if (threadIdx.x < 160)
+                ctab[threadIdx.x] = a.ctab[threadIdx.x];
+            __syncthreads();
+        }
+        const gptr<const uint64_t> codes = as_global<const uint64_t>(a.codes);
+        auto band_in = [&](const auto & bq, int k) -> float {
+            if constexpr (CB)
+                return ctab[k * 16 + (int)((bq >> (4 * k)) & 15u)];
+            else
+                return bq[k];
+        };
+        using BandIn = std::conditional_t<CB, uint64_t, float[10]>;
         __shared__ float c1r[K][3][kPostLanes];
         __shared__ float xr[K][3][kPostLanes];
         __shared__ float dr[K][3][kPostLanes];
@@ -997,12 +1038,17 @@ namespace
         // each slot's d, rhs and facc value (pq[q][0..2]); every lane loads,
         // at an index clamped to the arrays, with no branch between the
         // loads; step s decides which values count
-        auto load = [&](int s, float (&bq)[10], float (&pq)[K][3]) {
+        auto load = [&](int s, BandIn & bq, float (&pq)[K][3]) {
             const uint32_t i = (uint32_t)min(max((Rs + dir * (s - 3)) * m + c, 0), n - 1);
+            if constexpr (CB)
+                bq = ld32(codes, i);
+            else
+            {
 #pragma unroll
-            for (int k = 0; k < 9; ++k)
-                bq[k] = ld32(bands, i + (uint32_t)k * bs);
-            bq[9] = ld32(dinv, i);
+                for (int k = 0; k < 9; ++k)
+                    bq[k] = ld32(bands, i + (uint32_t)k * bs);
+                bq[9] = ld32(dinv, i);
+            }
 #pragma unroll
             for (int q = 0; q < K; ++q)
             {
@@ -1022,7 +1068,7 @@ namespace
             // (bc, ba, bb) shared, d, rhs, facc per slot (pc, pa, pb)
             float bc[10] = {}, ba[10] = {}, bb[10] = {};
             float pc[K][3] = {}, pa[K][3] = {}, pb[K][3] = {};
-            auto step = [&](int s, float (&bq)[10], float (&pq)[K][3]) {
+            auto step = [&](int s, BandIn & bq, float (&pq)[K][3]) {
                 // ring rows: positions p, p - 1, p - 2 (above / below as in v_post2_kernel)
                 const int cur = s % 3, prv = (s + 2) % 3, old = (s + 1) % 3;
                 const int above = dir > 0 ? old : cur, below = dir > 0 ? cur : old;
@@ -1034,7 +1080,7 @@ namespace
                     {
                         bb[k] = ba[k];
                         ba[k] = bc[k];
-                        bc[k] = valid ? bq[k] : 0.0f;
+                        bc[k] = valid ? band_in(bq, k) : 0.0f;
                     }
 #pragma unroll
                     for (int q = 0; q < K; ++q)
@@ -1117,7 +1163,8 @@ namespace
                     }
                 }
             };
-            float bA[10], bB[10], pA[K][3], pB[K][3];
+            BandIn bA, bB;
+            float pA[K][3], pB[K][3];
             load(0, bA, pA);
             load(1, bB, pB);
             for (int s = 0; s < steps; s += 2)
@@ -1690,7 +1737,10 @@ namespace
         (const void *)v_post2_kernel<0, 0, 0, KS>, (const void *)v_post2_kernel<0, 0, 1, KS>,                  \
         (const void *)v_post2_kernel<0, 1, 0, KS>, (const void *)v_post2_kernel<0, 1, 1, KS>,                  \
         (const void *)v_post2_kernel<2, 0, 0, KS>, (const void *)v_post2_kernel<2, 0, 1, KS>,                  \
-        (const void *)v_post2_kernel<2, 1, 0, KS>, (const void *)v_post2_kernel<2, 1, 1, KS>
+        (const void *)v_post2_kernel<2, 1, 0, KS>, (const void *)v_post2_kernel<2, 1, 1, KS>,                  \
+        (const void *)v_pre2r_kernel<0, 0, KS, true>, (const void *)v_pre2r_kernel<0, 1, KS, true>,            \
+        (const void *)v_post2_kernel<0, 0, 0, KS, true>, (const void *)v_post2_kernel<0, 0, 1, KS, true>,      \
+        (const void *)v_post2_kernel<0, 1, 0, KS, true>, (const void *)v_post2_kernel<0, 1, 1, KS, true>
         const void * fns[] = {HS_K(1), HS_K(kPostSlots)};
 #undef HS_K
         for (const void * f : fns)
@@ -1750,6 +1800,79 @@ void hierarchy_fill_poisson(Hierarchy & h, hipStream_t st)
                                h.B64(L), (int64_t)l.n, l.side);
     }
     HIP_CHECK(hipGetLastError());
+    hierarchy_codes(h);
+}
+
+// Value-indexed storage of a Q1 level's bands and dinv (the value-index
+// compression of Kourtis, Goumas and Koziris, "Optimizing sparse
+// matrix-vector multiplication using index and value compression", 2008):
+// when each of the ten arrays holds at most 16 distinct values (as bit
+// patterns) over the level's n nodes, node i's ten 4-bit indices go into one
+// 64-bit word and the values into a 16-entry table per array.  The one-pass
+// kernels then read 8 bytes per node instead of 40 and look the values up
+// in LDS: the same floats, so the same bits.  Levels whose arrays hold more
+// values keep the plain bands only.
+void hierarchy_codes(Hierarchy & h)
+{
+    if (h.kind != 0)
+        return;
+    HIP_CHECK(hipDeviceSynchronize());
+    for (uint64_t L = h.p.min_level; L <= h.max_level; ++L)
+    {
+        HierLevel & l = h.lv[L];
+        l.codes = nullptr;
+        l.ctab = nullptr;
+        if (l.n < 4225)     // the one-pass kernels run on levels of 65^2 and up
+            continue;
+        const uint64_t n = l.n;
+        std::vector<uint32_t> v(n);
+        std::vector<uint64_t> code(n, 0);
+        uint32_t tab[160] = {};
+        bool ok = true;
+        for (int k = 0; k < 10 && ok; ++k)
+        {
+            const float * src = k < 9 ? l.bands + (uint64_t)k * l.bs : l.dinv;
+            HIP_CHECK(hipMemcpy(v.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
+            int cnt = 0;
+            uint32_t last = 0;
+            uint64_t last_t = 0;
+            for (uint64_t i = 0; i < n; ++i)
+            {
+                const uint32_t bits = v[i];
+                uint64_t t = last_t;
+                if (cnt == 0 || bits != last)
+                {
+                    int f = -1;
+                    for (int q = 0; q < cnt; ++q)
+                        if (tab[16 * k + q] == bits)
+                        {
+                            f = q;
+                            break;
+                        }
+                    if (f < 0)
+                    {
+                        if (cnt == 16)
+                        {
+                            ok = false;
+                            break;
+                        }
+                        f = cnt;
+                        tab[16 * k + cnt++] = bits;
+                    }
+                    t = (uint64_t)f;
+                    last = bits;
+                    last_t = t;
+                }
+                code[i] |= t << (4 * k);
+            }
+        }
+        if (! ok)
+            continue;
+        l.codes = (uint64_t *)h.alloc(sizeof(uint64_t) * n);
+        l.ctab = (float *)h.alloc(sizeof(tab));
+        HIP_CHECK(hipMemcpy(l.codes, code.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(l.ctab, tab, sizeof(tab), hipMemcpyHostToDevice));
+    }
 }
 
 namespace
@@ -2025,6 +2148,8 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         a.n = (int64_t)l.n;
         a.m = l.side;
         a.bs = (int64_t)l.bs;
+        a.codes = l.codes;
+        a.ctab = l.ctab;
         return a;
     };
     auto norm_grid = [&](uint64_t nn) { return dim3(std::min<unsigned>(lanes4(nn), kNormBlocks), 1, S); };
@@ -2158,10 +2283,24 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                     auto go = [&](auto acc_tag, auto k_tag) {
                         constexpr int ACC = decltype(acc_tag)::value, KS = decltype(k_tag)::value;
                         const unsigned groups = (S + KS - 1) / KS;
-                        const int th = post_rows(a.m, groups, post_resident((const void *)v_pre2r_kernel<MK, ACC, KS>));
-                        hipLaunchKernelGGL((v_pre2r_kernel<MK, ACC, KS>), dim3(post_tiles(a.m, th), 1, groups),
-                                           dim3(kPostLanes), 0, st, a, T, (int)L, dsrc, ACC ? fx_top : (int)F_X, Nc, p.mask,
-                                           th, (int)S);
+                        auto launch = [&](auto cb_tag) {
+                            constexpr bool CB = decltype(cb_tag)::value;
+                            const int th =
+                                    post_rows(a.m, groups, post_resident((const void *)v_pre2r_kernel<MK, ACC, KS, CB>));
+                            hipLaunchKernelGGL((v_pre2r_kernel<MK, ACC, KS, CB>), dim3(post_tiles(a.m, th), 1, groups),
+                                               dim3(kPostLanes), 0, st, a, T, (int)L, dsrc, ACC ? fx_top : (int)F_X, Nc,
+                                               p.mask, th, (int)S);
+                        };
+                        // value-coded bands (Q1 levels whose bands allow them)
+                        if constexpr (MK == 0)
+                        {
+                            if (a.codes)
+                                launch(std::true_type{});
+                            else
+                                launch(std::false_type{});
+                        }
+                        else
+                            launch(std::false_type{});
                     };
                     if (L == maxl)
                         S > 1 ? go(IntC<1>{}, IntC<kPostSlots>{}) : go(IntC<1>{}, IntC<1>{});
@@ -2244,10 +2383,22 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         auto go = [&](auto def_tag, auto k_tag) {
             constexpr int DEF = decltype(def_tag)::value, KS = decltype(k_tag)::value;
             const unsigned groups = (S + KS - 1) / KS;
-            const int th = post_rows(a.m, groups, post_resident((const void *)v_post2_kernel<MK, DEF, PROL, KS>));
-            hipLaunchKernelGGL((v_post2_kernel<MK, DEF, PROL, KS>), dim3(post_tiles(a.m, th), 1, groups),
-                               dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X, (int)F_TMP, fcoarse, alpha, p.mask, th,
-                               (int)S);
+            auto launch = [&](auto cb_tag) {
+                constexpr bool CB = decltype(cb_tag)::value;
+                const int th = post_rows(a.m, groups, post_resident((const void *)v_post2_kernel<MK, DEF, PROL, KS, CB>));
+                hipLaunchKernelGGL((v_post2_kernel<MK, DEF, PROL, KS, CB>), dim3(post_tiles(a.m, th), 1, groups),
+                                   dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X, (int)F_TMP, fcoarse, alpha, p.mask,
+                                   th, (int)S);
+            };
+            if constexpr (MK == 0)
+            {
+                if (a.codes)
+                    launch(std::true_type{});
+                else
+                    launch(std::false_type{});
+            }
+            else
+                launch(std::false_type{});
         };
         if (L == maxl)
             S > 1 ? go(IntC<1>{}, IntC<kPostSlots>{}) : go(IntC<1>{}, IntC<1>{});

@@ -145,6 +145,7 @@ int hsolve_mg_create(const hsolve_mg_params * params, const double * const * ban
         else
             mg::hierarchy_dinv(h, l, 0);
     }
+    mg::hierarchy_codes(h);
     HIP_CHECK(hipDeviceSynchronize());
     *out = mg.release();
     HSOLVE_API_END
