@@ -149,6 +149,8 @@ namespace mg
         HierLevel lv[kMaxLevels];
         double * bands64[kMaxLevels] = {};   // FP64 bands of each finest level solved on
         uint64_t bs64[kMaxLevels] = {};
+        uint64_t * codes64[kMaxLevels] = {};  // value codes of the FP64 bands (as HierLevel::codes,
+        double * ctab64[kMaxLevels] = {};     // nine arrays), or null
         EllMat64 ell64[kMaxLevels];          // FP64 finest ELL matrix (ELL hierarchies)
         std::vector<void *> allocations;
         uint64_t bytes = 0;

@@ -205,10 +205,21 @@ namespace
     // od = b - A64 result and its norm; opens the next inner solve or ends
     // the outer loop.  On a start: the initial defect.  MK 0: Q1 bands B
     // (two rows per lane, 16-byte loads inside); MK 1: the ELL matrix E.
-    template <int MK>
+    //
+    // CB: the interior rows read the bands as value codes (one 64-bit word
+    // per row, band k in bits 4k .. 4k + 3) and look them up in ctab
+    // (16 per band, copied to LDS); the same values, the same sums.
+    template <int MK, bool CB = false>
     __global__ __launch_bounds__(256) void outer_defect_kernel(k::Bands B, EllMat64 E, const Slot * T, int64_t n,
-            int64_t m)
+            int64_t m, const uint64_t * __restrict__ codes, const double * __restrict__ ctab)
     {
+        __shared__ double tab[CB ? 144 : 1];
+        if constexpr (CB)
+        {
+            if (threadIdx.x < 144)
+                tab[threadIdx.x] = ctab[threadIdx.x];
+            __syncthreads();
+        }
         const VB vb = vblock();
         const Slot & sl = T[vb.z];
         if (! live(sl))
@@ -239,7 +250,39 @@ namespace
             {
                 // b and x are the caller's arrays (8-byte aligned), od is the slot's
                 double prod[2];
-                k::q1_rows_v<double, kOrd<MK>>(B, x, i0, m, prod);
+                if constexpr (CB)
+                {
+                    // q1_rows_v's sums (Product<Q1>'s order: dd first) with
+                    // the band values from the codes of rows i0, i0 + 1
+                    typedef uint64_t U2 __attribute__((ext_vector_type(2)));
+                    const U2 cw = __builtin_nontemporal_load((const U2 *)(codes + i0));
+                    double xm[4], xc[4], xp[4];
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj)
+                    {
+                        xm[jj] = x[i0 - m - 1 + jj];
+                        xc[jj] = x[i0 - 1 + jj];
+                        xp[jj] = x[i0 + m - 1 + jj];
+                    }
+#pragma unroll
+                    for (int v = 0; v < 2; ++v)
+                    {
+                        const uint64_t w = cw[v];
+                        auto bk = [&](int k) { return tab[16 * k + (int)((w >> (4 * k)) & 15u)]; };
+                        double sum = bk(4) * xc[v + 1];
+                        sum += bk(0) * xm[v];
+                        sum += bk(1) * xm[v + 1];
+                        sum += bk(2) * xm[v + 2];
+                        sum += bk(3) * xc[v];
+                        sum += bk(5) * xc[v + 2];
+                        sum += bk(6) * xp[v];
+                        sum += bk(7) * xp[v + 1];
+                        sum += bk(8) * xp[v + 2];
+                        prod[v] = sum;
+                    }
+                }
+                else
+                    k::q1_rows_v<double, kOrd<MK>>(B, x, i0, m, prod);
                 typedef k::Vec16<double>::type V2;
                 V2 o;
                 o[0] = __builtin_nontemporal_load(b + i0) - prod[0];
@@ -1873,6 +1916,60 @@ void hierarchy_codes(Hierarchy & h)
         HIP_CHECK(hipMemcpy(l.codes, code.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(l.ctab, tab, sizeof(tab), hipMemcpyHostToDevice));
     }
+    // the FP64 bands of the finest levels solved on (the outer defect)
+    for (uint64_t L = h.p.min_level; L <= h.max_level; ++L)
+    {
+        h.codes64[L] = nullptr;
+        h.ctab64[L] = nullptr;
+        if (! h.bands64[L] || size_of(L) < 4225)
+            continue;
+        const uint64_t n = size_of(L);
+        std::vector<uint64_t> v(n), code(n, 0);
+        uint64_t tab[144] = {};
+        bool ok = true;
+        for (int k = 0; k < 9 && ok; ++k)
+        {
+            HIP_CHECK(hipMemcpy(v.data(), h.bands64[L] + (uint64_t)k * h.bs64[L], sizeof(double) * n,
+                                hipMemcpyDeviceToHost));
+            int cnt = 0;
+            uint64_t last = 0, last_t = 0;
+            for (uint64_t i = 0; i < n; ++i)
+            {
+                const uint64_t bits = v[i];
+                uint64_t t = last_t;
+                if (cnt == 0 || bits != last)
+                {
+                    int f = -1;
+                    for (int q = 0; q < cnt; ++q)
+                        if (tab[16 * k + q] == bits)
+                        {
+                            f = q;
+                            break;
+                        }
+                    if (f < 0)
+                    {
+                        if (cnt == 16)
+                        {
+                            ok = false;
+                            break;
+                        }
+                        f = cnt;
+                        tab[16 * k + cnt++] = bits;
+                    }
+                    t = (uint64_t)f;
+                    last = bits;
+                    last_t = t;
+                }
+                code[i] |= t << (4 * k);
+            }
+        }
+        if (! ok)
+            continue;
+        h.codes64[L] = (uint64_t *)h.alloc(sizeof(uint64_t) * n);
+        h.ctab64[L] = (double *)h.alloc(sizeof(tab));
+        HIP_CHECK(hipMemcpy(h.codes64[L], code.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(h.ctab64[L], tab, sizeof(tab), hipMemcpyHostToDevice));
+    }
 }
 
 namespace
@@ -2228,8 +2325,21 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     hipLaunchKernelGGL(outer_accumulate_kernel, ggrid(grid_for(top.n, kBlock)), blk, 0, st, T, (int)maxl, n,
                        fx_top);
     const unsigned od_blocks = std::min<unsigned>(grid_for((top.n + 1) / 2, kBlock), kNormBlocks);
-    hipLaunchKernelGGL((outer_defect_kernel<MK>), dim3(od_blocks, 1, S), blk, 0, st,
-                       kBands<MK> ? h.B64(maxl) : k::Bands{}, h.ell64[maxl], T, n, top.side);
+    bool od_coded = false;
+    if constexpr (MK == 0)
+    {
+        if (h.codes64[maxl])
+        {
+            hipLaunchKernelGGL((outer_defect_kernel<MK, true>), dim3(od_blocks, 1, S), blk, 0, st, h.B64(maxl),
+                               h.ell64[maxl], T, n, top.side, (const uint64_t *)h.codes64[maxl],
+                               (const double *)h.ctab64[maxl]);
+            od_coded = true;
+        }
+    }
+    if (! od_coded)
+        hipLaunchKernelGGL((outer_defect_kernel<MK>), dim3(od_blocks, 1, S), blk, 0, st,
+                           kBands<MK> ? h.B64(maxl) : k::Bands{}, h.ell64[maxl], T, n, top.side,
+                           (const uint64_t *)nullptr, (const double *)nullptr);
     hipLaunchKernelGGL(outer_defect_finish_kernel, dim3(1, 1, S), blk, 0, st, T, od_blocks, (double)p.tolerance,
                        p.max_outer);
     hipLaunchKernelGGL(to_inner_kernel, ggrid(grid_for(top.n, kBlock)), blk, 0, st, T, (int)maxl, n);
