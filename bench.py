@@ -212,7 +212,9 @@ def mg_case(hip, P, level, min_level, reps, tol=1e-8):
                 ms_p50=p50, ms_p99=float(np.percentile(lat, 99)),
                 solves_per_s=float(1e3 / lat.mean()), device_bytes=mg.device_bytes,
                 roofline=dict(bound="hbm", bytes_per_solve=nbytes, achieved=round(gbps, 1), peak=HBM_PEAK_GBS,
-                              unit="GB/s", frac=round(gbps / HBM_PEAK_GBS, 4))), x
+                              unit="GB/s", frac=round(gbps / HBM_PEAK_GBS, 4),
+                              model="per-operation bytes of the reference's V-cycle and outer step (36 B per node "
+                                    "per band pass); the value-coded one-pass kernels read less, so frac can pass 1")), x
 
 
 def serve_trace(count, rate, seed):
