@@ -143,6 +143,7 @@ _SIGS = {
     "hsolve_mg_create_poisson": [C.POINTER(MGParams), C.POINTER(vp)],
     "hsolve_mg_destroy": [vp],
     "hsolve_mg_device_bytes": [vp, C.POINTER(u64)],
+    "hsolve_mg_set_value_codes": [vp, C.c_int],
     "hsolve_mg_solve": [vp, vp, vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl), vp],
     "hsolve_mg_create_ell": [C.POINTER(MGParams), vp, vp, vp, vp, vp, u64, C.POINTER(vp)],
     "hsolve_mg_slots": [vp, C.POINTER(u64)],

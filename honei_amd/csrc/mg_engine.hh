@@ -154,6 +154,7 @@ namespace mg
         EllMat64 ell64[kMaxLevels];          // FP64 finest ELL matrix (ELL hierarchies)
         std::vector<void *> allocations;
         uint64_t bytes = 0;
+        bool use_codes = true;               // read the value codes where a level has them
 
         void * alloc(size_t b);
         k::Bands B64(uint64_t L) const

@@ -2245,8 +2245,8 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
         a.n = (int64_t)l.n;
         a.m = l.side;
         a.bs = (int64_t)l.bs;
-        a.codes = l.codes;
-        a.ctab = l.ctab;
+        a.codes = h.use_codes ? l.codes : nullptr;
+        a.ctab = h.use_codes ? l.ctab : nullptr;
         return a;
     };
     auto norm_grid = [&](uint64_t nn) { return dim3(std::min<unsigned>(lanes4(nn), kNormBlocks), 1, S); };
@@ -2328,7 +2328,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
     bool od_coded = false;
     if constexpr (MK == 0)
     {
-        if (h.codes64[maxl])
+        if (h.use_codes && h.codes64[maxl])
         {
             hipLaunchKernelGGL((outer_defect_kernel<MK, true>), dim3(od_blocks, 1, S), blk, 0, st, h.B64(maxl),
                                h.ell64[maxl], T, n, top.side, (const uint64_t *)h.codes64[maxl],

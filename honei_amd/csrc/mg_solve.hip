@@ -170,6 +170,14 @@ int hsolve_mg_destroy(hsolve_mg mg)
     HSOLVE_API_END
 }
 
+int hsolve_mg_set_value_codes(hsolve_mg mg, int enable)
+{
+    HSOLVE_API_BEGIN
+    require(mg != nullptr, HSOLVE_ERR_INVALID, "mg_set_value_codes: null handle");
+    mg->h.use_codes = enable != 0;
+    HSOLVE_API_END
+}
+
 int hsolve_mg_device_bytes(hsolve_mg mg, uint64_t * bytes)
 {
     HSOLVE_API_BEGIN

@@ -467,6 +467,10 @@ class MGSolver:
         check(lib().hsolve_mg_device_bytes(vp(self.handle), C.byref(nb)), "mg_device_bytes")
         self.device_bytes = nb.value
 
+    def set_value_codes(self, enable: bool):
+        """Read the value-coded bands where the hierarchy has them (default) or the plain bands."""
+        check(lib().hsolve_mg_set_value_codes(vp(self.handle), int(bool(enable))), "mg_set_value_codes")
+
     @classmethod
     def from_ell(cls, params: MGParams, A64, a32, dinv32=None, prol32=None, res32=None, slots=1):
         """Multigrid<..., MIXED> on ELL matrices (hsolve_mg_create_ell).  A64: FP64 ELL dict

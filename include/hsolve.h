@@ -455,6 +455,13 @@ int hsolve_mg_create(const hsolve_mg_params *params, const double *const *bands6
 int hsolve_mg_create_poisson(const hsolve_mg_params *params, hsolve_mg *out);
 int hsolve_mg_destroy(hsolve_mg mg);
 int hsolve_mg_device_bytes(hsolve_mg mg, uint64_t *bytes);
+/* No counterpart in the reference (it concerns this backend's storage of the
+ * band values): a Q1 hierarchy whose levels' bands and inverted diagonals hold
+ * at most 16 distinct values each also keeps them as value codes (4-bit table
+ * indices, 8 bytes per node) and its V-cycle and outer-defect kernels read
+ * those.  enable = 0 makes the handle's solves read the plain bands, 1 (the
+ * default) the codes where they exist; both give the same bits. */
+int hsolve_mg_set_value_codes(hsolve_mg mg, int enable);
 /* b, x: FP64 device vectors of the finest size; x in: initial guess, out: result. */
 int hsolve_mg_solve(hsolve_mg mg, const void *b, void *x, uint64_t *outer_iters,
                     uint64_t *inner_iters, double *final_defect, double *initial_defect,

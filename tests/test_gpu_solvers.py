@@ -141,6 +141,74 @@ def test_mg_mixed_poisson_levels(hip, oracle, maxl, minl):
         info["final_defect"] < float(np.float32(1e-10)) * info["initial_defect"]
 
 
+def test_mg_value_codes_same_bits(hip):
+    """Poisson bands hold two values per band, so the hierarchy keeps value
+    codes and the V-cycle and outer-defect kernels read them; with
+    hsolve_mg_set_value_codes(0) the same handle reads the plain bands.  Both
+    give the same counts, defects and iterate bits (device-assembled and
+    host-uploaded hierarchies)."""
+    for maxl, minl, host in ((9, 2, False), (10, 2, False), (9, 2, True)):
+        n = P.side_of_level(maxl) ** 2
+        b = P.rhs_uniform(n)
+        if host:
+            a32, d32 = _levels(maxl, minl)
+            mg = hip.MGSolver(hip.mg_params(minl, maxl, tolerance=1e-10, max_outer=40), P.poisson_q1_bands(n),
+                              a32, d32)
+        else:
+            mg = hip.MGSolver(hip.mg_params(minl, maxl, tolerance=1e-10, max_outer=40))
+        bd = hip.DeviceVector.from_numpy(b)
+        runs = []
+        for enable in (True, False, True):
+            mg.set_value_codes(enable)
+            xd = hip.DeviceVector.zeros(n)
+            info = mg.solve(bd, xd)
+            runs.append((info["outer"], info["inner"], info["final_defect"], xd.get()))
+        for r in runs[1:]:
+            assert r[:3] == runs[0][:3], (maxl, host, r[:3], runs[0][:3])
+            assert np.array_equal(r[3], runs[0][3]), (maxl, host)
+
+
+def _scaled_diagonal(bands, seed):
+    """Q1 bands with the diagonal scaled node by node by 1 + k 2^-20, k in
+    0..31: 32 distinct diagonal values (FP32 too), more than a value code
+    holds.  (Steps of 2^-10 make the reference's MG-IR diverge: the oracle
+    ends in NaN.)"""
+    g = np.random.default_rng(seed)
+    out = [np.array(x, copy=True) for x in bands]
+    f = 1.0 + g.integers(0, 32, out[4].size) * 2.0 ** -20
+    out[4] = (out[4] * f).astype(out[4].dtype)
+    return out
+
+
+def test_mg_plain_bands_when_values_exceed_codes(hip, oracle):
+    """A finest level whose diagonal holds more than 16 values keeps the plain
+    bands, FP32 and FP64 (the coarser Poisson levels keep their codes: the
+    hierarchy holds fewer device bytes than the all-Poisson one), and the
+    solve still meets the oracle's Multigrid<MIXED>.  Both sides cap the outer
+    steps, so a solve that did not converge would end and fail the test."""
+    maxl, minl = 9, 2
+    n = P.side_of_level(maxl) ** 2
+    A64 = _scaled_diagonal(P.poisson_q1_bands(n), 1)
+    a32, d32 = {}, {}
+    for L in range(minl, maxl + 1):
+        nl = P.side_of_level(L) ** 2
+        a32[L] = [x.astype(np.float32) for x in (A64 if L == maxl else P.poisson_q1_bands(nl))]
+        d32[L] = dinv_reference(a32[L][4])
+    b = P.rhs_uniform(n)
+    ref = oracle.mg_mixed_q1(A64, b, np.zeros(n), a32, d32, min_level=minl, max_level=maxl, tolerance=1e-10,
+                             max_outer=40)
+    assert ref["status"] == 0 and ref["outer"] < 40, ref["outer"]
+    prm = hip.mg_params(minl, maxl, tolerance=1e-10, max_outer=40)
+    mg = hip.MGSolver(prm, A64, a32, d32)
+    pa32, pd32 = _levels(maxl, minl)
+    coded = hip.MGSolver(prm, P.poisson_q1_bands(n), pa32, pd32)
+    assert mg.device_bytes < coded.device_bytes
+    bd, xd = hip.DeviceVector.from_numpy(b), hip.DeviceVector.zeros(n)
+    info = mg.solve(bd, xd)
+    assert abs(info["outer"] - ref["outer"]) <= 1 and abs(info["inner"] - ref["inner"]) <= 2, (info, ref)
+    assert _rel(xd.get(), ref["x"]) <= 1e-6
+
+
 def _jac_inverse(E, dtype):
     n = E["rows"]
     Aj = E["Aj"].reshape(E["ncpr"], E["stride"])[:, :n]
