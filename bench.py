@@ -188,9 +188,11 @@ def mg_solve_bytes(level, min_level, inner, outer):
     return inner * cycle + outer * 128 * n(level)
 
 
-def mg_case(hip, P, level, min_level, reps, tol=1e-8):
+def mg_case(hip, P, level, min_level, reps, tol=1e-8, value_codes=True):
+    """value_codes False: the same solve reading the plain bands (hsolve_mg_set_value_codes(0))."""
     n = P.side_of_level(level) ** 2
     mg = hip.MGSolver(hip.mg_params(min_level, level, n_max_iter=2, tolerance=tol))
+    mg.set_value_codes(value_codes)
     b = hip.DeviceVector.from_numpy(P.rhs_uniform(n))
     x = hip.DeviceVector.zeros(n)
     lat = []
@@ -687,7 +689,9 @@ def main():
         del x_mg
         mg10, _ = mg_case(hip, P, 12, 2, reps=3, tol=1e-10)
         extra["mg_ir_poisson_4097_tol1e-10"] = mg10
-        log("mg", mg, mg10)
+        mgp, _ = mg_case(hip, P, 12, 2, reps=3, tol=1e-8, value_codes=False)
+        extra["mg_ir_poisson_4097_tol1e-8_plain_bands"] = mgp
+        log("mg", mg, mg10, mgp)
         extra["cg_fp64_poisson_257"] = cg257_case(hip, P, cpu=not args.no_cpu)
         log("cg 257", extra["cg_fp64_poisson_257"])
         cg = cg_case(hip, P, 4097, 100000)
