@@ -123,8 +123,8 @@ namespace mg
         EllMat32 ell;             // the level matrix (ELL hierarchies)
         EllMat32 prol, res;       // PROLMAT transfers to and from level - 1
         float * dinv = nullptr;
-        // value codes of the nine bands and dinv (Q1 hierarchies, when every
-        // one of the ten arrays holds at most 16 distinct values): node i's
+        // value codes of the nine bands and dinv (hierarchies on bands, when
+        // every one of the ten arrays holds at most 16 distinct values): node i's
         // ten 4-bit table indices in one word, array k in bits 4k .. 4k + 3,
         // and the tables, ctab[16 k + t]; null otherwise
         uint64_t * codes = nullptr;
@@ -180,8 +180,9 @@ namespace mg
     void hierarchy_fill_poisson(Hierarchy & h, hipStream_t st);
     // dinv = (float)((double)(1/dd) * 0.7) on level L
     void hierarchy_dinv(Hierarchy & h, uint64_t L, hipStream_t st);
-    // value codes (HierLevel::codes) of every Q1 level the one-pass kernels
-    // run on, where the bands and dinv allow them; call once they are final
+    // value codes (HierLevel::codes) of every level on bands the one-pass
+    // kernels run on, where the bands and dinv allow them, and of the FP64
+    // finest bands of a Q1 hierarchy; call once they are final
     void hierarchy_codes(Hierarchy & h);
 
     // An ELL matrix in HONEI's host layout (threads == 1: slot k of row r

@@ -1783,7 +1783,10 @@ namespace
         (const void *)v_post2_kernel<2, 1, 0, KS>, (const void *)v_post2_kernel<2, 1, 1, KS>,                  \
         (const void *)v_pre2r_kernel<0, 0, KS, true>, (const void *)v_pre2r_kernel<0, 1, KS, true>,            \
         (const void *)v_post2_kernel<0, 0, 0, KS, true>, (const void *)v_post2_kernel<0, 0, 1, KS, true>,      \
-        (const void *)v_post2_kernel<0, 1, 0, KS, true>, (const void *)v_post2_kernel<0, 1, 1, KS, true>
+        (const void *)v_post2_kernel<0, 1, 0, KS, true>, (const void *)v_post2_kernel<0, 1, 1, KS, true>,      \
+        (const void *)v_pre2r_kernel<2, 0, KS, true>, (const void *)v_pre2r_kernel<2, 1, KS, true>,            \
+        (const void *)v_post2_kernel<2, 0, 0, KS, true>, (const void *)v_post2_kernel<2, 0, 1, KS, true>,      \
+        (const void *)v_post2_kernel<2, 1, 0, KS, true>, (const void *)v_post2_kernel<2, 1, 1, KS, true>
         const void * fns[] = {HS_K(1), HS_K(kPostSlots)};
 #undef HS_K
         for (const void * f : fns)
@@ -1846,7 +1849,8 @@ void hierarchy_fill_poisson(Hierarchy & h, hipStream_t st)
     hierarchy_codes(h);
 }
 
-// Value-indexed storage of a Q1 level's bands and dinv (the value-index
+// Value-indexed storage of a level's bands (Q1, or a 9-point structured ELL
+// matrix kept as bands) and dinv (the value-index
 // compression of Kourtis, Goumas and Koziris, "Optimizing sparse
 // matrix-vector multiplication using index and value compression", 2008):
 // when each of the ten arrays holds at most 16 distinct values (as bit
@@ -1857,7 +1861,7 @@ void hierarchy_fill_poisson(Hierarchy & h, hipStream_t st)
 // values keep the plain bands only.
 void hierarchy_codes(Hierarchy & h)
 {
-    if (h.kind != 0)
+    if (h.kind == 1)    // ELL matrices: no bands
         return;
     HIP_CHECK(hipDeviceSynchronize());
     for (uint64_t L = h.p.min_level; L <= h.max_level; ++L)
@@ -1916,8 +1920,9 @@ void hierarchy_codes(Hierarchy & h)
         HIP_CHECK(hipMemcpy(l.codes, code.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(l.ctab, tab, sizeof(tab), hipMemcpyHostToDevice));
     }
-    // the FP64 bands of the finest levels solved on (the outer defect)
-    for (uint64_t L = h.p.min_level; L <= h.max_level; ++L)
+    // the FP64 bands of the finest levels solved on (the outer defect, whose
+    // coded path forms Product<Q1>'s order: Q1 hierarchies only)
+    for (uint64_t L = h.p.min_level; h.kind == 0 && L <= h.max_level; ++L)
     {
         h.codes64[L] = nullptr;
         h.ctab64[L] = nullptr;
@@ -2102,6 +2107,7 @@ void hierarchy_alloc_ell(Hierarchy & h, const Params & p, uint64_t max_level, co
         }
         const std::vector<double> b64 = ell_bands<double>(A64, max_level, h.bs64[max_level]);
         HIP_CHECK(hipMemcpy(h.bands64[max_level], b64.data(), sizeof(double) * b64.size(), hipMemcpyHostToDevice));
+        hierarchy_codes(h);
         return;
     }
     h.p = p;
@@ -2401,8 +2407,8 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                                                dim3(kPostLanes), 0, st, a, T, (int)L, dsrc, ACC ? fx_top : (int)F_X, Nc,
                                                p.mask, th, (int)S);
                         };
-                        // value-coded bands (Q1 levels whose bands allow them)
-                        if constexpr (MK == 0)
+                        // value-coded bands (levels whose bands allow them)
+                        if constexpr (MK != 1)
                         {
                             if (a.codes)
                                 launch(std::true_type{});
@@ -2500,7 +2506,7 @@ void tick_impl(const Hierarchy & h, uint64_t maxl, const Slot * T, unsigned S, h
                                    dim3(kPostLanes), 0, st, a, T, (int)L, (int)F_X, (int)F_TMP, fcoarse, alpha, p.mask,
                                    th, (int)S);
             };
-            if constexpr (MK == 0)
+            if constexpr (MK != 1)
             {
                 if (a.codes)
                     launch(std::true_type{});

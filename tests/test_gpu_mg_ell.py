@@ -67,3 +67,29 @@ def test_mg_ell_batch_equals_single(hip, oracle):
     ref = oracle.mg_mixed(kind="ell", b=P.rhs_uniform(n, seed=P.SEED + 2), x0=np.zeros(n), A64=A64, a32=a32,
                           dinv32=d32, min_level=minl, max_level=maxl, n_max_iter=2, tolerance=1e-8)
     assert _rel(xs[2].get(), ref["x"]) <= 1e-6
+
+
+def test_mg_ell_value_codes_same_bits(hip, oracle):
+    """The Poisson ELL levels kept as bands hold two values per band, so the
+    hierarchy keeps value codes for its levels of 65^2 and up; the solve with
+    them and with the plain bands (hsolve_mg_set_value_codes) gives the same
+    counts, defects and iterate bits, single and batched."""
+    maxl, minl = 8, 2
+    n = P.side_of_level(maxl) ** 2
+    a32, d32, _, _ = _hier(oracle, minl, maxl, False)
+    A64 = P.q1_to_ell(P.poisson_q1_bands(n))
+    mg = hip.MGSolver.from_ell(hip.mg_params(minl, maxl, tolerance=1e-8, max_outer=40), A64, a32, d32, slots=2)
+    bs = [hip.DeviceVector.from_numpy(P.rhs_uniform(n, seed=P.SEED + j)) for j in range(2)]
+    runs = []
+    for enable in (True, False):
+        mg.set_value_codes(enable)
+        x1 = hip.DeviceVector.zeros(n)
+        single = mg.solve(bs[0], x1)
+        xs = [hip.DeviceVector.zeros(n) for _ in range(2)]
+        infos = mg.solve_batch(bs, xs)
+        runs.append(((single["outer"], single["inner"], single["final_defect"]), x1.get(),
+                     [(i["outer"], i["inner"]) for i in infos], [x.get() for x in xs]))
+    assert runs[0][0] == runs[1][0] and runs[0][2] == runs[1][2]
+    assert np.array_equal(runs[0][1], runs[1][1])
+    for a, b in zip(runs[0][3], runs[1][3]):
+        assert np.array_equal(a, b)
